@@ -1,0 +1,182 @@
+"""Throughput of the simulated-bifurcation step on one MI355X.
+
+Workload (default): MaxCut dSB on a random 4-regular graph, N = 2^20 nodes,
+Z = 4 N = 4,194,304 symmetric nonzeros, weights +-1 uniform from seed, 256
+agents, dt = 0.5, max_steps = K (20,000 by default), windows every 100 steps.
+A "step" is one fused SB step over all N x 256 node-agents.  The headline
+form is the bit-packed-sign dSB kernel; the plain FP32 dSB kernel runs the
+same trajectory (bit-identical) beside it for the speed-up and the
+equal-steps best-cut comparison.
+
+The timed region is one sis_solver_run (pre-step + K steps + K/100 windows),
+with the state resident in HBM; update_window is off inside the timed region
+so that exactly K steps run.  The step-kernel time comes from HIP events the
+library records around the step launches on the solver's stream.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--form packed|f32|bf16|f64]
+                    [--baseline-form f32] [--cpu-seconds S] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "sparse-ising-solver_amd", "python"))
+
+import numpy as np  # noqa: E402
+
+import sis  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "agent-steps/s + HBM GB/s % of peak; best cut vs plain FP32 SB at equal steps"
+
+
+def step_bytes(form, N, Z, A):
+    """Algorithmic HBM bytes of one step launch over all agents."""
+    csr = 8 * (N + 1) + 4 * Z + 4 * Z  # row_ptr int64, col int32, val fp32
+    state = 16 * N * A                 # X, Y read + write, FP32
+    if form == "packed":
+        return state + N * A // 8 + Z * A // 8 + csr
+    if form == "bf16":
+        return state + 2 * N * A + 2 * Z * A + csr
+    if form == "f64":
+        return 32 * N * A + 8 * Z * A + 8 * (N + 1) + 4 * Z + 8 * Z
+    return state + 4 * Z * A + csr      # plain f32
+
+
+def time_form(J, form, A, dt, K, W, seed):
+    dyn = sis.BSB if form == "bf16" else sis.DSB
+    s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=A, dt=dt, form=sis.FORMS[form], dynamics=dyn))
+    if W > 0:  # untimed warm-up steps (same kernels, discarded state)
+        s.init_random(seed + 1)
+        s.advance(0, W, max(K, 1))
+    s.init_random(seed)
+    t0 = time.perf_counter()
+    info = s.run(K, 0)
+    wall = time.perf_counter() - t0
+    cuts, best = s.cuts()
+    s.close()
+    return dict(info=info, wall=wall, best_cut=float(cuts.max()), best_agent=int(best),
+                mean_cut=float(cuts.mean()))
+
+
+def cpu_baseline(J, A, dt, xi0, seconds, seed):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+
+    rp, col, val = J.arrays()
+    pb = orc.Problem(rp, col, val, S=J.sum)
+    X, Y = orc.init_state(seed, orc.MAXCUT, J.n, A)
+    T = 20000
+    t0 = time.perf_counter()
+    orc.advance(pb, orc.MAXCUT, orc.DSB, X, Y, 0, 1, T, dt, xi0, 1.0, "f32")
+    t1 = time.perf_counter() - t0
+    n = max(1, int(seconds / max(t1, 1e-6)) - 1)
+    t0 = time.perf_counter()
+    orc.advance(pb, orc.MAXCUT, orc.DSB, X, Y, 1, n, T, dt, xi0, 1.0, "f32")
+    el = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return dict(value=n * A / el, unit="agent-steps/s", cores=threads, kind="port",
+                sample=f"oracle FP32 dSB step (OpenMP, {threads} threads) on the same graph and "
+                       f"agents, {n} steps after 1 warm-up step, {el:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--form", default="packed", choices=list(sis.FORMS))
+    ap.add_argument("--baseline-form", default="f32", choices=list(sis.FORMS) + ["none"])
+    ap.add_argument("--nodes-log2", type=int, default=20)
+    ap.add_argument("--agents", type=int, default=256)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=2024)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: the solver runs on a single device (agents shard "
+              "across GPUs with no per-step communication, see DESIGN.md); use --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    N = 1 << a.nodes_log2
+    A, dt, K = a.agents, 0.5, a.steps
+    J = sis.Csr.gen_regular(N, 4, sis.W_PM1, a.seed)
+    Z = J.nnz
+    xi0 = J.xi0()
+
+    head = time_form(J, a.form, A, dt, K, a.warmup, a.seed)
+    base = None
+    if a.baseline_form != "none" and a.baseline_form != a.form:
+        base = time_form(J, a.baseline_form, A, dt, K, a.warmup, a.seed)
+
+    hi = head["info"]
+    value = hi["steps_executed"] * A / head["wall"]
+    avg_step_ms = hi["step_kernel_ms"] / max(hi["steps_executed"], 1)
+    nbytes = step_bytes(a.form, N, Z, A)
+    achieved = nbytes / (avg_step_ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "agent-steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": a.warmup,
+        "ms_per_step": head["wall"] * 1e3 / max(hi["steps_executed"], 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if a.form != "f64" else "f64",
+        "data": "synthetic (seeded random 4-regular graph, +-1 weights, seeded initial state)",
+        "config": {
+            "workload": f"maxcut dSB, random 4-regular N=2^{a.nodes_log2}, Z={Z}, +-1 weights, "
+                        f"{A} agents, dt {dt}, {K} steps, window every 100 steps",
+            "form": a.form,
+            "nodes": N,
+            "nnz": Z,
+            "agents": A,
+            "xi0": xi0,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": f"step ({a.form})",
+            "bytes_per_launch": nbytes,
+            "avg_launch_ms": avg_step_ms,
+        },
+        "device_ms": hi["device_ms"],
+        "step_kernel_ms": hi["step_kernel_ms"],
+        "best_cut": head["best_cut"],
+        "best_agent": head["best_agent"],
+        "mean_cut": head["mean_cut"],
+    }
+    if base is not None:
+        bi = base["info"]
+        bvalue = bi["steps_executed"] * A / base["wall"]
+        bstep = bi["step_kernel_ms"] / max(bi["steps_executed"], 1)
+        bbytes = step_bytes(a.baseline_form, N, Z, A)
+        out["plain"] = {
+            "form": a.baseline_form,
+            "value": bvalue,
+            "avg_launch_ms": bstep,
+            "achieved_GBs": bbytes / (bstep * 1e-3) / 1e9,
+            "frac": bbytes / (bstep * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "best_cut": base["best_cut"],
+            "mean_cut": base["mean_cut"],
+        }
+        out["speedup_vs_plain"] = value / bvalue
+        out["best_cut_rel_diff_vs_plain"] = (head["best_cut"] - base["best_cut"]) / abs(base["best_cut"])
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(J, A, dt, xi0, a.cpu_seconds, a.seed)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,233 @@
+// sis_cli.cpp -- command-line drop-in for the reference program
+// (src/main.cu:214-312):
+//
+//   sis <default?> <.mtx/.tsp> <output file>
+//   sis 0 <.mtx/.tsp> <output file> <maxcut/tsp> <update_window> <fused> <random>
+//       <time_step> <max_steps> <num_agents>
+//
+// The output file receives the reference's comma-separated fields, appended:
+//   MaxCut: time_us,min,max,avg,std,ps,<in>,N,nnz,max_steps,dt,agents
+//           (src/ising.cu:267 + src/main.cu:223)
+//   TSP:    time_us, min, max, avg, std, valid_agents,  (src/ising_tsp.cu:248)
+// Deliberate differences, each a fix of a reference defect:
+//   - time_step is parsed as a real number (src/main.cu:303 uses std::stoi);
+//   - TSP honours update_window / dt / steps / agents in custom mode (the
+//     reference fixes 10000 steps and 10000 agents, src/main.cu:238,
+//     src/csr.cuh:170-171); default mode keeps the reference's values;
+//   - the TSP output line ends with a newline;
+//   - values print with 6 significant digits like the reference unless that
+//     would not reproduce the value exactly, then with 17;
+//   - the seed comes from SIS_SEED (default: the clock, as srand(time(0))).
+// Extra knobs (environment, so the argv layout is unchanged):
+//   SIS_FORM = f32 | f64 | packed | bf16   precision / gather form (f32)
+//   SIS_DYNAMICS = dsb | bsb               MaxCut dynamics (dsb)
+//   SIS_EUC_ROUND = 0 | 1                  EUC_2D truncation (0, reference) or nint
+//   SIS_GRAPH = 0 | 1                      replay window periods as hipGraphs
+//   SIS_XI0 = <real>                       override the coupling scale
+//   SIS_WINDOW = <int>                     override the window period
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "sis.h"
+
+namespace {
+
+std::string fmt(double v) {
+  std::ostringstream a;
+  a << v;
+  const std::string s6 = a.str();
+  if (std::isnan(v) || std::isinf(v)) return s6;
+  if (std::strtod(s6.c_str(), nullptr) == v) return s6;
+  std::ostringstream b;
+  b.precision(17);
+  b << v;
+  return b.str();
+}
+
+bool ends_with(const std::string &v, const std::string &e) {
+  return e.size() <= v.size() && std::equal(e.rbegin(), e.rend(), v.rbegin());
+}
+
+const char *env(const char *k) {
+  const char *v = std::getenv(k);
+  return v && *v ? v : nullptr;
+}
+
+int die(const char *what) {
+  std::cerr << "sis: " << what << ": " << sis_last_error() << "\n";
+  return 1;
+}
+
+struct Opts {
+  bool uw = false, fused = true, random = false;
+  double dt = 0.1;
+  long long steps = 10000;
+  int agents = 1024;
+  uint64_t seed = 0;
+};
+
+void apply_env(sis_params &p, bool maxcut) {
+  if (const char *f = env("SIS_FORM")) {
+    std::string s = f;
+    if (s == "f64") p.form = SIS_FORM_F64;
+    else if (s == "packed") p.form = SIS_FORM_PACKED;
+    else if (s == "bf16") p.form = SIS_FORM_BF16;
+    else p.form = SIS_FORM_F32;
+  }
+  if (maxcut)
+    if (const char *d = env("SIS_DYNAMICS")) p.dynamics = std::string(d) == "bsb" ? SIS_BSB : SIS_DSB;
+  if (const char *g = env("SIS_GRAPH")) p.use_graph = std::atoi(g) != 0;
+  if (const char *x = env("SIS_XI0")) p.xi0 = std::atof(x);
+  if (const char *w = env("SIS_WINDOW")) p.window_period = std::atoi(w);
+}
+
+int max_cut(const Opts &o, const std::string &ifile, const std::string &ofile) {
+  std::ofstream file(ofile, std::ios::app);
+  sis_csr *J = nullptr;
+  if (sis_csr_read_mtx(ifile.c_str(), &J)) return die("reading the .mtx file");
+  int64_t n = 0, nnz = 0;
+  double S = 0;
+  sis_csr_shape(J, &n, &nnz, &S);
+  sis_params p;
+  sis_params_default(SIS_MAXCUT, &p);
+  p.agents = o.agents;
+  p.dt = o.dt;
+  p.fused = o.fused ? 1 : 0;
+  apply_env(p, true);
+  sis_solver *s = nullptr;
+  if (sis_solver_create(J, nullptr, &p, &s)) return die("creating the solver");
+  std::cout << "file: " << ifile << ", ";
+  sis_run_info info;
+  std::vector<double> cut((size_t)o.agents);
+  if (sis_solver_init_random(s, o.seed) || sis_solver_run(s, o.steps, o.uw, &info) ||
+      sis_solver_maxcut_cuts(s, cut.data(), nullptr))
+    return die("solving");
+  double st[5];
+  sis_cut_stats(cut.data(), o.agents, st);
+  const long long t = (long long)info.elapsed_us;
+  file << t << "," << fmt(st[0]) << "," << fmt(st[1]) << "," << fmt(st[2]) << "," << fmt(st[3])
+       << "," << fmt(st[4]) << ",";
+  std::cout << "time: " << t << ", steps: " << o.steps << ", max: " << fmt(st[1])
+            << ", min: " << fmt(st[0]) << ", average: " << fmt(st[2]) << ", Std Dev: " << fmt(st[3])
+            << ", P(s): " << fmt(st[4]) << ", time step: " << fmt(o.dt)
+            << ", # agents: " << o.agents << "\n";
+  file << ifile << "," << n << "," << nnz << "," << o.steps << "," << fmt(o.dt) << "," << o.agents
+       << "\n";
+  if (o.random) {
+    double rc = 0;
+    sis_maxcut_random_cut(J, o.seed ^ 0x5DEECE66Dull, &rc);
+    std::cout << "\nmax cut from random spin assignment: " << fmt(rc) << "\n";
+  }
+  sis_solver_destroy(s);
+  sis_csr_free(J);
+  return 0;
+}
+
+int traveling_salesman(const Opts &o, const std::string &ifile, const std::string &ofile) {
+  std::ofstream file(ofile, std::ios::app);
+  const char *er = env("SIS_EUC_ROUND");
+  sis_tsp *t = nullptr;
+  if (sis_tsp_read(ifile.c_str(), er ? std::atoi(er) : 0, &t)) return die("reading the .tsp file");
+  int64_t n = 0;
+  double maxd = 0;
+  sis_tsp_info(t, &n, &maxd, nullptr);
+  // The reference echoes DIMENSION, EDGE_WEIGHT_TYPE and the maximum distance
+  // while reading (src/tsp_cpu.cpp:16,21; src/tsp_gpu.cu:70-71).
+  int kind = SIS_TSP_EUC_2D;
+  sis_tsp_kind_of(t, &kind);
+  std::cout << n << "\n" << (kind == SIS_TSP_GEO ? "GEO" : "EUC_2D") << "\n\n" << fmt(maxd) << "\n";
+  sis_csr *J = nullptr;
+  std::vector<double> h((size_t)(n * n));
+  if (sis_tsp_qubo(t, &J, h.data())) return die("building the QUBO");
+  sis_params p;
+  sis_params_default(SIS_TSP, &p);
+  p.ncities = (int32_t)n;
+  p.agents = o.agents;
+  p.dt = o.dt;
+  apply_env(p, false);
+  sis_solver *s = nullptr;
+  if (sis_solver_create(J, h.data(), &p, &s)) return die("creating the solver");
+  sis_run_info info;
+  std::vector<double> len((size_t)o.agents);
+  if (sis_solver_init_random(s, o.seed) || sis_solver_run(s, o.steps, o.uw, &info) ||
+      sis_solver_tsp_lengths(s, t, len.data(), nullptr))
+    return die("solving");
+  double total = 0, mn = 1.7976931348623157e308, mx = -1.7976931348623157e308;
+  std::vector<double> all;
+  for (int a = 0; a < o.agents; a++)
+    if (len[(size_t)a] >= 0) {
+      all.push_back(len[(size_t)a]);
+      total += len[(size_t)a];
+      if (len[(size_t)a] < mn) mn = len[(size_t)a];
+      if (len[(size_t)a] > mx) mx = len[(size_t)a];
+    }
+  const long long valid = (long long)all.size();
+  if (valid == 0) std::cout << "no agent had a valid solution\n";
+  const double avg = total / (double)valid;
+  double sd = 0;
+  for (double d : all) sd += std::pow(d - avg, 2);
+  sd = std::pow(sd / (double)(valid - 1), 0.5);
+  const long long tus = (long long)info.elapsed_us;
+  file << tus << ", " << fmt(mn) << ", " << fmt(mx) << ", " << fmt(avg) << ", " << fmt(sd) << ", "
+       << valid << ", \n";
+  std::cout << "time: " << tus << ", vagents: " << valid << ", steps: " << o.steps
+            << ", max: " << fmt(mx) << ", min: " << fmt(mn) << ", average: " << fmt(avg)
+            << ", Std Dev: " << fmt(sd) << "\n";
+  double rl = 0;
+  sis_tsp_random_length(t, o.seed ^ 0x5DEECE66Dull, &rl);
+  std::cout << "\ntraveling salesman distance from random spin assignment: " << fmt(rl) << "\n";
+  sis_solver_destroy(s);
+  sis_csr_free(J);
+  sis_tsp_free(t);
+  return 0;
+}
+
+}  // namespace
+
+int main(int argc, char *argv[]) {
+  if (argc < 4) {
+    std::cout << "usage: ./program <default?> <.mtx/.tsp> <output file>\n";
+    return -1;
+  }
+  Opts o;
+  const char *sd = env("SIS_SEED");
+  o.seed = sd ? std::strtoull(sd, nullptr, 0) : (uint64_t)std::time(nullptr);
+  bool mtx = false, tsp = false;
+  const bool default_settings = std::atoi(argv[1]) != 0;
+  const std::string in = argv[2], out = argv[3];
+  if (default_settings) {
+    if (ends_with(in, ".mtx")) mtx = true;
+    else if (ends_with(in, ".tsp")) tsp = true;
+    if (tsp) o.agents = 10000;  // src/csr.cuh:170
+  } else {
+    if (argc != 11) {
+      std::cout << "usage: ./program <default?> <.mtx/.tsp> <output file> <maxcut/tsp> <update "
+                   "window?> <fused?> <random?> <time_step> <max_steps> <agents>\n";
+      return -1;
+    }
+    const std::string alg = argv[4];
+    if (alg == "maxcut") mtx = true;
+    else if (alg == "tsp") tsp = true;
+    o.uw = std::atoi(argv[5]) != 0;
+    o.fused = std::atoi(argv[6]) != 0;
+    o.random = std::atoi(argv[7]) != 0;
+    o.dt = std::strtod(argv[8], nullptr);
+    o.steps = std::atoll(argv[9]);
+    o.agents = std::atoi(argv[10]);
+  }
+  if (o.agents < 1 || o.steps < 0) {
+    std::cerr << "sis: agents must be >= 1 and max_steps >= 0\n";
+    return 1;
+  }
+  if (mtx) return max_cut(o, in, out);
+  if (tsp) return traveling_salesman(o, in, out);
+  return 0;
+}

@@ -1,0 +1,842 @@
+// sis_kernels.hip -- gfx950 kernels: initial state, pre-step, the fused SB
+// step in its four forms, the update window, cut / energy and tour lengths.
+//
+// Every `a += b * c` of the reference step is an explicit fused multiply-add
+// here and nothing else is contracted (-ffp-contract=off), so the FP32 and
+// FP64 forms are bit-exact with a CPU evaluation in the same order.
+#include "sis_kernels.hpp"
+
+namespace sis {
+
+// ------------------------------------------------------------ utilities
+
+__device__ __forceinline__ uint64_t d_splitmix(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ double d_uniform(uint64_t seed, uint64_t k) {
+  return (double)(d_splitmix(seed + k * 0x9E3779B97F4A7C15ull) >> 11) * 0x1.0p-53;
+}
+
+// Round to nearest even BF16 (NaN stays quiet NaN, inf stays inf).
+__device__ __forceinline__ uint16_t d_bf16(float x) {
+  uint32_t u = __float_as_uint(x);
+  if ((u & 0x7f800000u) == 0x7f800000u) {
+    u &= 0xffff0000u;
+    if (x != x) u |= 0x00400000u;
+    return (uint16_t)(u >> 16);
+  }
+  return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+__device__ __forceinline__ float d_from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ __forceinline__ int wave_index() {
+  return __builtin_amdgcn_readfirstlane((int)(blockIdx.x * kWaves + (threadIdx.x >> 6)));
+}
+
+__device__ __forceinline__ uint64_t wave_or64(uint64_t v) {
+  uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+  for (int m = 32; m >= 1; m >>= 1) {
+    lo |= (uint32_t)__shfl_xor((int)lo, m, 64);
+    hi |= (uint32_t)__shfl_xor((int)hi, m, 64);
+  }
+  return ((uint64_t)hi << 32) | lo;
+}
+
+template <typename T, int V>
+struct alignas(sizeof(T) * V) Vec {
+  T v[V];
+};
+
+template <typename T, int V>
+__device__ __forceinline__ Vec<T, V> ldv(const T *p) {
+  return *reinterpret_cast<const Vec<T, V> *>(p);
+}
+
+template <typename T, int V>
+__device__ __forceinline__ void stv(T *p, const Vec<T, V> &x) {
+  *reinterpret_cast<Vec<T, V> *>(p) = x;
+}
+
+__device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+static inline unsigned grid_for(int64_t work, int64_t per_block, int64_t cap = 1 << 20) {
+  int64_t b = (work + per_block - 1) / per_block;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return (unsigned)b;
+}
+
+// ------------------------------------------------------- state / prestep
+
+template <typename T>
+__global__ void k_init_state(T *X, T *Y, int64_t N, int64_t A, int64_t lda, uint64_t seed,
+                             int tsp) {
+  const int64_t n = N * lda;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = idx / lda, a = idx - i * lda;
+    T x = 0, y = 0;
+    if (a < A) {
+      const uint64_t e = (uint64_t)(i * A + a);
+      const double ux = d_uniform(seed, 2 * e), uy = d_uniform(seed, 2 * e + 1);
+      x = tsp ? (T)0 : (T)((ux * 2 - 1) / 10);
+      y = (T)((uy * 2 - 1) / 10);
+    }
+    X[idx] = x;
+    Y[idx] = y;
+  }
+}
+
+// Pre-step at t = 0 (src/kernels.cu:15-20): Y += dt(0-1) X; X += dt(0+1) Y.
+template <typename T>
+__global__ void k_prestep(T *X, T *Y, int64_t n, T cy, T cx) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    T y = fmaT(cy, X[e], Y[e]);
+    X[e] = fmaT(cx, y, X[e]);
+    Y[e] = y;
+  }
+}
+
+// Update_XY inside the unfused step (src/kernels.cu:15-20) with the step's
+// coefficients c0 = dt(p'+1), c1 = dt(p'-1), p' = 1.025 t / T unclamped.
+template <typename T>
+__global__ void k_xy_update(T *X, T *Y, int64_t n, const T *sched, const Ctrl *ctrl,
+                            int32_t offset) {
+  if (ctrl->done) return;
+  const T *sc = sched + 4 * (ctrl->base + offset);
+  const T c0 = sc[0], c1 = sc[1];
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    T y = fmaT(c1, X[e], Y[e]);
+    X[e] = fmaT(c0, y, X[e]);
+    Y[e] = y;
+  }
+}
+
+__global__ void k_bf16_shadow(const float *X, uint16_t *sh, int64_t n) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x)
+    sh[e] = d_bf16(X[e]);
+}
+
+// One wavefront per (row, word): ballot of the sign bits of 64 agents.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pack_signs(const T *X, uint64_t *S, int64_t N,
+                                                       int64_t W, int64_t lda) {
+  const int lane = threadIdx.x & 63;
+  const int64_t g = wave_index();
+  if (g >= N * W) return;
+  const int64_t i = g / W, w = g - i * W;
+  const uint64_t b = __ballot(signbit(X[i * lda + 64 * w + lane]));
+  if (lane == 0) S[g] = b;
+}
+
+// ------------------------------------------------------------ SB steps
+//
+// MaxCut (src/kernels.cu:236-262), step t with c = sched[t]:
+//   |x| > 1:  x = sgn x, y = 0          else  y += c2 * F
+//   x += c0 * y;  y += c1 * x           (every element)
+// TSP (src/kernels.cu:564-596):
+//   |x| > 1:  x = sgn x, y = 0 and nothing else
+//   else:     y += c0 * (c2 * (c3 * h_i + 2 F));  x += c0 * y;  y += c1 * x
+// with F = -sum_k J_k g(x_col_k) accumulated in ascending CSR order and
+// g = sgn (dSB) or coupling * x (bSB).
+
+template <typename T, int PROB>
+__device__ __forceinline__ void sb_update(T &x, T &y, T F, T c0, T c1, T c2, T c3, T h) {
+  const bool conf = (x < (T)0 ? -x : x) > (T)1;
+  if (PROB == PROB_UNFUSED) {
+    // mat_mult then confine (src/kernels.cu:198-207, 158-165); the X/Y
+    // update of this step already ran in k_xy_update.
+    y = fmaT(c2, F, y);
+    if (conf) {
+      x = signbit(x) ? (T)-1 : (T)1;
+      y = (T)0;
+    }
+  } else if (PROB == PROB_MAXCUT) {
+    if (conf) {
+      x = signbit(x) ? (T)-1 : (T)1;
+      y = (T)0;
+    } else {
+      y = fmaT(c2, F, y);
+    }
+    x = fmaT(c0, y, x);
+    y = fmaT(c1, x, y);
+  } else {
+    if (conf) {
+      x = signbit(x) ? (T)-1 : (T)1;
+      y = (T)0;
+    } else {
+      const T q = fmaT(c3, h, (T)2 * F);
+      y = fmaT(c0, c2 * q, y);
+      x = fmaT(c0, y, x);
+      y = fmaT(c1, x, y);
+    }
+  }
+}
+
+template <typename T, int GATHER>
+__device__ __forceinline__ T sb_accum(T F, T v, T xj, T coupling) {
+  if (GATHER == G_DSB) return signbit(xj) ? F + v : F - v;  // == fma(-v, sgn xj, F)
+  return fmaT(-v, xj * coupling, F);
+}
+
+// Plain form: X double-buffered (gather from the snapshot Xs, write Xn), Y in
+// place.  One wavefront = one row x (64 V) agents; lane owns V contiguous
+// agents, so own-row and neighbour-row accesses are 64 V-wide coalesced loads.
+template <typename T, int PROB, int GATHER, int V>
+__global__ __launch_bounds__(kBlock) void k_step_plain(StepArgs a) {
+  const Ctrl *ctrl = a.ctrl;
+  if (ctrl->done) return;
+  const int64_t tiles = a.lda / (64 * V);
+  const int64_t g = wave_index();
+  const int64_t row = g / tiles;
+  if (row >= a.N) return;
+  const int lane = threadIdx.x & 63;
+  const int64_t a0 = (g - row * tiles) * (64 * V) + lane * V;
+  const int64_t t = ctrl->base + a.offset;
+  const T *sc = (const T *)a.sched + 4 * t;
+  const T c0 = sc[0], c1 = sc[1], c2 = sc[2], c3 = sc[3];
+  const T coupling = (T)a.coupling;
+  const T *__restrict__ Xs = (const T *)a.gsrc;
+  const T *__restrict__ val = (const T *)a.val;
+  const int32_t *__restrict__ col = a.col;
+  T *Y = (T *)a.Y;
+  const int64_t lda = a.lda;
+
+  const Vec<T, V> xo = ldv<T, V>(Xs + row * lda + a0);
+  Vec<T, V> yo = ldv<T, V>(Y + row * lda + a0);
+  T F[V];
+#pragma unroll
+  for (int u = 0; u < V; u++) F[u] = (T)0;
+  const int64_t k0 = a.rp[row], k1 = a.rp[row + 1];
+  int64_t k = k0;
+  for (; k + 4 <= k1; k += 4) {
+    const int64_t j0 = col[k], j1 = col[k + 1], j2 = col[k + 2], j3 = col[k + 3];
+    const T v0 = val[k], v1 = val[k + 1], v2 = val[k + 2], v3 = val[k + 3];
+    const Vec<T, V> g0 = ldv<T, V>(Xs + j0 * lda + a0);
+    const Vec<T, V> g1 = ldv<T, V>(Xs + j1 * lda + a0);
+    const Vec<T, V> g2 = ldv<T, V>(Xs + j2 * lda + a0);
+    const Vec<T, V> g3 = ldv<T, V>(Xs + j3 * lda + a0);
+#pragma unroll
+    for (int u = 0; u < V; u++) {
+      F[u] = sb_accum<T, GATHER>(F[u], v0, g0.v[u], coupling);
+      F[u] = sb_accum<T, GATHER>(F[u], v1, g1.v[u], coupling);
+      F[u] = sb_accum<T, GATHER>(F[u], v2, g2.v[u], coupling);
+      F[u] = sb_accum<T, GATHER>(F[u], v3, g3.v[u], coupling);
+    }
+  }
+  for (; k < k1; k++) {
+    const int64_t j = col[k];
+    const T v = val[k];
+    const Vec<T, V> gj = ldv<T, V>(Xs + j * lda + a0);
+#pragma unroll
+    for (int u = 0; u < V; u++) F[u] = sb_accum<T, GATHER>(F[u], v, gj.v[u], coupling);
+  }
+  const T h = PROB == PROB_TSP ? ((const T *)a.field)[row] : (T)0;
+  Vec<T, V> xn;
+#pragma unroll
+  for (int u = 0; u < V; u++) {
+    T x = xo.v[u], y = yo.v[u];
+    sb_update<T, PROB>(x, y, F[u], c0, c1, c2, c3, h);
+    xn.v[u] = x;
+    yo.v[u] = y;
+  }
+  stv<T, V>((T *)a.gdst + row * lda + a0, xn);
+  stv<T, V>(Y + row * lda + a0, yo);
+}
+
+// BF16-shadow bSB form: X, Y FP32 in place; gathers read the BF16 shadow of
+// the step's snapshot (gsrc) and the step writes the next shadow (gdst).
+template <int PROB, int V>
+__global__ __launch_bounds__(kBlock) void k_step_bf16(StepArgs a) {
+  const Ctrl *ctrl = a.ctrl;
+  if (ctrl->done) return;
+  const int64_t tiles = a.lda / (64 * V);
+  const int64_t g = wave_index();
+  const int64_t row = g / tiles;
+  if (row >= a.N) return;
+  const int lane = threadIdx.x & 63;
+  const int64_t a0 = (g - row * tiles) * (64 * V) + lane * V;
+  const int64_t t = ctrl->base + a.offset;
+  const float *sc = (const float *)a.sched + 4 * t;
+  const float c0 = sc[0], c1 = sc[1], c2 = sc[2], c3 = sc[3];
+  const float coupling = (float)a.coupling;
+  const uint16_t *__restrict__ Sh = (const uint16_t *)a.gsrc;
+  const float *__restrict__ val = (const float *)a.val;
+  const int32_t *__restrict__ col = a.col;
+  float *X = (float *)a.X, *Y = (float *)a.Y;
+  const int64_t lda = a.lda;
+
+  Vec<float, V> xo = ldv<float, V>(X + row * lda + a0);
+  Vec<float, V> yo = ldv<float, V>(Y + row * lda + a0);
+  float F[V];
+#pragma unroll
+  for (int u = 0; u < V; u++) F[u] = 0.0f;
+  const int64_t k0 = a.rp[row], k1 = a.rp[row + 1];
+  int64_t k = k0;
+  for (; k + 4 <= k1; k += 4) {
+    const int64_t j0 = col[k], j1 = col[k + 1], j2 = col[k + 2], j3 = col[k + 3];
+    const float v0 = val[k], v1 = val[k + 1], v2 = val[k + 2], v3 = val[k + 3];
+    const Vec<uint16_t, V> g0 = ldv<uint16_t, V>(Sh + j0 * lda + a0);
+    const Vec<uint16_t, V> g1 = ldv<uint16_t, V>(Sh + j1 * lda + a0);
+    const Vec<uint16_t, V> g2 = ldv<uint16_t, V>(Sh + j2 * lda + a0);
+    const Vec<uint16_t, V> g3 = ldv<uint16_t, V>(Sh + j3 * lda + a0);
+#pragma unroll
+    for (int u = 0; u < V; u++) {
+      F[u] = sb_accum<float, G_BSB>(F[u], v0, d_from_bf16(g0.v[u]), coupling);
+      F[u] = sb_accum<float, G_BSB>(F[u], v1, d_from_bf16(g1.v[u]), coupling);
+      F[u] = sb_accum<float, G_BSB>(F[u], v2, d_from_bf16(g2.v[u]), coupling);
+      F[u] = sb_accum<float, G_BSB>(F[u], v3, d_from_bf16(g3.v[u]), coupling);
+    }
+  }
+  for (; k < k1; k++) {
+    const int64_t j = col[k];
+    const float v = val[k];
+    const Vec<uint16_t, V> gj = ldv<uint16_t, V>(Sh + j * lda + a0);
+#pragma unroll
+    for (int u = 0; u < V; u++) F[u] = sb_accum<float, G_BSB>(F[u], v, d_from_bf16(gj.v[u]), coupling);
+  }
+  const float h = PROB == PROB_TSP ? ((const float *)a.field)[row] : 0.0f;
+  Vec<uint16_t, V> sn;
+#pragma unroll
+  for (int u = 0; u < V; u++) {
+    float x = xo.v[u], y = yo.v[u];
+    sb_update<float, PROB>(x, y, F[u], c0, c1, c2, c3, h);
+    xo.v[u] = x;
+    yo.v[u] = y;
+    sn.v[u] = d_bf16(x);
+  }
+  stv<float, V>(X + row * lda + a0, xo);
+  stv<float, V>(Y + row * lda + a0, yo);
+  stv<uint16_t, V>((uint16_t *)a.gdst + row * lda + a0, sn);
+}
+
+// Bit-packed dSB form: X, Y FP32 in place; the gather reads one sign bit per
+// agent from the snapshot's sign words (gsrc, wave-uniform scalar loads) and
+// the step writes the next sign words (gdst) with one ballot per word.  Lane
+// l of word w owns agent 64 w + l.  F is the exact sum of +-J, so the
+// trajectory is bit-identical to the plain FP32 dSB form.
+template <int PROB, int WT>
+__global__ __launch_bounds__(kBlock) void k_step_packed(StepArgs a) {
+  const Ctrl *ctrl = a.ctrl;
+  if (ctrl->done) return;
+  const int64_t W = a.W;
+  const int64_t tiles = W / WT;
+  const int64_t g = wave_index();
+  const int64_t row = g / tiles;
+  if (row >= a.N) return;
+  const int lane = threadIdx.x & 63;
+  const int64_t w0 = (g - row * tiles) * WT;
+  const int64_t t = ctrl->base + a.offset;
+  const float *sc = (const float *)a.sched + 4 * t;
+  const float c0 = sc[0], c1 = sc[1], c2 = sc[2], c3 = sc[3];
+  const uint64_t *__restrict__ S = (const uint64_t *)a.gsrc;
+  const float *__restrict__ val = (const float *)a.val;
+  const int32_t *__restrict__ col = a.col;
+  float *X = (float *)a.X, *Y = (float *)a.Y;
+  const int64_t lda = a.lda;
+
+  float x[WT], y[WT], F[WT];
+#pragma unroll
+  for (int u = 0; u < WT; u++) {
+    x[u] = X[row * lda + 64 * (w0 + u) + lane];
+    y[u] = Y[row * lda + 64 * (w0 + u) + lane];
+    F[u] = 0.0f;
+  }
+  const int64_t k0 = a.rp[row], k1 = a.rp[row + 1];
+  int64_t k = k0;
+  for (; k + 4 <= k1; k += 4) {
+    const int64_t j0 = col[k], j1 = col[k + 1], j2 = col[k + 2], j3 = col[k + 3];
+    const float v0 = val[k], v1 = val[k + 1], v2 = val[k + 2], v3 = val[k + 3];
+    uint64_t s0[WT], s1[WT], s2[WT], s3[WT];
+#pragma unroll
+    for (int u = 0; u < WT; u++) {
+      s0[u] = S[j0 * W + w0 + u];
+      s1[u] = S[j1 * W + w0 + u];
+      s2[u] = S[j2 * W + w0 + u];
+      s3[u] = S[j3 * W + w0 + u];
+    }
+#pragma unroll
+    for (int u = 0; u < WT; u++) {
+      F[u] = ((s0[u] >> lane) & 1) ? F[u] + v0 : F[u] - v0;
+      F[u] = ((s1[u] >> lane) & 1) ? F[u] + v1 : F[u] - v1;
+      F[u] = ((s2[u] >> lane) & 1) ? F[u] + v2 : F[u] - v2;
+      F[u] = ((s3[u] >> lane) & 1) ? F[u] + v3 : F[u] - v3;
+    }
+  }
+  for (; k < k1; k++) {
+    const int64_t j = col[k];
+    const float v = val[k];
+#pragma unroll
+    for (int u = 0; u < WT; u++) {
+      const uint64_t s = S[j * W + w0 + u];
+      F[u] = ((s >> lane) & 1) ? F[u] + v : F[u] - v;
+    }
+  }
+  const float h = PROB == PROB_TSP ? ((const float *)a.field)[row] : 0.0f;
+  uint64_t *Sn = (uint64_t *)a.gdst;
+#pragma unroll
+  for (int u = 0; u < WT; u++) {
+    sb_update<float, PROB>(x[u], y[u], F[u], c0, c1, c2, c3, h);
+    X[row * lda + 64 * (w0 + u) + lane] = x[u];
+    Y[row * lda + 64 * (w0 + u) + lane] = y[u];
+    const uint64_t b = __ballot(signbit(x[u]));
+    if (lane == 0) Sn[row * W + w0 + u] = b;
+  }
+}
+
+// -------------------------------------------------------------- windows
+
+// Sign scan of the window (src/kernels.cu:37-44, 84-90), race-free: each
+// (row, word) is compared and overwritten by the one wavefront that owns it;
+// per-block OR of the changed bits goes to part[block][word].
+template <int SRC>
+__global__ __launch_bounds__(kBlock) void k_window_scan(const void *xsrc, uint64_t *cur,
+                                                        uint64_t *part, int64_t N, int64_t W,
+                                                        int64_t lda, int64_t rpb,
+                                                        const Ctrl *ctrl) {
+  extern __shared__ uint64_t s_acc[];
+  if (ctrl->done) return;
+  for (int64_t w = threadIdx.x; w < W; w += blockDim.x) s_acc[w] = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int64_t r0 = (int64_t)blockIdx.x * rpb;
+  const int64_t r1 = r0 + rpb < N ? r0 + rpb : N;
+  const int64_t npairs = (r1 - r0) * W;
+  for (int64_t p = q; p < npairs; p += kWaves) {
+    const int64_t i = r0 + p / W, w = p % W;
+    uint64_t b;
+    if (SRC == SRC_BITS) b = ((const uint64_t *)xsrc)[i * W + w];
+    else if (SRC == SRC_F32) b = __ballot(signbit(((const float *)xsrc)[i * lda + 64 * w + lane]));
+    else b = __ballot(signbit(((const double *)xsrc)[i * lda + 64 * w + lane]));
+    const uint64_t d = b ^ cur[i * W + w];
+    if (lane == 0) {
+      cur[i * W + w] = b;
+      if (d) atomicOr((unsigned long long *)&s_acc[w], (unsigned long long)d);
+    }
+  }
+  __syncthreads();
+  for (int64_t w = threadIdx.x; w < W; w += blockDim.x) part[(int64_t)blockIdx.x * W + w] = s_acc[w];
+}
+
+// Per-agent stability / lock update (src/kernels.cu:46-69) in one block.
+__global__ __launch_bounds__(1024) void k_window_finalize(const uint64_t *part, int64_t nblk,
+                                                          uint64_t *newmask, int32_t *stab,
+                                                          int32_t *bif, int64_t A, int64_t W,
+                                                          int K, int uw, int64_t P, Ctrl *ctrl) {
+  __shared__ int s_nbif, s_any;
+  if (ctrl->done) {
+    if (threadIdx.x == 0) ctrl->merge = 0;
+    return;
+  }
+  if (threadIdx.x == 0) {
+    s_nbif = 0;
+    s_any = 0;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  for (int64_t base = threadIdx.x & ~63; base < W * 64; base += blockDim.x) {
+    const int64_t w = base >> 6, ag = base + lane;
+    uint64_t m = 0;
+    for (int64_t b = lane; b < nblk; b += 64) m |= part[b * W + w];
+    m = wave_or64(m);
+    const int changed = (int)((m >> lane) & 1);
+    int now = 0, fresh = 0;
+    if (ag < A) {
+      const int prev = bif[ag];
+      int st = stab[ag];
+      if (!prev) st = changed ? 0 : st + 1;
+      now = st == K - 1;
+      stab[ag] = st;
+      bif[ag] = now;
+      fresh = now && !prev;
+    }
+    const uint64_t nm = __ballot(fresh), bm = __ballot(now);
+    if (lane == 0) {
+      newmask[w] = nm;
+      if (nm) atomicOr(&s_any, 1);
+      atomicAdd(&s_nbif, (int)__popcll(bm));
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ctrl->merge = s_any;
+    ctrl->nbif = s_nbif;
+    ctrl->base += P;
+    if (uw && (int64_t)s_nbif == A) ctrl->done = 1;
+  }
+}
+
+// final = cur for newly locked / valid agents (src/kernels.cu:73-79, 524-531).
+__global__ void k_window_merge(const uint64_t *cur, uint64_t *fin, const uint64_t *newmask,
+                               int64_t N, int64_t W, const Ctrl *ctrl) {
+  if (!ctrl->merge) return;
+  const int64_t n = N * W;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t nm = newmask[idx % W];
+    if (nm) fin[idx] = (fin[idx] & ~nm) | (cur[idx] & nm);
+  }
+}
+
+// TSP validity (src/kernels.cu:475-500): exactly one +1 per position and per
+// city.  One wavefront per 64-agent word; spin words are wave-uniform loads.
+__global__ __launch_bounds__(kBlock) void k_tsp_valid(const uint64_t *__restrict__ cur, int64_t n,
+                                                      int64_t W, int64_t A, int32_t *valid_list,
+                                                      uint64_t *newmask, int64_t P,
+                                                      const Ctrl *ctrl) {
+  const int64_t w = wave_index();
+  if (w >= W) return;
+  const int lane = threadIdx.x & 63;
+  int valid = 1;
+  for (int64_t i = 0; i < n; i++) {
+    int c = 0;
+    for (int64_t k = 0; k < n; k++) c += !((cur[(i * n + k) * W + w] >> lane) & 1);
+    valid &= c == 1;
+  }
+  for (int64_t k = 0; k < n; k++) {
+    int c = 0;
+    for (int64_t i = 0; i < n; i++) c += !((cur[(i * n + k) * W + w] >> lane) & 1);
+    valid &= c == 1;
+  }
+  const int64_t ag = 64 * w + lane;
+  valid &= ag < A;
+  const uint64_t nm = __ballot(valid);
+  if (lane == 0) newmask[w] = nm;
+  if (valid) valid_list[ag] = (int32_t)(ctrl->base + P);
+}
+
+__global__ void k_ctrl_advance(Ctrl *ctrl, int64_t P) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) ctrl->base += P;
+}
+
+// ---------------------------------------------------------- evaluation
+
+// Row terms of H = sum_i 0.5 s_i sum_k J_k s_col (src/kernels.cu:371-379) or
+// of E = sum_i s_i sum_k J_k s_col - h_i s_i, summed per agent over a block's
+// rows in a fixed order; part[block][agent].
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_cut_rows(const int64_t *__restrict__ rp,
+                                                     const int32_t *__restrict__ col,
+                                                     const double *__restrict__ val,
+                                                     const double *__restrict__ field,
+                                                     const uint64_t *__restrict__ spins,
+                                                     int64_t N, int64_t W, int64_t lda,
+                                                     int64_t rpb, double *part) {
+  __shared__ double s_red[kWaves][64];
+  const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int64_t r0 = (int64_t)blockIdx.x * rpb;
+  const int64_t r1 = r0 + rpb < N ? r0 + rpb : N;
+  for (int64_t w = 0; w < W; w++) {
+    double acc = 0.0;
+    for (int64_t i = r0 + q; i < r1; i += kWaves) {
+      const double si = ((spins[i * W + w] >> lane) & 1) ? -1.0 : 1.0;
+      double tmp = 0.0;
+      for (int64_t k = rp[i]; k < rp[i + 1]; k++) {
+        const double sj = ((spins[(int64_t)col[k] * W + w] >> lane) & 1) ? -1.0 : 1.0;
+        tmp = __builtin_fma(val[k], sj, tmp);
+      }
+      if (MODE == 0) acc += tmp * si * 0.5;
+      else acc += tmp * si - field[i] * si;
+    }
+    s_red[q][lane] = acc;
+    __syncthreads();
+    if (q == 0)
+      part[(int64_t)blockIdx.x * lda + 64 * w + lane] =
+          ((s_red[0][lane] + s_red[1][lane]) + s_red[2][lane]) + s_red[3][lane];
+    __syncthreads();
+  }
+}
+
+// mode 0: cut = -0.5 (H - S) (src/kernels.cu:393-398); mode 1: energy.
+__global__ void k_cut_reduce(const double *part, int64_t nblk, int64_t A, int64_t lda, int mode,
+                             double S, double *out) {
+  const int64_t ag = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ag >= A) return;
+  double H = 0.0;
+  for (int64_t b = 0; b < nblk; b++) H += part[b * lda + ag];
+  out[ag] = mode == 0 ? (-0.5) * (H - S) : H;
+}
+
+__device__ __forceinline__ int64_t d_tri(int64_t c1, int64_t c2, int64_t n) {
+  return c2 + c1 * n - ((c1 + 2) * (c1 + 1)) / 2;
+}
+
+// Tour of each valid agent from its final spins, in (position, city) order
+// (src/ising_tsp.cu:206-214), and its cyclic length (src/ising_tsp.cu:272-287).
+__global__ void k_tsp_len(const uint64_t *__restrict__ fin, const int32_t *valid_list, int64_t n,
+                          int64_t W, int64_t A, const double *__restrict__ dist, double *len) {
+  const int64_t ag = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t w = ag >> 6;
+  const int lane = (int)(ag & 63);
+  if (w >= W) return;
+  int64_t m = 0, first = -1, prev = -1;
+  double sum = 0.0;
+  for (int64_t o = 0; o < n; o++)
+    for (int64_t c = 0; c < n; c++) {
+      if ((fin[(o * n + c) * W + w] >> lane) & 1) continue;
+      if (m == 0) first = c;
+      else if (prev != c) sum += dist[d_tri(prev < c ? prev : c, prev < c ? c : prev, n)];
+      prev = c;
+      m++;
+    }
+  if (ag >= A) return;
+  if (valid_list[ag] <= 0 || m != n) {
+    len[ag] = -1.0;
+    return;
+  }
+  if (n > 1 && prev != first) sum += dist[d_tri(prev < first ? prev : first, prev < first ? first : prev, n)];
+  len[ag] = sum;
+}
+
+// Best agent: mode 0 = largest value, mode 1 = smallest value >= 0; ties go
+// to the lowest index; -1 if no candidate.
+__global__ __launch_bounds__(1024) void k_best(const double *v, int64_t A, int mode, int64_t *best) {
+  __shared__ double s_v[1024];
+  __shared__ int64_t s_i[1024];
+  double bv = 0.0;
+  int64_t bi = -1;
+  for (int64_t a = threadIdx.x; a < A; a += blockDim.x) {
+    const double x = v[a];
+    if (mode == 1 && !(x >= 0.0)) continue;
+    const bool better = bi < 0 || (mode == 0 ? x > bv : x < bv);
+    if (better) {
+      bv = x;
+      bi = a;
+    }
+  }
+  s_v[threadIdx.x] = bv;
+  s_i[threadIdx.x] = bi;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      const double ov = s_v[threadIdx.x + s];
+      const int64_t oi = s_i[threadIdx.x + s];
+      const int64_t mi = s_i[threadIdx.x];
+      const double mv = s_v[threadIdx.x];
+      bool take = false;
+      if (oi >= 0) {
+        if (mi < 0) take = true;
+        else if (mode == 0 ? ov > mv : ov < mv) take = true;
+        else if (ov == mv && oi < mi) take = true;
+      }
+      if (take) {
+        s_v[threadIdx.x] = ov;
+        s_i[threadIdx.x] = oi;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *best = s_i[0];
+}
+
+// ------------------------------------------------------------ launchers
+
+void launch_init_state(hipStream_t st, bool f64, void *X, void *Y, int64_t N, int64_t A,
+                       int64_t lda, uint64_t seed, bool tsp) {
+  const unsigned gb = grid_for(N * lda, 256, 16384);
+  if (f64)
+    hipLaunchKernelGGL(k_init_state<double>, dim3(gb), dim3(256), 0, st, (double *)X, (double *)Y,
+                       N, A, lda, seed, (int)tsp);
+  else
+    hipLaunchKernelGGL(k_init_state<float>, dim3(gb), dim3(256), 0, st, (float *)X, (float *)Y, N,
+                       A, lda, seed, (int)tsp);
+}
+
+void launch_prestep(hipStream_t st, bool f64, void *X, void *Y, int64_t n, double dt) {
+  const unsigned gb = grid_for(n, 256, 16384);
+  const double cy = dt * (0.0 - 1), cx = dt * (0.0 + 1);
+  if (f64)
+    hipLaunchKernelGGL(k_prestep<double>, dim3(gb), dim3(256), 0, st, (double *)X, (double *)Y, n,
+                       cy, cx);
+  else
+    hipLaunchKernelGGL(k_prestep<float>, dim3(gb), dim3(256), 0, st, (float *)X, (float *)Y, n,
+                       (float)cy, (float)cx);
+}
+
+void launch_bf16_shadow(hipStream_t st, const float *X, uint16_t *sh, int64_t n) {
+  hipLaunchKernelGGL(k_bf16_shadow, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, X, sh, n);
+}
+
+void launch_pack_signs(hipStream_t st, int src, const void *X, uint64_t *S, int64_t N, int64_t W,
+                       int64_t lda) {
+  const unsigned gb = grid_for(N * W, kWaves, INT32_MAX);
+  if (src == SRC_F64)
+    hipLaunchKernelGGL(k_pack_signs<double>, dim3(gb), dim3(kBlock), 0, st, (const double *)X, S,
+                       N, W, lda);
+  else
+    hipLaunchKernelGGL(k_pack_signs<float>, dim3(gb), dim3(kBlock), 0, st, (const float *)X, S, N,
+                       W, lda);
+}
+
+void launch_xy_update(hipStream_t st, bool f64, void *X, void *Y, int64_t n, const void *sched,
+                      const Ctrl *ctrl, int32_t offset) {
+  const unsigned gb = grid_for(n, 256, 16384);
+  if (f64)
+    hipLaunchKernelGGL(k_xy_update<double>, dim3(gb), dim3(256), 0, st, (double *)X, (double *)Y,
+                       n, (const double *)sched, ctrl, offset);
+  else
+    hipLaunchKernelGGL(k_xy_update<float>, dim3(gb), dim3(256), 0, st, (float *)X, (float *)Y, n,
+                       (const float *)sched, ctrl, offset);
+}
+
+bool step_supported(int form, int prob, int gather, int vec) {
+  if (prob == PROB_UNFUSED)
+    return (form == 0 && (vec == 1 || vec == 2 || vec == 4) && gather == G_DSB) ||
+           (form == 1 && (vec == 1 || vec == 2) && gather == G_DSB);
+  if (prob != PROB_MAXCUT && prob != PROB_TSP) return false;
+  switch (form) {
+    case 0: return (gather == G_DSB || gather == G_BSB) && (vec == 1 || vec == 2 || vec == 4);
+    case 1: return (gather == G_DSB || gather == G_BSB) && (vec == 1 || vec == 2);
+    case 2: return gather == G_DSB && (vec == 1 || vec == 2 || vec == 4 || vec == 8);
+    case 3: return gather == G_BSB && (vec == 1 || vec == 2 || vec == 4);
+  }
+  return false;
+}
+
+template <typename K>
+static void launch_waves(hipStream_t st, K kern, int64_t waves, const StepArgs &a) {
+  hipLaunchKernelGGL(kern, dim3(grid_for(waves, kWaves, INT32_MAX)), dim3(kBlock), 0, st, a);
+}
+
+#define SIS_PLAIN(T, P, G, V)                                                   \
+  if (prob == P && gather == G && vec == V) {                                   \
+    launch_waves(st, k_step_plain<T, P, G, V>, a.N * (a.lda / (64 * V)), a);    \
+    return;                                                                     \
+  }
+
+void launch_step(hipStream_t st, int form, int prob, int gather, int vec, const StepArgs &a) {
+  if (form == 0) {
+    SIS_PLAIN(float, PROB_MAXCUT, G_DSB, 1)
+    SIS_PLAIN(float, PROB_MAXCUT, G_DSB, 2)
+    SIS_PLAIN(float, PROB_MAXCUT, G_DSB, 4)
+    SIS_PLAIN(float, PROB_MAXCUT, G_BSB, 1)
+    SIS_PLAIN(float, PROB_MAXCUT, G_BSB, 2)
+    SIS_PLAIN(float, PROB_MAXCUT, G_BSB, 4)
+    SIS_PLAIN(float, PROB_TSP, G_DSB, 1)
+    SIS_PLAIN(float, PROB_TSP, G_DSB, 2)
+    SIS_PLAIN(float, PROB_TSP, G_DSB, 4)
+    SIS_PLAIN(float, PROB_TSP, G_BSB, 1)
+    SIS_PLAIN(float, PROB_TSP, G_BSB, 2)
+    SIS_PLAIN(float, PROB_TSP, G_BSB, 4)
+    SIS_PLAIN(float, PROB_UNFUSED, G_DSB, 1)
+    SIS_PLAIN(float, PROB_UNFUSED, G_DSB, 2)
+    SIS_PLAIN(float, PROB_UNFUSED, G_DSB, 4)
+  } else if (form == 1) {
+    SIS_PLAIN(double, PROB_UNFUSED, G_DSB, 1)
+    SIS_PLAIN(double, PROB_UNFUSED, G_DSB, 2)
+    SIS_PLAIN(double, PROB_MAXCUT, G_DSB, 1)
+    SIS_PLAIN(double, PROB_MAXCUT, G_DSB, 2)
+    SIS_PLAIN(double, PROB_MAXCUT, G_BSB, 1)
+    SIS_PLAIN(double, PROB_MAXCUT, G_BSB, 2)
+    SIS_PLAIN(double, PROB_TSP, G_DSB, 1)
+    SIS_PLAIN(double, PROB_TSP, G_DSB, 2)
+    SIS_PLAIN(double, PROB_TSP, G_BSB, 1)
+    SIS_PLAIN(double, PROB_TSP, G_BSB, 2)
+  } else if (form == 2) {
+    const int64_t waves = a.N * (a.W / vec);
+#define SIS_PACKED(P, V)                                        \
+  if (prob == P && vec == V) {                                  \
+    launch_waves(st, k_step_packed<P, V>, waves, a);            \
+    return;                                                     \
+  }
+    SIS_PACKED(PROB_MAXCUT, 1)
+    SIS_PACKED(PROB_MAXCUT, 2)
+    SIS_PACKED(PROB_MAXCUT, 4)
+    SIS_PACKED(PROB_MAXCUT, 8)
+    SIS_PACKED(PROB_TSP, 1)
+    SIS_PACKED(PROB_TSP, 2)
+    SIS_PACKED(PROB_TSP, 4)
+    SIS_PACKED(PROB_TSP, 8)
+#undef SIS_PACKED
+  } else if (form == 3) {
+#define SIS_BF16(P, V)                                                          \
+  if (prob == P && vec == V) {                                                  \
+    launch_waves(st, k_step_bf16<P, V>, a.N * (a.lda / (64 * V)), a);           \
+    return;                                                                     \
+  }
+    SIS_BF16(PROB_MAXCUT, 1)
+    SIS_BF16(PROB_MAXCUT, 2)
+    SIS_BF16(PROB_MAXCUT, 4)
+    SIS_BF16(PROB_TSP, 1)
+    SIS_BF16(PROB_TSP, 2)
+    SIS_BF16(PROB_TSP, 4)
+#undef SIS_BF16
+  }
+}
+#undef SIS_PLAIN
+
+void launch_window_scan(hipStream_t st, int src, const void *xsrc, uint64_t *cur, uint64_t *part,
+                        int64_t N, int64_t W, int64_t lda, int64_t nblk, int64_t rows_per_blk,
+                        const Ctrl *ctrl) {
+  const size_t shm = sizeof(uint64_t) * (size_t)W;
+  if (src == SRC_BITS)
+    hipLaunchKernelGGL(k_window_scan<SRC_BITS>, dim3((unsigned)nblk), dim3(kBlock), shm, st, xsrc,
+                       cur, part, N, W, lda, rows_per_blk, ctrl);
+  else if (src == SRC_F64)
+    hipLaunchKernelGGL(k_window_scan<SRC_F64>, dim3((unsigned)nblk), dim3(kBlock), shm, st, xsrc,
+                       cur, part, N, W, lda, rows_per_blk, ctrl);
+  else
+    hipLaunchKernelGGL(k_window_scan<SRC_F32>, dim3((unsigned)nblk), dim3(kBlock), shm, st, xsrc,
+                       cur, part, N, W, lda, rows_per_blk, ctrl);
+}
+
+void launch_window_finalize(hipStream_t st, const uint64_t *part, int64_t nblk, uint64_t *newmask,
+                            int32_t *stab, int32_t *bif, int64_t A, int64_t W, int K, int uw,
+                            int64_t P, Ctrl *ctrl) {
+  hipLaunchKernelGGL(k_window_finalize, dim3(1), dim3(1024), 0, st, part, nblk, newmask, stab, bif,
+                     A, W, K, uw, P, ctrl);
+}
+
+void launch_window_merge(hipStream_t st, const uint64_t *cur, uint64_t *fin,
+                         const uint64_t *newmask, int64_t N, int64_t W, const Ctrl *ctrl) {
+  hipLaunchKernelGGL(k_window_merge, dim3(grid_for(N * W, 256, 8192)), dim3(256), 0, st, cur, fin,
+                     newmask, N, W, ctrl);
+}
+
+void launch_tsp_valid(hipStream_t st, const uint64_t *cur, int64_t n, int64_t W, int64_t A,
+                      int32_t *valid_list, uint64_t *newmask, int64_t P, Ctrl *ctrl) {
+  hipLaunchKernelGGL(k_tsp_valid, dim3(grid_for(W, kWaves)), dim3(kBlock), 0, st, cur, n, W, A,
+                     valid_list, newmask, P, ctrl);
+}
+
+void launch_ctrl_advance(hipStream_t st, Ctrl *ctrl, int64_t P) {
+  hipLaunchKernelGGL(k_ctrl_advance, dim3(1), dim3(64), 0, st, ctrl, P);
+}
+
+void launch_cut_rows(hipStream_t st, const int64_t *rp, const int32_t *col, const double *val,
+                     const double *field, const uint64_t *spins, int64_t N, int64_t W,
+                     int64_t lda, int64_t nblk, int64_t rows_per_blk, double *part) {
+  if (field)
+    hipLaunchKernelGGL(k_cut_rows<1>, dim3((unsigned)nblk), dim3(kBlock), 0, st, rp, col, val,
+                       field, spins, N, W, lda, rows_per_blk, part);
+  else
+    hipLaunchKernelGGL(k_cut_rows<0>, dim3((unsigned)nblk), dim3(kBlock), 0, st, rp, col, val,
+                       field, spins, N, W, lda, rows_per_blk, part);
+}
+
+void launch_cut_reduce(hipStream_t st, const double *part, int64_t nblk, int64_t A, int64_t lda,
+                       int mode, double S, double *out) {
+  hipLaunchKernelGGL(k_cut_reduce, dim3(grid_for(A, 256)), dim3(256), 0, st, part, nblk, A, lda,
+                     mode, S, out);
+}
+
+void launch_tsp_len(hipStream_t st, const uint64_t *fin, const int32_t *valid_list, int64_t n,
+                    int64_t W, int64_t A, const double *dist, double *len) {
+  hipLaunchKernelGGL(k_tsp_len, dim3(grid_for(W * 64, 256)), dim3(256), 0, st, fin, valid_list, n,
+                     W, A, dist, len);
+}
+
+void launch_best(hipStream_t st, const double *v, int64_t A, int mode, int64_t *best) {
+  hipLaunchKernelGGL(k_best, dim3(1), dim3(1024), 0, st, v, A, mode, best);
+}
+
+}  // namespace sis

@@ -1,0 +1,99 @@
+// sis_kernels.hpp -- gfx950 kernels of the simulated-bifurcation step, the
+// update window and the cut / energy / tour evaluation.
+//
+// Layout in HBM (all node-major, the reference's layout, src/kernels.cu:115):
+//   X, Y      N x lda     FP32 or FP64, lda = A rounded up to 64 (padding
+//                         agents start at 0 and are never reported)
+//   shadow    N x lda     BF16 copy of X (bSB BF16 form), double-buffered
+//   signs     N x W       uint64 words, W = lda / 64; bit l of word w is
+//                         signbit(X[i, 64 w + l]) -- a wavefront ballot
+//   CSR       row_ptr int64 (N+1), col int32, val FP32/FP64 per form
+// A wavefront owns one row and a tile of agents; CSR indices and values are
+// wave-uniform (scalar loads), the agent axis is the lane axis, so every
+// gather of a neighbour row is one coalesced load per wavefront.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace sis {
+
+// Device control block: step counter and loop flags.  Kernels read the
+// counter instead of a per-launch argument so a captured window period can be
+// replayed; only the window kernels advance it (after every step of the
+// period has finished), which removes the reference's read/increment race
+// (src/kernels.cu:225,232).
+struct Ctrl {
+  int64_t base;   // fused steps executed before the current window period
+  int32_t done;   // 1: stop (all agents bifurcated with update_window)
+  int32_t merge;  // 1: the last window locked new agents (copy final spins)
+  int32_t nbif;   // agents bifurcated at the last window
+  int32_t pad[3];
+};
+
+// PROB_UNFUSED is the second half of the reference's four-kernel MaxCut step
+// (mat_mult + confine, src/kernels.cu:150-209); its first half is
+// launch_xy_update (Update_XY, src/kernels.cu:8-22).
+enum : int { PROB_MAXCUT = 0, PROB_TSP = 1, PROB_UNFUSED = 2 };
+enum : int { G_DSB = 0, G_BSB = 1 };
+enum : int { SRC_F32 = 0, SRC_F64 = 1, SRC_BITS = 2 };
+
+constexpr int kBlock = 256;  // 4 wavefronts of 64 lanes
+constexpr int kWaves = kBlock / 64;
+
+// Launchers (defined in sis_kernels.hip).  All take the solver's stream.
+void launch_init_state(hipStream_t st, bool f64, void *X, void *Y, int64_t N, int64_t A,
+                       int64_t lda, uint64_t seed, bool tsp);
+void launch_prestep(hipStream_t st, bool f64, void *X, void *Y, int64_t n, double dt);
+void launch_bf16_shadow(hipStream_t st, const float *X, uint16_t *sh, int64_t n);
+void launch_pack_signs(hipStream_t st, int src, const void *X, uint64_t *S, int64_t N, int64_t W,
+                       int64_t lda);
+
+struct StepArgs {
+  const int64_t *rp;
+  const int32_t *col;
+  const void *val;    // FP32 or FP64 per form
+  const void *field;  // per-row field (TSP) or nullptr
+  const void *gsrc;   // gather source: X snapshot / BF16 shadow / sign words
+  void *gdst;         // next X / next shadow / next sign words
+  void *X;            // own X (in place for BF16 and packed forms)
+  void *Y;
+  const void *sched;  // 4 coefficients per step, form precision
+  const Ctrl *ctrl;
+  int32_t offset;     // step index = ctrl->base + offset
+  int64_t N, lda, W;
+  double coupling;
+};
+
+// form: 0 f32, 1 f64, 2 packed, 3 bf16.  vec = agents per lane (f32/f64/bf16)
+// or sign words per wavefront (packed).
+bool step_supported(int form, int prob, int gather, int vec);
+void launch_step(hipStream_t st, int form, int prob, int gather, int vec, const StepArgs &a);
+// Update_XY at the step index ctrl->base + offset: y += c1 x; x += c0 y.
+void launch_xy_update(hipStream_t st, bool f64, void *X, void *Y, int64_t n, const void *sched,
+                      const Ctrl *ctrl, int32_t offset);
+
+void launch_window_scan(hipStream_t st, int src, const void *xsrc, uint64_t *cur, uint64_t *part,
+                        int64_t N, int64_t W, int64_t lda, int64_t nblk, int64_t rows_per_blk,
+                        const Ctrl *ctrl);
+void launch_window_finalize(hipStream_t st, const uint64_t *part, int64_t nblk, uint64_t *newmask,
+                            int32_t *stab, int32_t *bif, int64_t A, int64_t W, int K, int uw,
+                            int64_t P, Ctrl *ctrl);
+void launch_window_merge(hipStream_t st, const uint64_t *cur, uint64_t *fin,
+                         const uint64_t *newmask, int64_t N, int64_t W, const Ctrl *ctrl);
+void launch_tsp_valid(hipStream_t st, const uint64_t *cur, int64_t n, int64_t W, int64_t A,
+                      int32_t *valid_list, uint64_t *newmask, int64_t P, Ctrl *ctrl);
+void launch_ctrl_advance(hipStream_t st, Ctrl *ctrl, int64_t P);
+
+// Cut / energy: partial row sums per block, then an ordered reduction.
+void launch_cut_rows(hipStream_t st, const int64_t *rp, const int32_t *col, const double *val,
+                     const double *field, const uint64_t *spins, int64_t N, int64_t W,
+                     int64_t lda, int64_t nblk, int64_t rows_per_blk, double *part);
+void launch_cut_reduce(hipStream_t st, const double *part, int64_t nblk, int64_t A, int64_t lda,
+                       int mode, double S, double *out);
+void launch_tsp_len(hipStream_t st, const uint64_t *fin, const int32_t *valid_list, int64_t n,
+                    int64_t W, int64_t A, const double *dist, double *len);
+void launch_best(hipStream_t st, const double *v, int64_t A, int mode, int64_t *best);
+
+}  // namespace sis

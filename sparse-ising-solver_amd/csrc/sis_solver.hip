@@ -1,0 +1,724 @@
+// sis_solver.hip -- device half of the C ABI: solver handle, state upload,
+// the window-period loop (optionally replayed as a hipGraph), evaluation.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "sis_internal.hpp"
+#include "sis_kernels.hpp"
+
+using namespace sis;
+
+#define HIPCHK(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(SIS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));    \
+  } while (0)
+
+namespace {
+
+template <typename T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  int alloc(size_t count) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = count;
+    if (count == 0) return SIS_OK;
+    hipError_t e = hipMalloc(&p, sizeof(T) * count);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return fail(e == hipErrorOutOfMemory ? SIS_ERR_NOMEM : SIS_ERR_HIP,
+                  "hipMalloc of " + std::to_string(sizeof(T) * count) +
+                      " bytes: " + hipGetErrorString(e));
+    }
+    return SIS_OK;
+  }
+};
+
+}  // namespace
+
+struct sis_solver {
+  sis_params p{};
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  int64_t N = 0, Z = 0, A = 0, lda = 0, W = 0;
+  double S = 0.0;
+  int vec = 1;     // agents per lane (plain / bf16) or words per wavefront (packed)
+  int gather = G_DSB;
+  int prob = PROB_MAXCUT;  // PROB_MAXCUT, PROB_TSP or PROB_UNFUSED
+  bool f64 = false;
+
+  DevBuf<int64_t> rp;
+  DevBuf<int32_t> col;
+  DevBuf<float> valf;
+  DevBuf<double> vald;
+  DevBuf<float> hf;
+  DevBuf<double> hd;
+  bool has_field = false;
+
+  DevBuf<float> Xf[2], Yf;       // FP32 state (plain uses both X buffers)
+  DevBuf<double> Xd[2], Yd;      // FP64 state
+  DevBuf<uint16_t> sh[2];        // BF16 shadow
+  DevBuf<uint64_t> sg[2];        // packed sign words
+  int par = 0;                   // index of the current snapshot buffer
+
+  DevBuf<uint64_t> cur, fin, newmask, part;
+  DevBuf<int32_t> stab, bif, valid;
+  DevBuf<Ctrl> ctrl;
+  Ctrl *h_ctrl = nullptr;        // pinned mirror
+  int64_t scan_blocks = 0, scan_rows = 0;
+
+  DevBuf<float> schedf;
+  DevBuf<double> schedd;
+  int64_t sched_T = -1, sched_len = 0;
+
+  DevBuf<double> cutpart, outv;
+  DevBuf<int64_t> best;
+  int64_t cut_blocks = 0, cut_rows = 0;
+
+  DevBuf<double> dist;
+  int64_t dist_n = -1;
+  std::vector<double> dist_host;
+
+  std::vector<hipEvent_t> events;
+  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+  int64_t graph_P = -1;
+  int graph_uw = -1;
+  bool state_ready = false;
+
+  ~sis_solver() {
+    for (auto &g : graph_exec)
+      if (g) (void)hipGraphExecDestroy(g);
+    for (auto e : events) (void)hipEventDestroy(e);
+    if (h_ctrl) (void)hipHostFree(h_ctrl);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  void *Xbuf(int i) { return f64 ? (void *)Xd[i].p : (void *)Xf[i].p; }
+  void *Ybuf() { return f64 ? (void *)Yd.p : (void *)Yf.p; }
+  // X holding the current positions.
+  void *Xcur() { return p.form == SIS_FORM_F32 || p.form == SIS_FORM_F64 ? Xbuf(par) : Xbuf(0); }
+};
+
+static int resolve_period(const sis_solver *s, int64_t T) {
+  if (s->p.window_period > 0) return s->p.window_period;
+  if (s->p.problem == SIS_TSP) return (int)std::max<int64_t>(1, T / 10);
+  return 100;
+}
+
+extern "C" {
+
+int sis_params_default(int problem, sis_params *p) {
+  if (!p || (problem != SIS_MAXCUT && problem != SIS_TSP))
+    return fail(SIS_ERR_ARG, "sis_params_default: bad argument");
+  std::memset(p, 0, sizeof(*p));
+  p->problem = problem;
+  p->form = SIS_FORM_F32;
+  p->dt = 0.1;
+  p->xi0 = 0.0;
+  p->use_graph = 0;
+  p->fused = 1;
+  if (problem == SIS_MAXCUT) {
+    p->dynamics = SIS_DSB;
+    p->agents = 1024;
+    p->coupling = 1.0;
+    p->window_period = 100;
+    p->window_threshold = 25;
+  } else {
+    p->dynamics = SIS_BSB;
+    p->agents = 10000;
+    p->coupling = 1.005;
+    p->xi0 = 0.002;
+    p->window_period = 0;
+    p->window_threshold = 50;
+  }
+  return SIS_OK;
+}
+
+int sis_solver_create(const sis_csr *J, const double *field, const sis_params *pin,
+                      sis_solver **out) {
+  if (!J || !pin || !out) return fail(SIS_ERR_ARG, "sis_solver_create: null argument");
+  *out = nullptr;
+  std::unique_ptr<sis_solver> s(new (std::nothrow) sis_solver());
+  if (!s) return fail(SIS_ERR_NOMEM, "out of host memory");
+  sis_params p = *pin;
+  if (p.problem != SIS_MAXCUT && p.problem != SIS_TSP)
+    return fail(SIS_ERR_ARG, "sis_solver_create: unknown problem");
+  if (p.dynamics != SIS_DSB && p.dynamics != SIS_BSB)
+    return fail(SIS_ERR_ARG, "sis_solver_create: unknown dynamics");
+  if (p.form == SIS_FORM_PACKED && p.dynamics != SIS_DSB)
+    return fail(SIS_ERR_ARG, "sis_solver_create: the packed-sign form is dSB only");
+  if (p.form == SIS_FORM_BF16 && p.dynamics != SIS_BSB)
+    return fail(SIS_ERR_ARG, "sis_solver_create: the BF16 form is bSB only");
+  if (p.form < SIS_FORM_F32 || p.form > SIS_FORM_BF16)
+    return fail(SIS_ERR_ARG, "sis_solver_create: unknown form");
+  if (p.agents < 1 || p.agents > (1 << 18))
+    return fail(SIS_ERR_ARG, "sis_solver_create: agents must be in [1, 262144]");
+  if (!(p.dt == p.dt)) return fail(SIS_ERR_ARG, "sis_solver_create: dt is NaN");
+  if (p.window_threshold < 1) p.window_threshold = p.problem == SIS_MAXCUT ? 25 : 50;
+  s->N = J->n;
+  s->Z = (int64_t)J->col.size();
+  s->S = J->sum;
+  if (s->N < 1) return fail(SIS_ERR_ARG, "sis_solver_create: empty matrix");
+  if (p.problem == SIS_TSP) {
+    if (p.ncities < 1 || (int64_t)p.ncities * p.ncities != s->N)
+      return fail(SIS_ERR_ARG, "sis_solver_create: TSP needs ncities^2 == N");
+    if (!field) return fail(SIS_ERR_ARG, "sis_solver_create: TSP needs the field h");
+  }
+  if (p.xi0 <= 0.0) {
+    if (p.problem == SIS_TSP) {
+      p.xi0 = 0.002;
+    } else {
+      int st = sis_maxcut_xi0(J, &p.xi0);
+      if (st) return st;
+    }
+  }
+  s->A = p.agents;
+  s->lda = (s->A + 63) / 64 * 64;
+  s->W = s->lda / 64;
+  s->f64 = p.form == SIS_FORM_F64;
+  s->gather = p.dynamics == SIS_DSB ? G_DSB : G_BSB;
+  // Tile: agents per lane / words per wavefront.
+  int vec = p.agents_per_lane;
+  if (vec <= 0) {
+    if (p.form == SIS_FORM_PACKED) {
+      vec = 1;
+      while (vec < 4 && s->W % (vec * 2) == 0) vec *= 2;
+    } else {
+      const int vmax = s->f64 ? 2 : 4;
+      vec = 1;
+      while (vec < vmax && s->lda % (64 * vec * 2) == 0) vec *= 2;
+    }
+  }
+  if (p.problem == SIS_TSP) p.fused = 1;
+  const int prob = p.problem == SIS_TSP ? PROB_TSP : (p.fused ? PROB_MAXCUT : PROB_UNFUSED);
+  if (prob == PROB_UNFUSED &&
+      (s->gather != G_DSB || (p.form != SIS_FORM_F32 && p.form != SIS_FORM_F64)))
+    return fail(SIS_ERR_ARG, "sis_solver_create: the unfused step is plain-form dSB only");
+  s->prob = prob;
+  if (!step_supported(p.form, prob, s->gather, vec) ||
+      (p.form == SIS_FORM_PACKED ? s->W % vec : s->lda % (64 * vec)))
+    return fail(SIS_ERR_ARG, "sis_solver_create: unsupported tile " + std::to_string(vec) +
+                                 " for " + std::to_string(s->A) + " agents");
+  s->vec = vec;
+  const int64_t tiles = p.form == SIS_FORM_PACKED ? s->W / vec : s->lda / (64 * vec);
+  if (s->N * tiles >= INT32_MAX || s->N * s->W >= INT32_MAX)
+    return fail(SIS_ERR_ARG, "sis_solver_create: N x agents too large for one launch");
+  s->p = p;
+
+  HIPCHK(hipGetDevice(&s->dev));
+  HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  HIPCHK(hipHostMalloc((void **)&s->h_ctrl, sizeof(Ctrl), hipHostMallocDefault));
+
+  int st;
+  const int64_t N = s->N, Z = s->Z, NL = N * s->lda, NW = N * s->W;
+  if ((st = s->rp.alloc((size_t)N + 1)) || (st = s->col.alloc((size_t)Z)) ||
+      (st = s->vald.alloc((size_t)Z)))
+    return st;
+  HIPCHK(hipMemcpy(s->rp.p, J->rp.data(), sizeof(int64_t) * (N + 1), hipMemcpyHostToDevice));
+  if (Z) {
+    HIPCHK(hipMemcpy(s->col.p, J->col.data(), sizeof(int32_t) * Z, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(s->vald.p, J->val.data(), sizeof(double) * Z, hipMemcpyHostToDevice));
+  }
+  if (!s->f64) {
+    if ((st = s->valf.alloc((size_t)Z))) return st;
+    std::vector<float> vf(J->val.begin(), J->val.end());
+    if (Z) HIPCHK(hipMemcpy(s->valf.p, vf.data(), sizeof(float) * Z, hipMemcpyHostToDevice));
+  }
+  if (field) {
+    s->has_field = true;
+    if ((st = s->hd.alloc((size_t)N))) return st;
+    HIPCHK(hipMemcpy(s->hd.p, field, sizeof(double) * N, hipMemcpyHostToDevice));
+    if (!s->f64) {
+      if ((st = s->hf.alloc((size_t)N))) return st;
+      std::vector<float> hf(field, field + N);
+      HIPCHK(hipMemcpy(s->hf.p, hf.data(), sizeof(float) * N, hipMemcpyHostToDevice));
+    }
+  }
+  // State.
+  if (s->f64) {
+    if ((st = s->Xd[0].alloc((size_t)NL)) || (st = s->Xd[1].alloc((size_t)NL)) ||
+        (st = s->Yd.alloc((size_t)NL)))
+      return st;
+  } else {
+    if ((st = s->Xf[0].alloc((size_t)NL)) || (st = s->Yf.alloc((size_t)NL))) return st;
+    if (p.form == SIS_FORM_F32 && (st = s->Xf[1].alloc((size_t)NL))) return st;
+    if (p.form == SIS_FORM_BF16 &&
+        ((st = s->sh[0].alloc((size_t)NL)) || (st = s->sh[1].alloc((size_t)NL))))
+      return st;
+    if (p.form == SIS_FORM_PACKED &&
+        ((st = s->sg[0].alloc((size_t)NW)) || (st = s->sg[1].alloc((size_t)NW))))
+      return st;
+  }
+  // Window state.
+  s->scan_rows = std::max<int64_t>(1, std::min<int64_t>(512, (N + 2047) / 2048));
+  s->scan_blocks = (N + s->scan_rows - 1) / s->scan_rows;
+  s->cut_rows = std::max<int64_t>(4, std::min<int64_t>(256, (N + 1023) / 1024));
+  s->cut_blocks = (N + s->cut_rows - 1) / s->cut_rows;
+  if ((st = s->cur.alloc((size_t)NW)) || (st = s->fin.alloc((size_t)NW)) ||
+      (st = s->newmask.alloc((size_t)s->W)) ||
+      (st = s->part.alloc((size_t)(s->scan_blocks * s->W))) ||
+      (st = s->stab.alloc((size_t)s->lda)) || (st = s->bif.alloc((size_t)s->lda)) ||
+      (st = s->valid.alloc((size_t)s->lda)) || (st = s->ctrl.alloc(1)) ||
+      (st = s->cutpart.alloc((size_t)(s->cut_blocks * s->lda))) ||
+      (st = s->outv.alloc((size_t)s->lda)) || (st = s->best.alloc(1)))
+    return st;
+  *out = s.release();
+  return SIS_OK;
+}
+
+int sis_solver_destroy(sis_solver *s) {
+  if (!s) return SIS_OK;
+  (void)hipStreamSynchronize(s->stream);
+  delete s;
+  return SIS_OK;
+}
+
+int sis_solver_params(const sis_solver *s, sis_params *p) {
+  if (!s || !p) return fail(SIS_ERR_ARG, "sis_solver_params: null argument");
+  *p = s->p;
+  p->agents_per_lane = s->vec;
+  return SIS_OK;
+}
+
+}  // extern "C"
+
+// Spins and window state back to the reference's reset values
+// (src/ising.cu:127-131, plus stability = 0); step counter to zero.
+static int reset_windows(sis_solver *s) {
+  const int64_t NW = s->N * s->W;
+  HIPCHK(hipMemsetAsync(s->cur.p, 0xff, sizeof(uint64_t) * NW, s->stream));
+  HIPCHK(hipMemsetAsync(s->fin.p, 0xff, sizeof(uint64_t) * NW, s->stream));
+  HIPCHK(hipMemsetAsync(s->stab.p, 0, sizeof(int32_t) * s->lda, s->stream));
+  HIPCHK(hipMemsetAsync(s->bif.p, 0, sizeof(int32_t) * s->lda, s->stream));
+  HIPCHK(hipMemsetAsync(s->valid.p, 0, sizeof(int32_t) * s->lda, s->stream));
+  HIPCHK(hipMemsetAsync(s->newmask.p, 0, sizeof(uint64_t) * s->W, s->stream));
+  Ctrl c{};
+  c.merge = s->p.problem == SIS_TSP ? 1 : 0;
+  HIPCHK(hipMemcpyAsync(s->ctrl.p, &c, sizeof(Ctrl), hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return SIS_OK;
+}
+
+// Derived gather buffers (BF16 shadow / sign words) from the current X.
+static int refresh_shadow(sis_solver *s) {
+  const int64_t NL = s->N * s->lda;
+  if (s->p.form == SIS_FORM_BF16) launch_bf16_shadow(s->stream, s->Xf[0].p, s->sh[s->par].p, NL);
+  if (s->p.form == SIS_FORM_PACKED)
+    launch_pack_signs(s->stream, SRC_F32, s->Xf[0].p, s->sg[s->par].p, s->N, s->W, s->lda);
+  HIPCHK(hipGetLastError());
+  return SIS_OK;
+}
+
+// Schedule table for horizon T covering steps [0, len).
+static int ensure_schedule(sis_solver *s, int64_t T, int64_t len) {
+  if (s->sched_T == T && s->sched_len >= len) return SIS_OK;
+  len = std::max<int64_t>(len, 1);
+  std::vector<double> c((size_t)(4 * len));
+  const double dt = s->p.dt, xi0 = s->p.xi0;
+  for (int64_t t = 0; t < len; t++) {
+    double *q = &c[(size_t)(4 * t)];
+    if (s->p.problem == SIS_MAXCUT) {
+      // src/kernels.cu:225-226, 258-262; the unfused path leaves p
+      // unclamped (src/kernels.cu:15)
+      double pr = 1.025 * (double)t / (double)T;
+      if (s->prob != PROB_UNFUSED) pr = pr < 1.0 ? pr : 1.0;
+      q[0] = dt * (pr + 1);
+      q[1] = dt * (pr - 1);
+      q[2] = dt * xi0;
+      q[3] = 0.0;
+    } else {
+      // src/kernels.cu:553-559, 592-595 with ncap = 2 (src/ising_tsp.cu:134)
+      const int ncap = 2;
+      double at = ncap * 1.04 * (double)t / (double)T;
+      double bt = 0.5 + (0.5 * std::pow((double)t / (double)T, 1.9));
+      at = at < ncap ? at : ncap;
+      bt = bt < 1 ? bt : 1;
+      q[0] = dt;
+      q[1] = dt * -(1 - at);
+      q[2] = xi0;
+      q[3] = bt;
+    }
+  }
+  int st;
+  if (s->f64) {
+    if ((st = s->schedd.alloc(c.size()))) return st;
+    HIPCHK(hipMemcpy(s->schedd.p, c.data(), sizeof(double) * c.size(), hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> cf(c.begin(), c.end());
+    if ((st = s->schedf.alloc(cf.size()))) return st;
+    HIPCHK(hipMemcpy(s->schedf.p, cf.data(), sizeof(float) * cf.size(), hipMemcpyHostToDevice));
+  }
+  s->sched_T = T;
+  s->sched_len = len;
+  for (auto &g : s->graph_exec)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+  return SIS_OK;
+}
+
+// Enqueue one fused step at offset `off` from ctrl->base, flipping par.
+static void enqueue_step(sis_solver *s, int32_t off) {
+  StepArgs a{};
+  a.rp = s->rp.p;
+  a.col = s->col.p;
+  a.val = s->f64 ? (const void *)s->vald.p : (const void *)s->valf.p;
+  a.field = s->has_field ? (s->f64 ? (const void *)s->hd.p : (const void *)s->hf.p) : nullptr;
+  a.sched = s->f64 ? (const void *)s->schedd.p : (const void *)s->schedf.p;
+  a.ctrl = s->ctrl.p;
+  a.offset = off;
+  a.N = s->N;
+  a.lda = s->lda;
+  a.W = s->W;
+  a.coupling = s->p.coupling;
+  a.Y = s->Ybuf();
+  const int cur = s->par, nxt = s->par ^ 1;
+  switch (s->p.form) {
+    case SIS_FORM_F32:
+    case SIS_FORM_F64:
+      a.gsrc = s->Xbuf(cur);
+      a.gdst = s->Xbuf(nxt);
+      a.X = nullptr;
+      break;
+    case SIS_FORM_BF16:
+      a.gsrc = s->sh[cur].p;
+      a.gdst = s->sh[nxt].p;
+      a.X = s->Xf[0].p;
+      break;
+    case SIS_FORM_PACKED:
+      a.gsrc = s->sg[cur].p;
+      a.gdst = s->sg[nxt].p;
+      a.X = s->Xf[0].p;
+      break;
+  }
+  if (s->prob == PROB_UNFUSED)
+    launch_xy_update(s->stream, s->f64, s->Xbuf(cur), a.Y, s->N * s->lda, a.sched, a.ctrl, off);
+  launch_step(s->stream, s->p.form, s->prob, s->gather, s->vec, a);
+  s->par = nxt;
+}
+
+// Enqueue the window kernels that follow a period of P steps.
+static void enqueue_window(sis_solver *s, int64_t P, int uw) {
+  int src;
+  const void *x;
+  switch (s->p.form) {
+    case SIS_FORM_PACKED: src = SRC_BITS; x = s->sg[s->par].p; break;
+    case SIS_FORM_F64: src = SRC_F64; x = s->Xcur(); break;
+    default: src = SRC_F32; x = s->Xcur(); break;
+  }
+  launch_window_scan(s->stream, src, x, s->cur.p, s->part.p, s->N, s->W, s->lda, s->scan_blocks,
+                     s->scan_rows, s->ctrl.p);
+  if (s->p.problem == SIS_MAXCUT) {
+    launch_window_finalize(s->stream, s->part.p, s->scan_blocks, s->newmask.p, s->stab.p, s->bif.p,
+                           s->A, s->W, s->p.window_threshold, uw, P, s->ctrl.p);
+    launch_window_merge(s->stream, s->cur.p, s->fin.p, s->newmask.p, s->N, s->W, s->ctrl.p);
+  } else {
+    launch_tsp_valid(s->stream, s->cur.p, s->p.ncities, s->W, s->A, s->valid.p, s->newmask.p, P,
+                     s->ctrl.p);
+    launch_window_merge(s->stream, s->cur.p, s->fin.p, s->newmask.p, s->N, s->W, s->ctrl.p);
+    launch_ctrl_advance(s->stream, s->ctrl.p, P);
+  }
+}
+
+static int ensure_events(sis_solver *s, size_t n) {
+  while (s->events.size() < n) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    s->events.push_back(e);
+  }
+  return SIS_OK;
+}
+
+// Captures one window period (P steps + window) for each snapshot parity.
+static int build_graphs(sis_solver *s, int64_t P, int uw) {
+  if (s->graph_P == P && s->graph_uw == uw && s->graph_exec[0] && s->graph_exec[1]) return SIS_OK;
+  for (auto &g : s->graph_exec)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+  const int saved = s->par;
+  for (int ph = 0; ph < 2; ph++) {
+    s->par = ph;
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    for (int64_t k = 0; k < P; k++) enqueue_step(s, (int32_t)k);
+    enqueue_window(s, P, uw);
+    HIPCHK(hipStreamEndCapture(s->stream, &g));
+    hipError_t e = hipGraphInstantiate(&s->graph_exec[ph], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(SIS_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  s->par = saved;
+  s->graph_P = P;
+  s->graph_uw = uw;
+  return SIS_OK;
+}
+
+extern "C" {
+
+int sis_solver_init_random(sis_solver *s, uint64_t seed) {
+  if (!s) return fail(SIS_ERR_ARG, "sis_solver_init_random: null solver");
+  int st = reset_windows(s);
+  if (st) return st;
+  s->par = 0;
+  launch_init_state(s->stream, s->f64, s->Xbuf(0), s->Ybuf(), s->N, s->A, s->lda, seed,
+                    s->p.problem == SIS_TSP);
+  HIPCHK(hipGetLastError());
+  if ((st = refresh_shadow(s))) return st;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  s->state_ready = true;
+  return SIS_OK;
+}
+
+int sis_solver_set_state(sis_solver *s, const double *x, const double *y) {
+  if (!s || !x || !y) return fail(SIS_ERR_ARG, "sis_solver_set_state: null argument");
+  int st = reset_windows(s);
+  if (st) return st;
+  s->par = 0;
+  const int64_t N = s->N, A = s->A, lda = s->lda;
+  if (s->f64) {
+    std::vector<double> bx((size_t)(N * lda), 0.0), by((size_t)(N * lda), 0.0);
+    for (int64_t i = 0; i < N; i++)
+      for (int64_t a = 0; a < A; a++) {
+        bx[(size_t)(i * lda + a)] = x[i * A + a];
+        by[(size_t)(i * lda + a)] = y[i * A + a];
+      }
+    HIPCHK(hipMemcpy(s->Xd[0].p, bx.data(), sizeof(double) * bx.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(s->Yd.p, by.data(), sizeof(double) * by.size(), hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> bx((size_t)(N * lda), 0.0f), by((size_t)(N * lda), 0.0f);
+    for (int64_t i = 0; i < N; i++)
+      for (int64_t a = 0; a < A; a++) {
+        bx[(size_t)(i * lda + a)] = (float)x[i * A + a];
+        by[(size_t)(i * lda + a)] = (float)y[i * A + a];
+      }
+    HIPCHK(hipMemcpy(s->Xf[0].p, bx.data(), sizeof(float) * bx.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(s->Yf.p, by.data(), sizeof(float) * by.size(), hipMemcpyHostToDevice));
+  }
+  if ((st = refresh_shadow(s))) return st;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  s->state_ready = true;
+  return SIS_OK;
+}
+
+int sis_solver_get_state(sis_solver *s, double *x, double *y) {
+  if (!s) return fail(SIS_ERR_ARG, "sis_solver_get_state: null solver");
+  HIPCHK(hipStreamSynchronize(s->stream));
+  const int64_t N = s->N, A = s->A, lda = s->lda;
+  auto copy_out = [&](const void *src, double *dst) -> int {
+    if (!dst) return SIS_OK;
+    if (s->f64) {
+      std::vector<double> b((size_t)(N * lda));
+      HIPCHK(hipMemcpy(b.data(), src, sizeof(double) * b.size(), hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < N; i++)
+        for (int64_t a = 0; a < A; a++) dst[i * A + a] = b[(size_t)(i * lda + a)];
+    } else {
+      std::vector<float> b((size_t)(N * lda));
+      HIPCHK(hipMemcpy(b.data(), src, sizeof(float) * b.size(), hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < N; i++)
+        for (int64_t a = 0; a < A; a++) dst[i * A + a] = b[(size_t)(i * lda + a)];
+    }
+    return SIS_OK;
+  };
+  int st = copy_out(s->Xcur(), x);
+  if (st) return st;
+  return copy_out(s->Ybuf(), y);
+}
+
+int sis_solver_prestep(sis_solver *s) {
+  if (!s || !s->state_ready) return fail(SIS_ERR_STATE, "sis_solver_prestep: state not set");
+  launch_prestep(s->stream, s->f64, s->Xcur(), s->Ybuf(), s->N * s->lda, s->p.dt);
+  HIPCHK(hipGetLastError());
+  int st = refresh_shadow(s);
+  if (st) return st;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return SIS_OK;
+}
+
+int sis_solver_advance(sis_solver *s, int64_t t0, int64_t nsteps, int64_t T) {
+  if (!s || !s->state_ready) return fail(SIS_ERR_STATE, "sis_solver_advance: state not set");
+  if (t0 < 0 || nsteps < 0 || T < 1 || t0 + nsteps > (int64_t)INT32_MAX)
+    return fail(SIS_ERR_ARG, "sis_solver_advance: bad step range");
+  int st = ensure_schedule(s, T, t0 + nsteps);
+  if (st) return st;
+  Ctrl c{};
+  c.base = t0;
+  c.merge = s->p.problem == SIS_TSP ? 1 : 0;
+  HIPCHK(hipMemcpyAsync(s->ctrl.p, &c, sizeof(Ctrl), hipMemcpyHostToDevice, s->stream));
+  for (int64_t k = 0; k < nsteps; k++) enqueue_step(s, (int32_t)k);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return SIS_OK;
+}
+
+int sis_solver_run(sis_solver *s, int64_t max_steps, int update_window, sis_run_info *info) {
+  if (!s || !s->state_ready) return fail(SIS_ERR_STATE, "sis_solver_run: state not set");
+  if (max_steps < 0) return fail(SIS_ERR_ARG, "sis_solver_run: negative max_steps");
+  const int uw = s->p.problem == SIS_MAXCUT && update_window ? 1 : 0;
+  const int64_t P = resolve_period(s, max_steps);
+  const int64_t nwin = max_steps > 0 ? (max_steps + P - 1) / P : 0;
+  if (nwin * P > (int64_t)INT32_MAX) return fail(SIS_ERR_ARG, "sis_solver_run: too many steps");
+  int st = ensure_schedule(s, std::max<int64_t>(max_steps, 1), nwin * P);
+  if (st) return st;
+  if ((st = ensure_events(s, (size_t)(2 * nwin + 2)))) return st;
+  if (s->p.use_graph && nwin > 0 && (st = build_graphs(s, P, uw))) return st;
+  // Fresh spins, window state and counters for the run (the reference
+  // resets them in reset_initialization, src/ising.cu:127-131).
+  if ((st = reset_windows(s))) return st;
+  const int par0 = s->par;
+
+  auto t_start = std::chrono::high_resolution_clock::now();
+  HIPCHK(hipEventRecord(s->events[0], s->stream));
+  launch_prestep(s->stream, s->f64, s->Xcur(), s->Ybuf(), s->N * s->lda, s->p.dt);
+  if ((st = refresh_shadow(s))) return st;
+  // Early exit is a device flag; the host checks it every `chunk` windows.
+  const int64_t chunk = uw ? 4 : nwin;
+  int64_t enq = 0;
+  bool stopped = false;
+  while (enq < nwin && !stopped) {
+    const int64_t stop = std::min(nwin, enq + chunk);
+    for (; enq < stop; enq++) {
+      if (s->p.use_graph) {
+        HIPCHK(hipGraphLaunch(s->graph_exec[s->par], s->stream));
+        if (P & 1) s->par ^= 1;
+      } else {
+        HIPCHK(hipEventRecord(s->events[2 + 2 * enq], s->stream));
+        for (int64_t k = 0; k < P; k++) enqueue_step(s, (int32_t)k);
+        HIPCHK(hipEventRecord(s->events[3 + 2 * enq], s->stream));
+        enqueue_window(s, P, uw);
+      }
+    }
+    HIPCHK(hipGetLastError());
+    if (uw && enq < nwin) {
+      HIPCHK(hipMemcpyAsync(s->h_ctrl, s->ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, s->stream));
+      HIPCHK(hipStreamSynchronize(s->stream));
+      stopped = s->h_ctrl->done != 0;
+    }
+  }
+  HIPCHK(hipEventRecord(s->events[1], s->stream));
+  HIPCHK(hipMemcpyAsync(s->h_ctrl, s->ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  auto t_end = std::chrono::high_resolution_clock::now();
+  const int64_t executed = s->h_ctrl->base;
+  // Each executed step flipped the snapshot parity once.
+  s->par = (int)((par0 + executed) & 1);
+  if (info) {
+    std::memset(info, 0, sizeof(*info));
+    info->steps_executed = executed;
+    info->windows = P > 0 ? executed / P : 0;
+    info->all_bifurcated = s->p.problem == SIS_MAXCUT && s->h_ctrl->nbif == s->A && executed > 0;
+    info->elapsed_us =
+        (double)std::chrono::duration_cast<std::chrono::microseconds>(t_end - t_start).count();
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, s->events[0], s->events[1]));
+    info->device_ms = ms;
+    double sk = 0.0;
+    if (!s->p.use_graph) {
+      for (int64_t w = 0; w < info->windows && w < nwin; w++) {
+        float m = 0.f;
+        HIPCHK(hipEventElapsedTime(&m, s->events[2 + 2 * w], s->events[3 + 2 * w]));
+        sk += m;
+      }
+    }
+    info->step_kernel_ms = sk;
+  }
+  return SIS_OK;
+}
+
+int sis_solver_get_spins(sis_solver *s, int which, int32_t *out) {
+  if (!s || !out || (which != SIS_SPINS_CURRENT && which != SIS_SPINS_FINAL))
+    return fail(SIS_ERR_ARG, "sis_solver_get_spins: bad argument");
+  const int64_t N = s->N, W = s->W, A = s->A;
+  std::vector<uint64_t> b((size_t)(N * W));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  HIPCHK(hipMemcpy(b.data(), which == SIS_SPINS_CURRENT ? s->cur.p : s->fin.p,
+                   sizeof(uint64_t) * b.size(), hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < N; i++)
+    for (int64_t a = 0; a < A; a++)
+      out[i * A + a] = ((b[(size_t)(i * W + a / 64)] >> (a % 64)) & 1) ? -1 : 1;
+  return SIS_OK;
+}
+
+static int eval_rows(sis_solver *s, const uint64_t *spins, int mode, double *out,
+                     int64_t *best_agent, int best_mode) {
+  launch_cut_rows(s->stream, s->rp.p, s->col.p, s->vald.p, mode == 1 ? s->hd.p : nullptr, spins,
+                  s->N, s->W, s->lda, s->cut_blocks, s->cut_rows, s->cutpart.p);
+  launch_cut_reduce(s->stream, s->cutpart.p, s->cut_blocks, s->A, s->lda, mode, s->S, s->outv.p);
+  if (best_agent) launch_best(s->stream, s->outv.p, s->A, best_mode, s->best.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s->stream));
+  if (out) HIPCHK(hipMemcpy(out, s->outv.p, sizeof(double) * s->A, hipMemcpyDeviceToHost));
+  if (best_agent) HIPCHK(hipMemcpy(best_agent, s->best.p, sizeof(int64_t), hipMemcpyDeviceToHost));
+  return SIS_OK;
+}
+
+int sis_solver_maxcut_cuts(sis_solver *s, double *cut, int64_t *best_agent) {
+  if (!s) return fail(SIS_ERR_ARG, "sis_solver_maxcut_cuts: null solver");
+  return eval_rows(s, s->cur.p, 0, cut, best_agent, 0);
+}
+
+int sis_solver_energy(sis_solver *s, int which, double *energy) {
+  if (!s || !energy || (which != SIS_SPINS_CURRENT && which != SIS_SPINS_FINAL))
+    return fail(SIS_ERR_ARG, "sis_solver_energy: bad argument");
+  if (!s->has_field) {
+    // Without a field: s^T J s = 2 H, from the cut-mode row sums.
+    std::vector<double> c((size_t)s->A);
+    launch_cut_rows(s->stream, s->rp.p, s->col.p, s->vald.p, nullptr,
+                    which == SIS_SPINS_CURRENT ? s->cur.p : s->fin.p, s->N, s->W, s->lda,
+                    s->cut_blocks, s->cut_rows, s->cutpart.p);
+    launch_cut_reduce(s->stream, s->cutpart.p, s->cut_blocks, s->A, s->lda, 1, 0.0, s->outv.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(c.data(), s->outv.p, sizeof(double) * s->A, hipMemcpyDeviceToHost));
+    for (int64_t a = 0; a < s->A; a++) energy[a] = 2.0 * c[(size_t)a];
+    return SIS_OK;
+  }
+  return eval_rows(s, which == SIS_SPINS_CURRENT ? s->cur.p : s->fin.p, 1, energy, nullptr, 0);
+}
+
+int sis_solver_agent_state(sis_solver *s, int32_t *stability, int32_t *locked, int32_t *valid_step) {
+  if (!s) return fail(SIS_ERR_ARG, "sis_solver_agent_state: null solver");
+  HIPCHK(hipStreamSynchronize(s->stream));
+  if (stability) HIPCHK(hipMemcpy(stability, s->stab.p, sizeof(int32_t) * s->A, hipMemcpyDeviceToHost));
+  if (locked) HIPCHK(hipMemcpy(locked, s->bif.p, sizeof(int32_t) * s->A, hipMemcpyDeviceToHost));
+  if (valid_step) HIPCHK(hipMemcpy(valid_step, s->valid.p, sizeof(int32_t) * s->A, hipMemcpyDeviceToHost));
+  return SIS_OK;
+}
+
+int sis_solver_tsp_lengths(sis_solver *s, const sis_tsp *t, double *length, int64_t *best_agent) {
+  if (!s || !t) return fail(SIS_ERR_ARG, "sis_solver_tsp_lengths: null argument");
+  if (s->p.problem != SIS_TSP || t->n != s->p.ncities)
+    return fail(SIS_ERR_ARG, "sis_solver_tsp_lengths: solver is not this TSP instance");
+  if (s->dist_n != t->n || s->dist_host != t->dist) {
+    int st = s->dist.alloc(std::max<size_t>(1, t->dist.size()));
+    if (st) return st;
+    if (!t->dist.empty())
+      HIPCHK(hipMemcpy(s->dist.p, t->dist.data(), sizeof(double) * t->dist.size(), hipMemcpyHostToDevice));
+    s->dist_n = t->n;
+    s->dist_host = t->dist;
+  }
+  launch_tsp_len(s->stream, s->fin.p, s->valid.p, t->n, s->W, s->A, s->dist.p, s->outv.p);
+  if (best_agent) launch_best(s->stream, s->outv.p, s->A, 1, s->best.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s->stream));
+  if (length) HIPCHK(hipMemcpy(length, s->outv.p, sizeof(double) * s->A, hipMemcpyDeviceToHost));
+  if (best_agent) HIPCHK(hipMemcpy(best_agent, s->best.p, sizeof(int64_t), hipMemcpyDeviceToHost));
+  return SIS_OK;
+}
+
+}  // extern "C"
