@@ -327,6 +327,8 @@ static int refresh_shadow(sis_solver *s) {
 static int ensure_schedule(sis_solver *s, int64_t T, int64_t len) {
   if (s->sched_T == T && s->sched_len >= len) return SIS_OK;
   len = std::max<int64_t>(len, 1);
+  // The old table may still be read by queued kernels.
+  HIPCHK(hipStreamSynchronize(s->stream));
   std::vector<double> c((size_t)(4 * len));
   const double dt = s->p.dt, xi0 = s->p.xi0;
   for (int64_t t = 0; t < len; t++) {
