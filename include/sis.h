@@ -178,7 +178,10 @@ typedef struct sis_params {
   int32_t window_threshold; /* K: bifurcated after K-1 unchanged windows (25) */
   int32_t ncities;          /* TSP: n, with N = n^2 spins */
   int32_t use_graph;        /* 1: replay each window period as a hipGraph */
-  int32_t agents_per_lane;  /* tile override, 0 = tuned default */
+  int32_t tile;             /* wavefront tile, 0 = default: agents per lane (1, 2, 4)
+                               for the plain and BF16 forms; for the packed form
+                               words (1, 2, 4, 8) | rows (1, 2, 4) << 8 |
+                               non-temporal state (0, 1) << 16 */
   int32_t fused;            /* MaxCut: 1 fused step (src/kernels.cu:211-269), 0 the
                                four-kernel path (src/ising.cu:167-172); plain forms */
 } sis_params;
@@ -232,6 +235,13 @@ int sis_solver_get_state(sis_solver *s, double *x, double *y);
  * fewer than max_steps steps have run; with update_window (MaxCut) stop after
  * the window at which every agent has bifurcated.  info may be NULL. */
 int sis_solver_run(sis_solver *s, int64_t max_steps, int update_window, sis_run_info *info);
+
+/* Tile autotuner (no reference counterpart): times trial_steps fused steps
+ * for every tile the form supports on this problem and keeps the fastest.
+ * All tiles compute bit-identical results; the state is restored afterwards.
+ * tile (may be NULL) receives the chosen tile, ms_per_step (may be NULL) its
+ * time per step. */
+int sis_solver_autotune(sis_solver *s, int64_t trial_steps, int32_t *tile, double *ms_per_step);
 
 /* Pre-step alone (src/kernels.cu:8-22 at t = 0). */
 int sis_solver_prestep(sis_solver *s);

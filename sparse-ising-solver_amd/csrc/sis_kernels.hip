@@ -324,17 +324,32 @@ __global__ __launch_bounds__(kBlock) void k_step_bf16(StepArgs a) {
 // the step writes the next sign words (gdst) with one ballot per word.  Lane
 // l of word w owns agent 64 w + l.  F is the exact sum of +-J, so the
 // trajectory is bit-identical to the plain FP32 dSB form.
-template <int PROB, int WT>
+// Own-row state access, optionally non-temporal (streamed once per step, so
+// it need not displace the sign words the gathers re-read from cache).
+template <int NT>
+__device__ __forceinline__ float ld_state(const float *p) {
+  if (NT) return __builtin_nontemporal_load(p);
+  return *p;
+}
+template <int NT>
+__device__ __forceinline__ void st_state(float *p, float v) {
+  if (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
+// Wavefront tile: R consecutive rows x WT sign words (64 WT agents).  The
+// X / Y loads of all R rows are issued before the gathers.
+template <int PROB, int WT, int R, int NT>
 __global__ __launch_bounds__(kBlock) void k_step_packed(StepArgs a) {
   const Ctrl *ctrl = a.ctrl;
   if (ctrl->done) return;
   const int64_t W = a.W;
   const int64_t tiles = W / WT;
   const int64_t g = wave_index();
-  const int64_t row = g / tiles;
-  if (row >= a.N) return;
+  const int64_t row0 = (g / tiles) * R;
+  if (row0 >= a.N) return;
   const int lane = threadIdx.x & 63;
-  const int64_t w0 = (g - row * tiles) * WT;
+  const int64_t w0 = (g - (g / tiles) * tiles) * WT;
   const int64_t t = ctrl->base + a.offset;
   const float *sc = (const float *)a.sched + 4 * t;
   const float c0 = sc[0], c1 = sc[1], c2 = sc[2], c3 = sc[3];
@@ -343,53 +358,68 @@ __global__ __launch_bounds__(kBlock) void k_step_packed(StepArgs a) {
   const int32_t *__restrict__ col = a.col;
   float *X = (float *)a.X, *Y = (float *)a.Y;
   const int64_t lda = a.lda;
-
-  float x[WT], y[WT], F[WT];
-#pragma unroll
-  for (int u = 0; u < WT; u++) {
-    x[u] = X[row * lda + 64 * (w0 + u) + lane];
-    y[u] = Y[row * lda + 64 * (w0 + u) + lane];
-    F[u] = 0.0f;
-  }
-  const int64_t k0 = a.rp[row], k1 = a.rp[row + 1];
-  int64_t k = k0;
-  for (; k + 4 <= k1; k += 4) {
-    const int64_t j0 = col[k], j1 = col[k + 1], j2 = col[k + 2], j3 = col[k + 3];
-    const float v0 = val[k], v1 = val[k + 1], v2 = val[k + 2], v3 = val[k + 3];
-    uint64_t s0[WT], s1[WT], s2[WT], s3[WT];
-#pragma unroll
-    for (int u = 0; u < WT; u++) {
-      s0[u] = S[j0 * W + w0 + u];
-      s1[u] = S[j1 * W + w0 + u];
-      s2[u] = S[j2 * W + w0 + u];
-      s3[u] = S[j3 * W + w0 + u];
-    }
-#pragma unroll
-    for (int u = 0; u < WT; u++) {
-      F[u] = ((s0[u] >> lane) & 1) ? F[u] + v0 : F[u] - v0;
-      F[u] = ((s1[u] >> lane) & 1) ? F[u] + v1 : F[u] - v1;
-      F[u] = ((s2[u] >> lane) & 1) ? F[u] + v2 : F[u] - v2;
-      F[u] = ((s3[u] >> lane) & 1) ? F[u] + v3 : F[u] - v3;
-    }
-  }
-  for (; k < k1; k++) {
-    const int64_t j = col[k];
-    const float v = val[k];
-#pragma unroll
-    for (int u = 0; u < WT; u++) {
-      const uint64_t s = S[j * W + w0 + u];
-      F[u] = ((s >> lane) & 1) ? F[u] + v : F[u] - v;
-    }
-  }
-  const float h = PROB == PROB_TSP ? ((const float *)a.field)[row] : 0.0f;
   uint64_t *Sn = (uint64_t *)a.gdst;
+
+  float x[R][WT], y[R][WT];
 #pragma unroll
-  for (int u = 0; u < WT; u++) {
-    sb_update<float, PROB>(x[u], y[u], F[u], c0, c1, c2, c3, h);
-    X[row * lda + 64 * (w0 + u) + lane] = x[u];
-    Y[row * lda + 64 * (w0 + u) + lane] = y[u];
-    const uint64_t b = __ballot(signbit(x[u]));
-    if (lane == 0) Sn[row * W + w0 + u] = b;
+  for (int r = 0; r < R; r++) {
+    if (row0 + r < a.N) {
+#pragma unroll
+      for (int u = 0; u < WT; u++) {
+        x[r][u] = ld_state<NT>(X + (row0 + r) * lda + 64 * (w0 + u) + lane);
+        y[r][u] = ld_state<NT>(Y + (row0 + r) * lda + 64 * (w0 + u) + lane);
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int64_t row = row0 + r;
+    if (row >= a.N) break;
+    float F[WT];
+#pragma unroll
+    for (int u = 0; u < WT; u++) F[u] = 0.0f;
+    const int64_t k0 = a.rp[row], k1 = a.rp[row + 1];
+    int64_t k = k0;
+    for (; k + 4 <= k1; k += 4) {
+      const int64_t j0 = col[k], j1 = col[k + 1], j2 = col[k + 2], j3 = col[k + 3];
+      const float v0 = val[k], v1 = val[k + 1], v2 = val[k + 2], v3 = val[k + 3];
+      uint64_t s0[WT], s1[WT], s2[WT], s3[WT];
+#pragma unroll
+      for (int u = 0; u < WT; u++) {
+        s0[u] = S[j0 * W + w0 + u];
+        s1[u] = S[j1 * W + w0 + u];
+        s2[u] = S[j2 * W + w0 + u];
+        s3[u] = S[j3 * W + w0 + u];
+      }
+#pragma unroll
+      for (int u = 0; u < WT; u++) {
+        F[u] = ((s0[u] >> lane) & 1) ? F[u] + v0 : F[u] - v0;
+        F[u] = ((s1[u] >> lane) & 1) ? F[u] + v1 : F[u] - v1;
+        F[u] = ((s2[u] >> lane) & 1) ? F[u] + v2 : F[u] - v2;
+        F[u] = ((s3[u] >> lane) & 1) ? F[u] + v3 : F[u] - v3;
+      }
+    }
+    for (; k < k1; k++) {
+      const int64_t j = col[k];
+      const float v = val[k];
+#pragma unroll
+      for (int u = 0; u < WT; u++) {
+        const uint64_t s = S[j * W + w0 + u];
+        F[u] = ((s >> lane) & 1) ? F[u] + v : F[u] - v;
+      }
+    }
+    const float h = PROB == PROB_TSP ? ((const float *)a.field)[row] : 0.0f;
+    uint64_t mine = 0;
+#pragma unroll
+    for (int u = 0; u < WT; u++) {
+      sb_update<float, PROB>(x[r][u], y[r][u], F[u], c0, c1, c2, c3, h);
+      st_state<NT>(X + row * lda + 64 * (w0 + u) + lane, x[r][u]);
+      st_state<NT>(Y + row * lda + 64 * (w0 + u) + lane, y[r][u]);
+      const uint64_t b = __ballot(signbit(x[r][u]));
+      if (lane == u) mine = b;
+    }
+    // lanes 0 .. WT-1 write the row's WT contiguous sign words at once
+    if (lane < WT) Sn[row * W + w0 + lane] = mine;
   }
 }
 
@@ -696,7 +726,11 @@ bool step_supported(int form, int prob, int gather, int vec) {
   switch (form) {
     case 0: return (gather == G_DSB || gather == G_BSB) && (vec == 1 || vec == 2 || vec == 4);
     case 1: return (gather == G_DSB || gather == G_BSB) && (vec == 1 || vec == 2);
-    case 2: return gather == G_DSB && (vec == 1 || vec == 2 || vec == 4 || vec == 8);
+    case 2: {
+      const int wt = vec & 0xff, rows = (vec >> 8) & 0xff;
+      return gather == G_DSB && (vec >> 17) == 0 && (wt == 1 || wt == 2 || wt == 4 || wt == 8) &&
+             (rows == 1 || rows == 2 || rows == 4);
+    }
     case 3: return gather == G_BSB && (vec == 1 || vec == 2 || vec == 4);
   }
   return false;
@@ -742,20 +776,21 @@ void launch_step(hipStream_t st, int form, int prob, int gather, int vec, const 
     SIS_PLAIN(double, PROB_TSP, G_BSB, 1)
     SIS_PLAIN(double, PROB_TSP, G_BSB, 2)
   } else if (form == 2) {
-    const int64_t waves = a.N * (a.W / vec);
-#define SIS_PACKED(P, V)                                        \
-  if (prob == P && vec == V) {                                  \
-    launch_waves(st, k_step_packed<P, V>, waves, a);            \
-    return;                                                     \
+    const int wt = vec & 0xff, rows = (vec >> 8) & 0xff, nt = (vec >> 16) & 1;
+    const int64_t waves = ((a.N + rows - 1) / rows) * (a.W / wt);
+#define SIS_PACKED(P, V, R, NT)                                              \
+  if (prob == P && wt == V && rows == R && nt == NT) {                       \
+    launch_waves(st, k_step_packed<P, V, R, NT>, waves, a);                  \
+    return;                                                                  \
   }
-    SIS_PACKED(PROB_MAXCUT, 1)
-    SIS_PACKED(PROB_MAXCUT, 2)
-    SIS_PACKED(PROB_MAXCUT, 4)
-    SIS_PACKED(PROB_MAXCUT, 8)
-    SIS_PACKED(PROB_TSP, 1)
-    SIS_PACKED(PROB_TSP, 2)
-    SIS_PACKED(PROB_TSP, 4)
-    SIS_PACKED(PROB_TSP, 8)
+#define SIS_PACKED_R(P, V, NT) SIS_PACKED(P, V, 1, NT) SIS_PACKED(P, V, 2, NT) SIS_PACKED(P, V, 4, NT)
+#define SIS_PACKED_V(P, NT) SIS_PACKED_R(P, 1, NT) SIS_PACKED_R(P, 2, NT) SIS_PACKED_R(P, 4, NT) SIS_PACKED_R(P, 8, NT)
+    SIS_PACKED_V(PROB_MAXCUT, 0)
+    SIS_PACKED_V(PROB_MAXCUT, 1)
+    SIS_PACKED_V(PROB_TSP, 0)
+    SIS_PACKED_V(PROB_TSP, 1)
+#undef SIS_PACKED_V
+#undef SIS_PACKED_R
 #undef SIS_PACKED
   } else if (form == 3) {
 #define SIS_BF16(P, V)                                                          \

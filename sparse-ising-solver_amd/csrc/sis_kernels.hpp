@@ -66,8 +66,9 @@ struct StepArgs {
   double coupling;
 };
 
-// form: 0 f32, 1 f64, 2 packed, 3 bf16.  vec = agents per lane (f32/f64/bf16)
-// or sign words per wavefront (packed).
+// form: 0 f32, 1 f64, 2 packed, 3 bf16.  vec = agents per lane (f32/f64/bf16);
+// for the packed form vec = words per wavefront | rows per wavefront << 8 |
+// non-temporal state access << 16.
 bool step_supported(int form, int prob, int gather, int vec);
 void launch_step(hipStream_t st, int form, int prob, int gather, int vec, const StepArgs &a);
 // Update_XY at the step index ctrl->base + offset: y += c1 x; x += c0 y.

@@ -113,6 +113,40 @@ struct sis_solver {
   void *Xcur() { return p.form == SIS_FORM_F32 || p.form == SIS_FORM_F64 ? Xbuf(par) : Xbuf(0); }
 };
 
+// Tile encoding: plain / BF16 = agents per lane; packed = words | rows << 8 |
+// non-temporal << 16 (see sis_kernels.hpp).
+static bool tile_ok(const sis_solver *s, int vec) {
+  if (!step_supported(s->p.form, s->prob, s->gather, vec)) return false;
+  if (s->p.form == SIS_FORM_PACKED) return s->W % (vec & 0xff) == 0;
+  return s->lda % (64 * vec) == 0;
+}
+
+static int default_tile(const sis_solver *s) {
+  int v = 1;
+  if (s->p.form == SIS_FORM_PACKED) {
+    while (v < 4 && s->W % (v * 2) == 0) v *= 2;
+    return v | (1 << 8);
+  }
+  const int vmax = s->f64 ? 2 : 4;
+  while (v < vmax && s->lda % (64 * v * 2) == 0) v *= 2;
+  return v;
+}
+
+static std::vector<int> candidate_tiles(const sis_solver *s) {
+  std::vector<int> c;
+  if (s->p.form == SIS_FORM_PACKED) {
+    for (int wt : {1, 2, 4, 8})
+      for (int r : {1, 2, 4})
+        for (int nt : {0, 1}) c.push_back(wt | (r << 8) | (nt << 16));
+  } else {
+    for (int v : {1, 2, 4}) c.push_back(v);
+  }
+  std::vector<int> ok;
+  for (int v : c)
+    if (tile_ok(s, v)) ok.push_back(v);
+  return ok;
+}
+
 static int resolve_period(const sis_solver *s, int64_t T) {
   if (s->p.window_period > 0) return s->p.window_period;
   if (s->p.problem == SIS_TSP) return (int)std::max<int64_t>(1, T / 10);
@@ -191,33 +225,20 @@ int sis_solver_create(const sis_csr *J, const double *field, const sis_params *p
   s->W = s->lda / 64;
   s->f64 = p.form == SIS_FORM_F64;
   s->gather = p.dynamics == SIS_DSB ? G_DSB : G_BSB;
-  // Tile: agents per lane / words per wavefront.
-  int vec = p.agents_per_lane;
-  if (vec <= 0) {
-    if (p.form == SIS_FORM_PACKED) {
-      vec = 1;
-      while (vec < 4 && s->W % (vec * 2) == 0) vec *= 2;
-    } else {
-      const int vmax = s->f64 ? 2 : 4;
-      vec = 1;
-      while (vec < vmax && s->lda % (64 * vec * 2) == 0) vec *= 2;
-    }
-  }
   if (p.problem == SIS_TSP) p.fused = 1;
   const int prob = p.problem == SIS_TSP ? PROB_TSP : (p.fused ? PROB_MAXCUT : PROB_UNFUSED);
   if (prob == PROB_UNFUSED &&
       (s->gather != G_DSB || (p.form != SIS_FORM_F32 && p.form != SIS_FORM_F64)))
     return fail(SIS_ERR_ARG, "sis_solver_create: the unfused step is plain-form dSB only");
   s->prob = prob;
-  if (!step_supported(p.form, prob, s->gather, vec) ||
-      (p.form == SIS_FORM_PACKED ? s->W % vec : s->lda % (64 * vec)))
+  s->p = p;
+  const int vec = p.tile > 0 ? p.tile : default_tile(s.get());
+  if (!tile_ok(s.get(), vec))
     return fail(SIS_ERR_ARG, "sis_solver_create: unsupported tile " + std::to_string(vec) +
                                  " for " + std::to_string(s->A) + " agents");
   s->vec = vec;
-  const int64_t tiles = p.form == SIS_FORM_PACKED ? s->W / vec : s->lda / (64 * vec);
-  if (s->N * tiles >= INT32_MAX || s->N * s->W >= INT32_MAX)
+  if (s->N * s->W >= INT32_MAX || s->N * (s->lda / 64) >= INT32_MAX)
     return fail(SIS_ERR_ARG, "sis_solver_create: N x agents too large for one launch");
-  s->p = p;
 
   HIPCHK(hipGetDevice(&s->dev));
   HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
@@ -290,7 +311,7 @@ int sis_solver_destroy(sis_solver *s) {
 int sis_solver_params(const sis_solver *s, sis_params *p) {
   if (!s || !p) return fail(SIS_ERR_ARG, "sis_solver_params: null argument");
   *p = s->p;
-  p->agents_per_lane = s->vec;
+  p->tile = s->vec;
   return SIS_OK;
 }
 
@@ -566,6 +587,75 @@ int sis_solver_advance(sis_solver *s, int64_t t0, int64_t nsteps, int64_t T) {
   for (int64_t k = 0; k < nsteps; k++) enqueue_step(s, (int32_t)k);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s->stream));
+  return SIS_OK;
+}
+
+int sis_solver_autotune(sis_solver *s, int64_t trial_steps, int32_t *tile, double *ms_per_step) {
+  if (!s || !s->state_ready) return fail(SIS_ERR_STATE, "sis_solver_autotune: state not set");
+  if (trial_steps < 1 || trial_steps > 100000)
+    return fail(SIS_ERR_ARG, "sis_solver_autotune: trial_steps must be in [1, 100000]");
+  const int64_t warm = 2, T = warm + trial_steps;
+  int st = ensure_schedule(s, T, T);
+  if (st) return st;
+  if ((st = ensure_events(s, 2))) return st;
+  // Snapshot of the state every candidate starts from (and that is restored).
+  const int64_t NL = s->N * s->lda, NW = s->N * s->W;
+  const size_t xbytes = (size_t)NL * (s->f64 ? 8 : 4);
+  size_t gbytes = 0;
+  void *gcur = nullptr;
+  if (s->p.form == SIS_FORM_BF16) {
+    gbytes = (size_t)NL * 2;
+    gcur = s->sh[s->par].p;
+  } else if (s->p.form == SIS_FORM_PACKED) {
+    gbytes = (size_t)NW * 8;
+    gcur = s->sg[s->par].p;
+  }
+  DevBuf<char> sx, sy, sgb;
+  if ((st = sx.alloc(xbytes)) || (st = sy.alloc(xbytes)) || (st = sgb.alloc(gbytes))) return st;
+  const int par0 = s->par;
+  void *x0 = s->Xcur();
+  HIPCHK(hipMemcpyAsync(sx.p, x0, xbytes, hipMemcpyDeviceToDevice, s->stream));
+  HIPCHK(hipMemcpyAsync(sy.p, s->Ybuf(), xbytes, hipMemcpyDeviceToDevice, s->stream));
+  if (gbytes) HIPCHK(hipMemcpyAsync(sgb.p, gcur, gbytes, hipMemcpyDeviceToDevice, s->stream));
+  auto restore = [&]() -> int {
+    s->par = par0;
+    HIPCHK(hipMemcpyAsync(x0, sx.p, xbytes, hipMemcpyDeviceToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->Ybuf(), sy.p, xbytes, hipMemcpyDeviceToDevice, s->stream));
+    if (gbytes) HIPCHK(hipMemcpyAsync(gcur, sgb.p, gbytes, hipMemcpyDeviceToDevice, s->stream));
+    Ctrl c{};
+    c.merge = s->p.problem == SIS_TSP ? 1 : 0;
+    HIPCHK(hipMemcpyAsync(s->ctrl.p, &c, sizeof(Ctrl), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return SIS_OK;
+  };
+  double best_ms = 1e300;
+  int best = s->vec;
+  for (int v : candidate_tiles(s)) {
+    s->vec = v;
+    if ((st = restore())) return st;
+    for (int64_t k = 0; k < warm; k++) enqueue_step(s, (int32_t)k);
+    HIPCHK(hipEventRecord(s->events[0], s->stream));
+    for (int64_t k = 0; k < trial_steps; k++) enqueue_step(s, (int32_t)(warm + k));
+    HIPCHK(hipEventRecord(s->events[1], s->stream));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, s->events[0], s->events[1]));
+    const double per = (double)ms / (double)trial_steps;
+    if (per < best_ms) {
+      best_ms = per;
+      best = v;
+    }
+  }
+  s->vec = best;
+  for (auto &g : s->graph_exec)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+  if ((st = restore())) return st;
+  if (tile) *tile = best;
+  if (ms_per_step) *ms_per_step = best_ms;
   return SIS_OK;
 }
 

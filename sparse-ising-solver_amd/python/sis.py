@@ -31,7 +31,7 @@ class Params(C.Structure):
         ("agents", C.c_int32), ("dt", C.c_double), ("xi0", C.c_double),
         ("coupling", C.c_double), ("window_period", C.c_int32),
         ("window_threshold", C.c_int32), ("ncities", C.c_int32),
-        ("use_graph", C.c_int32), ("agents_per_lane", C.c_int32), ("fused", C.c_int32),
+        ("use_graph", C.c_int32), ("tile", C.c_int32), ("fused", C.c_int32),
     ]
 
 
@@ -88,6 +88,7 @@ _SIGS = {
     "sis_solver_get_state": (None, [_P, _DP, _DP]),
     "sis_solver_run": (None, [_P, C.c_int64, C.c_int, C.POINTER(RunInfo)]),
     "sis_solver_prestep": (None, [_P]),
+    "sis_solver_autotune": (None, [_P, C.c_int64, _I32P, _DP]),
     "sis_solver_advance": (None, [_P, C.c_int64, C.c_int64, C.c_int64]),
     "sis_solver_get_spins": (None, [_P, C.c_int, _I32P]),
     "sis_solver_maxcut_cuts": (None, [_P, _DP, _I64P]),
@@ -320,6 +321,13 @@ class Solver:
 
     def prestep(self):
         _ck(lib().sis_solver_prestep(self.h), "sis_solver_prestep")
+
+    def autotune(self, trial_steps=10):
+        tile, ms = C.c_int32(), C.c_double()
+        _ck(lib().sis_solver_autotune(self.h, trial_steps, C.byref(tile), C.byref(ms)),
+            "sis_solver_autotune")
+        _ck(lib().sis_solver_params(self.h, C.byref(self.p)), "sis_solver_params")
+        return tile.value, ms.value
 
     def advance(self, t0, nsteps, T):
         _ck(lib().sis_solver_advance(self.h, t0, nsteps, T), "sis_solver_advance")

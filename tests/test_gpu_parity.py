@@ -214,6 +214,47 @@ def test_runs_are_deterministic():
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
 
 
+PACKED_TILES = [1 | 1 << 8, 2 | 2 << 8 | 1 << 16, 4 | 4 << 8, 8 | 1 << 8 | 1 << 16,
+                8 | 4 << 8, 1 | 4 << 8 | 1 << 16]
+
+
+@pytest.mark.parametrize("tile", PACKED_TILES)
+def test_packed_tiles_bit_exact(tile):
+    # 203 rows: not a multiple of the rows-per-wavefront tile
+    J, pb = maxcut_problem(203, 0.04, sis.W_INT3, 12)
+    A, T, dt = 512, 300, 0.5
+    s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=A, dt=dt, form=sis.FORM_PACKED, tile=tile))
+    assert s.p.tile == tile
+    X0, Y0 = seeded_state(J.n, A, 13)
+    s.set_state(X0, Y0)
+    s.advance(0, 250, T)
+    X, Y = X0.astype(np.float32), Y0.astype(np.float32)
+    orc.advance(pb, orc.MAXCUT, orc.DSB, X, Y, 0, 250, T, dt, s.p.xi0, 1.0, "f32")
+    x, y = s.get_state()
+    assert np.array_equal(x, X.astype(np.float64)) and np.array_equal(y, Y.astype(np.float64))
+
+
+@pytest.mark.parametrize("form", ["packed", "f32", "bf16"])
+def test_autotune_keeps_state_and_results(form):
+    J, pb = maxcut_problem(300, 0.03, sis.W_PM1, 14)
+    A, dt = 256, 0.5
+    dyn = sis.BSB if form == "bf16" else sis.DSB
+    s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=A, dt=dt, form=sis.FORMS[form], dynamics=dyn))
+    X0, Y0 = seeded_state(J.n, A, 15)
+    s.set_state(X0, Y0)
+    tile, ms = s.autotune(4)
+    assert ms > 0 and s.p.tile == tile
+    x, y = s.get_state()
+    assert np.array_equal(x, X0) and np.array_equal(y, Y0)
+    info = s.run(700, 0)
+    X, Y = X0.astype(np.float32), Y0.astype(np.float32)
+    r = orc.solve(pb, orc.MAXCUT, dyn, X, Y, 700, 0, P=100, dt=dt, xi0=s.p.xi0,
+                  precision="bf16" if form == "bf16" else "f32")
+    assert info["steps_executed"] == r["steps"]
+    assert np.array_equal(s.get_state()[0], X.astype(np.float64))
+    assert np.array_equal(s.cuts()[0], orc.cut(pb, r["cur"]))
+
+
 def _sparse(J):
     rp, col, val = J.arrays()
     return sp.csr_matrix((val, col, rp), shape=(J.n, J.n))
