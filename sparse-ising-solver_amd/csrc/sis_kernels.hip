@@ -62,6 +62,40 @@ __device__ __forceinline__ void stv(T *p, const Vec<T, V> &x) {
   *reinterpret_cast<Vec<T, V> *>(p) = x;
 }
 
+// Own-row state access; NT = 1 marks it non-temporal (streamed once per step).
+template <typename T, int V, int NT>
+__device__ __forceinline__ Vec<T, V> ldvs(const T *p) {
+  if constexpr (NT == 0) {
+    return ldv<T, V>(p);
+  } else if constexpr (V == 1) {
+    Vec<T, V> o;
+    o.v[0] = __builtin_nontemporal_load(p);
+    return o;
+  } else {
+    typedef T vt __attribute__((ext_vector_type(V)));
+    const vt r = __builtin_nontemporal_load(reinterpret_cast<const vt *>(p));
+    Vec<T, V> o;
+#pragma unroll
+    for (int u = 0; u < V; u++) o.v[u] = r[u];
+    return o;
+  }
+}
+
+template <typename T, int V, int NT>
+__device__ __forceinline__ void stvs(T *p, const Vec<T, V> &x) {
+  if constexpr (NT == 0) {
+    stv<T, V>(p, x);
+  } else if constexpr (V == 1) {
+    __builtin_nontemporal_store(x.v[0], p);
+  } else {
+    typedef T vt __attribute__((ext_vector_type(V)));
+    vt r;
+#pragma unroll
+    for (int u = 0; u < V; u++) r[u] = x.v[u];
+    __builtin_nontemporal_store(r, reinterpret_cast<vt *>(p));
+  }
+}
+
 __device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
@@ -191,7 +225,7 @@ __device__ __forceinline__ T sb_accum(T F, T v, T xj, T coupling) {
 // Plain form: X double-buffered (gather from the snapshot Xs, write Xn), Y in
 // place.  One wavefront = one row x (64 V) agents; lane owns V contiguous
 // agents, so own-row and neighbour-row accesses are 64 V-wide coalesced loads.
-template <typename T, int PROB, int GATHER, int V>
+template <typename T, int PROB, int GATHER, int V, int NT>
 __global__ __launch_bounds__(kBlock) void k_step_plain(StepArgs a) {
   const Ctrl *ctrl = a.ctrl;
   if (ctrl->done) return;
@@ -211,8 +245,8 @@ __global__ __launch_bounds__(kBlock) void k_step_plain(StepArgs a) {
   T *Y = (T *)a.Y;
   const int64_t lda = a.lda;
 
-  const Vec<T, V> xo = ldv<T, V>(Xs + row * lda + a0);
-  Vec<T, V> yo = ldv<T, V>(Y + row * lda + a0);
+  const Vec<T, V> xo = ldvs<T, V, NT>(Xs + row * lda + a0);
+  Vec<T, V> yo = ldvs<T, V, NT>(Y + row * lda + a0);
   T F[V];
 #pragma unroll
   for (int u = 0; u < V; u++) F[u] = (T)0;
@@ -249,13 +283,13 @@ __global__ __launch_bounds__(kBlock) void k_step_plain(StepArgs a) {
     xn.v[u] = x;
     yo.v[u] = y;
   }
-  stv<T, V>((T *)a.gdst + row * lda + a0, xn);
-  stv<T, V>(Y + row * lda + a0, yo);
+  stvs<T, V, NT>((T *)a.gdst + row * lda + a0, xn);
+  stvs<T, V, NT>(Y + row * lda + a0, yo);
 }
 
 // BF16-shadow bSB form: X, Y FP32 in place; gathers read the BF16 shadow of
 // the step's snapshot (gsrc) and the step writes the next shadow (gdst).
-template <int PROB, int V>
+template <int PROB, int V, int NT>
 __global__ __launch_bounds__(kBlock) void k_step_bf16(StepArgs a) {
   const Ctrl *ctrl = a.ctrl;
   if (ctrl->done) return;
@@ -275,8 +309,8 @@ __global__ __launch_bounds__(kBlock) void k_step_bf16(StepArgs a) {
   float *X = (float *)a.X, *Y = (float *)a.Y;
   const int64_t lda = a.lda;
 
-  Vec<float, V> xo = ldv<float, V>(X + row * lda + a0);
-  Vec<float, V> yo = ldv<float, V>(Y + row * lda + a0);
+  Vec<float, V> xo = ldvs<float, V, NT>(X + row * lda + a0);
+  Vec<float, V> yo = ldvs<float, V, NT>(Y + row * lda + a0);
   float F[V];
 #pragma unroll
   for (int u = 0; u < V; u++) F[u] = 0.0f;
@@ -314,8 +348,8 @@ __global__ __launch_bounds__(kBlock) void k_step_bf16(StepArgs a) {
     yo.v[u] = y;
     sn.v[u] = d_bf16(x);
   }
-  stv<float, V>(X + row * lda + a0, xo);
-  stv<float, V>(Y + row * lda + a0, yo);
+  stvs<float, V, NT>(X + row * lda + a0, xo);
+  stvs<float, V, NT>(Y + row * lda + a0, yo);
   stv<uint16_t, V>((uint16_t *)a.gdst + row * lda + a0, sn);
 }
 
@@ -719,19 +753,20 @@ void launch_xy_update(hipStream_t st, bool f64, void *X, void *Y, int64_t n, con
 }
 
 bool step_supported(int form, int prob, int gather, int vec) {
-  if (prob == PROB_UNFUSED)
-    return (form == 0 && (vec == 1 || vec == 2 || vec == 4) && gather == G_DSB) ||
-           (form == 1 && (vec == 1 || vec == 2) && gather == G_DSB);
+  if (form == 2) {
+    const int wt = vec & 0xff, rows = (vec >> 8) & 0xff;
+    return (prob == PROB_MAXCUT || prob == PROB_TSP) && gather == G_DSB && (vec >> 17) == 0 &&
+           (wt == 1 || wt == 2 || wt == 4 || wt == 8) && (rows == 1 || rows == 2 || rows == 4);
+  }
+  // plain / BF16: agents per lane | non-temporal << 16
+  const int v = vec & 0xffff;
+  if ((vec >> 17) != 0 || (v != 1 && v != 2 && v != 4)) return false;
+  if (prob == PROB_UNFUSED) return (form == 0 || (form == 1 && v != 4)) && gather == G_DSB;
   if (prob != PROB_MAXCUT && prob != PROB_TSP) return false;
   switch (form) {
-    case 0: return (gather == G_DSB || gather == G_BSB) && (vec == 1 || vec == 2 || vec == 4);
-    case 1: return (gather == G_DSB || gather == G_BSB) && (vec == 1 || vec == 2);
-    case 2: {
-      const int wt = vec & 0xff, rows = (vec >> 8) & 0xff;
-      return gather == G_DSB && (vec >> 17) == 0 && (wt == 1 || wt == 2 || wt == 4 || wt == 8) &&
-             (rows == 1 || rows == 2 || rows == 4);
-    }
-    case 3: return gather == G_BSB && (vec == 1 || vec == 2 || vec == 4);
+    case 0: return gather == G_DSB || gather == G_BSB;
+    case 1: return (gather == G_DSB || gather == G_BSB) && v != 4;
+    case 3: return gather == G_BSB;
   }
   return false;
 }
@@ -741,13 +776,15 @@ static void launch_waves(hipStream_t st, K kern, int64_t waves, const StepArgs &
   hipLaunchKernelGGL(kern, dim3(grid_for(waves, kWaves, INT32_MAX)), dim3(kBlock), 0, st, a);
 }
 
-#define SIS_PLAIN(T, P, G, V)                                                   \
-  if (prob == P && gather == G && vec == V) {                                   \
-    launch_waves(st, k_step_plain<T, P, G, V>, a.N * (a.lda / (64 * V)), a);    \
-    return;                                                                     \
+#define SIS_PLAIN1(T, P, G, V, NT)                                                  \
+  if (prob == P && gather == G && v == V && nt == NT) {                             \
+    launch_waves(st, k_step_plain<T, P, G, V, NT>, a.N * (a.lda / (64 * V)), a);    \
+    return;                                                                         \
   }
+#define SIS_PLAIN(T, P, G, V) SIS_PLAIN1(T, P, G, V, 0) SIS_PLAIN1(T, P, G, V, 1)
 
 void launch_step(hipStream_t st, int form, int prob, int gather, int vec, const StepArgs &a) {
+  const int v = vec & 0xffff, nt = (vec >> 16) & 1;
   if (form == 0) {
     SIS_PLAIN(float, PROB_MAXCUT, G_DSB, 1)
     SIS_PLAIN(float, PROB_MAXCUT, G_DSB, 2)
@@ -776,7 +813,7 @@ void launch_step(hipStream_t st, int form, int prob, int gather, int vec, const 
     SIS_PLAIN(double, PROB_TSP, G_BSB, 1)
     SIS_PLAIN(double, PROB_TSP, G_BSB, 2)
   } else if (form == 2) {
-    const int wt = vec & 0xff, rows = (vec >> 8) & 0xff, nt = (vec >> 16) & 1;
+    const int wt = vec & 0xff, rows = (vec >> 8) & 0xff;
     const int64_t waves = ((a.N + rows - 1) / rows) * (a.W / wt);
 #define SIS_PACKED(P, V, R, NT)                                              \
   if (prob == P && wt == V && rows == R && nt == NT) {                       \
@@ -793,21 +830,28 @@ void launch_step(hipStream_t st, int form, int prob, int gather, int vec, const 
 #undef SIS_PACKED_R
 #undef SIS_PACKED
   } else if (form == 3) {
-#define SIS_BF16(P, V)                                                          \
-  if (prob == P && vec == V) {                                                  \
-    launch_waves(st, k_step_bf16<P, V>, a.N * (a.lda / (64 * V)), a);           \
+#define SIS_BF16(P, V, NT)                                                      \
+  if (prob == P && v == V && nt == NT) {                                        \
+    launch_waves(st, k_step_bf16<P, V, NT>, a.N * (a.lda / (64 * V)), a);       \
     return;                                                                     \
   }
-    SIS_BF16(PROB_MAXCUT, 1)
-    SIS_BF16(PROB_MAXCUT, 2)
-    SIS_BF16(PROB_MAXCUT, 4)
-    SIS_BF16(PROB_TSP, 1)
-    SIS_BF16(PROB_TSP, 2)
-    SIS_BF16(PROB_TSP, 4)
+    SIS_BF16(PROB_MAXCUT, 1, 0)
+    SIS_BF16(PROB_MAXCUT, 2, 0)
+    SIS_BF16(PROB_MAXCUT, 4, 0)
+    SIS_BF16(PROB_TSP, 1, 0)
+    SIS_BF16(PROB_TSP, 2, 0)
+    SIS_BF16(PROB_TSP, 4, 0)
+    SIS_BF16(PROB_MAXCUT, 1, 1)
+    SIS_BF16(PROB_MAXCUT, 2, 1)
+    SIS_BF16(PROB_MAXCUT, 4, 1)
+    SIS_BF16(PROB_TSP, 1, 1)
+    SIS_BF16(PROB_TSP, 2, 1)
+    SIS_BF16(PROB_TSP, 4, 1)
 #undef SIS_BF16
   }
 }
 #undef SIS_PLAIN
+#undef SIS_PLAIN1
 
 void launch_window_scan(hipStream_t st, int src, const void *xsrc, uint64_t *cur, uint64_t *part,
                         int64_t N, int64_t W, int64_t lda, int64_t nblk, int64_t rows_per_blk,

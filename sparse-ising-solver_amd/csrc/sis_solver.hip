@@ -113,12 +113,12 @@ struct sis_solver {
   void *Xcur() { return p.form == SIS_FORM_F32 || p.form == SIS_FORM_F64 ? Xbuf(par) : Xbuf(0); }
 };
 
-// Tile encoding: plain / BF16 = agents per lane; packed = words | rows << 8 |
-// non-temporal << 16 (see sis_kernels.hpp).
+// Tile encoding: plain / BF16 = agents per lane | non-temporal << 16;
+// packed = words | rows << 8 | non-temporal << 16 (see sis_kernels.hpp).
 static bool tile_ok(const sis_solver *s, int vec) {
   if (!step_supported(s->p.form, s->prob, s->gather, vec)) return false;
   if (s->p.form == SIS_FORM_PACKED) return s->W % (vec & 0xff) == 0;
-  return s->lda % (64 * vec) == 0;
+  return s->lda % (64 * (vec & 0xffff)) == 0;
 }
 
 static int default_tile(const sis_solver *s) {
@@ -139,7 +139,8 @@ static std::vector<int> candidate_tiles(const sis_solver *s) {
       for (int r : {1, 2, 4})
         for (int nt : {0, 1}) c.push_back(wt | (r << 8) | (nt << 16));
   } else {
-    for (int v : {1, 2, 4}) c.push_back(v);
+    for (int v : {1, 2, 4})
+      for (int nt : {0, 1}) c.push_back(v | (nt << 16));
   }
   std::vector<int> ok;
   for (int v : c)

@@ -234,6 +234,26 @@ def test_packed_tiles_bit_exact(tile):
     assert np.array_equal(x, X.astype(np.float64)) and np.array_equal(y, Y.astype(np.float64))
 
 
+@pytest.mark.parametrize("form,dyn,tile", [("f32", sis.DSB, 4 | 1 << 16), ("f32", sis.BSB, 2 | 1 << 16),
+                                           ("f32", sis.DSB, 1 | 1 << 16), ("f64", sis.BSB, 2 | 1 << 16),
+                                           ("bf16", sis.BSB, 4 | 1 << 16), ("bf16", sis.BSB, 1)])
+def test_plain_and_bf16_tiles_bit_exact(form, dyn, tile):
+    J, pb = maxcut_problem(150, 0.05, sis.W_INT3, 16)
+    A, T, dt = 256, 300, 0.3
+    s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=A, dt=dt, form=sis.FORMS[form], dynamics=dyn,
+                           tile=tile))
+    assert s.p.tile == tile
+    X0, Y0 = seeded_state(J.n, A, 17)
+    s.set_state(X0, Y0)
+    s.advance(0, 200, T)
+    prec = {"f32": "f32", "f64": "f64", "bf16": "bf16"}[form]
+    np_dt = np.float64 if form == "f64" else np.float32
+    X, Y = X0.astype(np_dt), Y0.astype(np_dt)
+    orc.advance(pb, orc.MAXCUT, dyn, X, Y, 0, 200, T, dt, s.p.xi0, 1.0, prec)
+    x, y = s.get_state()
+    assert np.array_equal(x, X.astype(np.float64)) and np.array_equal(y, Y.astype(np.float64))
+
+
 @pytest.mark.parametrize("form", ["packed", "f32", "bf16"])
 def test_autotune_keeps_state_and_results(form):
     J, pb = maxcut_problem(300, 0.03, sis.W_PM1, 14)
