@@ -255,3 +255,145 @@ def test_tsp_window_validity():
     assert valid.tolist() == [700, 0, 0]
     v, tour = orc.tsp_tour_length(n, np.arange(1, 7, dtype=float), fin, 0)
     assert tour.tolist() == [2, 0, 3, 1]
+
+
+def test_splitmix_stream_known_answers():
+    # Published splitmix64 outputs for state 0 (Vigna's reference generator):
+    # u(0, k) is the k-th output mapped to [0, 1) by its top 53 bits.
+    for k, z in enumerate((0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F)):
+        assert orc.uniform(0, k) == (z >> 11) * 2.0 ** -53
+    X, Y = orc.init_state(5, orc.MAXCUT, 200, 64, f64=True)
+    assert X.min() >= -0.1 and X.max() < 0.1 and abs(X.mean()) < 0.005
+    assert np.std(Y) == pytest.approx(0.2 / math.sqrt(12), rel=0.03)
+    Xt, _ = orc.init_state(5, orc.TSP, 20, 8)
+    assert not Xt.any()
+
+
+def test_prestep_order():
+    # y first, then x with the new y (src/kernels.cu:19-20): (0, y) -> (dt y, y);
+    # (x, 0) -> (x - dt^2 x, -dt x).
+    dt = 0.25
+    X = np.array([[0.0, 0.5]])
+    Y = np.array([[0.3, 0.0]])
+    orc.prestep(X, Y, dt)
+    assert X[0, 0] == dt * 0.3 and Y[0, 0] == 0.3
+    assert Y[0, 1] == -dt * 0.5 and X[0, 1] == 0.5 - dt * dt * 0.5
+
+
+def test_tsp_step_structure():
+    # TSP step with c = (1, 0, 1, b): y' = 2 F + b h with F = -J (1.005 x)
+    # (src/kernels.cu:578-593), x' = x + y'; confined elements skip the update.
+    n = 4
+    d, D, mx = _tsp_instance(n, 3)
+    J = orc.tsp_qubo_dense(n, d, mx)
+    h = orc.tsp_field(n, D, mx)
+    csr = sp.csr_matrix(J)
+    csr.sort_indices()
+    pb = orc.Problem(csr.indptr, csr.indices, csr.data, field=h, ncities=n)
+    rng = np.random.default_rng(4)
+    X = rng.uniform(-0.9, 0.9, (n * n, 3))
+    X[5, 1] = 1.5
+    Y = np.zeros_like(X)
+    Xn = orc.step(pb, orc.TSP, orc.BSB, X, Y, [1.0, 0.0, 1.0, 0.75], 1.005, "f64")
+    want_y = 2 * (-(csr @ (X * 1.005))) + 0.75 * h[:, None]
+    mask = np.ones_like(X, bool)
+    mask[5, 1] = False
+    np.testing.assert_allclose(Y[mask], want_y[mask], rtol=1e-12, atol=1e-9)
+    np.testing.assert_allclose(Xn[mask], (X + want_y)[mask], rtol=1e-12, atol=1e-9)
+    assert Xn[5, 1] == 1.0 and Y[5, 1] == 0.0
+
+
+def test_unfused_kick_uses_updated_x():
+    # Four-kernel path: Update_XY, then mat_mult on the updated X, then confine.
+    pb, M = random_problem(29, 0.25, 8, symmetric=False)
+    rng = np.random.default_rng(9)
+    X = rng.uniform(-0.9, 0.9, (29, 6))
+    Y = rng.uniform(-0.9, 0.9, (29, 6))
+    c = [1.3, -0.4, 0.05, 0.0]
+    Yu = Y + c[1] * X
+    Xu = X + c[0] * Yu
+    want_y = Yu + c[2] * (-(sp.csr_matrix(M) @ np.where(np.signbit(Xu), -1.0, 1.0)))
+    want_x = Xu.copy()
+    conf = np.abs(Xu) > 1
+    want_x[conf] = np.sign(Xu[conf])
+    want_y[conf] = 0.0
+    Xn = orc.step(pb, orc.MAXCUT_UNFUSED, orc.DSB, X, Y, c, 1.0, "f64")
+    assert conf.any()
+    np.testing.assert_allclose(Xn, want_x, rtol=1e-13, atol=1e-14)
+    np.testing.assert_allclose(Y, want_y, rtol=1e-12, atol=1e-13)
+
+
+def test_window_stability_resets_on_change():
+    # A sign change at window 3 resets stability; the lock comes 24 unchanged
+    # windows after the last change, and final spins are the spins at lock.
+    n, A, K = 5, 2, 25
+    cur = np.full((n, A), -1, np.int32)
+    fin = cur.copy()
+    stab = np.zeros(A, np.int32)
+    bif = np.zeros(A, np.uint8)
+    Xa = np.full((n, A), -0.5, np.float32)
+    Xb = Xa.copy()
+    Xb[2, 0] = 0.5  # agent 0 differs, agent 1 does not
+    seq = [Xa, Xa, Xb] + [Xb] * 30
+    lock_at = None
+    for w, Xw in enumerate(seq, 1):
+        nb = orc.window_maxcut(np.ascontiguousarray(Xw), K, cur, fin, stab, bif)
+        if bif[0] and lock_at is None:
+            lock_at = w
+            assert np.array_equal(fin[:, 0], np.where(np.signbit(Xb[:, 0]), -1, 1))
+        if w == 24:
+            assert bif[1] == 1 and stab[1] == 24  # never changed: lock at window 24
+    assert lock_at == 3 + 24 and nb == 2
+
+
+def test_executed_steps_closed_form():
+    pb, _ = random_problem(10, 0.3, 11)
+    for T, P, want in ((0, 100, 0), (1, 100, 100), (250, 100, 300), (300, 100, 300), (17, 5, 20)):
+        X, Y = orc.init_state(1, orc.MAXCUT, 10, 3)
+        assert orc.solve(pb, orc.MAXCUT, orc.DSB, X, Y, T, 0, P, dt=0.1, xi0=0.2)["steps"] == want
+
+
+def test_tsp_totals_and_tour_length_direct():
+    rng = np.random.default_rng(12)
+    n = 9
+    x = rng.integers(0, 500, n).astype(float)
+    y = rng.integers(0, 500, n).astype(float)
+    d = orc.tsp_distances(x, y, 0, 1)
+    Dm = np.floor(np.hypot(x[:, None] - x[None, :], y[:, None] - y[None, :]) + 0.5)
+    D, mx = orc.tsp_totals(n, d)
+    assert np.array_equal(D, Dm.sum(1)) and mx == Dm.max()
+    perm = rng.permutation(n)
+    spins = _tour_spins(perm)[:, None].copy()
+    L, tour = orc.tsp_tour_length(n, d, spins, 0)
+    assert tour.tolist() == perm.tolist()
+    assert L == sum(Dm[perm[i], perm[(i + 1) % n]] for i in range(n))
+
+
+@pytest.mark.parametrize("n,seed", [(3, 1), (4, 2), (4, 5)])
+def test_tsp_field_ground_state_is_optimal_tour(n, seed):
+    # Brute force over all 2^(n^2) spin states: the minimum of
+    # s^T J s - h.s is a valid one-hot tour of minimum length.  (The
+    # valid-tour identity alone cannot see terms of h that are constant over
+    # valid tours; a flipped sign, a missing (n-2)(B+C) or a wrong factor on
+    # D_k all make the ground state invalid.)
+    d, D, mx = _tsp_instance(n, seed)
+    J = orc.tsp_qubo_dense(n, d, mx)
+    h = orc.tsp_field(n, D, mx)
+    S = np.array(list(itertools.product([-1, 1], repeat=n * n)), dtype=float)
+    E = np.einsum("si,ij,sj->s", S, J, S) - S @ h
+    s = S[int(np.argmin(E))].reshape(n, n)
+    assert ((s == 1).sum(0) == 1).all() and ((s == 1).sum(1) == 1).all()
+    perm = [int(np.nonzero(s[i] == 1)[0][0]) for i in range(n)]
+    L = lambda p: sum(d[orc.tri(min(p[i], p[(i + 1) % n]), max(p[i], p[(i + 1) % n]), n)] for i in range(n))
+    assert L(perm) == min(L(p) for p in itertools.permutations(range(n)))
+
+
+def test_bf16_rounding_matches_torch():
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(13)
+    vals = np.concatenate([rng.normal(size=2000).astype(np.float32),
+                           np.array([0.0, -0.0, 1.0, 1.00390625, 1.0078125, 3.0e38, -1e-40,
+                                     np.float32(1 + 2 ** -8), np.float32(1 + 3 * 2 ** -9)], np.float32)])
+    want = torch.from_numpy(vals).to(torch.bfloat16).to(torch.float32).numpy()
+    got = np.array([orc.bf16(float(v)) for v in vals], np.float32)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
