@@ -1,0 +1,91 @@
+"""The drop-in command line (src/main.cu:255-306 contract)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as orc
+import sis
+
+ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
+CLI = os.path.join(ROOT, "sparse-ising-solver_amd", "build", "sis")
+INPUTS = os.path.join(ROOT, "tests", "golden", "inputs")
+
+
+def run(args, env=None):
+    e = dict(os.environ)
+    e.update(env or {})
+    return subprocess.run([CLI] + [str(a) for a in args], capture_output=True, text=True, env=e,
+                          timeout=600)
+
+
+def test_usage_errors(tmp_path):
+    r = run(["1", "x.mtx"])
+    assert r.returncode == 255 and r.stdout.startswith("usage: ./program <default?>")
+    r = run(["0", "x.mtx", tmp_path / "o.csv", "maxcut", "0", "1"])
+    assert r.returncode == 255 and "<max_steps> <agents>" in r.stdout
+
+
+def test_unknown_extension_does_nothing(tmp_path):
+    out = tmp_path / "o.csv"
+    r = run(["1", "graph.txt", out])
+    assert r.returncode == 0 and r.stdout == "" and not out.exists() or out.read_text() == ""
+
+
+def test_bad_input_file_fails_cleanly(tmp_path):
+    p = tmp_path / "bad.mtx"
+    p.write_text("%%MatrixMarket matrix coordinate integer\n2 2 1\n1 2 1\n")
+    r = run(["0", p, tmp_path / "o.csv", "maxcut", "0", "1", "0", "0.5", "100", "64"])
+    assert r.returncode == 1 and "neither symmetric nor general" in r.stderr
+
+
+def _fields(line):
+    return [f.strip() for f in line.rstrip("\n").split(",")]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fused", [1, 0])
+def test_maxcut_line_matches_oracle(tmp_path, fused):
+    J = sis.Csr.gen_er(300, 0.05, sis.W_PM1, 3)
+    mtx = tmp_path / "g.mtx"
+    J.write_mtx(mtx)
+    out = tmp_path / "out.csv"
+    A, T, dt, seed = 128, 1500, 0.5, 99
+    r = run(["0", mtx, out, "maxcut", "1", fused, "1", dt, T, A], env={"SIS_SEED": str(seed)})
+    assert r.returncode == 0, r.stderr
+    f = _fields(out.read_text())
+    assert f[6:] == [str(mtx), "300", str(J.nnz), str(T), "0.5", str(A)]
+    rp, col, val = J.arrays()
+    pb = orc.Problem(rp, col, val, S=J.sum)
+    X, Y = orc.init_state(seed, orc.MAXCUT, J.n, A)
+    prob = orc.MAXCUT if fused else orc.MAXCUT_UNFUSED
+    res = orc.solve(pb, prob, orc.DSB, X, Y, T, 1, P=100, dt=dt, xi0=J.xi0())
+    st = orc.cut_stats(orc.cut(pb, res["cur"]))
+    assert [float(v) for v in f[1:6]] == list(st)
+    assert "max cut from random spin assignment" in r.stdout
+    assert r.stdout.startswith(f"file: {mtx}, time: ")
+
+
+@pytest.mark.gpu
+def test_tsp_line_matches_oracle(tmp_path):
+    src = os.path.join(INPUTS, "euc8.tsp")
+    out = tmp_path / "out.csv"
+    A, T, dt, seed = 256, 2000, 0.1, 5
+    r = run(["0", src, out, "tsp", "0", "1", "0", dt, T, A], env={"SIS_SEED": str(seed)})
+    assert r.returncode == 0, r.stderr
+    f = _fields(out.read_text())
+    t = sis.Tsp.read(src)
+    J, h = t.qubo()
+    rp, col, val = J.arrays()
+    pb = orc.Problem(rp, col, val, field=h, ncities=8)
+    X, Y = orc.init_state(seed, orc.TSP, 64, A)
+    res = orc.solve(pb, orc.TSP, orc.BSB, X, Y, T, 0, P=T // 10, K=50, dt=dt, xi0=0.002,
+                    coupling=1.005)
+    _, _, d, _ = t.arrays()
+    lens = np.array([orc.tsp_tour_length(8, d, res["fin"], a)[0] for a in range(A) if res["valid"][a] > 0])
+    assert int(f[5]) == len(lens)
+    if len(lens):
+        assert float(f[1]) == lens.min() and float(f[2]) == lens.max()
+    assert r.stdout.startswith("8\nEUC_2D\n\n")
+    assert "traveling salesman distance from random spin assignment" in r.stdout
