@@ -273,6 +273,66 @@ int sis_solver_agent_state(sis_solver *s, int32_t *stability, int32_t *locked, i
  * valid window), and the best (shortest) valid agent (-1 if none). */
 int sis_solver_tsp_lengths(sis_solver *s, const sis_tsp *t, double *length, int64_t *best_agent);
 
+/* ------------------------------------------------- batched serving (MaxCut) */
+/* No reference counterpart: the reference solves one instance per process
+ * (src/main.cu:214-227).  A batch engine keeps many small MaxCut dSB solves
+ * resident in one concatenated CSR and one state array, runs all of them with
+ * one persistent step kernel per step (replayed as a hipGraph per chunk of
+ * steps), admits queued requests when rows / nonzeros / slots free up, and
+ * retires each instance when its run ends (max_steps, or every agent locked
+ * with update_window).  Each instance follows exactly the single-solver packed
+ * dSB semantics (sis_solver_run with SIS_FORM_PACKED and sis_solver_init_random
+ * with the same seed), so its results are bit-identical alone and in the mix. */
+
+typedef struct sis_batch sis_batch;
+
+typedef struct sis_batch_params {
+  int32_t agents_max;    /* agents per instance <= agents_max; 64, 128 or 256 */
+  int32_t max_instances; /* resident slots */
+  int64_t max_rows;      /* capacity: sum of N over resident instances */
+  int64_t max_nnz;       /* capacity: sum of stored nonzeros */
+  int32_t chunk_steps;   /* steps per graph replay between admissions (>= 1) */
+  int32_t tile;          /* rows per task (1, 2, 4) | non-temporal << 16; 0 = default */
+} sis_batch_params;
+
+typedef struct sis_batch_status {
+  int32_t state;           /* 0 queued, 1 running, 2 done */
+  int32_t agents;
+  int64_t n;               /* rows of the instance */
+  int64_t steps_executed;
+  int64_t best_agent;      /* lowest index among maximal cuts */
+  double best_cut;
+  double submit_s, admit_s, done_s; /* host monotonic clock, seconds */
+} sis_batch_status;
+
+int sis_batch_create(const sis_batch_params *p, sis_batch **out);
+int sis_batch_destroy(sis_batch *b);
+
+/* Queues one MaxCut solve.  p supplies agents (<= agents_max), dt, xi0 (<= 0:
+ * 0.7 / (sigma sqrt N)), window period and threshold; J is copied.  The
+ * instance starts from sis_solver_init_random's state for `seed`. */
+int sis_batch_submit(sis_batch *b, const sis_csr *J, const sis_params *p, int64_t max_steps,
+                     int update_window, uint64_t seed, int64_t *ticket);
+
+/* Retires finished instances, admits queued ones that fit, then runs `steps`
+ * steps (rounded up to whole chunks) of every resident instance.  running /
+ * queued (may be NULL) receive the counts after the call. */
+int sis_batch_advance(sis_batch *b, int64_t steps, int32_t *running, int32_t *queued);
+
+/* State and timing of a ticket. */
+int sis_batch_poll(sis_batch *b, int64_t ticket, sis_batch_status *st);
+
+/* Per-agent cuts (agents doubles) and current spins (N x agents int32, may be
+ * NULL) of a finished ticket. */
+int sis_batch_result(sis_batch *b, int64_t ticket, double *cuts, int32_t *spins);
+
+/* Forgets a finished ticket's stored result. */
+int sis_batch_release(sis_batch *b, int64_t ticket);
+
+/* Times each tile on the resident instances (state restored afterwards) and
+ * keeps the fastest; all tiles are bit-identical. */
+int sis_batch_autotune(sis_batch *b, int64_t trial_steps, int32_t *tile, double *ms_per_step);
+
 #ifdef __cplusplus
 }
 #endif

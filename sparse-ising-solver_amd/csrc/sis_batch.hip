@@ -1,0 +1,849 @@
+// sis_batch.hip -- continuous batching of many small MaxCut dSB solves.
+//
+// Layout: every resident instance owns a contiguous range of global rows and
+// of nonzeros.  Column indices are stored as global rows, so one gather path
+// serves all instances.  State arrays (X, Y: rows x lda FP32; sign, current
+// and final spin words: rows x W) are indexed by global row.  A task is
+// (slot, first row, rows) and the persistent step kernel walks the task table
+// with a grid-stride loop, so the kernel and the captured graph do not change
+// when instances come and go: only the task table and the slot records do.
+//
+// Per instance the arithmetic is exactly that of k_step_packed (MaxCut): the
+// coefficients of step t are evaluated in FP64 from (t, T, dt, xi0) with the
+// same operations as the single solver's host schedule, the window follows
+// k_window_scan / k_window_finalize / k_window_merge, and the cut follows
+// k_cut_rows / k_cut_reduce -- so an instance gives bit-identical results
+// alone (sis_solver_*, SIS_FORM_PACKED) and in the mix.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <new>
+#include <unordered_map>
+#include <vector>
+
+#include "sis_device.hpp"
+#include "sis_hip_util.hpp"
+#include "sis_internal.hpp"
+
+namespace sis {
+
+enum : int32_t { SLOT_FREE = 0, SLOT_ACTIVE = 1, SLOT_DONE = 2 };
+
+struct BInst {
+  int64_t row_off;   // first global row
+  int32_t n;         // rows
+  int32_t agents;    // A <= 64 W
+  int32_t t;         // steps executed
+  int32_t T;         // max_steps (>= 1 while active)
+  int32_t P, K;      // window period and threshold
+  int32_t uw, status;
+  double dt, xi0, S; // time step, coupling scale, sum of file weights
+  int32_t nbif, pad;
+};
+
+struct BTask {
+  int32_t slot, nrows;
+  int64_t row0;
+};
+
+struct BArgs {
+  const BTask *tasks;
+  const int32_t *ntasks;
+  BInst *inst;
+  const int64_t *rbeg, *rend;
+  const int32_t *col;
+  const float *val;
+  const double *vald;
+  float *X, *Y;
+  uint64_t *S0, *S1, *cur, *fin;
+  int32_t *stab, *bif;
+  double *cuts;
+  int64_t lda, W;
+};
+
+// One step of every active instance.  Wavefront task = R rows x W words.
+template <int R, int WT, int NT>
+__global__ __launch_bounds__(kBlock) void k_batch_step(BArgs b) {
+  const int lane = threadIdx.x & 63;
+  const int nwaves = gridDim.x * kWaves;
+  const int ntasks = *b.ntasks;
+  const int64_t lda = b.lda, W = WT;
+  for (int task = wave_index(); task < ntasks; task += nwaves) {
+    const BTask tk = b.tasks[task];
+    const BInst &in = b.inst[tk.slot];
+    if (in.status != SLOT_ACTIVE) continue;
+    const int32_t t = in.t;
+    // src/kernels.cu:225-226, 258-262, evaluated as the host schedule does
+    double p = 1.025 * (double)t / (double)in.T;
+    p = p < 1.0 ? p : 1.0;
+    const float c0 = (float)(in.dt * (p + 1)), c1 = (float)(in.dt * (p - 1));
+    const float c2 = (float)(in.dt * in.xi0);
+    const uint64_t *__restrict__ S = (t & 1) ? b.S1 : b.S0;
+    uint64_t *Sn = (t & 1) ? b.S0 : b.S1;
+    float x[R][WT], y[R][WT];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      if (r < tk.nrows) {
+#pragma unroll
+        for (int u = 0; u < WT; u++) {
+          x[r][u] = ld_state<NT>(b.X + (tk.row0 + r) * lda + 64 * u + lane);
+          y[r][u] = ld_state<NT>(b.Y + (tk.row0 + r) * lda + 64 * u + lane);
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      if (r >= tk.nrows) break;
+      const int64_t row = tk.row0 + r;
+      float F[WT];
+#pragma unroll
+      for (int u = 0; u < WT; u++) F[u] = 0.0f;
+      const int64_t k1 = b.rend[row];
+      int64_t k = b.rbeg[row];
+      for (; k + 4 <= k1; k += 4) {
+        const int64_t j0 = b.col[k], j1 = b.col[k + 1], j2 = b.col[k + 2], j3 = b.col[k + 3];
+        const float v0 = b.val[k], v1 = b.val[k + 1], v2 = b.val[k + 2], v3 = b.val[k + 3];
+        uint64_t s0[WT], s1[WT], s2[WT], s3[WT];
+#pragma unroll
+        for (int u = 0; u < WT; u++) {
+          s0[u] = S[j0 * W + u];
+          s1[u] = S[j1 * W + u];
+          s2[u] = S[j2 * W + u];
+          s3[u] = S[j3 * W + u];
+        }
+#pragma unroll
+        for (int u = 0; u < WT; u++) {
+          F[u] = ((s0[u] >> lane) & 1) ? F[u] + v0 : F[u] - v0;
+          F[u] = ((s1[u] >> lane) & 1) ? F[u] + v1 : F[u] - v1;
+          F[u] = ((s2[u] >> lane) & 1) ? F[u] + v2 : F[u] - v2;
+          F[u] = ((s3[u] >> lane) & 1) ? F[u] + v3 : F[u] - v3;
+        }
+      }
+      for (; k < k1; k++) {
+        const int64_t j = b.col[k];
+        const float v = b.val[k];
+#pragma unroll
+        for (int u = 0; u < WT; u++) {
+          const uint64_t s = S[j * W + u];
+          F[u] = ((s >> lane) & 1) ? F[u] + v : F[u] - v;
+        }
+      }
+      uint64_t mine = 0;
+#pragma unroll
+      for (int u = 0; u < WT; u++) {
+        sb_update<float, PROB_MAXCUT>(x[r][u], y[r][u], F[u], c0, c1, c2, 0.0f, 0.0f);
+        st_state<NT>(b.X + row * lda + 64 * u + lane, x[r][u]);
+        st_state<NT>(b.Y + row * lda + 64 * u + lane, y[r][u]);
+        const uint64_t bits = __ballot(signbit(x[r][u]));
+        if (lane == u) mine = bits;
+      }
+      if (lane < WT) Sn[row * W + lane] = mine;
+    }
+  }
+}
+
+// After each step: one block per slot advances the instance's step count and,
+// at a window boundary, runs its window (src/kernels.cu:24-91, race-free) and
+// decides whether the instance is finished.
+__global__ __launch_bounds__(kBlock) void k_batch_post(BArgs b, int32_t nslots) {
+  __shared__ uint64_t s_mism[4], s_new[4];
+  __shared__ int s_nbif;
+  const int slot = blockIdx.x;
+  if (slot >= nslots) return;
+  BInst &in = b.inst[slot];
+  if (in.status != SLOT_ACTIVE) return;
+  const int32_t t1 = in.t + 1;
+  const int64_t W = b.W, r0 = in.row_off, n = in.n;
+  __syncthreads();
+  if (t1 % in.P == 0) {
+    if (threadIdx.x < 4) {
+      s_mism[threadIdx.x] = 0;
+      s_new[threadIdx.x] = 0;
+    }
+    if (threadIdx.x == 0) s_nbif = 0;
+    __syncthreads();
+    const uint64_t *Snew = (t1 & 1) ? b.S1 : b.S0;
+    for (int64_t p = threadIdx.x; p < n * W; p += blockDim.x) {
+      const int64_t idx = r0 * W + p;
+      const uint64_t nw = Snew[idx];
+      const uint64_t d = nw ^ b.cur[idx];
+      b.cur[idx] = nw;
+      if (d) atomicOr((unsigned long long *)&s_mism[p % W], (unsigned long long)d);
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    for (int64_t base = threadIdx.x & ~63; base < W * 64; base += blockDim.x) {
+      const int64_t w = base >> 6, ag = base + lane;
+      int now = 0, fresh = 0;
+      if (ag < in.agents) {
+        const int64_t ai = (int64_t)slot * W * 64 + ag;
+        const int prev = b.bif[ai];
+        int st = b.stab[ai];
+        if (!prev) st = ((s_mism[w] >> lane) & 1) ? 0 : st + 1;
+        now = st == in.K - 1;
+        b.stab[ai] = st;
+        b.bif[ai] = now;
+        fresh = now && !prev;
+      }
+      const uint64_t nm = __ballot(fresh), bm = __ballot(now);
+      if (lane == 0) {
+        s_new[w] = nm;
+        atomicAdd(&s_nbif, (int)__popcll(bm));
+      }
+    }
+    __syncthreads();
+    for (int64_t p = threadIdx.x; p < n * W; p += blockDim.x) {
+      const uint64_t nm = s_new[p % W];
+      if (nm) {
+        const int64_t idx = r0 * W + p;
+        b.fin[idx] = (b.fin[idx] & ~nm) | (b.cur[idx] & nm);
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      in.nbif = s_nbif;
+      if (t1 >= in.T || (in.uw && s_nbif == in.agents)) in.status = SLOT_DONE;
+    }
+  }
+  if (threadIdx.x == 0) in.t = t1;
+}
+
+// Admission: counter-based initial state of the slot's agents (as
+// k_init_state), pre-step (as k_prestep) and sign words of the result.
+// One wavefront per (row, word).
+__global__ __launch_bounds__(kBlock) void k_batch_init(BArgs b, int32_t slot, uint64_t seed) {
+  const BInst &in = b.inst[slot];
+  const int64_t W = b.W;
+  const int64_t g = wave_index();
+  if (g >= (int64_t)in.n * W) return;
+  const int lane = threadIdx.x & 63;
+  const int64_t i = g / W, w = g - i * W;
+  const int64_t row = in.row_off + i, a = 64 * w + lane;
+  float x = 0.0f, y = 0.0f;
+  if (a < in.agents) {
+    const uint64_t e = (uint64_t)(i * in.agents + a);
+    x = (float)((d_uniform(seed, 2 * e) * 2 - 1) / 10);
+    y = (float)((d_uniform(seed, 2 * e + 1) * 2 - 1) / 10);
+  }
+  const float cy = (float)(in.dt * (0.0 - 1)), cx = (float)(in.dt * (0.0 + 1));
+  y = fmaT(cy, x, y);
+  x = fmaT(cx, y, x);
+  b.X[row * b.lda + a] = x;
+  b.Y[row * b.lda + a] = y;
+  const uint64_t bits = __ballot(signbit(x));
+  if (lane == 0) {
+    b.S0[row * W + w] = bits;
+    b.cur[row * W + w] = ~0ull;
+    b.fin[row * W + w] = ~0ull;
+  }
+  if (i == 0) {
+    b.stab[(int64_t)slot * W * 64 + a] = 0;
+    b.bif[(int64_t)slot * W * 64 + a] = 0;
+  }
+}
+
+// Per-agent cut of a finished instance's current spins, -0.5 (H - S), one
+// block per listed slot; rows in a fixed order (exact for integer weights).
+__global__ __launch_bounds__(kBlock) void k_batch_cut(BArgs b, const int32_t *slots) {
+  __shared__ double s_red[kWaves][64];
+  const int32_t slot = slots[blockIdx.x];
+  const BInst &in = b.inst[slot];
+  const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int64_t W = b.W;
+  for (int64_t w = 0; w < W; w++) {
+    double acc = 0.0;
+    for (int64_t i = in.row_off + q; i < in.row_off + in.n; i += kWaves) {
+      const double si = ((b.cur[i * W + w] >> lane) & 1) ? -1.0 : 1.0;
+      double tmp = 0.0;
+      for (int64_t k = b.rbeg[i]; k < b.rend[i]; k++) {
+        const double sj = ((b.cur[(int64_t)b.col[k] * W + w] >> lane) & 1) ? -1.0 : 1.0;
+        tmp = __builtin_fma(b.vald[k], sj, tmp);
+      }
+      acc += tmp * si * 0.5;
+    }
+    s_red[q][lane] = acc;
+    __syncthreads();
+    if (q == 0)
+      b.cuts[(int64_t)slot * W * 64 + 64 * w + lane] =
+          (-0.5) * ((((s_red[0][lane] + s_red[1][lane]) + s_red[2][lane]) + s_red[3][lane]) - in.S);
+    __syncthreads();
+  }
+}
+
+}  // namespace sis
+
+using namespace sis;
+
+namespace {
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// First-fit range allocator with coalescing.
+struct RangeAlloc {
+  std::map<int64_t, int64_t> free_;  // start -> length
+  explicit RangeAlloc(int64_t cap = 0) {
+    if (cap > 0) free_[0] = cap;
+  }
+  int64_t take(int64_t len) {
+    if (len == 0) return 0;
+    for (auto it = free_.begin(); it != free_.end(); ++it)
+      if (it->second >= len) {
+        const int64_t s = it->first, l = it->second;
+        free_.erase(it);
+        if (l > len) free_[s + len] = l - len;
+        return s;
+      }
+    return -1;
+  }
+  void give(int64_t s, int64_t len) {
+    if (len == 0) return;
+    auto it = free_.emplace(s, len).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+};
+
+struct Request {
+  int64_t ticket;
+  int64_t n, T;
+  std::vector<int64_t> rp;
+  std::vector<int32_t> col;
+  std::vector<double> val;
+  double S, dt, xi0;
+  int32_t agents, P, K, uw;
+  uint64_t seed;
+  double submit_s;
+};
+
+struct Live {
+  Request req;
+  int32_t slot;
+  int64_t row_off, nnz_off;
+  double admit_s;
+};
+
+struct Result {
+  int32_t state = 0;
+  int32_t agents = 0;
+  int64_t n = 0, steps = 0, best = -1;
+  double best_cut = 0.0, submit_s = 0.0, admit_s = 0.0, done_s = 0.0;
+  std::vector<double> cuts;
+  std::vector<uint64_t> words;
+};
+
+}  // namespace
+
+struct sis_batch {
+  sis_batch_params p{};
+  int64_t W = 1, lda = 64;
+  hipStream_t stream = nullptr;
+  DevBuf<float> X, Y;
+  DevBuf<uint64_t> S0, S1, cur, fin;
+  DevBuf<int64_t> rbeg, rend;
+  DevBuf<int32_t> col;
+  DevBuf<float> val;
+  DevBuf<double> vald;
+  DevBuf<BInst> inst;
+  DevBuf<BTask> tasks;
+  DevBuf<int32_t> ntasks, stab, bif, cutslots;
+  DevBuf<double> cuts;
+  BInst *h_inst = nullptr;  // pinned mirror of the slot table
+  RangeAlloc rows, nnz;
+  std::vector<int32_t> free_slots;
+  std::deque<Request> queue;
+  std::map<int32_t, Live> live;
+  std::unordered_map<int64_t, Result> results;
+  int64_t next_ticket = 1;
+  int32_t tile = 1;
+  int32_t grid = 1;
+  int32_t ntask_cap = 0;
+  bool tasks_dirty = true;
+  hipGraphExec_t graph = nullptr;
+  int32_t graph_tile = -1;
+
+  ~sis_batch() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+    if (h_inst) (void)hipHostFree(h_inst);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  BArgs args() {
+    BArgs a{};
+    a.tasks = tasks.p;
+    a.ntasks = ntasks.p;
+    a.inst = inst.p;
+    a.rbeg = rbeg.p;
+    a.rend = rend.p;
+    a.col = col.p;
+    a.val = val.p;
+    a.vald = vald.p;
+    a.X = X.p;
+    a.Y = Y.p;
+    a.S0 = S0.p;
+    a.S1 = S1.p;
+    a.cur = cur.p;
+    a.fin = fin.p;
+    a.stab = stab.p;
+    a.bif = bif.p;
+    a.cuts = cuts.p;
+    a.lda = lda;
+    a.W = W;
+    return a;
+  }
+};
+
+namespace {
+
+bool batch_tile_ok(int W, int tile) {
+  const int r = tile & 0xffff, nt = tile >> 16;
+  return (r == 1 || r == 2 || r == 4) && (nt == 0 || nt == 1) && (W == 1 || W == 2 || W == 4);
+}
+
+template <int R, int WT, int NT>
+void launch_bstep(const sis_batch *b, const BArgs &a) {
+  hipLaunchKernelGGL((k_batch_step<R, WT, NT>), dim3(b->grid), dim3(kBlock), 0, b->stream, a);
+}
+
+void launch_batch_step(sis_batch *b) {
+  const BArgs a = b->args();
+  const int r = b->tile & 0xffff, nt = b->tile >> 16, w = (int)b->W;
+#define SIS_BSTEP(R, WT, NT) \
+  if (r == R && w == WT && nt == NT) return launch_bstep<R, WT, NT>(b, a);
+#define SIS_BSTEP_W(R, NT) SIS_BSTEP(R, 1, NT) SIS_BSTEP(R, 2, NT) SIS_BSTEP(R, 4, NT)
+  SIS_BSTEP_W(1, 0)
+  SIS_BSTEP_W(2, 0)
+  SIS_BSTEP_W(4, 0)
+  SIS_BSTEP_W(1, 1)
+  SIS_BSTEP_W(2, 1)
+  SIS_BSTEP_W(4, 1)
+#undef SIS_BSTEP_W
+#undef SIS_BSTEP
+}
+
+void launch_batch_post(sis_batch *b) {
+  hipLaunchKernelGGL(k_batch_post, dim3(b->p.max_instances), dim3(kBlock), 0, b->stream, b->args(),
+                     b->p.max_instances);
+}
+
+int rebuild_tasks(sis_batch *b) {
+  if (!b->tasks_dirty) return SIS_OK;
+  const int R = b->tile & 0xffff;
+  std::vector<BTask> t;
+  for (auto &kv : b->live) {
+    const Live &l = kv.second;
+    for (int64_t r = 0; r < l.req.n; r += R)
+      t.push_back(BTask{l.slot, (int32_t)std::min<int64_t>(R, l.req.n - r), l.row_off + r});
+  }
+  if ((int64_t)t.size() > b->ntask_cap) return fail(SIS_ERR_STATE, "batch task table overflow");
+  if (!t.empty())
+    HIPCHK(hipMemcpyAsync(b->tasks.p, t.data(), sizeof(BTask) * t.size(), hipMemcpyHostToDevice,
+                          b->stream));
+  const int32_t nt = (int32_t)t.size();
+  HIPCHK(hipMemcpyAsync(b->ntasks.p, &nt, sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  b->tasks_dirty = false;
+  return SIS_OK;
+}
+
+// Copies results of finished slots, frees their ranges.
+int retire(sis_batch *b) {
+  if (b->live.empty()) return SIS_OK;
+  HIPCHK(hipMemcpyAsync(b->h_inst, b->inst.p, sizeof(BInst) * b->p.max_instances,
+                        hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  std::vector<int32_t> done;
+  for (auto &kv : b->live)
+    if (b->h_inst[kv.first].status == SLOT_DONE) done.push_back(kv.first);
+  if (done.empty()) return SIS_OK;
+  const double t_done = now_s();
+  HIPCHK(hipMemcpyAsync(b->cutslots.p, done.data(), sizeof(int32_t) * done.size(),
+                        hipMemcpyHostToDevice, b->stream));
+  hipLaunchKernelGGL(k_batch_cut, dim3((unsigned)done.size()), dim3(kBlock), 0, b->stream,
+                     b->args(), b->cutslots.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(b->stream));
+  const int64_t L = b->W * 64;
+  for (int32_t slot : done) {
+    Live &l = b->live[slot];
+    Result &r = b->results[l.req.ticket];
+    r.state = 2;
+    r.agents = l.req.agents;
+    r.n = l.req.n;
+    r.steps = b->h_inst[slot].t;
+    r.submit_s = l.req.submit_s;
+    r.admit_s = l.admit_s;
+    r.done_s = t_done;
+    r.cuts.resize((size_t)l.req.agents);
+    HIPCHK(hipMemcpy(r.cuts.data(), b->cuts.p + (int64_t)slot * L, sizeof(double) * l.req.agents,
+                     hipMemcpyDeviceToHost));
+    r.words.resize((size_t)(l.req.n * b->W));
+    HIPCHK(hipMemcpy(r.words.data(), b->cur.p + l.row_off * b->W,
+                     sizeof(uint64_t) * r.words.size(), hipMemcpyDeviceToHost));
+    r.best = 0;
+    r.best_cut = r.cuts[0];
+    for (int32_t a = 1; a < l.req.agents; a++)
+      if (r.cuts[(size_t)a] > r.best_cut) {
+        r.best_cut = r.cuts[(size_t)a];
+        r.best = a;
+      }
+    b->h_inst[slot].status = SLOT_FREE;
+    HIPCHK(hipMemcpyAsync(b->inst.p + slot, &b->h_inst[slot], sizeof(BInst), hipMemcpyHostToDevice,
+                          b->stream));
+    b->rows.give(l.row_off, l.req.n);
+    b->nnz.give(l.nnz_off, (int64_t)l.req.col.size());
+    b->free_slots.push_back(slot);
+    b->live.erase(slot);
+  }
+  HIPCHK(hipStreamSynchronize(b->stream));
+  b->tasks_dirty = true;
+  return SIS_OK;
+}
+
+// Admits queued requests in order while the head fits.
+int admit(sis_batch *b) {
+  while (!b->queue.empty() && !b->free_slots.empty()) {
+    Request &q = b->queue.front();
+    const int64_t z = (int64_t)q.col.size();
+    const int64_t r0 = b->rows.take(q.n);
+    if (r0 < 0) break;
+    const int64_t z0 = b->nnz.take(z);
+    if (z0 < 0) {
+      b->rows.give(r0, q.n);
+      break;
+    }
+    const int32_t slot = b->free_slots.back();
+    b->free_slots.pop_back();
+    std::vector<int64_t> rb((size_t)q.n), re((size_t)q.n);
+    for (int64_t i = 0; i < q.n; i++) {
+      rb[(size_t)i] = z0 + q.rp[(size_t)i];
+      re[(size_t)i] = z0 + q.rp[(size_t)i + 1];
+    }
+    std::vector<int32_t> gcol((size_t)z);
+    std::vector<float> vf((size_t)z);
+    for (int64_t k = 0; k < z; k++) {
+      gcol[(size_t)k] = (int32_t)(r0 + q.col[(size_t)k]);
+      vf[(size_t)k] = (float)q.val[(size_t)k];
+    }
+    HIPCHK(hipMemcpy(b->rbeg.p + r0, rb.data(), sizeof(int64_t) * q.n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->rend.p + r0, re.data(), sizeof(int64_t) * q.n, hipMemcpyHostToDevice));
+    if (z) {
+      HIPCHK(hipMemcpy(b->col.p + z0, gcol.data(), sizeof(int32_t) * z, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(b->val.p + z0, vf.data(), sizeof(float) * z, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(b->vald.p + z0, q.val.data(), sizeof(double) * z, hipMemcpyHostToDevice));
+    }
+    BInst in{};
+    in.row_off = r0;
+    in.n = (int32_t)q.n;
+    in.agents = q.agents;
+    in.t = 0;
+    in.T = (int32_t)std::max<int64_t>(q.T, 1);
+    in.P = q.P;
+    in.K = q.K;
+    in.uw = q.uw;
+    in.status = q.T > 0 ? SLOT_ACTIVE : SLOT_DONE;
+    in.dt = q.dt;
+    in.xi0 = q.xi0;
+    in.S = q.S;
+    b->h_inst[slot] = in;
+    HIPCHK(hipMemcpy(b->inst.p + slot, &in, sizeof(BInst), hipMemcpyHostToDevice));
+    const int64_t waves = q.n * b->W;
+    hipLaunchKernelGGL(k_batch_init, dim3((unsigned)((waves + kWaves - 1) / kWaves)), dim3(kBlock), 0,
+                       b->stream, b->args(), slot, q.seed);
+    HIPCHK(hipGetLastError());
+    Live l;
+    l.req = std::move(q);
+    l.slot = slot;
+    l.row_off = r0;
+    l.nnz_off = z0;
+    l.admit_s = now_s();
+    Result &res = b->results[l.req.ticket];
+    res.state = 1;
+    res.admit_s = l.admit_s;
+    b->live.emplace(slot, std::move(l));
+    b->queue.pop_front();
+    b->tasks_dirty = true;
+  }
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return SIS_OK;
+}
+
+int build_graph(sis_batch *b) {
+  if (b->graph && b->graph_tile == b->tile) return SIS_OK;
+  if (b->graph) {
+    (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  for (int32_t k = 0; k < b->p.chunk_steps; k++) {
+    launch_batch_step(b);
+    launch_batch_post(b);
+  }
+  HIPCHK(hipStreamEndCapture(b->stream, &g));
+  hipError_t e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return fail(SIS_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  b->graph_tile = b->tile;
+  return SIS_OK;
+}
+
+int run_chunks(sis_batch *b, int64_t chunks) {
+  int st = build_graph(b);
+  if (st) return st;
+  for (int64_t c = 0; c < chunks; c++) HIPCHK(hipGraphLaunch(b->graph, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return SIS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sis_batch_create(const sis_batch_params *pin, sis_batch **out) {
+  if (!pin || !out) return fail(SIS_ERR_ARG, "sis_batch_create: null argument");
+  *out = nullptr;
+  sis_batch_params p = *pin;
+  if (p.agents_max != 64 && p.agents_max != 128 && p.agents_max != 256)
+    return fail(SIS_ERR_ARG, "sis_batch_create: agents_max must be 64, 128 or 256");
+  if (p.max_instances < 1 || p.max_instances > (1 << 20) || p.max_rows < 1 || p.max_nnz < 0 ||
+      p.max_rows >= INT32_MAX || p.max_nnz >= INT32_MAX)
+    return fail(SIS_ERR_ARG, "sis_batch_create: bad capacity");
+  if (p.chunk_steps < 1 || p.chunk_steps > 100000)
+    return fail(SIS_ERR_ARG, "sis_batch_create: chunk_steps must be in [1, 100000]");
+  std::unique_ptr<sis_batch> b(new (std::nothrow) sis_batch());
+  if (!b) return fail(SIS_ERR_NOMEM, "out of host memory");
+  b->W = p.agents_max / 64;
+  b->lda = p.agents_max;
+  if (p.tile == 0) p.tile = 2 | (1 << 16);
+  if (!batch_tile_ok((int)b->W, p.tile)) return fail(SIS_ERR_ARG, "sis_batch_create: unsupported tile");
+  b->tile = p.tile;
+  b->p = p;
+  b->rows = RangeAlloc(p.max_rows);
+  b->nnz = RangeAlloc(p.max_nnz);
+  for (int32_t s = p.max_instances - 1; s >= 0; s--) b->free_slots.push_back(s);
+  b->ntask_cap = (int32_t)std::min<int64_t>(INT32_MAX - 1, p.max_rows);
+  HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  HIPCHK(hipHostMalloc((void **)&b->h_inst, sizeof(BInst) * p.max_instances, hipHostMallocDefault));
+  std::memset(b->h_inst, 0, sizeof(BInst) * p.max_instances);
+  const int64_t R = p.max_rows, RW = R * b->W, L = b->W * 64 * (int64_t)p.max_instances;
+  int st;
+  if ((st = b->X.alloc((size_t)(R * b->lda))) || (st = b->Y.alloc((size_t)(R * b->lda))) ||
+      (st = b->S0.alloc((size_t)RW)) || (st = b->S1.alloc((size_t)RW)) ||
+      (st = b->cur.alloc((size_t)RW)) || (st = b->fin.alloc((size_t)RW)) ||
+      (st = b->rbeg.alloc((size_t)R)) || (st = b->rend.alloc((size_t)R)) ||
+      (st = b->col.alloc((size_t)std::max<int64_t>(1, p.max_nnz))) ||
+      (st = b->val.alloc((size_t)std::max<int64_t>(1, p.max_nnz))) ||
+      (st = b->vald.alloc((size_t)std::max<int64_t>(1, p.max_nnz))) ||
+      (st = b->inst.alloc((size_t)p.max_instances)) || (st = b->tasks.alloc((size_t)b->ntask_cap)) ||
+      (st = b->ntasks.alloc(1)) || (st = b->stab.alloc((size_t)L)) || (st = b->bif.alloc((size_t)L)) ||
+      (st = b->cutslots.alloc((size_t)p.max_instances)) || (st = b->cuts.alloc((size_t)L)))
+    return st;
+  HIPCHK(hipMemset(b->inst.p, 0, sizeof(BInst) * p.max_instances));
+  HIPCHK(hipMemset(b->ntasks.p, 0, sizeof(int32_t)));
+  int cus = 256;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, 0) == hipSuccess && prop.multiProcessorCount > 0)
+    cus = prop.multiProcessorCount;
+  b->grid = cus * 8;  // 32 wavefronts per CU, grid-stride over the task table
+  *out = b.release();
+  return SIS_OK;
+}
+
+int sis_batch_destroy(sis_batch *b) {
+  if (!b) return SIS_OK;
+  (void)hipStreamSynchronize(b->stream);
+  delete b;
+  return SIS_OK;
+}
+
+int sis_batch_submit(sis_batch *b, const sis_csr *J, const sis_params *p, int64_t max_steps,
+                     int update_window, uint64_t seed, int64_t *ticket) {
+  if (!b || !J || !p || !ticket) return fail(SIS_ERR_ARG, "sis_batch_submit: null argument");
+  if (p->problem != SIS_MAXCUT || p->dynamics != SIS_DSB || !p->fused)
+    return fail(SIS_ERR_ARG, "sis_batch_submit: batched serving runs fused MaxCut dSB");
+  if (p->agents < 1 || p->agents > b->p.agents_max)
+    return fail(SIS_ERR_ARG, "sis_batch_submit: agents exceed the engine's agents_max");
+  if (J->n < 1 || J->n > b->p.max_rows || (int64_t)J->col.size() > b->p.max_nnz)
+    return fail(SIS_ERR_ARG, "sis_batch_submit: instance larger than the engine capacity");
+  if (max_steps < 0 || max_steps >= INT32_MAX) return fail(SIS_ERR_ARG, "sis_batch_submit: bad max_steps");
+  Request q;
+  q.ticket = b->next_ticket++;
+  q.n = J->n;
+  q.T = max_steps;
+  q.rp = J->rp;
+  q.col = J->col;
+  q.val = J->val;
+  q.S = J->sum;
+  q.dt = p->dt;
+  q.xi0 = p->xi0;
+  if (q.xi0 <= 0.0) {
+    int st = sis_maxcut_xi0(J, &q.xi0);
+    if (st) return st;
+  }
+  q.agents = p->agents;
+  q.P = p->window_period > 0 ? p->window_period : 100;
+  q.K = p->window_threshold > 0 ? p->window_threshold : 25;
+  q.uw = update_window ? 1 : 0;
+  q.seed = seed;
+  q.submit_s = now_s();
+  Result &r = b->results[q.ticket];
+  r.state = 0;
+  r.submit_s = q.submit_s;
+  r.agents = q.agents;
+  r.n = q.n;
+  *ticket = q.ticket;
+  b->queue.push_back(std::move(q));
+  return SIS_OK;
+}
+
+int sis_batch_advance(sis_batch *b, int64_t steps, int32_t *running, int32_t *queued) {
+  if (!b || steps < 0) return fail(SIS_ERR_ARG, "sis_batch_advance: bad argument");
+  int st;
+  if ((st = retire(b)) || (st = admit(b)) || (st = retire(b)) || (st = rebuild_tasks(b))) return st;
+  if (!b->live.empty() && steps > 0) {
+    const int64_t chunks = (steps + b->p.chunk_steps - 1) / b->p.chunk_steps;
+    if ((st = run_chunks(b, chunks))) return st;
+  }
+  if (running) *running = (int32_t)b->live.size();
+  if (queued) *queued = (int32_t)b->queue.size();
+  return SIS_OK;
+}
+
+int sis_batch_poll(sis_batch *b, int64_t ticket, sis_batch_status *s) {
+  if (!b || !s) return fail(SIS_ERR_ARG, "sis_batch_poll: null argument");
+  auto it = b->results.find(ticket);
+  if (it == b->results.end()) return fail(SIS_ERR_ARG, "sis_batch_poll: unknown ticket");
+  const Result &r = it->second;
+  std::memset(s, 0, sizeof(*s));
+  s->state = r.state;
+  s->agents = r.agents;
+  s->n = r.n;
+  s->steps_executed = r.steps;
+  s->best_agent = r.best;
+  s->best_cut = r.best_cut;
+  s->submit_s = r.submit_s;
+  s->admit_s = r.admit_s;
+  s->done_s = r.done_s;
+  return SIS_OK;
+}
+
+int sis_batch_result(sis_batch *b, int64_t ticket, double *cuts, int32_t *spins) {
+  if (!b) return fail(SIS_ERR_ARG, "sis_batch_result: null batch");
+  auto it = b->results.find(ticket);
+  if (it == b->results.end() || it->second.state != 2)
+    return fail(SIS_ERR_STATE, "sis_batch_result: ticket not finished");
+  const Result &r = it->second;
+  if (cuts) std::memcpy(cuts, r.cuts.data(), sizeof(double) * r.cuts.size());
+  if (spins)
+    for (int64_t i = 0; i < r.n; i++)
+      for (int64_t a = 0; a < r.agents; a++)
+        spins[i * r.agents + a] = ((r.words[(size_t)(i * b->W + a / 64)] >> (a % 64)) & 1) ? -1 : 1;
+  return SIS_OK;
+}
+
+int sis_batch_release(sis_batch *b, int64_t ticket) {
+  if (!b) return fail(SIS_ERR_ARG, "sis_batch_release: null batch");
+  auto it = b->results.find(ticket);
+  if (it == b->results.end() || it->second.state != 2)
+    return fail(SIS_ERR_STATE, "sis_batch_release: ticket not finished");
+  b->results.erase(it);
+  return SIS_OK;
+}
+
+int sis_batch_autotune(sis_batch *b, int64_t trial_steps, int32_t *tile, double *ms_per_step) {
+  if (!b || trial_steps < 1 || trial_steps > 100000)
+    return fail(SIS_ERR_ARG, "sis_batch_autotune: bad argument");
+  int st;
+  if ((st = retire(b)) || (st = admit(b)) || (st = rebuild_tasks(b))) return st;
+  if (b->live.empty()) return fail(SIS_ERR_STATE, "sis_batch_autotune: no resident instance");
+  // Snapshot of everything a step mutates.
+  const int64_t R = b->p.max_rows, RW = R * b->W, L = b->W * 64 * (int64_t)b->p.max_instances;
+  DevBuf<float> sx, sy;
+  DevBuf<uint64_t> s0, s1, sc, sf;
+  DevBuf<int32_t> sst, sbf;
+  DevBuf<BInst> si;
+  if ((st = sx.alloc((size_t)(R * b->lda))) || (st = sy.alloc((size_t)(R * b->lda))) ||
+      (st = s0.alloc((size_t)RW)) || (st = s1.alloc((size_t)RW)) || (st = sc.alloc((size_t)RW)) ||
+      (st = sf.alloc((size_t)RW)) || (st = sst.alloc((size_t)L)) || (st = sbf.alloc((size_t)L)) ||
+      (st = si.alloc((size_t)b->p.max_instances)))
+    return st;
+  auto copy_all = [&](bool save) -> int {
+    auto cp = [&](void *dst, void *src, size_t bytes) -> int {
+      HIPCHK(hipMemcpyAsync(save ? dst : src, save ? src : dst, bytes, hipMemcpyDeviceToDevice,
+                            b->stream));
+      return SIS_OK;
+    };
+    int s2;
+    if ((s2 = cp(sx.p, b->X.p, sizeof(float) * R * b->lda)) ||
+        (s2 = cp(sy.p, b->Y.p, sizeof(float) * R * b->lda)) ||
+        (s2 = cp(s0.p, b->S0.p, sizeof(uint64_t) * RW)) || (s2 = cp(s1.p, b->S1.p, sizeof(uint64_t) * RW)) ||
+        (s2 = cp(sc.p, b->cur.p, sizeof(uint64_t) * RW)) || (s2 = cp(sf.p, b->fin.p, sizeof(uint64_t) * RW)) ||
+        (s2 = cp(sst.p, b->stab.p, sizeof(int32_t) * L)) || (s2 = cp(sbf.p, b->bif.p, sizeof(int32_t) * L)) ||
+        (s2 = cp(si.p, b->inst.p, sizeof(BInst) * b->p.max_instances)))
+      return s2;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return SIS_OK;
+  };
+  if ((st = copy_all(true))) return st;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  double best_ms = 1e300;
+  int32_t best = b->tile;
+  const int32_t saved = b->tile;
+  for (int r : {1, 2, 4})
+    for (int nt : {0, 1}) {
+      b->tile = r | (nt << 16);
+      b->tasks_dirty = true;
+      if ((st = copy_all(false)) || (st = rebuild_tasks(b))) break;
+      launch_batch_step(b);
+      launch_batch_post(b);
+      (void)hipEventRecord(e0, b->stream);
+      for (int64_t k = 0; k < trial_steps; k++) {
+        launch_batch_step(b);
+        launch_batch_post(b);
+      }
+      (void)hipEventRecord(e1, b->stream);
+      if (hipStreamSynchronize(b->stream) != hipSuccess) {
+        st = fail(SIS_ERR_HIP, "sis_batch_autotune: trial failed");
+        break;
+      }
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (ms / trial_steps < best_ms) {
+        best_ms = ms / trial_steps;
+        best = b->tile;
+      }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  b->tile = st ? saved : best;
+  b->tasks_dirty = true;
+  int st2 = copy_all(false);
+  if (!st) st = st2;
+  if (!st) st = rebuild_tasks(b);
+  if (st) return st;
+  if (tile) *tile = b->tile;
+  if (ms_per_step) *ms_per_step = best_ms;
+  return SIS_OK;
+}
+
+}  // extern "C"

@@ -47,6 +47,24 @@ class RunInfo(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
+class BatchParams(C.Structure):
+    _fields_ = [
+        ("agents_max", C.c_int32), ("max_instances", C.c_int32), ("max_rows", C.c_int64),
+        ("max_nnz", C.c_int64), ("chunk_steps", C.c_int32), ("tile", C.c_int32),
+    ]
+
+
+class BatchStatus(C.Structure):
+    _fields_ = [
+        ("state", C.c_int32), ("agents", C.c_int32), ("n", C.c_int64),
+        ("steps_executed", C.c_int64), ("best_agent", C.c_int64), ("best_cut", C.c_double),
+        ("submit_s", C.c_double), ("admit_s", C.c_double), ("done_s", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _P = C.c_void_p
 _I64P = C.POINTER(C.c_int64)
 _I32P = C.POINTER(C.c_int32)
@@ -95,6 +113,14 @@ _SIGS = {
     "sis_solver_energy": (None, [_P, C.c_int, _DP]),
     "sis_solver_agent_state": (None, [_P, _I32P, _I32P, _I32P]),
     "sis_solver_tsp_lengths": (None, [_P, _P, _DP, _I64P]),
+    "sis_batch_create": (None, [C.POINTER(BatchParams), C.POINTER(_P)]),
+    "sis_batch_destroy": (None, [_P]),
+    "sis_batch_submit": (None, [_P, _P, C.POINTER(Params), C.c_int64, C.c_int, C.c_uint64, _I64P]),
+    "sis_batch_advance": (None, [_P, C.c_int64, _I32P, _I32P]),
+    "sis_batch_poll": (None, [_P, C.c_int64, C.POINTER(BatchStatus)]),
+    "sis_batch_result": (None, [_P, C.c_int64, _DP, _I32P]),
+    "sis_batch_release": (None, [_P, C.c_int64]),
+    "sis_batch_autotune": (None, [_P, C.c_int64, _I32P, _DP]),
 }
 
 
@@ -372,6 +398,64 @@ class Solver:
     def close(self):
         if getattr(self, "h", None) is not None and _lib is not None:
             _lib.sis_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Batch:
+    """Continuous-batching engine for many small MaxCut dSB solves."""
+
+    def __init__(self, agents_max=64, max_instances=1024, max_rows=1 << 20, max_nnz=1 << 23,
+                 chunk_steps=100, tile=0):
+        bp = BatchParams(agents_max, max_instances, max_rows, max_nnz, chunk_steps, tile)
+        h = C.c_void_p()
+        _ck(lib().sis_batch_create(C.byref(bp), C.byref(h)), "sis_batch_create")
+        self.h = h
+        self.agents = {}
+        self.sizes = {}
+
+    def submit(self, J, agents, dt, max_steps, update_window=1, seed=1, xi0=0.0, window_period=100):
+        p = default_params(MAXCUT)
+        p.agents, p.dt, p.xi0, p.window_period, p.form = agents, dt, xi0, window_period, FORM_PACKED
+        t = C.c_int64()
+        _ck(lib().sis_batch_submit(self.h, J.h, C.byref(p), max_steps, update_window, seed,
+                                   C.byref(t)), "sis_batch_submit")
+        self.agents[t.value] = agents
+        self.sizes[t.value] = J.n
+        return t.value
+
+    def advance(self, steps):
+        r, q = C.c_int32(), C.c_int32()
+        _ck(lib().sis_batch_advance(self.h, steps, C.byref(r), C.byref(q)), "sis_batch_advance")
+        return r.value, q.value
+
+    def poll(self, ticket):
+        s = BatchStatus()
+        _ck(lib().sis_batch_poll(self.h, ticket, C.byref(s)), "sis_batch_poll")
+        return s.as_dict()
+
+    def result(self, ticket, spins=False):
+        A, n = self.agents[ticket], self.sizes[ticket]
+        cuts = np.zeros(A)
+        sp = np.zeros((n, A), np.int32) if spins else None
+        _ck(lib().sis_batch_result(self.h, ticket, _ptr(cuts, C.c_double), _ptr(sp, C.c_int32)),
+            "sis_batch_result")
+        return (cuts, sp) if spins else cuts
+
+    def release(self, ticket):
+        _ck(lib().sis_batch_release(self.h, ticket), "sis_batch_release")
+
+    def autotune(self, trial_steps=20):
+        tile, ms = C.c_int32(), C.c_double()
+        _ck(lib().sis_batch_autotune(self.h, trial_steps, C.byref(tile), C.byref(ms)),
+            "sis_batch_autotune")
+        return tile.value, ms.value
+
+    def close(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.sis_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -1,0 +1,93 @@
+"""Batched serving: every instance's result is bit-identical alone and in the
+mix (cuts, current spins and executed steps), with admission under capacity
+pressure, retirement on lock, uneven sizes and schedules."""
+import numpy as np
+import pytest
+
+import oracle as orc
+import sis
+
+pytestmark = pytest.mark.gpu
+
+
+def instances(count, seed, nmin=64, nmax=700):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        n = int(np.exp(rng.uniform(np.log(nmin), np.log(nmax))))
+        J = sis.Csr.gen_er(n, min(1.0, 8.0 / (n - 1)), sis.W_INT3, 1000 + k)
+        out.append(J)
+    return out
+
+
+def solve_alone(J, agents, dt, T, uw, seed, P=100):
+    s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=agents, dt=dt, form=sis.FORM_PACKED,
+                           window_period=P))
+    s.init_random(seed)
+    info = s.run(T, uw)
+    cuts, best = s.cuts()
+    spins = s.spins()
+    s.close()
+    return info["steps_executed"], cuts, spins
+
+
+@pytest.mark.parametrize("agents_max,agents", [(64, 64), (128, 100)])
+def test_batch_results_equal_alone(agents_max, agents):
+    Js = instances(24, 5)
+    T = [800, 1500, 0, 1200] * 6
+    dts = [0.5, 0.3, 0.7] * 8
+    # capacity for about a third of the rows: admission happens in waves
+    total = sum(J.n for J in Js)
+    b = sis.Batch(agents_max=agents_max, max_instances=8, max_rows=max(total // 3, 700),
+                  max_nnz=sum(J.nnz for J in Js), chunk_steps=100)
+    tickets = [b.submit(J, agents, dts[k], T[k], update_window=k % 2, seed=77 + k)
+               for k, J in enumerate(Js)]
+    seen_queued = False
+    for _ in range(1000):
+        running, queued = b.advance(200)
+        seen_queued |= queued > 0
+        if running == 0 and queued == 0:
+            break
+    assert seen_queued
+    b.advance(0)
+    for k, (J, t) in enumerate(zip(Js, tickets)):
+        st = b.poll(t)
+        assert st["state"] == 2, (k, st)
+        steps, cuts, spins = solve_alone(J, agents, dts[k], T[k], k % 2, 77 + k)
+        bc, bs = b.result(t, spins=True)
+        assert st["steps_executed"] == steps, k
+        assert np.array_equal(bc, cuts), k
+        assert np.array_equal(bs, spins), k
+        assert st["best_cut"] == cuts.max() and st["best_agent"] == int(np.argmax(cuts))
+        assert st["submit_s"] <= st["admit_s"] <= st["done_s"]
+
+
+def test_batch_matches_oracle_directly():
+    J = sis.Csr.gen_er(150, 8 / 149, sis.W_INT3, 3)
+    rp, col, val = J.arrays()
+    pb = orc.Problem(rp, col, val, S=J.sum)
+    b = sis.Batch(agents_max=64, max_instances=4, max_rows=1000, max_nnz=10000, chunk_steps=50)
+    t = b.submit(J, 64, 0.5, 1000, update_window=1, seed=9)
+    while b.advance(500)[0]:
+        pass
+    b.advance(0)
+    X, Y = orc.init_state(9, orc.MAXCUT, J.n, 64)
+    r = orc.solve(pb, orc.MAXCUT, orc.DSB, X, Y, 1000, 1, P=100, dt=0.5, xi0=J.xi0())
+    cuts, spins = b.result(t, spins=True)
+    assert b.poll(t)["steps_executed"] == r["steps"]
+    assert np.array_equal(spins, r["cur"]) and np.array_equal(cuts, orc.cut(pb, r["cur"]))
+
+
+def test_batch_autotune_midrun_keeps_results():
+    Js = instances(10, 8, 100, 400)
+    b = sis.Batch(agents_max=64, max_instances=16, max_rows=10000, max_nnz=200000, chunk_steps=100)
+    tickets = [b.submit(J, 64, 0.5, 1000, update_window=0, seed=5 + k) for k, J in enumerate(Js)]
+    b.advance(300)
+    tile, ms = b.autotune(10)
+    assert ms > 0 and (tile & 0xffff) in (1, 2, 4)
+    while b.advance(300)[0]:
+        pass
+    b.advance(0)
+    for k, (J, t) in enumerate(zip(Js, tickets)):
+        steps, cuts, spins = solve_alone(J, 64, 0.5, 1000, 0, 5 + k)
+        assert np.array_equal(b.result(t), cuts) and b.poll(t)["steps_executed"] == steps
