@@ -87,6 +87,105 @@ def cpu_baseline(J, A, dt, xi0, seconds, seed):
                        f"agents, {n} steps after 1 warm-up step, {el:.1f} s")
 
 
+def serving(a):
+    """Batched serving of many small MaxCut dSB instances: N log-uniform in
+    [64, 4096], Erdos-Renyi with mean degree 8, weights uniform in
+    {-3..3}\\{0}, 64 agents, dt 0.5, 5000 steps, update window on (instances
+    retire when every agent has locked)."""
+    rng = np.random.default_rng(a.seed)
+    count, A, dt, T = a.instances, 64, 0.5, a.steps
+    sizes = np.floor(np.exp(rng.uniform(np.log(64), np.log(4096), count))).astype(int)
+    Js = [sis.Csr.gen_er(int(n), 8.0 / (n - 1), sis.W_INT3, 7000 + k) for k, n in enumerate(sizes)]
+    rows, nnz = int(sizes.sum()), int(sum(J.nnz for J in Js))
+
+    def engine(tile=0):
+        return sis.Batch(agents_max=A, max_instances=count, max_rows=rows, max_nnz=nnz,
+                         chunk_steps=a.chunk, tile=tile)
+
+    # Tile choice on the full resident set, outside the timed regions.
+    b = engine()
+    for k, J in enumerate(Js):
+        b.submit(J, A, dt, T, 1, seed=k)
+    b.advance(0)
+    tile, _ = b.autotune(10)
+    b.close()
+
+    # Closed loop: everything submitted at t = 0.
+    b = engine(tile)
+    t0 = time.perf_counter()
+    tickets = [b.submit(J, A, dt, T, 1, seed=k) for k, J in enumerate(Js)]
+    while True:
+        running, queued = b.advance(a.chunk)
+        if running == 0 and queued == 0:
+            break
+    wall = time.perf_counter() - t0
+    st = [b.poll(t) for t in tickets]
+    steps_done = sum(s["steps_executed"] for s in st)
+    closed = dict(solves_per_s=count / wall, wall_s=wall,
+                  mean_steps=steps_done / count,
+                  node_agent_steps_per_s=sum(s["steps_executed"] * s["n"] * A for s in st) / wall)
+    b.close()
+
+    # Poisson arrivals at rate lam (default: 70% of the closed-loop rate).
+    lam = a.rate if a.rate > 0 else 0.7 * closed["solves_per_s"]
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, count))
+    b = engine(tile)
+    start = time.perf_counter()
+    nxt, delay = 0, {}
+    while True:
+        now = time.perf_counter() - start
+        while nxt < count and arrivals[nxt] <= now:
+            t = b.submit(Js[nxt], A, dt, T, 1, seed=nxt)
+            delay[t] = (time.perf_counter() - start) - arrivals[nxt]  # arrival -> submit
+            nxt += 1
+        running, queued = b.advance(a.chunk)
+        if nxt == count and running == 0 and queued == 0:
+            break
+        if running == 0 and queued == 0:
+            time.sleep(max(0.0, arrivals[nxt] - (time.perf_counter() - start)))
+    done = {t: b.poll(t) for t in delay}
+    # latency = scheduled arrival -> observed retirement (library clock for
+    # submit -> done, plus the loop's arrival -> submit delay)
+    lats = np.array([s["done_s"] - s["submit_s"] + delay[t] for t, s in done.items()])
+    span = max(s["done_s"] for s in done.values()) - min(s["submit_s"] for s in done.values())
+    poisson = dict(rate_per_s=lam, solves_per_s=count / span, p50_s=float(np.percentile(lats, 50)),
+                   p99_s=float(np.percentile(lats, 99)), mean_s=float(lats.mean()))
+    b.close()
+
+    # Sequential plain FP32 solves of a sample (graph-replayed window periods).
+    sample = list(range(0, count, max(1, count // a.seq_sample)))[: a.seq_sample]
+    t0 = time.perf_counter()
+    for k in sample:
+        s = sis.Solver(Js[k], dict(problem=sis.MAXCUT, agents=A, dt=dt, form=sis.FORM_F32,
+                                   use_graph=1))
+        s.init_random(k)
+        s.run(T, 1)
+        s.cuts()
+        s.close()
+    seq = len(sample) / (time.perf_counter() - t0)
+    out = {
+        "metric": "solves/s (batched serving) vs sequential plain FP32 solves",
+        "value": closed["solves_per_s"],
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": T,
+        "warmup": 0,
+        "higher_is_better": True,
+        "dtype": "f32",
+        "data": "synthetic (seeded Erdos-Renyi instances, seeded initial state)",
+        "config": {"workload": f"{count} MaxCut instances, N log-uniform [64,4096] (sum {rows}), "
+                               f"ER mean degree 8, weights {{-3..3}}\\{{0}}, {A} agents, dt {dt}, "
+                               f"{T} steps, update window on, chunk {a.chunk} steps",
+                   "autotuned_tile": tile, "nnz": nnz},
+        "closed_loop": closed,
+        "poisson": poisson,
+        "sequential_plain_solves_per_s": seq,
+        "sequential_sample": len(sample),
+        "speedup_vs_sequential_plain": closed["solves_per_s"] / seq,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,12 +198,22 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--workload", default="c3", choices=["c3", "serving"])
+    ap.add_argument("--instances", type=int, default=4096)
+    ap.add_argument("--chunk", type=int, default=100)
+    ap.add_argument("--rate", type=float, default=0.0)
+    ap.add_argument("--seq-sample", type=int, default=16)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: the solver runs on a single device (agents shard "
               "across GPUs with no per-step communication, see DESIGN.md); use --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
+    if a.workload == "serving":
+        if a.steps == 20000:
+            a.steps = 5000
+        serving(a)
+        return
 
     N = 1 << a.nodes_log2
     A, dt, K = a.agents, 0.5, a.steps

@@ -97,47 +97,156 @@ __global__ __launch_bounds__(kBlock) void k_batch_step(BArgs b) {
         }
       }
     }
+    // The R rows' gathers are interleaved so their dependent load chains
+    // (row range -> column -> sign word) overlap.  Positions past a row's end
+    // use the row itself with weight 0: F never holds -0 (it starts at +0 and
+    // is only changed by nonzero-sum steps), so adding +-0 leaves it unchanged
+    // and each row still accumulates its own entries in ascending order.
+    int64_t kb[R], deg[R];
+    int64_t maxdeg = 0;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int64_t row = tk.row0 + (r < tk.nrows ? r : 0);
+      kb[r] = b.rbeg[row];
+      deg[r] = r < tk.nrows ? b.rend[row] - kb[r] : 0;
+      maxdeg = deg[r] > maxdeg ? deg[r] : maxdeg;
+    }
+    float F[R][WT];
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+      for (int u = 0; u < WT; u++) F[r][u] = 0.0f;
+    for (int64_t d = 0; d < maxdeg; d += 2) {
+      int64_t j[R][2];
+      float v[R][2];
+#pragma unroll
+      for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+          const bool ok = d + i < deg[r];
+          const int64_t k = ok ? kb[r] + d + i : 0;
+          j[r][i] = ok ? (int64_t)b.col[k] : tk.row0;
+          v[r][i] = ok ? b.val[k] : 0.0f;
+        }
+      uint64_t s[R][2][WT];
+#pragma unroll
+      for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+          for (int u = 0; u < WT; u++) s[r][i][u] = S[j[r][i] * W + u];
+#pragma unroll
+      for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+          for (int u = 0; u < WT; u++)
+            F[r][u] = ((s[r][i][u] >> lane) & 1) ? F[r][u] + v[r][i] : F[r][u] - v[r][i];
+    }
 #pragma unroll
     for (int r = 0; r < R; r++) {
       if (r >= tk.nrows) break;
       const int64_t row = tk.row0 + r;
-      float F[WT];
-#pragma unroll
-      for (int u = 0; u < WT; u++) F[u] = 0.0f;
-      const int64_t k1 = b.rend[row];
-      int64_t k = b.rbeg[row];
-      for (; k + 4 <= k1; k += 4) {
-        const int64_t j0 = b.col[k], j1 = b.col[k + 1], j2 = b.col[k + 2], j3 = b.col[k + 3];
-        const float v0 = b.val[k], v1 = b.val[k + 1], v2 = b.val[k + 2], v3 = b.val[k + 3];
-        uint64_t s0[WT], s1[WT], s2[WT], s3[WT];
-#pragma unroll
-        for (int u = 0; u < WT; u++) {
-          s0[u] = S[j0 * W + u];
-          s1[u] = S[j1 * W + u];
-          s2[u] = S[j2 * W + u];
-          s3[u] = S[j3 * W + u];
-        }
-#pragma unroll
-        for (int u = 0; u < WT; u++) {
-          F[u] = ((s0[u] >> lane) & 1) ? F[u] + v0 : F[u] - v0;
-          F[u] = ((s1[u] >> lane) & 1) ? F[u] + v1 : F[u] - v1;
-          F[u] = ((s2[u] >> lane) & 1) ? F[u] + v2 : F[u] - v2;
-          F[u] = ((s3[u] >> lane) & 1) ? F[u] + v3 : F[u] - v3;
-        }
-      }
-      for (; k < k1; k++) {
-        const int64_t j = b.col[k];
-        const float v = b.val[k];
-#pragma unroll
-        for (int u = 0; u < WT; u++) {
-          const uint64_t s = S[j * W + u];
-          F[u] = ((s >> lane) & 1) ? F[u] + v : F[u] - v;
-        }
-      }
       uint64_t mine = 0;
 #pragma unroll
       for (int u = 0; u < WT; u++) {
-        sb_update<float, PROB_MAXCUT>(x[r][u], y[r][u], F[u], c0, c1, c2, 0.0f, 0.0f);
+        sb_update<float, PROB_MAXCUT>(x[r][u], y[r][u], F[r][u], c0, c1, c2, 0.0f, 0.0f);
+        st_state<NT>(b.X + row * lda + 64 * u + lane, x[r][u]);
+        st_state<NT>(b.Y + row * lda + 64 * u + lane, y[r][u]);
+        const uint64_t bits = __ballot(signbit(x[r][u]));
+        if (lane == u) mine = bits;
+      }
+      if (lane < WT) Sn[row * W + lane] = mine;
+    }
+  }
+}
+
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Same step as k_batch_step with the gathers on the vector memory path:
+// lane l serves slot l of the task (row l / (64/R), neighbour position
+// l % (64/R) of the pass) and loads that neighbour's column, weight and sign
+// words; every lane then reads the slots back with readlane, row by row and
+// in CSR order, so the arithmetic and its order are unchanged.
+template <int R, int WT, int NT>
+__global__ __launch_bounds__(kBlock) void k_batch_step_vg(BArgs b) {
+  constexpr int SL = 64 / R;  // neighbour slots per row and pass
+  const int lane = threadIdx.x & 63;
+  const int nwaves = gridDim.x * kWaves;
+  const int ntasks = *b.ntasks;
+  const int64_t lda = b.lda, W = WT;
+  const int my_r = lane / SL, my_i = lane % SL;
+  for (int task = wave_index(); task < ntasks; task += nwaves) {
+    const BTask tk = b.tasks[task];
+    const BInst &in = b.inst[tk.slot];
+    if (in.status != SLOT_ACTIVE) continue;
+    const int32_t t = in.t;
+    double p = 1.025 * (double)t / (double)in.T;
+    p = p < 1.0 ? p : 1.0;
+    const float c0 = (float)(in.dt * (p + 1)), c1 = (float)(in.dt * (p - 1));
+    const float c2 = (float)(in.dt * in.xi0);
+    const uint64_t *S = (t & 1) ? b.S1 : b.S0;
+    uint64_t *Sn = (t & 1) ? b.S0 : b.S1;
+    float x[R][WT], y[R][WT];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      if (r < tk.nrows) {
+#pragma unroll
+        for (int u = 0; u < WT; u++) {
+          x[r][u] = ld_state<NT>(b.X + (tk.row0 + r) * lda + 64 * u + lane);
+          y[r][u] = ld_state<NT>(b.Y + (tk.row0 + r) * lda + 64 * u + lane);
+        }
+      }
+    }
+    // this lane's row range
+    const int64_t myrow = tk.row0 + (my_r < tk.nrows ? my_r : 0);
+    const int64_t mykb = b.rbeg[myrow];
+    const int mydeg = my_r < tk.nrows ? (int)(b.rend[myrow] - mykb) : 0;
+    int maxdeg = mydeg;
+    for (int m = 32; m >= 1; m >>= 1) {
+      const int o = __shfl_xor(maxdeg, m, 64);
+      maxdeg = o > maxdeg ? o : maxdeg;
+    }
+    maxdeg = __builtin_amdgcn_readfirstlane(maxdeg);
+    float F[R][WT];
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+      for (int u = 0; u < WT; u++) F[r][u] = 0.0f;
+    for (int d0 = 0; d0 < maxdeg; d0 += SL) {
+      const bool ok = d0 + my_i < mydeg;
+      const int64_t k = ok ? mykb + d0 + my_i : 0;
+      const int64_t j = ok ? (int64_t)b.col[k] : tk.row0;
+      const float v = ok ? b.val[k] : 0.0f;
+      uint64_t w[WT];
+#pragma unroll
+      for (int u = 0; u < WT; u++) w[u] = S[j * W + u];
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+#pragma unroll
+        for (int i = 0; i < SL; i++) {
+          const int l = r * SL + i;
+          const float vv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+#pragma unroll
+          for (int u = 0; u < WT; u++) {
+            const uint64_t ww = readlane64(w[u], l);
+            F[r][u] = ((ww >> lane) & 1) ? F[r][u] + vv : F[r][u] - vv;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      if (r >= tk.nrows) break;
+      const int64_t row = tk.row0 + r;
+      uint64_t mine = 0;
+#pragma unroll
+      for (int u = 0; u < WT; u++) {
+        sb_update<float, PROB_MAXCUT>(x[r][u], y[r][u], F[r][u], c0, c1, c2, 0.0f, 0.0f);
         st_state<NT>(b.X + row * lda + 64 * u + lane, x[r][u]);
         st_state<NT>(b.Y + row * lda + 64 * u + lane, y[r][u]);
         const uint64_t bits = __ballot(signbit(x[r][u]));
@@ -412,9 +521,14 @@ struct sis_batch {
 
 namespace {
 
+// Tile: rows per task (1, 2, 4, 8) | non-temporal << 16 | vector gathers << 17
+// (vector gathers need 2, 4 or 8 rows per task).
 bool batch_tile_ok(int W, int tile) {
-  const int r = tile & 0xffff, nt = tile >> 16;
-  return (r == 1 || r == 2 || r == 4) && (nt == 0 || nt == 1) && (W == 1 || W == 2 || W == 4);
+  const int r = tile & 0xffff, nt = (tile >> 16) & 1, vg = (tile >> 17) & 1;
+  if ((tile >> 18) != 0 || (nt != 0 && nt != 1)) return false;
+  if (!(W == 1 || W == 2 || W == 4)) return false;
+  if (vg) return r == 2 || r == 4 || r == 8;
+  return r == 1 || r == 2 || r == 4 || r == 8;
 }
 
 template <int R, int WT, int NT>
@@ -422,20 +536,39 @@ void launch_bstep(const sis_batch *b, const BArgs &a) {
   hipLaunchKernelGGL((k_batch_step<R, WT, NT>), dim3(b->grid), dim3(kBlock), 0, b->stream, a);
 }
 
+template <int R, int WT, int NT>
+void launch_bstep_vg(const sis_batch *b, const BArgs &a) {
+  hipLaunchKernelGGL((k_batch_step_vg<R, WT, NT>), dim3(b->grid), dim3(kBlock), 0, b->stream, a);
+}
+
 void launch_batch_step(sis_batch *b) {
   const BArgs a = b->args();
-  const int r = b->tile & 0xffff, nt = b->tile >> 16, w = (int)b->W;
+  const int r = b->tile & 0xffff, nt = (b->tile >> 16) & 1, vg = (b->tile >> 17) & 1;
+  const int w = (int)b->W;
 #define SIS_BSTEP(R, WT, NT) \
-  if (r == R && w == WT && nt == NT) return launch_bstep<R, WT, NT>(b, a);
+  if (!vg && r == R && w == WT && nt == NT) return launch_bstep<R, WT, NT>(b, a);
 #define SIS_BSTEP_W(R, NT) SIS_BSTEP(R, 1, NT) SIS_BSTEP(R, 2, NT) SIS_BSTEP(R, 4, NT)
   SIS_BSTEP_W(1, 0)
   SIS_BSTEP_W(2, 0)
   SIS_BSTEP_W(4, 0)
+  SIS_BSTEP_W(8, 0)
   SIS_BSTEP_W(1, 1)
   SIS_BSTEP_W(2, 1)
   SIS_BSTEP_W(4, 1)
+  SIS_BSTEP_W(8, 1)
 #undef SIS_BSTEP_W
 #undef SIS_BSTEP
+#define SIS_BSTEPV(R, WT, NT) \
+  if (vg && r == R && w == WT && nt == NT) return launch_bstep_vg<R, WT, NT>(b, a);
+#define SIS_BSTEPV_W(R, NT) SIS_BSTEPV(R, 1, NT) SIS_BSTEPV(R, 2, NT) SIS_BSTEPV(R, 4, NT)
+  SIS_BSTEPV_W(2, 0)
+  SIS_BSTEPV_W(4, 0)
+  SIS_BSTEPV_W(8, 0)
+  SIS_BSTEPV_W(2, 1)
+  SIS_BSTEPV_W(4, 1)
+  SIS_BSTEPV_W(8, 1)
+#undef SIS_BSTEPV_W
+#undef SIS_BSTEPV
 }
 
 void launch_batch_post(sis_batch *b) {
@@ -809,9 +942,14 @@ int sis_batch_autotune(sis_batch *b, int64_t trial_steps, int32_t *tile, double 
   double best_ms = 1e300;
   int32_t best = b->tile;
   const int32_t saved = b->tile;
-  for (int r : {1, 2, 4})
-    for (int nt : {0, 1}) {
-      b->tile = r | (nt << 16);
+  std::vector<int32_t> cand;
+  for (int r : {1, 2, 4, 8})
+    for (int nt : {0, 1})
+      for (int vg : {0, 1})
+        if (batch_tile_ok((int)b->W, r | (nt << 16) | (vg << 17))) cand.push_back(r | (nt << 16) | (vg << 17));
+  for (int32_t tl : cand) {
+    {
+      b->tile = tl;
       b->tasks_dirty = true;
       if ((st = copy_all(false)) || (st = rebuild_tasks(b))) break;
       launch_batch_step(b);
@@ -833,6 +971,7 @@ int sis_batch_autotune(sis_batch *b, int64_t trial_steps, int32_t *tile, double 
         best = b->tile;
       }
     }
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   b->tile = st ? saved : best;

@@ -62,6 +62,24 @@ def test_batch_results_equal_alone(agents_max, agents):
         assert st["submit_s"] <= st["admit_s"] <= st["done_s"]
 
 
+@pytest.mark.parametrize("agents_max,tile", [(64, 1), (64, 8 | 1 << 16), (64, 2 | 1 << 17),
+                                             (64, 8 | 1 << 16 | 1 << 17), (128, 4 | 1 << 17),
+                                             (256, 2 | 1 << 16 | 1 << 17)])
+def test_batch_tiles_bit_exact(agents_max, tile):
+    Js = instances(9, 21, 64, 500)
+    A = min(agents_max, 100) if agents_max > 64 else 64
+    b = sis.Batch(agents_max=agents_max, max_instances=9, max_rows=5000, max_nnz=100000,
+                  chunk_steps=100, tile=tile)
+    tickets = [b.submit(J, A, 0.5, 600, update_window=0, seed=40 + k) for k, J in enumerate(Js)]
+    while b.advance(600)[0]:
+        pass
+    b.advance(0)
+    for k, (J, t) in enumerate(zip(Js, tickets)):
+        steps, cuts, spins = solve_alone(J, A, 0.5, 600, 0, 40 + k)
+        bc, bs = b.result(t, spins=True)
+        assert np.array_equal(bc, cuts) and np.array_equal(bs, spins), (tile, k)
+
+
 def test_batch_matches_oracle_directly():
     J = sis.Csr.gen_er(150, 8 / 149, sis.W_INT3, 3)
     rp, col, val = J.arrays()
