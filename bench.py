@@ -100,7 +100,7 @@ def serving(a):
 
     def engine(tile=0):
         return sis.Batch(agents_max=A, max_instances=count, max_rows=rows, max_nnz=nnz,
-                         chunk_steps=a.chunk, tile=tile)
+                         chunk_steps=a.chunk, tile=tile, eager=a.eager)
 
     # Tile choice on the full resident set, outside the timed regions.
     b = engine()
@@ -203,6 +203,8 @@ def main():
     ap.add_argument("--chunk", type=int, default=100)
     ap.add_argument("--rate", type=float, default=0.0)
     ap.add_argument("--seq-sample", type=int, default=16)
+    ap.add_argument("--eager", action="store_true",
+                    help="serving: launch chunk kernels directly instead of replaying a hipGraph")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: the solver runs on a single device (agents shard "

@@ -56,7 +56,7 @@ struct BArgs {
   const BTask *tasks;
   const int32_t *ntasks;
   BInst *inst;
-  const int64_t *rbeg, *rend;
+  const int64_t *rbeg, *rend;  // written only by k_batch_admit, through BCsr
   const int32_t *col;
   const float *val;
   const double *vald;
@@ -323,58 +323,123 @@ __global__ __launch_bounds__(kBlock) void k_batch_post(BArgs b, int32_t nslots) 
   if (threadIdx.x == 0) in.t = t1;
 }
 
-// Admission: counter-based initial state of the slot's agents (as
-// k_init_state), pre-step (as k_prestep) and sign words of the result.
-// One wavefront per (row, word).
-__global__ __launch_bounds__(kBlock) void k_batch_init(BArgs b, int32_t slot, uint64_t seed) {
-  const BInst &in = b.inst[slot];
-  const int64_t W = b.W;
-  const int64_t g = wave_index();
-  if (g >= (int64_t)in.n * W) return;
+// Staged description of one admitted instance.
+struct BAdmit {
+  BInst inst;
+  int32_t slot, pad;
+  uint64_t seed;
+  int64_t src_rp, src_nz, nnz_off;  // offsets into the staged row pointers / nonzeros
+};
+
+// Admission of every instance staged in one round, one block per instance:
+// CSR copy into its row / nonzero ranges (columns rebased to global rows),
+// counter-based initial state (as k_init_state), pre-step (as k_prestep),
+// the sign words of the result, reset window state, then the slot record.
+struct BCsr {
+  int64_t *rbeg, *rend;
+  int32_t *col;
+  float *val;
+  double *vald;
+};
+
+__global__ __launch_bounds__(kBlock) void k_batch_admit(BArgs b, BCsr c, const BAdmit *adm,
+                                                        const int64_t *srp, const int32_t *scol,
+                                                        const double *sval) {
+  const BAdmit d = adm[blockIdx.x];
+  const BInst &in = d.inst;
+  const int64_t W = b.W, r0 = in.row_off, n = in.n;
+  const int64_t z = srp[d.src_rp + n] - srp[d.src_rp];
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    c.rbeg[r0 + i] = d.nnz_off + srp[d.src_rp + i];
+    c.rend[r0 + i] = d.nnz_off + srp[d.src_rp + i + 1];
+  }
+  for (int64_t k = threadIdx.x; k < z; k += blockDim.x) {
+    c.col[d.nnz_off + k] = (int32_t)(r0 + scol[d.src_nz + k]);
+    c.val[d.nnz_off + k] = (float)sval[d.src_nz + k];
+    c.vald[d.nnz_off + k] = sval[d.src_nz + k];
+  }
   const int lane = threadIdx.x & 63;
-  const int64_t i = g / W, w = g - i * W;
-  const int64_t row = in.row_off + i, a = 64 * w + lane;
-  float x = 0.0f, y = 0.0f;
-  if (a < in.agents) {
-    const uint64_t e = (uint64_t)(i * in.agents + a);
-    x = (float)((d_uniform(seed, 2 * e) * 2 - 1) / 10);
-    y = (float)((d_uniform(seed, 2 * e + 1) * 2 - 1) / 10);
-  }
   const float cy = (float)(in.dt * (0.0 - 1)), cx = (float)(in.dt * (0.0 + 1));
-  y = fmaT(cy, x, y);
-  x = fmaT(cx, y, x);
-  b.X[row * b.lda + a] = x;
-  b.Y[row * b.lda + a] = y;
-  const uint64_t bits = __ballot(signbit(x));
-  if (lane == 0) {
-    b.S0[row * W + w] = bits;
-    b.cur[row * W + w] = ~0ull;
-    b.fin[row * W + w] = ~0ull;
+  for (int64_t g = threadIdx.x >> 6; g < n * W; g += kWaves) {
+    const int64_t i = g / W, w = g - i * W;
+    const int64_t row = r0 + i, a = 64 * w + lane;
+    float x = 0.0f, y = 0.0f;
+    if (a < in.agents) {
+      const uint64_t e = (uint64_t)(i * in.agents + a);
+      x = (float)((d_uniform(d.seed, 2 * e) * 2 - 1) / 10);
+      y = (float)((d_uniform(d.seed, 2 * e + 1) * 2 - 1) / 10);
+    }
+    y = fmaT(cy, x, y);
+    x = fmaT(cx, y, x);
+    b.X[row * b.lda + a] = x;
+    b.Y[row * b.lda + a] = y;
+    const uint64_t bits = __ballot(signbit(x));
+    if (lane == 0) {
+      b.S0[row * W + w] = bits;
+      b.cur[row * W + w] = ~0ull;
+      b.fin[row * W + w] = ~0ull;
+    }
   }
-  if (i == 0) {
-    b.stab[(int64_t)slot * W * 64 + a] = 0;
-    b.bif[(int64_t)slot * W * 64 + a] = 0;
+  for (int64_t a = threadIdx.x; a < W * 64; a += blockDim.x) {
+    b.stab[(int64_t)d.slot * W * 64 + a] = 0;
+    b.bif[(int64_t)d.slot * W * 64 + a] = 0;
   }
+  if (threadIdx.x == 0) b.inst[d.slot] = in;
 }
 
 // Per-agent cut of a finished instance's current spins, -0.5 (H - S), one
-// block per listed slot; rows in a fixed order (exact for integer weights).
+// block per listed slot.  Each wavefront takes groups of 8 rows; lane l loads
+// neighbour slot l % 8 of row l / 8 (vector gathers) and every lane then
+// accumulates each row's terms in CSR order from readlane broadcasts.
+// Padding slots add fma(0, +-1, tmp) = tmp (tmp starts at +0 and is never
+// -0).  Exact for integer weights.
 __global__ __launch_bounds__(kBlock) void k_batch_cut(BArgs b, const int32_t *slots) {
   __shared__ double s_red[kWaves][64];
   const int32_t slot = slots[blockIdx.x];
   const BInst &in = b.inst[slot];
   const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
-  const int64_t W = b.W;
+  const int64_t W = b.W, r0 = in.row_off, n = in.n;
+  const int my_r = lane >> 3, my_i = lane & 7;
   for (int64_t w = 0; w < W; w++) {
     double acc = 0.0;
-    for (int64_t i = in.row_off + q; i < in.row_off + in.n; i += kWaves) {
-      const double si = ((b.cur[i * W + w] >> lane) & 1) ? -1.0 : 1.0;
-      double tmp = 0.0;
-      for (int64_t k = b.rbeg[i]; k < b.rend[i]; k++) {
-        const double sj = ((b.cur[(int64_t)b.col[k] * W + w] >> lane) & 1) ? -1.0 : 1.0;
-        tmp = __builtin_fma(b.vald[k], sj, tmp);
+    for (int64_t g0 = 8 * q; g0 < n; g0 += 8 * kWaves) {
+      const bool rv = g0 + my_r < n;
+      const int64_t myrow = r0 + (rv ? g0 + my_r : 0);
+      const int64_t kb = b.rbeg[myrow];
+      const int mydeg = rv ? (int)(b.rend[myrow] - kb) : 0;
+      int maxdeg = mydeg;
+      for (int m = 32; m >= 1; m >>= 1) {
+        const int o = __shfl_xor(maxdeg, m, 64);
+        maxdeg = o > maxdeg ? o : maxdeg;
       }
-      acc += tmp * si * 0.5;
+      maxdeg = __builtin_amdgcn_readfirstlane(maxdeg);
+      double tmp[8];
+#pragma unroll
+      for (int rr = 0; rr < 8; rr++) tmp[rr] = 0.0;
+      for (int d0 = 0; d0 < maxdeg; d0 += 8) {
+        const bool ok = d0 + my_i < mydeg;
+        const int64_t k = ok ? kb + d0 + my_i : 0;
+        const int64_t j = ok ? (int64_t)b.col[k] : r0;
+        const double v = ok ? b.vald[k] : 0.0;
+        const uint64_t word = b.cur[j * W + w];
+#pragma unroll
+        for (int rr = 0; rr < 8; rr++)
+#pragma unroll
+          for (int ii = 0; ii < 8; ii++) {
+            const int l = rr * 8 + ii;
+            const uint64_t vb = readlane64((uint64_t)__double_as_longlong(v), l);
+            const uint64_t ww = readlane64(word, l);
+            const double sj = ((ww >> lane) & 1) ? -1.0 : 1.0;
+            tmp[rr] = __builtin_fma(__longlong_as_double((long long)vb), sj, tmp[rr]);
+          }
+      }
+#pragma unroll
+      for (int rr = 0; rr < 8; rr++) {
+        if (g0 + rr < n) {
+          const double si = ((b.cur[(r0 + g0 + rr) * W + w] >> lane) & 1) ? -1.0 : 1.0;
+          acc += tmp[rr] * si * 0.5;
+        }
+      }
     }
     s_red[q][lane] = acc;
     __syncthreads();
@@ -383,6 +448,20 @@ __global__ __launch_bounds__(kBlock) void k_batch_cut(BArgs b, const int32_t *sl
           (-0.5) * ((((s_red[0][lane] + s_red[1][lane]) + s_red[2][lane]) + s_red[3][lane]) - in.S);
     __syncthreads();
   }
+}
+
+// Gathers retired slots' cuts and current spin words into contiguous
+// staging areas (one block per slot) for a single copy to the host.
+__global__ __launch_bounds__(kBlock) void k_batch_collect(BArgs b, const int32_t *slots,
+                                                          const int64_t *woff, uint64_t *words,
+                                                          double *cuts) {
+  const int32_t slot = slots[blockIdx.x];
+  const BInst &in = b.inst[slot];
+  const int64_t W = b.W, L = W * 64;
+  for (int64_t p = threadIdx.x; p < (int64_t)in.n * W; p += blockDim.x)
+    words[woff[blockIdx.x] + p] = b.cur[in.row_off * W + p];
+  for (int64_t a = threadIdx.x; a < L; a += blockDim.x)
+    cuts[(int64_t)blockIdx.x * L + a] = b.cuts[(int64_t)slot * L + a];
 }
 
 }  // namespace sis
@@ -474,6 +553,12 @@ struct sis_batch {
   DevBuf<BTask> tasks;
   DevBuf<int32_t> ntasks, stab, bif, cutslots;
   DevBuf<double> cuts;
+  // admission / retirement staging (device side; grown on demand)
+  DevBuf<BAdmit> st_adm;
+  DevBuf<int64_t> st_rp, st_woff;
+  DevBuf<int32_t> st_col;
+  DevBuf<double> st_val, st_cuts;
+  DevBuf<uint64_t> st_words;
   BInst *h_inst = nullptr;  // pinned mirror of the slot table
   RangeAlloc rows, nnz;
   std::vector<int32_t> free_slots;
@@ -596,7 +681,14 @@ int rebuild_tasks(sis_batch *b) {
   return SIS_OK;
 }
 
-// Copies results of finished slots, frees their ranges.
+template <typename T>
+int ensure_cap(DevBuf<T> &buf, size_t n) {
+  if (buf.n >= n && buf.p) return SIS_OK;
+  return buf.alloc(std::max<size_t>(n, 2 * buf.n));
+}
+
+// Copies the results of finished slots (one cut kernel, one gather kernel and
+// one copy per array for the whole round) and frees their ranges.
 int retire(sis_batch *b) {
   if (b->live.empty()) return SIS_OK;
   HIPCHK(hipMemcpyAsync(b->h_inst, b->inst.p, sizeof(BInst) * b->p.max_instances,
@@ -607,14 +699,35 @@ int retire(sis_batch *b) {
     if (b->h_inst[kv.first].status == SLOT_DONE) done.push_back(kv.first);
   if (done.empty()) return SIS_OK;
   const double t_done = now_s();
+  const int64_t L = b->W * 64;
+  std::vector<int64_t> woff(done.size());
+  int64_t nw = 0;
+  for (size_t i = 0; i < done.size(); i++) {
+    woff[i] = nw;
+    nw += b->live[done[i]].req.n * b->W;
+  }
+  int st;
+  if ((st = ensure_cap(b->st_woff, done.size())) || (st = ensure_cap(b->st_words, (size_t)nw)) ||
+      (st = ensure_cap(b->st_cuts, done.size() * (size_t)L)))
+    return st;
   HIPCHK(hipMemcpyAsync(b->cutslots.p, done.data(), sizeof(int32_t) * done.size(),
+                        hipMemcpyHostToDevice, b->stream));
+  HIPCHK(hipMemcpyAsync(b->st_woff.p, woff.data(), sizeof(int64_t) * done.size(),
                         hipMemcpyHostToDevice, b->stream));
   hipLaunchKernelGGL(k_batch_cut, dim3((unsigned)done.size()), dim3(kBlock), 0, b->stream,
                      b->args(), b->cutslots.p);
+  hipLaunchKernelGGL(k_batch_collect, dim3((unsigned)done.size()), dim3(kBlock), 0, b->stream,
+                     b->args(), b->cutslots.p, b->st_woff.p, b->st_words.p, b->st_cuts.p);
   HIPCHK(hipGetLastError());
+  std::vector<uint64_t> words((size_t)nw);
+  std::vector<double> cuts(done.size() * (size_t)L);
+  HIPCHK(hipMemcpyAsync(words.data(), b->st_words.p, sizeof(uint64_t) * words.size(),
+                        hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipMemcpyAsync(cuts.data(), b->st_cuts.p, sizeof(double) * cuts.size(),
+                        hipMemcpyDeviceToHost, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
-  const int64_t L = b->W * 64;
-  for (int32_t slot : done) {
+  for (size_t i = 0; i < done.size(); i++) {
+    const int32_t slot = done[i];
     Live &l = b->live[slot];
     Result &r = b->results[l.req.ticket];
     r.state = 2;
@@ -624,12 +737,8 @@ int retire(sis_batch *b) {
     r.submit_s = l.req.submit_s;
     r.admit_s = l.admit_s;
     r.done_s = t_done;
-    r.cuts.resize((size_t)l.req.agents);
-    HIPCHK(hipMemcpy(r.cuts.data(), b->cuts.p + (int64_t)slot * L, sizeof(double) * l.req.agents,
-                     hipMemcpyDeviceToHost));
-    r.words.resize((size_t)(l.req.n * b->W));
-    HIPCHK(hipMemcpy(r.words.data(), b->cur.p + l.row_off * b->W,
-                     sizeof(uint64_t) * r.words.size(), hipMemcpyDeviceToHost));
+    r.cuts.assign(cuts.begin() + (int64_t)i * L, cuts.begin() + (int64_t)i * L + l.req.agents);
+    r.words.assign(words.begin() + woff[i], words.begin() + woff[i] + l.req.n * b->W);
     r.best = 0;
     r.best_cut = r.cuts[0];
     for (int32_t a = 1; a < l.req.agents; a++)
@@ -638,20 +747,28 @@ int retire(sis_batch *b) {
         r.best = a;
       }
     b->h_inst[slot].status = SLOT_FREE;
-    HIPCHK(hipMemcpyAsync(b->inst.p + slot, &b->h_inst[slot], sizeof(BInst), hipMemcpyHostToDevice,
-                          b->stream));
     b->rows.give(l.row_off, l.req.n);
     b->nnz.give(l.nnz_off, (int64_t)l.req.col.size());
     b->free_slots.push_back(slot);
     b->live.erase(slot);
   }
+  // Freed slot records (status only) back to the device in one copy.
+  HIPCHK(hipMemcpyAsync(b->inst.p, b->h_inst, sizeof(BInst) * b->p.max_instances,
+                        hipMemcpyHostToDevice, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
   b->tasks_dirty = true;
   return SIS_OK;
 }
 
-// Admits queued requests in order while the head fits.
+// Admits queued requests in order while the head fits: all of a round's CSR
+// data goes to the device in one staged copy per array and one kernel
+// installs and initialises every admitted instance.
 int admit(sis_batch *b) {
+  std::vector<BAdmit> adm;
+  std::vector<int64_t> srp;
+  std::vector<int32_t> scol;
+  std::vector<double> sval;
+  const double t_admit = now_s();
   while (!b->queue.empty() && !b->free_slots.empty()) {
     Request &q = b->queue.front();
     const int64_t z = (int64_t)q.col.size();
@@ -664,25 +781,8 @@ int admit(sis_batch *b) {
     }
     const int32_t slot = b->free_slots.back();
     b->free_slots.pop_back();
-    std::vector<int64_t> rb((size_t)q.n), re((size_t)q.n);
-    for (int64_t i = 0; i < q.n; i++) {
-      rb[(size_t)i] = z0 + q.rp[(size_t)i];
-      re[(size_t)i] = z0 + q.rp[(size_t)i + 1];
-    }
-    std::vector<int32_t> gcol((size_t)z);
-    std::vector<float> vf((size_t)z);
-    for (int64_t k = 0; k < z; k++) {
-      gcol[(size_t)k] = (int32_t)(r0 + q.col[(size_t)k]);
-      vf[(size_t)k] = (float)q.val[(size_t)k];
-    }
-    HIPCHK(hipMemcpy(b->rbeg.p + r0, rb.data(), sizeof(int64_t) * q.n, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(b->rend.p + r0, re.data(), sizeof(int64_t) * q.n, hipMemcpyHostToDevice));
-    if (z) {
-      HIPCHK(hipMemcpy(b->col.p + z0, gcol.data(), sizeof(int32_t) * z, hipMemcpyHostToDevice));
-      HIPCHK(hipMemcpy(b->val.p + z0, vf.data(), sizeof(float) * z, hipMemcpyHostToDevice));
-      HIPCHK(hipMemcpy(b->vald.p + z0, q.val.data(), sizeof(double) * z, hipMemcpyHostToDevice));
-    }
-    BInst in{};
+    BAdmit d{};
+    BInst &in = d.inst;
     in.row_off = r0;
     in.n = (int32_t)q.n;
     in.agents = q.agents;
@@ -695,25 +795,49 @@ int admit(sis_batch *b) {
     in.dt = q.dt;
     in.xi0 = q.xi0;
     in.S = q.S;
+    d.slot = slot;
+    d.seed = q.seed;
+    d.src_rp = (int64_t)srp.size();
+    d.src_nz = (int64_t)scol.size();
+    d.nnz_off = z0;
+    srp.insert(srp.end(), q.rp.begin(), q.rp.end());
+    scol.insert(scol.end(), q.col.begin(), q.col.end());
+    sval.insert(sval.end(), q.val.begin(), q.val.end());
+    adm.push_back(d);
     b->h_inst[slot] = in;
-    HIPCHK(hipMemcpy(b->inst.p + slot, &in, sizeof(BInst), hipMemcpyHostToDevice));
-    const int64_t waves = q.n * b->W;
-    hipLaunchKernelGGL(k_batch_init, dim3((unsigned)((waves + kWaves - 1) / kWaves)), dim3(kBlock), 0,
-                       b->stream, b->args(), slot, q.seed);
-    HIPCHK(hipGetLastError());
     Live l;
     l.req = std::move(q);
     l.slot = slot;
     l.row_off = r0;
     l.nnz_off = z0;
-    l.admit_s = now_s();
+    l.admit_s = t_admit;
     Result &res = b->results[l.req.ticket];
     res.state = 1;
-    res.admit_s = l.admit_s;
+    res.admit_s = t_admit;
     b->live.emplace(slot, std::move(l));
     b->queue.pop_front();
     b->tasks_dirty = true;
   }
+  if (adm.empty()) return SIS_OK;
+  int st;
+  if ((st = ensure_cap(b->st_adm, adm.size())) || (st = ensure_cap(b->st_rp, srp.size())) ||
+      (st = ensure_cap(b->st_col, std::max<size_t>(1, scol.size()))) ||
+      (st = ensure_cap(b->st_val, std::max<size_t>(1, sval.size()))))
+    return st;
+  HIPCHK(hipMemcpyAsync(b->st_adm.p, adm.data(), sizeof(BAdmit) * adm.size(), hipMemcpyHostToDevice,
+                        b->stream));
+  HIPCHK(hipMemcpyAsync(b->st_rp.p, srp.data(), sizeof(int64_t) * srp.size(), hipMemcpyHostToDevice,
+                        b->stream));
+  if (!scol.empty()) {
+    HIPCHK(hipMemcpyAsync(b->st_col.p, scol.data(), sizeof(int32_t) * scol.size(),
+                          hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(b->st_val.p, sval.data(), sizeof(double) * sval.size(),
+                          hipMemcpyHostToDevice, b->stream));
+  }
+  const BCsr c{b->rbeg.p, b->rend.p, b->col.p, b->val.p, b->vald.p};
+  hipLaunchKernelGGL(k_batch_admit, dim3((unsigned)adm.size()), dim3(kBlock), 0, b->stream,
+                     b->args(), c, b->st_adm.p, b->st_rp.p, b->st_col.p, b->st_val.p);
+  HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(b->stream));
   return SIS_OK;
 }
@@ -739,6 +863,15 @@ int build_graph(sis_batch *b) {
 }
 
 int run_chunks(sis_batch *b, int64_t chunks) {
+  if (b->p.eager) {
+    for (int64_t c = 0; c < chunks * b->p.chunk_steps; c++) {
+      launch_batch_step(b);
+      launch_batch_post(b);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return SIS_OK;
+  }
   int st = build_graph(b);
   if (st) return st;
   for (int64_t c = 0; c < chunks; c++) HIPCHK(hipGraphLaunch(b->graph, b->stream));

@@ -51,6 +51,7 @@ class BatchParams(C.Structure):
     _fields_ = [
         ("agents_max", C.c_int32), ("max_instances", C.c_int32), ("max_rows", C.c_int64),
         ("max_nnz", C.c_int64), ("chunk_steps", C.c_int32), ("tile", C.c_int32),
+        ("eager", C.c_int32),
     ]
 
 
@@ -408,8 +409,8 @@ class Batch:
     """Continuous-batching engine for many small MaxCut dSB solves."""
 
     def __init__(self, agents_max=64, max_instances=1024, max_rows=1 << 20, max_nnz=1 << 23,
-                 chunk_steps=100, tile=0):
-        bp = BatchParams(agents_max, max_instances, max_rows, max_nnz, chunk_steps, tile)
+                 chunk_steps=100, tile=0, eager=False):
+        bp = BatchParams(agents_max, max_instances, max_rows, max_nnz, chunk_steps, tile, int(eager))
         h = C.c_void_p()
         _ck(lib().sis_batch_create(C.byref(bp), C.byref(h)), "sis_batch_create")
         self.h = h

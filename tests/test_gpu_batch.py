@@ -80,6 +80,22 @@ def test_batch_tiles_bit_exact(agents_max, tile):
         assert np.array_equal(bc, cuts) and np.array_equal(bs, spins), (tile, k)
 
 
+def test_batch_eager_equals_graph():
+    Js = instances(6, 33, 64, 300)
+    res = []
+    for eager in (False, True):
+        b = sis.Batch(agents_max=64, max_instances=6, max_rows=3000, max_nnz=50000,
+                      chunk_steps=50, eager=eager)
+        ts = [b.submit(J, 64, 0.5, 700, update_window=1, seed=3 + k) for k, J in enumerate(Js)]
+        while b.advance(200)[0]:
+            pass
+        b.advance(0)
+        res.append([(b.result(t), b.poll(t)["steps_executed"]) for t in ts])
+        b.close()
+    for (ca, sa), (cb, sb) in zip(*res):
+        assert np.array_equal(ca, cb) and sa == sb
+
+
 def test_batch_matches_oracle_directly():
     J = sis.Csr.gen_er(150, 8 / 149, sis.W_INT3, 3)
     rp, col, val = J.arrays()
