@@ -87,6 +87,76 @@ def cpu_baseline(J, A, dt, xi0, seconds, seed):
                        f"agents, {n} steps after 1 warm-up step, {el:.1f} s")
 
 
+def run_solver(J, params, T, uw, seed, field=None, autotune=True, tsp=None):
+    """Timed sis_solver_run (pre-step + T steps + windows) of one configuration."""
+    s = sis.Solver(J, params, field=field)
+    tune = None
+    if autotune:
+        s.init_random(seed + 1)
+        tune = s.autotune(10)[0]
+    s.init_random(seed)
+    s.run(min(T, 200), uw)  # warm-up run (graph capture, caches); discarded
+    s.init_random(seed)
+    t0 = time.perf_counter()
+    info = s.run(T, uw)
+    wall = time.perf_counter() - t0
+    out = dict(info=info, wall=wall, tile=tune,
+               agent_steps_per_s=info["steps_executed"] * s.A / wall)
+    if tsp is None:
+        cuts, best = s.cuts()
+        out.update(best=float(cuts.max()), mean=float(cuts.mean()))
+    else:
+        lens, best = s.tsp_lengths(tsp)
+        ok = lens[lens >= 0]
+        out.update(valid=int(len(ok)), best=float(ok.min()) if len(ok) else None,
+                   mean=float(ok.mean()) if len(ok) else None)
+        E = s.energy(sis.SPINS_FINAL)
+        out["energy_min"] = float(E.min())
+    s.close()
+    return out
+
+
+def small_configs(a):
+    """The remaining configurations, one JSON line each (not the headline)."""
+    lines = []
+    if a.workload == "c1":
+        # ER G(800, 0.06), +1 weights, 128 agents, bSB, dt 0.1, 2000 steps, no
+        # window exit; c0 = 0.5 / (sqrt(mean degree) * std(J)) as configured.
+        J = sis.Csr.gen_er(800, 0.06, sis.W_ONE, a.seed)
+        mean_deg = J.nnz / J.n
+        sigma = 0.7 / (J.xi0() * np.sqrt(J.n))  # sample std of all N^2 entries
+        c0 = 0.5 / (np.sqrt(mean_deg) * sigma)
+        for g in (0, 1, 2):
+            r = run_solver(J, dict(problem=sis.MAXCUT, agents=128, dt=0.1, dynamics=sis.BSB,
+                                   form=sis.FORM_F32, xi0=c0, use_graph=g), 2000, 0, a.seed)
+            lines.append(dict(workload="c1 ER N=800 p=0.06 bSB plain FP32, 128 agents, 2000 steps",
+                              use_graph=g, nnz=J.nnz, xi0=c0, **{k: v for k, v in r.items() if k != "info"},
+                              ms_per_step=r["wall"] * 1e3 / r["info"]["steps_executed"]))
+    elif a.workload == "c2":
+        J = sis.Csr.gen_torus(100, 100, sis.W_PM1, a.seed)
+        for form in ("f32", "packed"):
+            r = run_solver(J, dict(problem=sis.MAXCUT, agents=1024, dt=0.5, form=sis.FORMS[form],
+                                   use_graph=1), a.steps if a.steps != 20000 else 10000, 1, a.seed)
+            ms = r["info"]["step_kernel_ms"] / max(r["info"]["steps_executed"], 1)
+            lines.append(dict(workload="c2 torus 100x100 +-1 dSB, 1024 agents, 10000 steps, uw=1",
+                              form=form, steps_executed=r["info"]["steps_executed"],
+                              **{k: v for k, v in r.items() if k != "info"},
+                              ms_per_step=r["wall"] * 1e3 / max(r["info"]["steps_executed"], 1)))
+    elif a.workload == "c4":
+        t = sis.Tsp.generate(48, 1000, 1, a.seed)
+        J, h = t.qubo()
+        for form in ("f32", "bf16"):
+            r = run_solver(J, dict(problem=sis.TSP, agents=1024, dt=0.1, form=sis.FORMS[form],
+                                   dynamics=sis.BSB, ncities=48, use_graph=1),
+                           a.steps if a.steps != 20000 else 10000, 0, a.seed, field=h, tsp=t)
+            lines.append(dict(workload="c4 TSP n=48 (2304 spins) bSB, 1024 agents, 10000 steps",
+                              form=form, nnz=J.nnz, **{k: v for k, v in r.items() if k != "info"},
+                              ms_per_step=r["wall"] * 1e3 / r["info"]["steps_executed"],
+                              step_kernel_ms=r["info"]["step_kernel_ms"]))
+    for ln in lines:
+        print(json.dumps(ln), flush=True)
+
+
 def serving(a):
     """Batched serving of many small MaxCut dSB instances: N log-uniform in
     [64, 4096], Erdos-Renyi with mean degree 8, weights uniform in
@@ -198,7 +268,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--workload", default="c3", choices=["c3", "serving"])
+    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4", "serving"])
     ap.add_argument("--instances", type=int, default=4096)
     ap.add_argument("--chunk", type=int, default=100)
     ap.add_argument("--rate", type=float, default=0.0)
@@ -215,6 +285,9 @@ def main():
         if a.steps == 20000:
             a.steps = 5000
         serving(a)
+        return
+    if a.workload in ("c1", "c2", "c4"):
+        small_configs(a)
         return
 
     N = 1 << a.nodes_log2

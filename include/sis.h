@@ -177,7 +177,9 @@ typedef struct sis_params {
   int32_t window_period;    /* P; <= 0 selects 100 (MaxCut) or max(1, T/10) (TSP) */
   int32_t window_threshold; /* K: bifurcated after K-1 unchanged windows (25) */
   int32_t ncities;          /* TSP: n, with N = n^2 spins */
-  int32_t use_graph;        /* 1: replay each window period as a hipGraph */
+  int32_t use_graph;        /* 0: direct launches; 1: replay each window period as a
+                               hipGraph; 2: capture the whole max_steps loop as one
+                               hipGraph (up to ~40k launches, else as 1) */
   int32_t tile;             /* wavefront tile, 0 = default: agents per lane (1, 2, 4)
                                for the plain and BF16 forms; for the packed form
                                words (1, 2, 4, 8) | rows (1, 2, 4) << 8 |

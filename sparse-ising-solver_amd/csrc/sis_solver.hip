@@ -64,11 +64,26 @@ struct sis_solver {
   hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
   int64_t graph_P = -1;
   int graph_uw = -1;
+  // whole-run graph (use_graph == 2): every window period of one run
+  hipGraphExec_t run_graph = nullptr;
+  int64_t run_P = -1, run_nwin = -1;
+  int run_uw = -1, run_par = -1;
   bool state_ready = false;
 
-  ~sis_solver() {
+  void drop_graphs() {
     for (auto &g : graph_exec)
-      if (g) (void)hipGraphExecDestroy(g);
+      if (g) {
+        (void)hipGraphExecDestroy(g);
+        g = nullptr;
+      }
+    if (run_graph) {
+      (void)hipGraphExecDestroy(run_graph);
+      run_graph = nullptr;
+    }
+  }
+
+  ~sis_solver() {
+    drop_graphs();
     for (auto e : events) (void)hipEventDestroy(e);
     if (h_ctrl) (void)hipHostFree(h_ctrl);
     if (stream) (void)hipStreamDestroy(stream);
@@ -355,11 +370,7 @@ static int ensure_schedule(sis_solver *s, int64_t T, int64_t len) {
   }
   s->sched_T = T;
   s->sched_len = len;
-  for (auto &g : s->graph_exec)
-    if (g) {
-      (void)hipGraphExecDestroy(g);
-      g = nullptr;
-    }
+  s->drop_graphs();  // captured launches hold the old table's address
   return SIS_OK;
 }
 
@@ -458,6 +469,38 @@ static int build_graphs(sis_solver *s, int64_t P, int uw) {
   s->par = saved;
   s->graph_P = P;
   s->graph_uw = uw;
+  return SIS_OK;
+}
+
+// Captures a whole run (nwin window periods) as one graph.  Kernels after an
+// update-window exit see the device flag and return at once.
+static int build_run_graph(sis_solver *s, int64_t P, int64_t nwin, int uw) {
+  if (s->run_graph && s->run_P == P && s->run_nwin == nwin && s->run_uw == uw &&
+      s->run_par == s->par)
+    return SIS_OK;
+  if (s->run_graph) {
+    (void)hipGraphExecDestroy(s->run_graph);
+    s->run_graph = nullptr;
+  }
+  const int saved = s->par;
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  for (int64_t w = 0; w < nwin; w++) {
+    for (int64_t k = 0; k < P; k++) enqueue_step(s, (int32_t)k);
+    enqueue_window(s, P, uw);
+  }
+  HIPCHK(hipStreamEndCapture(s->stream, &g));
+  hipError_t e = hipGraphInstantiate(&s->run_graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  s->par = saved;
+  if (e != hipSuccess) {
+    s->run_graph = nullptr;
+    return fail(SIS_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  s->run_P = P;
+  s->run_nwin = nwin;
+  s->run_uw = uw;
+  s->run_par = saved;
   return SIS_OK;
 }
 
@@ -616,11 +659,7 @@ int sis_solver_autotune(sis_solver *s, int64_t trial_steps, int32_t *tile, doubl
     }
   }
   s->vec = best;
-  for (auto &g : s->graph_exec)
-    if (g) {
-      (void)hipGraphExecDestroy(g);
-      g = nullptr;
-    }
+  s->drop_graphs();
   if ((st = restore())) return st;
   if (tile) *tile = best;
   if (ms_per_step) *ms_per_step = best_ms;
@@ -637,7 +676,11 @@ int sis_solver_run(sis_solver *s, int64_t max_steps, int update_window, sis_run_
   int st = ensure_schedule(s, std::max<int64_t>(max_steps, 1), nwin * P);
   if (st) return st;
   if ((st = ensure_events(s, (size_t)(2 * nwin + 2)))) return st;
-  if (s->p.use_graph && nwin > 0 && (st = build_graphs(s, P, uw))) return st;
+  // use_graph 2: the whole run as one graph when it stays below ~40k nodes;
+  // otherwise (and for use_graph 1) one graph per window period.
+  const bool whole = s->p.use_graph == 2 && nwin > 0 && nwin * (P + 4) <= 40000;
+  if (whole && (st = build_run_graph(s, P, nwin, uw))) return st;
+  if (!whole && s->p.use_graph && nwin > 0 && (st = build_graphs(s, P, uw))) return st;
   // Fresh spins, window state and counters for the run (the reference
   // resets them in reset_initialization, src/ising.cu:127-131).
   if ((st = reset_windows(s))) return st;
@@ -651,6 +694,10 @@ int sis_solver_run(sis_solver *s, int64_t max_steps, int update_window, sis_run_
   const int64_t chunk = uw ? 4 : nwin;
   int64_t enq = 0;
   bool stopped = false;
+  if (whole) {
+    HIPCHK(hipGraphLaunch(s->run_graph, s->stream));
+    enq = nwin;
+  }
   while (enq < nwin && !stopped) {
     const int64_t stop = std::min(nwin, enq + chunk);
     for (; enq < stop; enq++) {

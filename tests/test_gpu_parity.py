@@ -103,7 +103,7 @@ def test_tsp_trajectory_bit_exact(form, dyn, prec):
 @pytest.mark.parametrize("form,dyn,prec", [("f32", sis.DSB, "f32"), ("packed", sis.DSB, "f32"),
                                            ("f64", sis.DSB, "f64"), ("bf16", sis.BSB, "bf16")])
 @pytest.mark.parametrize("uw,T", [(1, 6000), (0, 2345)])
-@pytest.mark.parametrize("graph", [0, 1])
+@pytest.mark.parametrize("graph", [0, 1, 2])
 def test_maxcut_full_run_matches_oracle(form, dyn, prec, uw, T, graph):
     J, pb = maxcut_problem(120, 0.06, sis.W_PM1, 5)
     A = 100  # not a multiple of 64: padded agents must stay invisible
