@@ -46,6 +46,21 @@ def step_bytes(form, N, Z, A):
     return state + 4 * Z * A + csr      # plain f32
 
 
+def measured_traffic(form, tile, N, Z, A):
+    """HBM bytes per launch of this kernel tile from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs (calibrated on k_prestep), or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_maxcut_4regular_N2p20_A256.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except OSError:
+        return None
+    if (t["nodes"], t["nnz"], t["agents"]) != (N, Z, A):
+        return None
+    k = t["kernels"].get(f"{form}:{tile}")
+    return k["traffic_bytes"] if k else None
+
+
 def time_form(J, form, A, dt, K, W, seed, dyn, autotune=True):
     s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=A, dt=dt, form=sis.FORMS[form], dynamics=dyn))
     tune = None
@@ -227,7 +242,7 @@ def serving(a):
     t0 = time.perf_counter()
     for k in sample:
         s = sis.Solver(Js[k], dict(problem=sis.MAXCUT, agents=A, dt=dt, form=sis.FORM_F32,
-                                   use_graph=1))
+                                   use_graph=0 if a.eager else 1))
         s.init_random(k)
         s.run(T, 1)
         s.cuts()
@@ -337,7 +352,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(a.form, (head["tune"] or {}).get("tile"), N, Z, A),
             "kernel": f"step ({a.form})",
             "bytes_per_launch": nbytes,
             "avg_launch_ms": avg_step_ms,
