@@ -76,9 +76,29 @@ def time_form(J, form, A, dt, K, W, seed, dyn, autotune=True):
     info = s.run(K, 0)
     wall = time.perf_counter() - t0
     cuts, best = s.cuts()
+    check = energy_check(J, s, cuts, best)
     s.close()
     return dict(info=info, wall=wall, best_cut=float(cuts.max()), best_agent=int(best),
-                mean_cut=float(cuts.mean()), tune=tune)
+                mean_cut=float(cuts.mean()), tune=tune, energy_check=check)
+
+
+def energy_check(J, s, cuts, best, nagents=16):
+    """Device FP64 cut and energy of the returned spins against an FP64 host
+    evaluation (scipy) of the same spins, for the best agent and the first
+    agents; the error is 0 for integer weights."""
+    import scipy.sparse as sp
+
+    rp, col, val = J.arrays()
+    M = sp.csr_matrix((val, col, rp), shape=(J.n, J.n))
+    idx = sorted(set([int(best)] + list(range(min(nagents, s.A)))))
+    spins = s.spins(sis.SPINS_CURRENT)[:, idx].astype(np.float64)
+    e_host = np.einsum("ia,ia->a", spins, M @ spins)      # s^T J s
+    cut_host = (J.sum - 0.5 * e_host) / 2                  # -(1/2)(H - S), H = s^T J s / 2
+    e_dev = s.energy(sis.SPINS_CURRENT)[idx]
+    return dict(agents=len(idx), best_agent_included=True,
+                max_abs_err_cut=float(np.abs(cut_host - cuts[idx]).max()),
+                max_abs_err_energy=float(np.abs(e_host - e_dev).max()),
+                host="scipy FP64 sparse product of the device's current spins")
 
 
 def cpu_baseline(J, A, dt, xi0, seconds, seed):
@@ -89,17 +109,26 @@ def cpu_baseline(J, A, dt, xi0, seconds, seed):
     pb = orc.Problem(rp, col, val, S=J.sum)
     X, Y = orc.init_state(seed, orc.MAXCUT, J.n, A)
     T = 20000
-    t0 = time.perf_counter()
-    orc.advance(pb, orc.MAXCUT, orc.DSB, X, Y, 0, 1, T, dt, xi0, 1.0, "f32")
-    t1 = time.perf_counter() - t0
-    n = max(1, int(seconds / max(t1, 1e-6)) - 1)
-    t0 = time.perf_counter()
-    orc.advance(pb, orc.MAXCUT, orc.DSB, X, Y, 1, n, T, dt, xi0, 1.0, "f32")
-    el = time.perf_counter() - t0
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = orc.max_threads()
+
+    def timed(t0, seconds):
+        c = time.perf_counter()
+        orc.advance(pb, orc.MAXCUT, orc.DSB, X, Y, t0, 1, T, dt, xi0, 1.0, "f32")
+        first = time.perf_counter() - c
+        n = max(1, int(seconds / max(first, 1e-6)) - 1)
+        c = time.perf_counter()
+        orc.advance(pb, orc.MAXCUT, orc.DSB, X, Y, t0 + 1, n, T, dt, xi0, 1.0, "f32")
+        return n, time.perf_counter() - c
+
+    n, el = timed(0, seconds)
+    orc.set_threads(1)
+    n1, el1 = timed(n + 1, min(seconds, 5.0) / 2)
+    orc.set_threads(threads)
     return dict(value=n * A / el, unit="agent-steps/s", cores=threads, kind="port",
+                one_core=n1 * A / el1,
                 sample=f"oracle FP32 dSB step (OpenMP, {threads} threads) on the same graph and "
-                       f"agents, {n} steps after 1 warm-up step, {el:.1f} s")
+                       f"agents, {n} steps after 1 warm-up step, {el:.1f} s; one_core: "
+                       f"{n1} steps on 1 thread, {el1:.1f} s")
 
 
 def run_solver(J, params, T, uw, seed, field=None, autotune=True, tsp=None):
@@ -362,6 +391,7 @@ def main():
         "best_cut": head["best_cut"],
         "best_agent": head["best_agent"],
         "mean_cut": head["mean_cut"],
+        "energy_check": head["energy_check"],
     }
     if base is not None:
         bi = base["info"]
@@ -377,6 +407,7 @@ def main():
             "best_cut": base["best_cut"],
             "mean_cut": base["mean_cut"],
             "autotuned_tile": base["tune"],
+            "energy_check": base["energy_check"],
         }
         out["speedup_vs_plain"] = value / bvalue
         out["best_cut_rel_diff_vs_plain"] = (head["best_cut"] - base["best_cut"]) / abs(base["best_cut"])

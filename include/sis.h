@@ -186,6 +186,13 @@ typedef struct sis_params {
                                non-temporal state (0, 1) << 16 */
   int32_t fused;            /* MaxCut: 1 fused step (src/kernels.cu:211-269), 0 the
                                four-kernel path (src/ising.cu:167-172); plain forms */
+  int32_t split;            /* row split: rows with more than `split` stored entries
+                               are summed by several wavefronts (chunks of `split`
+                               entries, added in chunk order); 0 off, -1 chosen by
+                               sis_solver_autotune.  Bit-identical to the unsplit sum
+                               when every partial sum is exact (dSB with integer
+                               weights); otherwise within FP rounding of it. */
+  int32_t reserved;
 } sis_params;
 
 typedef struct sis_run_info {

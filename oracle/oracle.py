@@ -34,6 +34,15 @@ def lib():
     return _lib
 
 
+def set_threads(n):
+    """Threads of the oracle's OpenMP loops."""
+    lib().orc_set_threads(int(n))
+
+
+def max_threads():
+    return int(lib().orc_max_threads())
+
+
 def _p(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 

@@ -31,11 +31,17 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <omp.h>
+
 #define ORC_MAXCUT 0
 #define ORC_TSP 1
 #define ORC_UNFUSED 2 /* MaxCut four-kernel path (src/ising.cu:167-172) */
 #define ORC_DSB 0 /* gather sgn(x)  (src/kernels.cu:251-254) */
 #define ORC_BSB 1 /* gather x * coupling (src/kernels.cu:582-585) */
+
+/* Threads used by the OpenMP loops below (the CPU baseline also times one). */
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int orc_max_threads(void) { return omp_get_max_threads(); }
 
 /* ------------------------------------------------------------------ RNG */
 

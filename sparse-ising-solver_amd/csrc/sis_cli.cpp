@@ -22,9 +22,12 @@
 //   SIS_FORM = f32 | f64 | packed | bf16   precision / gather form (f32)
 //   SIS_DYNAMICS = dsb | bsb               MaxCut dynamics (dsb)
 //   SIS_EUC_ROUND = 0 | 1                  EUC_2D truncation (0, reference) or nint
-//   SIS_GRAPH = 0 | 1                      replay window periods as hipGraphs
+//   SIS_GRAPH = 0 | 1 | 2                  hipGraph per window period / whole run
 //   SIS_XI0 = <real>                       override the coupling scale
 //   SIS_WINDOW = <int>                     override the window period
+//   SIS_SPLIT = <int>                      row-split threshold (0 off, -1 tuned)
+//   SIS_TUNE = 0 | 1                       autotune the tile before the run
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -61,6 +64,14 @@ const char *env(const char *k) {
   return v && *v ? v : nullptr;
 }
 
+// SIS_TUNE=1: pick the fastest tile (and, with SIS_SPLIT=-1, the row-split
+// threshold) on the initial state before the timed run; results do not change.
+int tune(sis_solver *s) {
+  const char *t = env("SIS_TUNE");
+  if (!t || std::atoi(t) == 0) return 0;
+  return sis_solver_autotune(s, 10, nullptr, nullptr);
+}
+
 int die(const char *what) {
   std::cerr << "sis: " << what << ": " << sis_last_error() << "\n";
   return 1;
@@ -84,7 +95,8 @@ void apply_env(sis_params &p, bool maxcut) {
   }
   if (maxcut)
     if (const char *d = env("SIS_DYNAMICS")) p.dynamics = std::string(d) == "bsb" ? SIS_BSB : SIS_DSB;
-  if (const char *g = env("SIS_GRAPH")) p.use_graph = std::atoi(g) != 0;
+  if (const char *g = env("SIS_GRAPH")) p.use_graph = std::min(2, std::max(0, std::atoi(g)));
+  if (const char *r = env("SIS_SPLIT")) p.split = std::max(-1, std::atoi(r));
   if (const char *x = env("SIS_XI0")) p.xi0 = std::atof(x);
   if (const char *w = env("SIS_WINDOW")) p.window_period = std::atoi(w);
 }
@@ -107,7 +119,7 @@ int max_cut(const Opts &o, const std::string &ifile, const std::string &ofile) {
   std::cout << "file: " << ifile << ", ";
   sis_run_info info;
   std::vector<double> cut((size_t)o.agents);
-  if (sis_solver_init_random(s, o.seed) || sis_solver_run(s, o.steps, o.uw, &info) ||
+  if (sis_solver_init_random(s, o.seed) || tune(s) || sis_solver_run(s, o.steps, o.uw, &info) ||
       sis_solver_maxcut_cuts(s, cut.data(), nullptr))
     return die("solving");
   double st[5];
@@ -157,7 +169,7 @@ int traveling_salesman(const Opts &o, const std::string &ifile, const std::strin
   if (sis_solver_create(J, h.data(), &p, &s)) return die("creating the solver");
   sis_run_info info;
   std::vector<double> len((size_t)o.agents);
-  if (sis_solver_init_random(s, o.seed) || sis_solver_run(s, o.steps, o.uw, &info) ||
+  if (sis_solver_init_random(s, o.seed) || tune(s) || sis_solver_run(s, o.steps, o.uw, &info) ||
       sis_solver_tsp_lengths(s, t, len.data(), nullptr))
     return die("solving");
   double total = 0, mn = 1.7976931348623157e308, mx = -1.7976931348623157e308;

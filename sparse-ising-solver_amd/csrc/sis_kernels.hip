@@ -109,12 +109,13 @@ __global__ __launch_bounds__(kBlock) void k_step_plain(StepArgs a) {
   T *Y = (T *)a.Y;
   const int64_t lda = a.lda;
 
+  const int64_t k0 = a.rp[row], k1 = a.rp[row + 1];
+  if (a.split > 0 && k1 - k0 > a.split) return;  // k_step_split owns the row
   const Vec<T, V> xo = ldvs<T, V, NT>(Xs + row * lda + a0);
   Vec<T, V> yo = ldvs<T, V, NT>(Y + row * lda + a0);
   T F[V];
 #pragma unroll
   for (int u = 0; u < V; u++) F[u] = (T)0;
-  const int64_t k0 = a.rp[row], k1 = a.rp[row + 1];
   int64_t k = k0;
   for (; k + 4 <= k1; k += 4) {
     const int64_t j0 = col[k], j1 = col[k + 1], j2 = col[k + 2], j3 = col[k + 3];
@@ -173,12 +174,13 @@ __global__ __launch_bounds__(kBlock) void k_step_bf16(StepArgs a) {
   float *X = (float *)a.X, *Y = (float *)a.Y;
   const int64_t lda = a.lda;
 
+  const int64_t k0 = a.rp[row], k1 = a.rp[row + 1];
+  if (a.split > 0 && k1 - k0 > a.split) return;  // k_step_split owns the row
   Vec<float, V> xo = ldvs<float, V, NT>(X + row * lda + a0);
   Vec<float, V> yo = ldvs<float, V, NT>(Y + row * lda + a0);
   float F[V];
 #pragma unroll
   for (int u = 0; u < V; u++) F[u] = 0.0f;
-  const int64_t k0 = a.rp[row], k1 = a.rp[row + 1];
   int64_t k = k0;
   for (; k + 4 <= k1; k += 4) {
     const int64_t j0 = col[k], j1 = col[k + 1], j2 = col[k + 2], j3 = col[k + 3];
@@ -260,10 +262,11 @@ __global__ __launch_bounds__(kBlock) void k_step_packed(StepArgs a) {
   for (int r = 0; r < R; r++) {
     const int64_t row = row0 + r;
     if (row >= a.N) break;
+    const int64_t k0 = a.rp[row], k1 = a.rp[row + 1];
+    if (a.split > 0 && k1 - k0 > a.split) continue;  // k_step_split owns the row
     float F[WT];
 #pragma unroll
     for (int u = 0; u < WT; u++) F[u] = 0.0f;
-    const int64_t k0 = a.rp[row], k1 = a.rp[row + 1];
     int64_t k = k0;
     for (; k + 4 <= k1; k += 4) {
       const int64_t j0 = col[k], j1 = col[k + 1], j2 = col[k + 2], j3 = col[k + 3];
@@ -306,6 +309,130 @@ __global__ __launch_bounds__(kBlock) void k_step_packed(StepArgs a) {
     // lanes 0 .. WT-1 write the row's WT contiguous sign words at once
     if (lane < WT) Sn[row * W + w0 + lane] = mine;
   }
+}
+
+// ------------------------------------------------------------ row split
+//
+// FORM as in launch_step: 0 plain FP32, 1 plain FP64, 2 packed signs, 3 BF16
+// shadow.  Lane l of a workgroup owns agent 64 tile + l; every wavefront
+// gathers with the same per-entry arithmetic as the row kernels above.
+template <int FORM> struct SplitT { typedef float real; typedef float item; };
+template <> struct SplitT<1> { typedef double real; typedef double item; };
+template <> struct SplitT<2> { typedef float real; typedef uint64_t item; };
+template <> struct SplitT<3> { typedef float real; typedef uint16_t item; };
+
+template <int FORM>
+__device__ __forceinline__ typename SplitT<FORM>::item split_load(const StepArgs &a, int64_t j,
+                                                                  int64_t tile, int lane) {
+  typedef typename SplitT<FORM>::item I;
+  if constexpr (FORM == 2) return ((const I *)a.gsrc)[j * a.W + tile];  // wave-uniform word
+  else return ((const I *)a.gsrc)[j * a.lda + tile * 64 + lane];
+}
+
+template <int FORM, int GATHER>
+__device__ __forceinline__ typename SplitT<FORM>::real split_acc(typename SplitT<FORM>::real F,
+                                                                 typename SplitT<FORM>::real v,
+                                                                 typename SplitT<FORM>::item g,
+                                                                 typename SplitT<FORM>::real c,
+                                                                 int lane) {
+  if constexpr (FORM == 2) return ((g >> lane) & 1) ? F + v : F - v;
+  else if constexpr (FORM == 3) return sb_accum<float, G_BSB>(F, v, d_from_bf16(g), c);
+  else return sb_accum<typename SplitT<FORM>::real, GATHER>(F, v, g, c);
+}
+
+// F over entries [kb, ke) of one row, in CSR order.
+template <int FORM, int GATHER>
+__device__ __forceinline__ typename SplitT<FORM>::real split_chunk(const StepArgs &a, int64_t kb,
+                                                                   int64_t ke, int64_t tile,
+                                                                   int lane) {
+  typedef typename SplitT<FORM>::real T;
+  typedef typename SplitT<FORM>::item I;
+  const T *__restrict__ val = (const T *)a.val;
+  const int32_t *__restrict__ col = a.col;
+  const T c = (T)a.coupling;
+  T F = (T)0;
+  int64_t k = kb;
+  for (; k + 4 <= ke; k += 4) {
+    const T v0 = val[k], v1 = val[k + 1], v2 = val[k + 2], v3 = val[k + 3];
+    const I g0 = split_load<FORM>(a, col[k], tile, lane);
+    const I g1 = split_load<FORM>(a, col[k + 1], tile, lane);
+    const I g2 = split_load<FORM>(a, col[k + 2], tile, lane);
+    const I g3 = split_load<FORM>(a, col[k + 3], tile, lane);
+    F = split_acc<FORM, GATHER>(F, v0, g0, c, lane);
+    F = split_acc<FORM, GATHER>(F, v1, g1, c, lane);
+    F = split_acc<FORM, GATHER>(F, v2, g2, c, lane);
+    F = split_acc<FORM, GATHER>(F, v3, g3, c, lane);
+  }
+  for (; k < ke; k++) F = split_acc<FORM, GATHER>(F, val[k], split_load<FORM>(a, col[k], tile, lane), c, lane);
+  return F;
+}
+
+// The element update and stores of one row for one 64-agent tile, as the
+// row kernels above do them.  Needs the whole wavefront (packed: ballot).
+template <int FORM, int PROB>
+__device__ __forceinline__ void split_finish(const StepArgs &a, int64_t row, int64_t tile, int lane,
+                                             typename SplitT<FORM>::real F) {
+  typedef typename SplitT<FORM>::real T;
+  const int64_t i = row * a.lda + tile * 64 + lane;
+  const int64_t t = a.ctrl->base + a.offset;
+  const T *sc = (const T *)a.sched + 4 * t;
+  const T h = PROB == PROB_TSP ? ((const T *)a.field)[row] : (T)0;
+  T *Y = (T *)a.Y;
+  if constexpr (FORM == 0 || FORM == 1) {
+    T x = ((const T *)a.gsrc)[i], y = Y[i];
+    sb_update<T, PROB>(x, y, F, sc[0], sc[1], sc[2], sc[3], h);
+    ((T *)a.gdst)[i] = x;
+    Y[i] = y;
+  } else {
+    float *X = (float *)a.X;
+    float x = X[i], y = Y[i];
+    sb_update<float, PROB>(x, y, F, sc[0], sc[1], sc[2], sc[3], h);
+    X[i] = x;
+    Y[i] = y;
+    if constexpr (FORM == 3) {
+      ((uint16_t *)a.gdst)[i] = d_bf16(x);
+    } else {
+      const uint64_t b = __ballot(signbit(x));
+      if (lane == 0) ((uint64_t *)a.gdst)[row * a.W + tile] = b;
+    }
+  }
+}
+
+template <int FORM, int PROB, int GATHER>
+__global__ __launch_bounds__(kBlock) void k_step_split(StepArgs a, SplitArgs sp) {
+  typedef typename SplitT<FORM>::real T;
+  __shared__ T sF[kWaves][64];
+  if (a.ctrl->done) return;
+  const int64_t tiles = a.W;
+  const int64_t b = blockIdx.x / tiles;
+  const int64_t tile = blockIdx.x - b * tiles;
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  const SplitEnt e = sp.ent[b];
+  const int64_t kb = e.kb + (int64_t)w * sp.chunk;
+  const int64_t ke = kb + sp.chunk < e.ke ? kb + sp.chunk : e.ke;
+  sF[w][lane] = kb < ke ? split_chunk<FORM, GATHER>(a, kb, ke, tile, lane) : (T)0;
+  __syncthreads();
+  if (w != 0) return;
+  // chunk sums in chunk order (empty chunks skipped)
+  T G = sF[0][lane];
+  for (int q = 1; q < kWaves; q++)
+    if (e.kb + (int64_t)q * sp.chunk < e.ke) G = G + sF[q][lane];
+  if (e.ng > 1) {
+    T *gs = (T *)sp.part;
+    const int64_t a0 = tile * 64 + lane;
+    gs[(e.pbase + e.g) * a.lda + a0] = G;
+    __threadfence();  // the group sum is visible device-wide before the arrival
+    int last = 0;
+    if (lane == 0) last = atomicAdd(&sp.cnt[(int64_t)e.hr * tiles + tile], 1) == e.ng - 1;
+    last = __builtin_amdgcn_readfirstlane(last);
+    if (!last) return;
+    __threadfence();  // every group sum of the row is visible to this workgroup
+    G = gs[e.pbase * a.lda + a0];
+    for (int32_t g = 1; g < e.ng; g++) G = G + gs[(e.pbase + g) * a.lda + a0];
+    if (lane == 0) atomicExch(&sp.cnt[(int64_t)e.hr * tiles + tile], 0);
+  }
+  split_finish<FORM, PROB>(a, e.row, tile, lane, G);
 }
 
 // -------------------------------------------------------------- windows
@@ -703,6 +830,43 @@ void launch_step(hipStream_t st, int form, int prob, int gather, int vec, const 
 }
 #undef SIS_PLAIN
 #undef SIS_PLAIN1
+
+bool split_supported(int form, int prob, int gather) {
+  switch (form) {
+    case 0:
+    case 1:
+      return ((prob == PROB_MAXCUT || prob == PROB_TSP) && (gather == G_DSB || gather == G_BSB)) ||
+             (prob == PROB_UNFUSED && gather == G_DSB);
+    case 2: return (prob == PROB_MAXCUT || prob == PROB_TSP) && gather == G_DSB;
+    case 3: return (prob == PROB_MAXCUT || prob == PROB_TSP) && gather == G_BSB;
+  }
+  return false;
+}
+
+void launch_step_split(hipStream_t st, int form, int prob, int gather, int64_t nent,
+                       const StepArgs &a, const SplitArgs &sp) {
+  const dim3 grid((unsigned)(nent * a.W));
+#define SIS_SPLIT(F, P, G)                                                      \
+  if (form == F && prob == P && gather == G) {                                  \
+    hipLaunchKernelGGL((k_step_split<F, P, G>), grid, dim3(kBlock), 0, st, a, sp); \
+    return;                                                                     \
+  }
+  SIS_SPLIT(0, PROB_MAXCUT, G_DSB)
+  SIS_SPLIT(0, PROB_MAXCUT, G_BSB)
+  SIS_SPLIT(0, PROB_TSP, G_DSB)
+  SIS_SPLIT(0, PROB_TSP, G_BSB)
+  SIS_SPLIT(0, PROB_UNFUSED, G_DSB)
+  SIS_SPLIT(1, PROB_MAXCUT, G_DSB)
+  SIS_SPLIT(1, PROB_MAXCUT, G_BSB)
+  SIS_SPLIT(1, PROB_TSP, G_DSB)
+  SIS_SPLIT(1, PROB_TSP, G_BSB)
+  SIS_SPLIT(1, PROB_UNFUSED, G_DSB)
+  SIS_SPLIT(2, PROB_MAXCUT, G_DSB)
+  SIS_SPLIT(2, PROB_TSP, G_DSB)
+  SIS_SPLIT(3, PROB_MAXCUT, G_BSB)
+  SIS_SPLIT(3, PROB_TSP, G_BSB)
+#undef SIS_SPLIT
+}
 
 void launch_window_scan(hipStream_t st, int src, const void *xsrc, uint64_t *cur, uint64_t *part,
                         int64_t N, int64_t W, int64_t lda, int64_t nblk, int64_t rows_per_blk,

@@ -64,7 +64,33 @@ struct StepArgs {
   int32_t offset;     // step index = ctrl->base + offset
   int64_t N, lda, W;
   double coupling;
+  int64_t split;      // rows with more stored entries are left to k_step_split (0: none)
 };
+
+// Row split: a row with more than `chunk` entries is cut into groups of
+// kWaves chunks; one workgroup per (group, 64-agent tile).  Each wavefront
+// sums one chunk in CSR order, the workgroup adds its chunk sums in order
+// through LDS, and a row of one group is finished by that workgroup.  For a
+// row of several groups every workgroup stores its group sum and the last to
+// arrive (per-row counter) adds the group sums in group order and finishes the
+// row, so the result does not depend on scheduling.
+struct SplitEnt {
+  int64_t row, kb, ke;  // row and its entries [kb, ke) for this group
+  int64_t pbase;        // first group-sum slot of the row (rows of >1 group)
+  int32_t hr, g, ng;    // heavy-row index, group index, groups of the row
+  int32_t pad;
+};
+
+struct SplitArgs {
+  const SplitEnt *ent;
+  void *part;           // group sums, (slot, lda) in the form's precision
+  int32_t *cnt;         // arrivals per (heavy row, tile); the last resets it
+  int64_t chunk;        // entries per wavefront
+};
+
+bool split_supported(int form, int prob, int gather);
+void launch_step_split(hipStream_t st, int form, int prob, int gather, int64_t nent,
+                       const StepArgs &a, const SplitArgs &sp);
 
 // form: 0 f32, 1 f64, 2 packed, 3 bf16.  vec = agents per lane (f32/f64/bf16);
 // for the packed form vec = words per wavefront | rows per wavefront << 8 |

@@ -60,6 +60,15 @@ struct sis_solver {
   int64_t dist_n = -1;
   std::vector<double> dist_host;
 
+  // Row split (k_step_split): rows with more than `split` stored entries.
+  std::vector<int64_t> rp_host;  // row pointers, for building the split tables
+  int64_t max_deg = 0;
+  int64_t split = 0;             // active threshold, 0: none
+  int64_t split_nent = 0;        // SplitEnt records (0: no row exceeds the threshold)
+  DevBuf<SplitEnt> split_ent;
+  DevBuf<char> split_part;
+  DevBuf<int32_t> split_cnt;
+
   std::vector<hipEvent_t> events;
   hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
   int64_t graph_P = -1;
@@ -130,6 +139,60 @@ static std::vector<int> candidate_tiles(const sis_solver *s) {
   return ok;
 }
 
+// Row-split tables for threshold tau (0: none).  A row with deg > tau gets
+// ceil(deg / (kWaves tau)) groups of kWaves chunks of tau entries; rows of more
+// than one group get consecutive group-sum slots and one arrival counter per
+// 64-agent tile.  Waits for queued work (the old tables may be in use).
+static constexpr size_t kSplitPartMax = size_t(1) << 30;  // group-sum bytes
+
+static int build_split(sis_solver *s, int64_t tau) {
+  if (tau < 0) return fail(SIS_ERR_ARG, "row split threshold must be >= 0");
+  if (s->stream) HIPCHK(hipStreamSynchronize(s->stream));
+  s->drop_graphs();  // captured launches hold the old tables
+  s->split = 0;
+  s->split_nent = 0;
+  if (tau == 0 || tau >= s->max_deg) return SIS_OK;
+  if (!split_supported(s->p.form, s->prob, s->gather))
+    return fail(SIS_ERR_ARG, "row split: unsupported form");
+  const int64_t per = (int64_t)kWaves * tau;
+  std::vector<SplitEnt> ent;
+  int64_t heavy = 0, slots = 0;
+  for (int64_t i = 0; i < s->N; i++) {
+    const int64_t k0 = s->rp_host[i], k1 = s->rp_host[i + 1];
+    if (k1 - k0 <= tau) continue;
+    const int64_t ng = (k1 - k0 + per - 1) / per;
+    for (int64_t g = 0; g < ng; g++) {
+      SplitEnt e{};
+      e.row = i;
+      e.kb = k0 + g * per;
+      e.ke = std::min(k1, e.kb + per);
+      e.pbase = ng > 1 ? slots : 0;
+      e.hr = (int32_t)heavy;
+      e.g = (int32_t)g;
+      e.ng = (int32_t)ng;
+      ent.push_back(e);
+    }
+    if (ng > 1) slots += ng;
+    heavy++;
+  }
+  const int64_t nent = (int64_t)ent.size();
+  const size_t esz = s->f64 ? sizeof(double) : sizeof(float);
+  if (nent * s->W >= INT32_MAX || heavy * s->W >= INT32_MAX)
+    return fail(SIS_ERR_ARG, "row split: too many groups for one launch");
+  if ((size_t)slots * (size_t)s->lda * esz > kSplitPartMax)
+    return fail(SIS_ERR_ARG, "row split: group sums exceed 1 GiB; raise the threshold");
+  int st;
+  if ((st = s->split_ent.alloc((size_t)nent)) ||
+      (st = s->split_part.alloc(std::max<size_t>(1, (size_t)slots * (size_t)s->lda * esz))) ||
+      (st = s->split_cnt.alloc((size_t)std::max<int64_t>(1, heavy * s->W))))
+    return st;
+  HIPCHK(hipMemcpy(s->split_ent.p, ent.data(), sizeof(SplitEnt) * nent, hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(s->split_cnt.p, 0, sizeof(int32_t) * s->split_cnt.n));
+  s->split = tau;
+  s->split_nent = nent;
+  return SIS_OK;
+}
+
 static int resolve_period(const sis_solver *s, int64_t T) {
   if (s->p.window_period > 0) return s->p.window_period;
   if (s->p.problem == SIS_TSP) return (int)std::max<int64_t>(1, T / 10);
@@ -185,6 +248,7 @@ int sis_solver_create(const sis_csr *J, const double *field, const sis_params *p
   if (p.agents < 1 || p.agents > (1 << 18))
     return fail(SIS_ERR_ARG, "sis_solver_create: agents must be in [1, 262144]");
   if (!(p.dt == p.dt)) return fail(SIS_ERR_ARG, "sis_solver_create: dt is NaN");
+  if (p.split < -1) return fail(SIS_ERR_ARG, "sis_solver_create: split must be >= -1");
   if (p.window_threshold < 1) p.window_threshold = p.problem == SIS_MAXCUT ? 25 : 50;
   s->N = J->n;
   s->Z = (int64_t)J->col.size();
@@ -233,6 +297,8 @@ int sis_solver_create(const sis_csr *J, const double *field, const sis_params *p
       (st = s->vald.alloc((size_t)Z)))
     return st;
   HIPCHK(hipMemcpy(s->rp.p, J->rp.data(), sizeof(int64_t) * (N + 1), hipMemcpyHostToDevice));
+  s->rp_host.assign(J->rp.begin(), J->rp.begin() + N + 1);
+  for (int64_t i = 0; i < N; i++) s->max_deg = std::max(s->max_deg, J->rp[i + 1] - J->rp[i]);
   if (Z) {
     HIPCHK(hipMemcpy(s->col.p, J->col.data(), sizeof(int32_t) * Z, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(s->vald.p, J->val.data(), sizeof(double) * Z, hipMemcpyHostToDevice));
@@ -280,6 +346,7 @@ int sis_solver_create(const sis_csr *J, const double *field, const sis_params *p
       (st = s->cutpart.alloc((size_t)(s->cut_blocks * s->lda))) ||
       (st = s->outv.alloc((size_t)s->lda)) || (st = s->best.alloc(1)))
     return st;
+  if (p.split > 0 && (st = build_split(s.get(), p.split))) return st;
   *out = s.release();
   return SIS_OK;
 }
@@ -295,6 +362,7 @@ int sis_solver_params(const sis_solver *s, sis_params *p) {
   if (!s || !p) return fail(SIS_ERR_ARG, "sis_solver_params: null argument");
   *p = s->p;
   p->tile = s->vec;
+  p->split = (int32_t)s->split;
   return SIS_OK;
 }
 
@@ -410,6 +478,11 @@ static void enqueue_step(sis_solver *s, int32_t off) {
   }
   if (s->prob == PROB_UNFUSED)
     launch_xy_update(s->stream, s->f64, s->Xbuf(cur), a.Y, s->N * s->lda, a.sched, a.ctrl, off);
+  a.split = s->split_nent > 0 ? s->split : 0;
+  if (a.split) {
+    SplitArgs sp{s->split_ent.p, s->split_part.p, s->split_cnt.p, s->split};
+    launch_step_split(s->stream, s->p.form, s->prob, s->gather, s->split_nent, a, sp);
+  }
   launch_step(s->stream, s->p.form, s->prob, s->gather, s->vec, a);
   s->par = nxt;
 }
@@ -639,11 +712,8 @@ int sis_solver_autotune(sis_solver *s, int64_t trial_steps, int32_t *tile, doubl
     HIPCHK(hipStreamSynchronize(s->stream));
     return SIS_OK;
   };
-  double best_ms = 1e300;
-  int best = s->vec;
-  for (int v : candidate_tiles(s)) {
-    s->vec = v;
-    if ((st = restore())) return st;
+  auto trial = [&](double *per) -> int {
+    if (int e = restore()) return e;
     for (int64_t k = 0; k < warm; k++) enqueue_step(s, (int32_t)k);
     HIPCHK(hipEventRecord(s->events[0], s->stream));
     for (int64_t k = 0; k < trial_steps; k++) enqueue_step(s, (int32_t)(warm + k));
@@ -652,13 +722,36 @@ int sis_solver_autotune(sis_solver *s, int64_t trial_steps, int32_t *tile, doubl
     HIPCHK(hipStreamSynchronize(s->stream));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, s->events[0], s->events[1]));
-    const double per = (double)ms / (double)trial_steps;
+    *per = (double)ms / (double)trial_steps;
+    return SIS_OK;
+  };
+  double best_ms = 1e300;
+  int best = s->vec;
+  for (int v : candidate_tiles(s)) {
+    s->vec = v;
+    double per;
+    if ((st = trial(&per))) return st;
     if (per < best_ms) {
       best_ms = per;
       best = v;
     }
   }
   s->vec = best;
+  // Row-split threshold (split = -1 lets the tuner choose): powers of two
+  // from 16 below the largest row, against no split, with the chosen tile.
+  if (s->p.split == -1 && s->max_deg > 16) {
+    int64_t best_tau = s->split;
+    for (int64_t tau = 0; tau < s->max_deg; tau = tau ? tau * 2 : 16) {
+      if ((st = build_split(s, tau))) return st;
+      double per;
+      if ((st = trial(&per))) return st;
+      if (per < best_ms) {
+        best_ms = per;
+        best_tau = tau;
+      }
+    }
+    if ((st = build_split(s, best_tau))) return st;
+  }
   s->drop_graphs();
   if ((st = restore())) return st;
   if (tile) *tile = best;

@@ -32,6 +32,7 @@ class Params(C.Structure):
         ("coupling", C.c_double), ("window_period", C.c_int32),
         ("window_threshold", C.c_int32), ("ncities", C.c_int32),
         ("use_graph", C.c_int32), ("tile", C.c_int32), ("fused", C.c_int32),
+        ("split", C.c_int32), ("reserved", C.c_int32),
     ]
 
 

@@ -33,6 +33,33 @@ def test_binding_covers_header():
     assert set(declared_symbols()) == set(sis._SIGS)
 
 
+def test_binding_struct_layouts_match_header(tmp_path):
+    """Every ctypes structure has the size and field offsets the C compiler
+    gives the header's struct."""
+    import ctypes
+    import subprocess
+
+    structs = {"sis_params": sis.Params, "sis_run_info": sis.RunInfo,
+               "sis_batch_params": sis.BatchParams, "sis_batch_status": sis.BatchStatus}
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "sis.h"', "int main(void) {"]
+    for cname, cls in structs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in cls._fields_:
+            lines.append(f'  printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)],
+                   check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, cls in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(cls), cname
+        for f, _ in cls._fields_:
+            assert got[(cname, f)] == getattr(cls, f).offset, (cname, f)
+
+
 @pytest.mark.parametrize("n,seed,euc_round", [(2, 1, 0), (3, 2, 0), (5, 3, 1), (7, 4, 0), (12, 5, 1)])
 def test_tsp_qubo_direct_csr_equals_dense(n, seed, euc_round):
     rng = np.random.default_rng(seed)
