@@ -144,7 +144,7 @@ def run_solver(J, params, T, uw, seed, field=None, autotune=True, tsp=None):
     t0 = time.perf_counter()
     info = s.run(T, uw)
     wall = time.perf_counter() - t0
-    out = dict(info=info, wall=wall, tile=tune,
+    out = dict(info=info, wall=wall, tile=tune, split=s.p.split,
                agent_steps_per_s=info["steps_executed"] * s.A / wall)
     if tsp is None:
         cuts, best = s.cuts()
@@ -189,14 +189,36 @@ def small_configs(a):
     elif a.workload == "c4":
         t = sis.Tsp.generate(48, 1000, 1, a.seed)
         J, h = t.qubo()
-        for form in ("f32", "bf16"):
+        for form, split in (("f32", 0), ("bf16", 0), ("f32", -1), ("bf16", -1)):
             r = run_solver(J, dict(problem=sis.TSP, agents=1024, dt=0.1, form=sis.FORMS[form],
-                                   dynamics=sis.BSB, ncities=48, use_graph=1),
+                                   dynamics=sis.BSB, ncities=48, use_graph=1, split=split),
                            a.steps if a.steps != 20000 else 10000, 0, a.seed, field=h, tsp=t)
             lines.append(dict(workload="c4 TSP n=48 (2304 spins) bSB, 1024 agents, 10000 steps",
-                              form=form, nnz=J.nnz, **{k: v for k, v in r.items() if k != "info"},
+                              form=form, split_requested=split, nnz=J.nnz,
+                              **{k: v for k, v in r.items() if k != "info"},
                               ms_per_step=r["wall"] * 1e3 / r["info"]["steps_executed"],
                               step_kernel_ms=r["info"]["step_kernel_ms"]))
+    elif a.workload == "powerlaw":
+        # Chung-Lu N = 2^20, mean degree 8, exponent 2.1 (largest row ~1.2e5
+        # entries), +-1 weights, 256 agents, dSB dt 0.5; without and with the
+        # row split (threshold chosen by the autotuner).  dSB with integer
+        # weights is bit-identical either way, so the cuts must agree.
+        J = sis.Csr.gen_chung_lu(1 << 20, 8.0, 2.1, sis.W_PM1, a.seed)
+        rp, _, _ = J.arrays()
+        T = a.steps if a.steps != 20000 else 2000
+        for form in ("packed", "f32"):
+            for split in (0, -1):
+                r = run_solver(J, dict(problem=sis.MAXCUT, agents=256, dt=0.5, form=sis.FORMS[form],
+                                       split=split), T, 0, a.seed)
+                ms = r["info"]["step_kernel_ms"] / max(r["info"]["steps_executed"], 1)
+                nbytes = step_bytes(form, J.n, J.nnz, 256)
+                lines.append(dict(workload=f"powerlaw Chung-Lu N=2^20 avg 8 gamma 2.1, +-1, dSB, 256 "
+                                           f"agents, {T} steps", form=form, nnz=J.nnz,
+                                  max_row=int(np.diff(rp).max()), split_requested=split,
+                                  **{k: v for k, v in r.items() if k != "info"},
+                                  step_kernel_ms_per_step=ms,
+                                  achieved_GBs=nbytes / (ms * 1e-3) / 1e9,
+                                  ms_per_step=r["wall"] * 1e3 / max(r["info"]["steps_executed"], 1)))
     for ln in lines:
         print(json.dumps(ln), flush=True)
 
@@ -312,7 +334,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4", "serving"])
+    ap.add_argument("--workload", default="c3",
+                    choices=["c1", "c2", "c3", "c4", "powerlaw", "serving"])
     ap.add_argument("--instances", type=int, default=4096)
     ap.add_argument("--chunk", type=int, default=100)
     ap.add_argument("--rate", type=float, default=0.0)
@@ -330,7 +353,7 @@ def main():
             a.steps = 5000
         serving(a)
         return
-    if a.workload in ("c1", "c2", "c4"):
+    if a.workload in ("c1", "c2", "c4", "powerlaw"):
         small_configs(a)
         return
 

@@ -161,6 +161,12 @@ int sis_gen_torus(int64_t rows, int64_t cols, int weights, uint64_t seed, sis_cs
 /* Random d-regular simple graph: configuration-model pairing with random
  * edge swaps that repair loops and multi-edges. */
 int sis_gen_regular(int64_t n, int d, int weights, uint64_t seed, sis_csr **out);
+/* Chung-Lu power-law graph: node of rank r (ranks given in a random order)
+ * has expected degree proportional to (r + 1)^(-1 / (exponent - 1)), mean
+ * about avg_degree; n avg_degree / 2 endpoint pairs are drawn, self-loops and
+ * repeated pairs dropped.  Hub rows of such graphs exercise the row split. */
+int sis_gen_chung_lu(int64_t n, double avg_degree, double exponent, int weights, uint64_t seed,
+                     sis_csr **out);
 /* n cities with integer coordinates uniform in [0, span]^2, EUC_2D. */
 int sis_gen_tsp(int64_t n, int64_t span, int euc_round, uint64_t seed, sis_tsp **out);
 

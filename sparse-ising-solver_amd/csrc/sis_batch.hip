@@ -161,12 +161,6 @@ __global__ __launch_bounds__(kBlock) void k_batch_step(BArgs b) {
   }
 }
 
-__device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
-  return ((uint64_t)hi << 32) | lo;
-}
-
 // Same step as k_batch_step with the gathers on the vector memory path:
 // lane l serves slot l of the task (row l / (64/R), neighbour position
 // l % (64/R) of the pass) and loads that neighbour's column, weight and sign

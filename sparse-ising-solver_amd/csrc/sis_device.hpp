@@ -50,6 +50,13 @@ __device__ __forceinline__ uint64_t wave_or64(uint64_t v) {
   return ((uint64_t)hi << 32) | lo;
 }
 
+// Lane l's 64-bit value, broadcast (l wave-uniform).
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
+  return ((uint64_t)hi << 32) | lo;
+}
+
 template <typename T, int V>
 struct alignas(sizeof(T) * V) Vec {
   T v[V];

@@ -693,6 +693,46 @@ int sis_gen_er(int64_t n, double p, int weights, uint64_t seed, sis_csr **out) {
   return edges_to_csr(n, u, v, w, out);
 }
 
+int sis_gen_chung_lu(int64_t n, double avg_degree, double exponent, int weights, uint64_t seed,
+                     sis_csr **out) {
+  if (!out || n < 2 || n > INT32_MAX || !(avg_degree > 0.0) || !(exponent > 1.0) ||
+      avg_degree * (double)n > 4e9)
+    return fail(SIS_ERR_ARG, "sis_gen_chung_lu: bad argument");
+  Rng rng(seed);
+  // Expected-degree weights (r + 1)^(-1 / (exponent - 1)) by rank r, given to
+  // the nodes in a random order; endpoints are drawn from their cumulative sum.
+  std::vector<int32_t> perm((size_t)n);
+  for (int64_t i = 0; i < n; i++) perm[(size_t)i] = (int32_t)i;
+  for (int64_t i = n - 1; i > 0; i--) std::swap(perm[(size_t)i], perm[(size_t)rng.below((uint64_t)i + 1)]);
+  std::vector<double> cdf((size_t)n);
+  double acc = 0.0;
+  const double e = -1.0 / (exponent - 1.0);
+  for (int64_t r = 0; r < n; r++) cdf[(size_t)r] = (acc += std::pow((double)(r + 1), e));
+  auto draw = [&]() -> int32_t {
+    const double x = rng.uniform() * acc;
+    const int64_t r = std::upper_bound(cdf.begin(), cdf.end(), x) - cdf.begin();
+    return perm[(size_t)std::min<int64_t>(r, n - 1)];
+  };
+  const int64_t m = (int64_t)std::llround(avg_degree * (double)n / 2.0);
+  std::vector<uint64_t> pairs;
+  pairs.reserve((size_t)m);
+  for (int64_t k = 0; k < m; k++) {
+    const uint32_t a = (uint32_t)draw(), b = (uint32_t)draw();
+    if (a == b) continue;  // no self-loops
+    pairs.push_back(a > b ? ((uint64_t)a << 32) | b : ((uint64_t)b << 32) | a);
+  }
+  std::sort(pairs.begin(), pairs.end());
+  pairs.erase(std::unique(pairs.begin(), pairs.end()), pairs.end());  // simple graph
+  std::vector<int32_t> u(pairs.size()), v(pairs.size());
+  std::vector<double> w(pairs.size());
+  for (size_t k = 0; k < pairs.size(); k++) {
+    u[k] = (int32_t)(pairs[k] >> 32);
+    v[k] = (int32_t)(pairs[k] & 0xffffffffu);
+    w[k] = draw_weight(rng, weights);
+  }
+  return edges_to_csr(n, u, v, w, out);
+}
+
 int sis_gen_torus(int64_t rows, int64_t cols, int weights, uint64_t seed, sis_csr **out) {
   if (!out || rows < 3 || cols < 3 || rows * cols > INT32_MAX)
     return fail(SIS_ERR_ARG, "sis_gen_torus: need rows, cols >= 3");

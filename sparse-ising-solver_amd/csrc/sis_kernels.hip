@@ -351,6 +351,26 @@ __device__ __forceinline__ typename SplitT<FORM>::real split_chunk(const StepArg
   const int32_t *__restrict__ col = a.col;
   const T c = (T)a.coupling;
   T F = (T)0;
+  if constexpr (FORM == 2) {
+    // 64 entries per pass on the vector memory path: lane l loads entry
+    // kk + l's column, weight and sign word; the words are then taken in
+    // CSR order with readlane.
+    const uint64_t *__restrict__ S = (const uint64_t *)a.gsrc;
+    for (int64_t kk = kb; kk < ke; kk += 64) {
+      const int64_t k = kk + lane;
+      const bool ok = k < ke;
+      const int64_t j = ok ? (int64_t)col[k] : 0;
+      const float v = ok ? val[k] : 0.0f;
+      const uint64_t sw = ok ? S[j * a.W + tile] : 0;
+      const int cnt = ke - kk < 64 ? (int)(ke - kk) : 64;
+      for (int e = 0; e < cnt; e++) {
+        const uint64_t we = readlane64(sw, e);
+        const float ve = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), e));
+        F = ((we >> lane) & 1) ? F + ve : F - ve;
+      }
+    }
+    return F;
+  }
   int64_t k = kb;
   for (; k + 4 <= ke; k += 4) {
     const T v0 = val[k], v1 = val[k + 1], v2 = val[k + 2], v3 = val[k + 3];
@@ -562,9 +582,14 @@ __global__ void k_ctrl_advance(Ctrl *ctrl, int64_t P) {
 
 // ---------------------------------------------------------- evaluation
 
-// Row terms of H = sum_i 0.5 s_i sum_k J_k s_col (src/kernels.cu:371-379) or
-// of E = sum_i s_i sum_k J_k s_col - h_i s_i, summed per agent over a block's
-// rows in a fixed order; part[block][agent].
+// Per-block partial sums of H = sum_i 0.5 s_i sum_k J_k s_col
+// (src/kernels.cu:371-379), written as H = 0.5 sum_k J_k s_row(k) s_col(k)
+// over the stored entries, or of E = sum_k J_k s_row s_col - sum_i h_i s_i.
+// Every wavefront takes an equal slice of the entries (so one long row does
+// not hold up the sum) and finds the row of its first entry by binary search
+// on the row pointers; the field term is taken over the block's row range.
+// Partials of the kWaves wavefronts are added in a fixed order:
+// part[block][agent].  Integer weights give exact sums in any order.
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_cut_rows(const int64_t *__restrict__ rp,
                                                      const int32_t *__restrict__ col,
@@ -574,21 +599,50 @@ __global__ __launch_bounds__(kBlock) void k_cut_rows(const int64_t *__restrict__
                                                      int64_t N, int64_t W, int64_t lda,
                                                      int64_t rpb, double *part) {
   __shared__ double s_red[kWaves][64];
-  const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int q = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int64_t Z = rp[N];
+  const int64_t nchunk = (int64_t)gridDim.x * kWaves;
+  const int64_t per = (Z + nchunk - 1) / nchunk;
+  const int64_t c = (int64_t)blockIdx.x * kWaves + q;
+  const int64_t ka = c * per < Z ? c * per : Z;
+  const int64_t kb = ka + per < Z ? ka + per : Z;
+  // last row i < N with rp[i] <= ka (rp[N] = Z > ka when the slice is not empty)
+  int64_t i0 = 0;
+  if (ka < kb) {
+    int64_t lo = 0, hi = N;
+    while (hi - lo > 1) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (rp[mid] <= ka) lo = mid;
+      else hi = mid;
+    }
+    i0 = lo;
+  }
   const int64_t r0 = (int64_t)blockIdx.x * rpb;
   const int64_t r1 = r0 + rpb < N ? r0 + rpb : N;
   for (int64_t w = 0; w < W; w++) {
     double acc = 0.0;
-    for (int64_t i = r0 + q; i < r1; i += kWaves) {
-      const double si = ((spins[i * W + w] >> lane) & 1) ? -1.0 : 1.0;
-      double tmp = 0.0;
-      for (int64_t k = rp[i]; k < rp[i + 1]; k++) {
+    if (ka < kb) {
+      int64_t i = i0, iend = rp[i0 + 1];
+      double si = ((spins[i * W + w] >> lane) & 1) ? -1.0 : 1.0;
+      for (int64_t k = ka; k < kb; k++) {
+        if (k >= iend) {
+          do {
+            i++;
+            iend = rp[i + 1];
+          } while (k >= iend);
+          si = ((spins[i * W + w] >> lane) & 1) ? -1.0 : 1.0;
+        }
         const double sj = ((spins[(int64_t)col[k] * W + w] >> lane) & 1) ? -1.0 : 1.0;
-        tmp = __builtin_fma(val[k], sj, tmp);
+        acc = __builtin_fma(val[k], si * sj, acc);
       }
-      if (MODE == 0) acc += tmp * si * 0.5;
-      else acc += tmp * si - field[i] * si;
+      if (MODE == 0) acc *= 0.5;
     }
+    if (MODE == 1)
+      for (int64_t r = r0 + q; r < r1; r += kWaves) {
+        const double sr = ((spins[r * W + w] >> lane) & 1) ? -1.0 : 1.0;
+        acc = __builtin_fma(-field[r], sr, acc);
+      }
     s_red[q][lane] = acc;
     __syncthreads();
     if (q == 0)

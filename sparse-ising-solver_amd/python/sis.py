@@ -98,6 +98,8 @@ _SIGS = {
     "sis_gen_er": (None, [C.c_int64, C.c_double, C.c_int, C.c_uint64, C.POINTER(_P)]),
     "sis_gen_torus": (None, [C.c_int64, C.c_int64, C.c_int, C.c_uint64, C.POINTER(_P)]),
     "sis_gen_regular": (None, [C.c_int64, C.c_int, C.c_int, C.c_uint64, C.POINTER(_P)]),
+    "sis_gen_chung_lu": (None, [C.c_int64, C.c_double, C.c_double, C.c_int, C.c_uint64,
+                                C.POINTER(_P)]),
     "sis_gen_tsp": (None, [C.c_int64, C.c_int64, C.c_int, C.c_uint64, C.POINTER(_P)]),
     "sis_params_default": (None, [C.c_int, C.POINTER(Params)]),
     "sis_solver_create": (None, [_P, _DP, C.POINTER(Params), C.POINTER(_P)]),
@@ -202,6 +204,13 @@ class Csr:
     def gen_regular(cls, n, d, weights, seed):
         h = C.c_void_p()
         _ck(lib().sis_gen_regular(n, d, weights, seed, C.byref(h)), "sis_gen_regular")
+        return cls(h)
+
+    @classmethod
+    def gen_chung_lu(cls, n, avg_degree, exponent, weights, seed):
+        h = C.c_void_p()
+        _ck(lib().sis_gen_chung_lu(n, avg_degree, exponent, weights, seed, C.byref(h)),
+            "sis_gen_chung_lu")
         return cls(h)
 
     def arrays(self):

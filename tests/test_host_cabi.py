@@ -142,6 +142,29 @@ def test_generators_shapes_and_determinism():
     assert set(val) <= {-3.0, -2.0, -1.0, 1.0, 2.0, 3.0}
 
 
+def test_chung_lu_power_law():
+    n, avg, gamma = 1 << 16, 8.0, 2.1
+    g = sis.Csr.gen_chung_lu(n, avg, gamma, sis.W_PM1, 3)
+    rp, col, val = g.arrays()
+    deg = np.diff(rp)
+    M = sp.csr_matrix((np.ones_like(val), col, rp), shape=(n, n))
+    assert M.max() == 1 and M.diagonal().sum() == 0 and (M != M.T).nnz == 0  # simple, symmetric
+    assert set(val) <= {-1.0, 1.0}
+    # drawn pairs: n avg / 2, minus self-loops and repeats (many at the hubs)
+    assert 0.6 * n * avg < g.nnz <= n * avg
+    # expected degree of rank r is proportional to (r + 1)^(-1/(gamma - 1)):
+    # the top rank expects avg n w_0 / sum_r w_r draws; repeated pairs with
+    # the other hubs collapse, leaving about 40% of that as distinct neighbours
+    w = np.arange(1, n + 1, dtype=np.float64) ** (-1 / (gamma - 1))
+    top = avg * n * w[0] / w.sum()
+    d = np.sort(deg)[::-1]
+    assert 0.3 * top < d[0] < 1.2 * top
+    assert d[100] < d[10] < d[0]
+    assert np.median(deg) < avg
+    g2 = sis.Csr.gen_chung_lu(n, avg, gamma, sis.W_PM1, 3)
+    assert np.array_equal(g2.arrays()[1], col)
+
+
 def test_tour_length_and_random_tour():
     t = sis.Tsp.from_coords([0, 3, 3, 0], [0, 0, 4, 4])
     assert t.tour_length([0, 1, 2, 3]) == 14.0
