@@ -308,7 +308,9 @@ typedef struct sis_batch_params {
   int64_t max_nnz;       /* capacity: sum of stored nonzeros */
   int32_t chunk_steps;   /* steps per graph replay between admissions (>= 1) */
   int32_t tile;          /* rows per task (1, 2, 4, 8) | non-temporal << 16 |
-                            vector gathers << 17 (2, 4 or 8 rows); 0 = default */
+                            vector gathers << 17 (2, 4 or 8 rows) | XCD-affine
+                            task order << 18 (each instance's tasks on one XCD,
+                            so its sign words stay in that XCD's L2); 0 = default */
   int32_t eager;         /* 1: launch each chunk's kernels directly instead of
                             replaying the captured hipGraph */
 } sis_batch_params;

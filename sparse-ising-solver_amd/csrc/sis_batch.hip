@@ -44,7 +44,8 @@ struct BInst {
   int32_t P, K;      // window period and threshold
   int32_t uw, status;
   double dt, xi0, S; // time step, coupling scale, sum of file weights
-  int32_t nbif, pad;
+  int32_t nbif;
+  int32_t intw;      // every weight an integer in [-127, 127] (bit-transpose path)
 };
 
 struct BTask {
@@ -52,9 +53,12 @@ struct BTask {
   int64_t row0;
 };
 
+constexpr int kXcds = 8;  // MI355X: workgroups go to the 8 XCDs round-robin
+
 struct BArgs {
   const BTask *tasks;
-  const int32_t *ntasks;
+  const int32_t *ntasks;  // [0] tasks; [1 .. kXcds + 1] per-XCD task offsets
+  int32_t xcd;            // 1: wavefronts of XCD x take tasks [off[x], off[x + 1])
   BInst *inst;
   const int64_t *rbeg, *rend;  // written only by k_batch_admit, through BCsr
   const int32_t *col;
@@ -67,14 +71,39 @@ struct BArgs {
   int64_t lda, W;
 };
 
+// Task range of the calling wavefront: the whole table with a grid stride,
+// or (XCD-affine order) the slice of tasks placed on the wavefront's XCD, so
+// an instance's sign words stay in that XCD's L2.  Any placement is correct;
+// only the cache reuse depends on it.
+struct TaskRange {
+  int first, end, stride;
+};
+
+__device__ __forceinline__ TaskRange task_range(const BArgs &b) {
+  TaskRange r;
+  if (b.xcd) {
+    const int x = blockIdx.x % kXcds;
+    const int nb = gridDim.x / kXcds;
+    r.first = b.ntasks[1 + x] + (int)(blockIdx.x / kXcds) * kWaves + (int)(threadIdx.x >> 6);
+    r.end = b.ntasks[2 + x];
+    r.stride = nb * kWaves;
+  } else {
+    r.first = wave_index();
+    r.end = *b.ntasks;
+    r.stride = gridDim.x * kWaves;
+  }
+  r.first = __builtin_amdgcn_readfirstlane(r.first);
+  r.end = __builtin_amdgcn_readfirstlane(r.end);
+  return r;
+}
+
 // One step of every active instance.  Wavefront task = R rows x W words.
 template <int R, int WT, int NT>
 __global__ __launch_bounds__(kBlock) void k_batch_step(BArgs b) {
   const int lane = threadIdx.x & 63;
-  const int nwaves = gridDim.x * kWaves;
-  const int ntasks = *b.ntasks;
+  const TaskRange tr = task_range(b);
   const int64_t lda = b.lda, W = WT;
-  for (int task = wave_index(); task < ntasks; task += nwaves) {
+  for (int task = tr.first; task < tr.end; task += tr.stride) {
     const BTask tk = b.tasks[task];
     const BInst &in = b.inst[tk.slot];
     if (in.status != SLOT_ACTIVE) continue;
@@ -170,11 +199,10 @@ template <int R, int WT, int NT>
 __global__ __launch_bounds__(kBlock) void k_batch_step_vg(BArgs b) {
   constexpr int SL = 64 / R;  // neighbour slots per row and pass
   const int lane = threadIdx.x & 63;
-  const int nwaves = gridDim.x * kWaves;
-  const int ntasks = *b.ntasks;
+  const TaskRange tr = task_range(b);
   const int64_t lda = b.lda, W = WT;
   const int my_r = lane / SL, my_i = lane % SL;
-  for (int task = wave_index(); task < ntasks; task += nwaves) {
+  for (int task = tr.first; task < tr.end; task += tr.stride) {
     const BTask tk = b.tasks[task];
     const BInst &in = b.inst[tk.slot];
     if (in.status != SLOT_ACTIVE) continue;
@@ -229,6 +257,178 @@ __global__ __launch_bounds__(kBlock) void k_batch_step_vg(BArgs b) {
           for (int u = 0; u < WT; u++) {
             const uint64_t ww = readlane64(w[u], l);
             F[r][u] = ((ww >> lane) & 1) ? F[r][u] + vv : F[r][u] - vv;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      if (r >= tk.nrows) break;
+      const int64_t row = tk.row0 + r;
+      uint64_t mine = 0;
+#pragma unroll
+      for (int u = 0; u < WT; u++) {
+        sb_update<float, PROB_MAXCUT>(x[r][u], y[r][u], F[r][u], c0, c1, c2, 0.0f, 0.0f);
+        st_state<NT>(b.X + row * lda + 64 * u + lane, x[r][u]);
+        st_state<NT>(b.Y + row * lda + 64 * u + lane, y[r][u]);
+        const uint64_t bits = __ballot(signbit(x[r][u]));
+        if (lane == u) mine = bits;
+      }
+      if (lane < WT) Sn[row * W + lane] = mine;
+    }
+  }
+}
+
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
+  const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, m, 64);
+  const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), m, 64);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// 64 x 64 bit-matrix transpose across the wavefront: lane l holds word w_l
+// (bit a = agent a's sign at neighbour slot l); afterwards lane a holds the
+// word whose bit l is bit a of w_l.  Six butterfly stages swap the
+// off-diagonal blocks of side 32, 16, ..., 1 between lanes l and l ^ s.
+__device__ __forceinline__ uint64_t bit_transpose64(uint64_t w, int lane) {
+  constexpr uint64_t kMask[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull,
+                                 0x00FF00FF00FF00FFull, 0x0F0F0F0F0F0F0F0Full,
+                                 0x3333333333333333ull, 0x5555555555555555ull};
+#pragma unroll
+  for (int st = 0; st < 6; st++) {
+    const int s = 32 >> st;
+    const uint64_t m = kMask[st];
+    const uint64_t p = shfl_xor64(w, s);
+    w = (lane & s) ? (((p >> s) & m) | (w & ~m)) : ((w & m) | ((p & m) << s));
+  }
+  return w;
+}
+
+// Same step as k_batch_step_vg.  For instances whose weights are all small
+// integers the neighbour sums are taken on bit masks instead of one readlane
+// per entry: after the transpose lane a holds, for each of the pass's 64
+// slots, bit l = 1 when agent a's neighbour at slot l is negative; with the
+// slots' weights split into bit planes p and signs (wave-uniform ballot
+// masks P_p, N_p),
+//   sum_l v_l bit_l = sum_p 2^p (popc(m & P_p) - popc(m & N_p))
+// restricted to a row's slots, and F = 2 sum_l v_l bit_l - sum_l v_l, which
+// is the sequential sum of +-v exactly (every partial sum is an integer
+// below 2^24), so results are bit-identical to the other step kernels.
+// Instances with other weights take the readlane path of k_batch_step_vg.
+template <int R, int WT, int NT>
+__global__ __launch_bounds__(kBlock) void k_batch_step_bt(BArgs b) {
+  constexpr int SL = 64 / R;  // neighbour slots per row and pass
+  constexpr int kPlanes = 7;  // |v| <= 127
+  const int lane = threadIdx.x & 63;
+  const TaskRange tr = task_range(b);
+  const int64_t lda = b.lda, W = WT;
+  const int my_r = lane / SL, my_i = lane % SL;
+  for (int task = tr.first; task < tr.end; task += tr.stride) {
+    const BTask tk = b.tasks[task];
+    const BInst &in = b.inst[tk.slot];
+    if (in.status != SLOT_ACTIVE) continue;
+    const int32_t t = in.t;
+    double p = 1.025 * (double)t / (double)in.T;
+    p = p < 1.0 ? p : 1.0;
+    const float c0 = (float)(in.dt * (p + 1)), c1 = (float)(in.dt * (p - 1));
+    const float c2 = (float)(in.dt * in.xi0);
+    const uint64_t *S = (t & 1) ? b.S1 : b.S0;
+    uint64_t *Sn = (t & 1) ? b.S0 : b.S1;
+    float x[R][WT], y[R][WT];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      if (r < tk.nrows) {
+#pragma unroll
+        for (int u = 0; u < WT; u++) {
+          x[r][u] = ld_state<NT>(b.X + (tk.row0 + r) * lda + 64 * u + lane);
+          y[r][u] = ld_state<NT>(b.Y + (tk.row0 + r) * lda + 64 * u + lane);
+        }
+      }
+    }
+    const int64_t myrow = tk.row0 + (my_r < tk.nrows ? my_r : 0);
+    const int64_t mykb = b.rbeg[myrow];
+    const int mydeg = my_r < tk.nrows ? (int)(b.rend[myrow] - mykb) : 0;
+    int maxdeg = mydeg;
+    for (int m = 32; m >= 1; m >>= 1) {
+      const int o = __shfl_xor(maxdeg, m, 64);
+      maxdeg = o > maxdeg ? o : maxdeg;
+    }
+    maxdeg = __builtin_amdgcn_readfirstlane(maxdeg);
+    float F[R][WT];
+    if (in.intw) {
+      int Sa[R][WT], Vr[R];
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        Vr[r] = 0;
+#pragma unroll
+        for (int u = 0; u < WT; u++) Sa[r][u] = 0;
+      }
+      for (int d0 = 0; d0 < maxdeg; d0 += SL) {
+        const bool ok = d0 + my_i < mydeg;
+        const int64_t k = ok ? mykb + d0 + my_i : 0;
+        const int64_t j = ok ? (int64_t)b.col[k] : tk.row0;
+        const int vi = ok ? (int)b.val[k] : 0;
+        const int av = vi < 0 ? -vi : vi;
+        uint64_t w[WT];
+#pragma unroll
+        for (int u = 0; u < WT; u++) w[u] = S[j * W + u];
+        // bit planes of the slots' weights: P_p (positive), N_p (negative)
+        uint64_t Pm[kPlanes], Nm[kPlanes];
+        int np = 0;
+#pragma unroll
+        for (int q = 0; q < kPlanes; q++) {
+          Pm[q] = __ballot(vi > 0 && ((av >> q) & 1));
+          Nm[q] = __ballot(vi < 0 && ((av >> q) & 1));
+          if (Pm[q] | Nm[q]) np = q + 1;
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const uint64_t rm = (SL == 64 ? ~0ull : ((1ull << SL) - 1)) << (r * SL);
+#pragma unroll
+          for (int q = 0; q < kPlanes; q++)
+            if (q < np) Vr[r] += (__popcll(Pm[q] & rm) - __popcll(Nm[q] & rm)) * (1 << q);
+        }
+#pragma unroll
+        for (int u = 0; u < WT; u++) {
+          const uint64_t m = bit_transpose64(w[u], lane);
+#pragma unroll
+          for (int r = 0; r < R; r++) {
+            const uint64_t rm = (SL == 64 ? ~0ull : ((1ull << SL) - 1)) << (r * SL);
+            int acc = 0;
+#pragma unroll
+            for (int q = 0; q < kPlanes; q++)
+              if (q < np) acc += (__popcll(m & Pm[q] & rm) - __popcll(m & Nm[q] & rm)) * (1 << q);
+            Sa[r][u] += acc;
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int u = 0; u < WT; u++) F[r][u] = (float)(2 * Sa[r][u] - Vr[r]);
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int u = 0; u < WT; u++) F[r][u] = 0.0f;
+      for (int d0 = 0; d0 < maxdeg; d0 += SL) {
+        const bool ok = d0 + my_i < mydeg;
+        const int64_t k = ok ? mykb + d0 + my_i : 0;
+        const int64_t j = ok ? (int64_t)b.col[k] : tk.row0;
+        const float v = ok ? b.val[k] : 0.0f;
+        uint64_t w[WT];
+#pragma unroll
+        for (int u = 0; u < WT; u++) w[u] = S[j * W + u];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+#pragma unroll
+          for (int i = 0; i < SL; i++) {
+            const int l = r * SL + i;
+            const float vv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+#pragma unroll
+            for (int u = 0; u < WT; u++) {
+              const uint64_t ww = readlane64(w[u], l);
+              F[r][u] = ((ww >> lane) & 1) ? F[r][u] + vv : F[r][u] - vv;
+            }
           }
         }
       }
@@ -561,7 +761,7 @@ struct sis_batch {
   std::unordered_map<int64_t, Result> results;
   int64_t next_ticket = 1;
   int32_t tile = 1;
-  int32_t grid = 1;
+  int32_t cus = 1;  // compute units of the device
   int32_t ntask_cap = 0;
   bool tasks_dirty = true;
   hipGraphExec_t graph = nullptr;
@@ -577,6 +777,7 @@ struct sis_batch {
     BArgs a{};
     a.tasks = tasks.p;
     a.ntasks = ntasks.p;
+    a.xcd = (tile >> 18) & 1;
     a.inst = inst.p;
     a.rbeg = rbeg.p;
     a.rend = rend.p;
@@ -601,31 +802,73 @@ struct sis_batch {
 namespace {
 
 // Tile: rows per task (1, 2, 4, 8) | non-temporal << 16 | vector gathers << 17
-// (vector gathers need 2, 4 or 8 rows per task).
+// (vector gathers need 2, 4 or 8 rows per task) | XCD-affine task order << 18 |
+// bit-transpose integer sums << 19 (2, 4 or 8 rows, no vector-gather bit).
 bool batch_tile_ok(int W, int tile) {
   const int r = tile & 0xffff, nt = (tile >> 16) & 1, vg = (tile >> 17) & 1;
-  if ((tile >> 18) != 0 || (nt != 0 && nt != 1)) return false;
+  const int bt = (tile >> 19) & 1;
+  if ((tile >> 20) != 0 || (nt != 0 && nt != 1)) return false;
   if (!(W == 1 || W == 2 || W == 4)) return false;
+  if (bt) return !vg && (r == 2 || r == 4 || r == 8);
   if (vg) return r == 2 || r == 4 || r == 8;
   return r == 1 || r == 2 || r == 4 || r == 8;
 }
 
-template <int R, int WT, int NT>
-void launch_bstep(const sis_batch *b, const BArgs &a) {
-  hipLaunchKernelGGL((k_batch_step<R, WT, NT>), dim3(b->grid), dim3(kBlock), 0, b->stream, a);
+// Persistent grid of a step kernel: as many workgroups as can be resident at
+// once (occupancy of that kernel), a multiple of the XCD count.  A larger
+// grid would run its last workgroups in a second round with a full share of
+// the task table each.
+template <typename K>
+unsigned persistent_grid(const sis_batch *b, K kern) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, 0) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  const int64_t g = (int64_t)per_cu * b->cus;
+  return (unsigned)std::max<int64_t>(kXcds, g / kXcds * kXcds);
 }
 
 template <int R, int WT, int NT>
-void launch_bstep_vg(const sis_batch *b, const BArgs &a) {
-  hipLaunchKernelGGL((k_batch_step_vg<R, WT, NT>), dim3(b->grid), dim3(kBlock), 0, b->stream, a);
+void launch_bstep(const sis_batch *b, const BArgs &a, bool launch) {
+  static unsigned grid = 0;  // per kernel; the engine runs on one kind of device
+  if (!grid) grid = persistent_grid(b, k_batch_step<R, WT, NT>);
+  if (launch) hipLaunchKernelGGL((k_batch_step<R, WT, NT>), dim3(grid), dim3(kBlock), 0, b->stream, a);
 }
 
-void launch_batch_step(sis_batch *b) {
+template <int R, int WT, int NT>
+void launch_bstep_vg(const sis_batch *b, const BArgs &a, bool launch) {
+  static unsigned grid = 0;  // per kernel; the engine runs on one kind of device
+  if (!grid) grid = persistent_grid(b, k_batch_step_vg<R, WT, NT>);
+  if (launch) hipLaunchKernelGGL((k_batch_step_vg<R, WT, NT>), dim3(grid), dim3(kBlock), 0, b->stream, a);
+}
+
+template <int R, int WT, int NT>
+void launch_bstep_bt(const sis_batch *b, const BArgs &a, bool launch) {
+  static unsigned grid = 0;  // per kernel; the engine runs on one kind of device
+  if (!grid) grid = persistent_grid(b, k_batch_step_bt<R, WT, NT>);
+  if (launch) hipLaunchKernelGGL((k_batch_step_bt<R, WT, NT>), dim3(grid), dim3(kBlock), 0, b->stream, a);
+}
+
+// launch = false only resolves the kernel's grid (done before graph capture,
+// where the occupancy query should not run).
+void launch_batch_step(sis_batch *b, bool launch = true) {
   const BArgs a = b->args();
   const int r = b->tile & 0xffff, nt = (b->tile >> 16) & 1, vg = (b->tile >> 17) & 1;
+  const int bt = (b->tile >> 19) & 1;
   const int w = (int)b->W;
+#define SIS_BSTEPB(R, WT, NT) \
+  if (bt && r == R && w == WT && nt == NT) return launch_bstep_bt<R, WT, NT>(b, a, launch);
+#define SIS_BSTEPB_W(R, NT) SIS_BSTEPB(R, 1, NT) SIS_BSTEPB(R, 2, NT) SIS_BSTEPB(R, 4, NT)
+  SIS_BSTEPB_W(2, 0)
+  SIS_BSTEPB_W(4, 0)
+  SIS_BSTEPB_W(8, 0)
+  SIS_BSTEPB_W(2, 1)
+  SIS_BSTEPB_W(4, 1)
+  SIS_BSTEPB_W(8, 1)
+#undef SIS_BSTEPB_W
+#undef SIS_BSTEPB
 #define SIS_BSTEP(R, WT, NT) \
-  if (!vg && r == R && w == WT && nt == NT) return launch_bstep<R, WT, NT>(b, a);
+  if (!vg && r == R && w == WT && nt == NT) return launch_bstep<R, WT, NT>(b, a, launch);
 #define SIS_BSTEP_W(R, NT) SIS_BSTEP(R, 1, NT) SIS_BSTEP(R, 2, NT) SIS_BSTEP(R, 4, NT)
   SIS_BSTEP_W(1, 0)
   SIS_BSTEP_W(2, 0)
@@ -638,7 +881,7 @@ void launch_batch_step(sis_batch *b) {
 #undef SIS_BSTEP_W
 #undef SIS_BSTEP
 #define SIS_BSTEPV(R, WT, NT) \
-  if (vg && r == R && w == WT && nt == NT) return launch_bstep_vg<R, WT, NT>(b, a);
+  if (vg && r == R && w == WT && nt == NT) return launch_bstep_vg<R, WT, NT>(b, a, launch);
 #define SIS_BSTEPV_W(R, NT) SIS_BSTEPV(R, 1, NT) SIS_BSTEPV(R, 2, NT) SIS_BSTEPV(R, 4, NT)
   SIS_BSTEPV_W(2, 0)
   SIS_BSTEPV_W(4, 0)
@@ -655,21 +898,54 @@ void launch_batch_post(sis_batch *b) {
                      b->p.max_instances);
 }
 
+// Task table.  Plain order: instance after instance.  XCD-affine order:
+// instances are placed whole on the least-loaded XCD (largest first, load =
+// state bytes + gathers); one too large for a fair share (over a sixteenth of
+// the total) has its tasks dealt round-robin to all XCDs instead.
 int rebuild_tasks(sis_batch *b) {
   if (!b->tasks_dirty) return SIS_OK;
   const int R = b->tile & 0xffff;
-  std::vector<BTask> t;
-  for (auto &kv : b->live) {
-    const Live &l = kv.second;
-    for (int64_t r = 0; r < l.req.n; r += R)
-      t.push_back(BTask{l.slot, (int32_t)std::min<int64_t>(R, l.req.n - r), l.row_off + r});
+  const bool xcd = (b->tile >> 18) & 1;
+  std::vector<std::vector<BTask>> per(xcd ? kXcds : 1);
+  std::vector<const Live *> order;
+  for (auto &kv : b->live) order.push_back(&kv.second);
+  auto load = [&](const Live *l) {
+    return (double)l->req.n * (double)b->lda * 16.0 + (double)l->req.col.size() * 8.0;
+  };
+  double total = 0.0;
+  for (const Live *l : order) total += load(l);
+  if (xcd)
+    std::stable_sort(order.begin(), order.end(),
+                     [&](const Live *x, const Live *y) { return load(x) > load(y); });
+  std::vector<double> xl(per.size(), 0.0);
+  for (const Live *l : order) {
+    const bool spread = xcd && load(l) > total / (2.0 * kXcds);
+    size_t dst = 0;
+    if (xcd && !spread)
+      dst = (size_t)(std::min_element(xl.begin(), xl.end()) - xl.begin());
+    int64_t k = 0;
+    for (int64_t r = 0; r < l->req.n; r += R, k++) {
+      const size_t x = spread ? (size_t)(k % kXcds) : dst;
+      per[x].push_back(BTask{l->slot, (int32_t)std::min<int64_t>(R, l->req.n - r), l->row_off + r});
+    }
+    if (spread)
+      for (auto &v : xl) v += load(l) / kXcds;
+    else
+      xl[dst] += load(l);
   }
+  std::vector<BTask> t;
+  int32_t hdr[2 + kXcds] = {0};
+  for (size_t x = 0; x < per.size(); x++) {
+    hdr[1 + x] = (int32_t)t.size();
+    t.insert(t.end(), per[x].begin(), per[x].end());
+  }
+  for (size_t x = per.size(); x <= (size_t)kXcds; x++) hdr[1 + x] = (int32_t)t.size();
+  hdr[0] = (int32_t)t.size();
   if ((int64_t)t.size() > b->ntask_cap) return fail(SIS_ERR_STATE, "batch task table overflow");
   if (!t.empty())
     HIPCHK(hipMemcpyAsync(b->tasks.p, t.data(), sizeof(BTask) * t.size(), hipMemcpyHostToDevice,
                           b->stream));
-  const int32_t nt = (int32_t)t.size();
-  HIPCHK(hipMemcpyAsync(b->ntasks.p, &nt, sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
+  HIPCHK(hipMemcpyAsync(b->ntasks.p, hdr, sizeof(hdr), hipMemcpyHostToDevice, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
   b->tasks_dirty = false;
   return SIS_OK;
@@ -789,6 +1065,12 @@ int admit(sis_batch *b) {
     in.dt = q.dt;
     in.xi0 = q.xi0;
     in.S = q.S;
+    in.intw = 1;
+    for (double v : q.val)
+      if (!(v == std::floor(v) && v >= -127.0 && v <= 127.0)) {
+        in.intw = 0;
+        break;
+      }
     d.slot = slot;
     d.seed = q.seed;
     d.src_rp = (int64_t)srp.size();
@@ -843,6 +1125,7 @@ int build_graph(sis_batch *b) {
     b->graph = nullptr;
   }
   hipGraph_t g = nullptr;
+  launch_batch_step(b, false);  // resolve the grid outside the capture
   HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   for (int32_t k = 0; k < b->p.chunk_steps; k++) {
     launch_batch_step(b);
@@ -913,16 +1196,17 @@ int sis_batch_create(const sis_batch_params *pin, sis_batch **out) {
       (st = b->val.alloc((size_t)std::max<int64_t>(1, p.max_nnz))) ||
       (st = b->vald.alloc((size_t)std::max<int64_t>(1, p.max_nnz))) ||
       (st = b->inst.alloc((size_t)p.max_instances)) || (st = b->tasks.alloc((size_t)b->ntask_cap)) ||
-      (st = b->ntasks.alloc(1)) || (st = b->stab.alloc((size_t)L)) || (st = b->bif.alloc((size_t)L)) ||
+      (st = b->ntasks.alloc(2 + kXcds)) || (st = b->stab.alloc((size_t)L)) || (st = b->bif.alloc((size_t)L)) ||
       (st = b->cutslots.alloc((size_t)p.max_instances)) || (st = b->cuts.alloc((size_t)L)))
     return st;
   HIPCHK(hipMemset(b->inst.p, 0, sizeof(BInst) * p.max_instances));
-  HIPCHK(hipMemset(b->ntasks.p, 0, sizeof(int32_t)));
-  int cus = 256;
+  HIPCHK(hipMemset(b->ntasks.p, 0, sizeof(int32_t) * (2 + kXcds)));
+  int dev = 0, cus = 256;
   hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, 0) == hipSuccess && prop.multiProcessorCount > 0)
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+      prop.multiProcessorCount > 0)
     cus = prop.multiProcessorCount;
-  b->grid = cus * 8;  // 32 wavefronts per CU, grid-stride over the task table
+  b->cus = cus;
   *out = b.release();
   return SIS_OK;
 }
@@ -1073,7 +1357,11 @@ int sis_batch_autotune(sis_batch *b, int64_t trial_steps, int32_t *tile, double 
   for (int r : {1, 2, 4, 8})
     for (int nt : {0, 1})
       for (int vg : {0, 1})
-        if (batch_tile_ok((int)b->W, r | (nt << 16) | (vg << 17))) cand.push_back(r | (nt << 16) | (vg << 17));
+        for (int xc : {0, 1})
+          for (int bt : {0, 1}) {
+            const int32_t tl = r | (nt << 16) | (vg << 17) | (xc << 18) | (bt << 19);
+            if (batch_tile_ok((int)b->W, tl)) cand.push_back(tl);
+          }
   for (int32_t tl : cand) {
     {
       b->tile = tl;

@@ -62,9 +62,18 @@ def test_batch_results_equal_alone(agents_max, agents):
         assert st["submit_s"] <= st["admit_s"] <= st["done_s"]
 
 
+XCD = 1 << 18  # XCD-affine task order: with 9 uneven instances both placements
+# (whole instance on one XCD, and the largest dealt to every XCD) occur
+BT = 1 << 19  # bit-transpose integer sums (instances with small integer weights)
+
+
 @pytest.mark.parametrize("agents_max,tile", [(64, 1), (64, 8 | 1 << 16), (64, 2 | 1 << 17),
                                              (64, 8 | 1 << 16 | 1 << 17), (128, 4 | 1 << 17),
-                                             (256, 2 | 1 << 16 | 1 << 17)])
+                                             (256, 2 | 1 << 16 | 1 << 17), (64, 1 | XCD),
+                                             (64, 8 | 1 << 16 | 1 << 17 | XCD),
+                                             (256, 2 | 1 << 16 | 1 << 17 | XCD),
+                                             (64, 4 | 1 << 16 | BT), (64, 8 | BT | XCD),
+                                             (64, 2 | BT), (128, 4 | BT), (256, 2 | 1 << 16 | BT)])
 def test_batch_tiles_bit_exact(agents_max, tile):
     Js = instances(9, 21, 64, 500)
     A = min(agents_max, 100) if agents_max > 64 else 64
@@ -118,10 +127,49 @@ def test_batch_autotune_midrun_keeps_results():
     tickets = [b.submit(J, 64, 0.5, 1000, update_window=0, seed=5 + k) for k, J in enumerate(Js)]
     b.advance(300)
     tile, ms = b.autotune(10)
-    assert ms > 0 and (tile & 0xffff) in (1, 2, 4)
+    assert ms > 0 and (tile & 0xffff) in (1, 2, 4, 8)
     while b.advance(300)[0]:
         pass
     b.advance(0)
     for k, (J, t) in enumerate(zip(Js, tickets)):
         steps, cuts, spins = solve_alone(J, 64, 0.5, 1000, 0, 5 + k)
         assert np.array_equal(b.result(t), cuts) and b.poll(t)["steps_executed"] == steps
+
+
+def _weighted(n, p, seed, kind):
+    """ER graph with weights of a given kind: 'wide' integers up to 127
+    (all seven bit planes), 'real' non-integers, 'big' integers beyond 127."""
+    rng = np.random.default_rng(seed)
+    iu, ju = np.triu_indices(n, 1)
+    keep = rng.random(len(iu)) < p
+    rows, cols = iu[keep].astype(np.int64), ju[keep].astype(np.int64)
+    if kind == "wide":
+        vals = rng.choice([-127, -100, -64, -37, -5, -1, 1, 3, 37, 64, 100, 127], size=len(rows))
+    elif kind == "real":
+        vals = rng.uniform(-2.0, 2.0, size=len(rows))
+    else:
+        vals = rng.choice([-300, 200, 129, -1, 1], size=len(rows))
+    return sis.Csr.from_coo(n, rows, cols, vals.astype(np.float64), mirror=True)
+
+
+@pytest.mark.parametrize("tile", [4 | BT, 8 | 1 << 16 | BT | XCD, 2 | BT])
+def test_batch_bit_transpose_mixed_weights(tile):
+    """Small-integer instances take the bit-plane sums, others the readlane
+    path, in the same launch; every result equals the solo solve."""
+    Js = instances(4, 17, 64, 400) + [_weighted(150, 0.06, 1, "wide"), _weighted(90, 0.1, 2, "real"),
+                                      _weighted(120, 0.08, 3, "big"), _weighted(300, 0.04, 4, "wide")]
+    b = sis.Batch(agents_max=64, max_instances=8, max_rows=6000, max_nnz=200000, chunk_steps=100,
+                  tile=tile)
+    tickets = [b.submit(J, 64, 0.5, 700, update_window=1, seed=60 + k) for k, J in enumerate(Js)]
+    while b.advance(700)[0]:
+        pass
+    b.advance(0)
+    for k, (J, t) in enumerate(zip(Js, tickets)):
+        steps, cuts, spins = solve_alone(J, 64, 0.5, 700, 1, 60 + k)
+        bc, bs = b.result(t, spins=True)
+        assert b.poll(t)["steps_executed"] == steps, k
+        assert np.array_equal(bs, spins), (tile, k)
+        if k == 5:  # real weights: the two cut kernels add FP64 terms in different orders
+            np.testing.assert_allclose(bc, cuts, rtol=1e-12, atol=0)
+        else:
+            assert np.array_equal(bc, cuts), (tile, k)
