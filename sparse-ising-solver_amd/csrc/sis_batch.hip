@@ -45,7 +45,7 @@ struct BInst {
   int32_t uw, status;
   double dt, xi0, S; // time step, coupling scale, sum of file weights
   int32_t nbif;
-  int32_t intw;      // every weight an integer in [-127, 127] (bit-transpose path)
+  int32_t intw;      // every weight an integer in [-3, 3] (bit-transpose path)
 };
 
 struct BTask {
@@ -303,8 +303,8 @@ __device__ __forceinline__ uint64_t bit_transpose64(uint64_t w, int lane) {
   return w;
 }
 
-// Same step as k_batch_step_vg.  For instances whose weights are all small
-// integers the neighbour sums are taken on bit masks instead of one readlane
+// Same step as k_batch_step_vg.  For instances whose weights are all integers
+// in [-3, 3] the neighbour sums are taken on bit masks instead of one readlane
 // per entry: after the transpose lane a holds, for each of the pass's 64
 // slots, bit l = 1 when agent a's neighbour at slot l is negative; with the
 // slots' weights split into bit planes p and signs (wave-uniform ballot
@@ -315,9 +315,9 @@ __device__ __forceinline__ uint64_t bit_transpose64(uint64_t w, int lane) {
 // below 2^24), so results are bit-identical to the other step kernels.
 // Instances with other weights take the readlane path of k_batch_step_vg.
 template <int R, int WT, int NT>
-__global__ __launch_bounds__(kBlock) void k_batch_step_bt(BArgs b) {
+__device__ __forceinline__ void batch_step_bt(const BArgs &b) {
   constexpr int SL = 64 / R;  // neighbour slots per row and pass
-  constexpr int kPlanes = 7;  // |v| <= 127
+  constexpr int kPlanes = 2;  // |v| <= 3
   const int lane = threadIdx.x & 63;
   const TaskRange tr = task_range(b);
   const int64_t lda = b.lda, W = WT;
@@ -373,19 +373,17 @@ __global__ __launch_bounds__(kBlock) void k_batch_step_bt(BArgs b) {
         for (int u = 0; u < WT; u++) w[u] = S[j * W + u];
         // bit planes of the slots' weights: P_p (positive), N_p (negative)
         uint64_t Pm[kPlanes], Nm[kPlanes];
-        int np = 0;
 #pragma unroll
         for (int q = 0; q < kPlanes; q++) {
           Pm[q] = __ballot(vi > 0 && ((av >> q) & 1));
           Nm[q] = __ballot(vi < 0 && ((av >> q) & 1));
-          if (Pm[q] | Nm[q]) np = q + 1;
         }
 #pragma unroll
         for (int r = 0; r < R; r++) {
           const uint64_t rm = (SL == 64 ? ~0ull : ((1ull << SL) - 1)) << (r * SL);
 #pragma unroll
           for (int q = 0; q < kPlanes; q++)
-            if (q < np) Vr[r] += (__popcll(Pm[q] & rm) - __popcll(Nm[q] & rm)) * (1 << q);
+            Vr[r] += (__popcll(Pm[q] & rm) - __popcll(Nm[q] & rm)) * (1 << q);
         }
 #pragma unroll
         for (int u = 0; u < WT; u++) {
@@ -396,7 +394,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_step_bt(BArgs b) {
             int acc = 0;
 #pragma unroll
             for (int q = 0; q < kPlanes; q++)
-              if (q < np) acc += (__popcll(m & Pm[q] & rm) - __popcll(m & Nm[q] & rm)) * (1 << q);
+              acc += (__popcll(m & Pm[q] & rm) - __popcll(m & Nm[q] & rm)) * (1 << q);
             Sa[r][u] += acc;
           }
         }
@@ -449,6 +447,11 @@ __global__ __launch_bounds__(kBlock) void k_batch_step_bt(BArgs b) {
       if (lane < WT) Sn[row * W + lane] = mine;
     }
   }
+}
+
+template <int R, int WT, int NT>
+__global__ __launch_bounds__(kBlock) void k_batch_step_bt(BArgs b) {
+  batch_step_bt<R, WT, NT>(b);
 }
 
 // After each step: one block per slot advances the instance's step count and,
@@ -1067,7 +1070,7 @@ int admit(sis_batch *b) {
     in.S = q.S;
     in.intw = 1;
     for (double v : q.val)
-      if (!(v == std::floor(v) && v >= -127.0 && v <= 127.0)) {
+      if (!(v == std::floor(v) && v >= -3.0 && v <= 3.0)) {
         in.intw = 0;
         break;
       }

@@ -137,13 +137,16 @@ def test_batch_autotune_midrun_keeps_results():
 
 
 def _weighted(n, p, seed, kind):
-    """ER graph with weights of a given kind: 'wide' integers up to 127
-    (all seven bit planes), 'real' non-integers, 'big' integers beyond 127."""
+    """ER graph with weights of a given kind: 'pm1' +-1 (one bit plane),
+    'wide' integers up to 127 and 'big' beyond it (both outside the bit-plane
+    range), 'real' non-integers."""
     rng = np.random.default_rng(seed)
     iu, ju = np.triu_indices(n, 1)
     keep = rng.random(len(iu)) < p
     rows, cols = iu[keep].astype(np.int64), ju[keep].astype(np.int64)
-    if kind == "wide":
+    if kind == "pm1":
+        vals = rng.choice([-1, 1], size=len(rows))
+    elif kind == "wide":
         vals = rng.choice([-127, -100, -64, -37, -5, -1, 1, 3, 37, 64, 100, 127], size=len(rows))
     elif kind == "real":
         vals = rng.uniform(-2.0, 2.0, size=len(rows))
@@ -157,7 +160,7 @@ def test_batch_bit_transpose_mixed_weights(tile):
     """Small-integer instances take the bit-plane sums, others the readlane
     path, in the same launch; every result equals the solo solve."""
     Js = instances(4, 17, 64, 400) + [_weighted(150, 0.06, 1, "wide"), _weighted(90, 0.1, 2, "real"),
-                                      _weighted(120, 0.08, 3, "big"), _weighted(300, 0.04, 4, "wide")]
+                                      _weighted(120, 0.08, 3, "big"), _weighted(300, 0.04, 4, "pm1")]
     b = sis.Batch(agents_max=64, max_instances=8, max_rows=6000, max_nnz=200000, chunk_steps=100,
                   tile=tile)
     tickets = [b.submit(J, 64, 0.5, 700, update_window=1, seed=60 + k) for k, J in enumerate(Js)]
