@@ -472,17 +472,47 @@ __global__ __launch_bounds__(kBlock) void k_window_scan(const void *xsrc, uint64
   const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
   const int64_t r0 = (int64_t)blockIdx.x * rpb;
   const int64_t r1 = r0 + rpb < N ? r0 + rpb : N;
-  const int64_t npairs = (r1 - r0) * W;
-  for (int64_t p = q; p < npairs; p += kWaves) {
-    const int64_t i = r0 + p / W, w = p % W;
-    uint64_t b;
-    if (SRC == SRC_BITS) b = ((const uint64_t *)xsrc)[i * W + w];
-    else if (SRC == SRC_F32) b = __ballot(signbit(((const float *)xsrc)[i * lda + 64 * w + lane]));
-    else b = __ballot(signbit(((const double *)xsrc)[i * lda + 64 * w + lane]));
-    const uint64_t d = b ^ cur[i * W + w];
-    if (lane == 0) {
-      cur[i * W + w] = b;
-      if (d) atomicOr((unsigned long long *)&s_acc[w], (unsigned long long)d);
+  if (SRC == SRC_BITS) {
+    // the words are already packed: one (row, word) per thread, coalesced
+    for (int64_t idx = r0 * W + threadIdx.x; idx < r1 * W; idx += blockDim.x) {
+      const uint64_t b = ((const uint64_t *)xsrc)[idx];
+      const uint64_t d = b ^ cur[idx];
+      cur[idx] = b;
+      if (d) atomicOr((unsigned long long *)&s_acc[idx % W], (unsigned long long)d);
+    }
+    __syncthreads();
+    for (int64_t w = threadIdx.x; w < W; w += blockDim.x) part[(int64_t)blockIdx.x * W + w] = s_acc[w];
+    return;
+  }
+  // one (row, word) per wavefront: ballot of 64 agents' signs; the X values
+  // and old words of U pairs are all loaded before the first ballot.  Pair
+  // indices within a block fit 32 bits (rows per block x words).
+  constexpr int U = 8;
+  const uint32_t npairs = (uint32_t)((r1 - r0) * W), uW = (uint32_t)W;
+  for (uint32_t p0 = (uint32_t)q * U; p0 < npairs; p0 += (uint32_t)kWaves * U) {
+    bool neg[U];
+    uint64_t old[U];
+    uint32_t wu[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const uint32_t p = p0 + u < npairs ? p0 + u : npairs - 1;
+      const uint32_t di = p / uW;
+      wu[u] = p - di * uW;
+      const int64_t i = r0 + di;
+      if (SRC == SRC_F32) neg[u] = signbit(((const float *)xsrc)[i * lda + 64 * wu[u] + lane]);
+      else neg[u] = signbit(((const double *)xsrc)[i * lda + 64 * wu[u] + lane]);
+      old[u] = cur[i * W + wu[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (p0 + u >= npairs) break;
+      const int64_t idx = r0 * W + (int64_t)(p0 + u);
+      const uint64_t b = __ballot(neg[u]);
+      const uint64_t d = b ^ old[u];
+      if (lane == 0) {
+        cur[idx] = b;
+        if (d) atomicOr((unsigned long long *)&s_acc[wu[u]], (unsigned long long)d);
+      }
     }
   }
   __syncthreads();
@@ -653,13 +683,23 @@ __global__ __launch_bounds__(kBlock) void k_cut_rows(const int64_t *__restrict__
 }
 
 // mode 0: cut = -0.5 (H - S) (src/kernels.cu:393-398); mode 1: energy.
-__global__ void k_cut_reduce(const double *part, int64_t nblk, int64_t A, int64_t lda, int mode,
-                             double S, double *out) {
-  const int64_t ag = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (ag >= A) return;
+// One workgroup per 64 agents: wavefront q adds the block partials
+// q, q + 16, ... in order, then the 16 wavefront sums are added in order.
+constexpr int kRedWaves = 16;
+__global__ __launch_bounds__(64 * kRedWaves) void k_cut_reduce(const double *part, int64_t nblk,
+                                                               int64_t A, int64_t lda, int mode,
+                                                               double S, double *out) {
+  __shared__ double s_h[kRedWaves][64];
+  const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int64_t ag = (int64_t)blockIdx.x * 64 + lane;
   double H = 0.0;
-  for (int64_t b = 0; b < nblk; b++) H += part[b * lda + ag];
-  out[ag] = mode == 0 ? (-0.5) * (H - S) : H;
+  for (int64_t b = q; b < nblk; b += kRedWaves) H += part[b * lda + ag];
+  s_h[q][lane] = H;
+  __syncthreads();
+  if (q != 0) return;
+  H = s_h[0][lane];
+  for (int k = 1; k < kRedWaves; k++) H += s_h[k][lane];
+  if (ag < A) out[ag] = mode == 0 ? (-0.5) * (H - S) : H;
 }
 
 __device__ __forceinline__ int64_t d_tri(int64_t c1, int64_t c2, int64_t n) {
@@ -973,8 +1013,8 @@ void launch_cut_rows(hipStream_t st, const int64_t *rp, const int32_t *col, cons
 
 void launch_cut_reduce(hipStream_t st, const double *part, int64_t nblk, int64_t A, int64_t lda,
                        int mode, double S, double *out) {
-  hipLaunchKernelGGL(k_cut_reduce, dim3(grid_for(A, 256)), dim3(256), 0, st, part, nblk, A, lda,
-                     mode, S, out);
+  hipLaunchKernelGGL(k_cut_reduce, dim3((unsigned)(lda / 64)), dim3(64 * kRedWaves), 0, st, part,
+                     nblk, A, lda, mode, S, out);
 }
 
 void launch_tsp_len(hipStream_t st, const uint64_t *fin, const int32_t *valid_list, int64_t n,

@@ -884,6 +884,7 @@ int sis_solver_energy(sis_solver *s, int which, double *energy) {
                     s->cut_blocks, s->cut_rows, s->cutpart.p);
     launch_cut_reduce(s->stream, s->cutpart.p, s->cut_blocks, s->A, s->lda, 1, 0.0, s->outv.p);
     HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));  // the stream does not order with hipMemcpy
     HIPCHK(hipMemcpy(c.data(), s->outv.p, sizeof(double) * s->A, hipMemcpyDeviceToHost));
     for (int64_t a = 0; a < s->A; a++) energy[a] = 2.0 * c[(size_t)a];
     return SIS_OK;

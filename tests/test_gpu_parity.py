@@ -302,3 +302,23 @@ def test_full_size_packed_equals_plain(n):
     s64 = sa.astype(np.int64)
     H = 0.5 * np.einsum("ia,ia->a", s64, (M @ s64))
     assert np.array_equal(ca, -0.5 * (H - J.sum))
+
+
+@pytest.mark.parametrize("form", ["packed", "f32"])
+def test_maxcut_energy_matches_oracle_after_cuts(form):
+    """Energy of the current and final spins (s^T J s without a field) right
+    after the cuts, which leave other values in the same device buffer."""
+    J, _ = maxcut_problem(300, 0.05, sis.W_INT3, 8)
+    rp, col, val = J.arrays()
+    pb0 = orc.Problem(rp, col, val, field=np.zeros(J.n), S=J.sum)
+    s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=128, dt=0.5, form=sis.FORMS[form]))
+    s.init_random(3)
+    s.run(3000, 1)
+    for which in (sis.SPINS_CURRENT, sis.SPINS_FINAL):
+        cuts, _ = s.cuts()
+        E = s.energy(which)
+        spins = s.spins(which)
+        assert np.array_equal(E, orc.energy_field(pb0, spins)), which
+    # cut = -(1/2)(H - S) with H = s^T J s / 2, on the current spins
+    cuts, _ = s.cuts()
+    assert np.array_equal(cuts, (J.sum - s.energy(sis.SPINS_CURRENT) / 2) / 2)
