@@ -314,7 +314,7 @@ __device__ __forceinline__ uint64_t bit_transpose64(uint64_t w, int lane) {
 // is the sequential sum of +-v exactly (every partial sum is an integer
 // below 2^24), so results are bit-identical to the other step kernels.
 // Instances with other weights take the readlane path of k_batch_step_vg.
-template <int R, int WT, int NT>
+template <int R, int WT, int NT, int PF>
 __device__ __forceinline__ void batch_step_bt(const BArgs &b) {
   constexpr int SL = 64 / R;  // neighbour slots per row and pass
   constexpr int kPlanes = 2;  // |v| <= 3
@@ -322,10 +322,45 @@ __device__ __forceinline__ void batch_step_bt(const BArgs &b) {
   const TaskRange tr = task_range(b);
   const int64_t lda = b.lda, W = WT;
   const int my_r = lane / SL, my_i = lane % SL;
+  // PF: the next task's record, state rows and row ranges are loaded while
+  // this task finishes, so a task waits only on column -> sign word.
+  BTask ntk{};
+  bool nact = false;
+  float nx[R][WT], ny[R][WT];
+  int64_t nkb = 0;
+  int ndeg = 0;
+  auto stage = [&](int task) {
+    ntk = b.tasks[task];
+    nact = b.inst[ntk.slot].status == SLOT_ACTIVE;
+    if (!nact) return;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      if (r < ntk.nrows) {
+#pragma unroll
+        for (int u = 0; u < WT; u++) {
+          nx[r][u] = ld_state<NT>(b.X + (ntk.row0 + r) * lda + 64 * u + lane);
+          ny[r][u] = ld_state<NT>(b.Y + (ntk.row0 + r) * lda + 64 * u + lane);
+        }
+      }
+    }
+    const int64_t row = ntk.row0 + (my_r < ntk.nrows ? my_r : 0);
+    nkb = b.rbeg[row];
+    ndeg = my_r < ntk.nrows ? (int)(b.rend[row] - nkb) : 0;
+  };
+  if (PF && tr.first < tr.end) stage(tr.first);
   for (int task = tr.first; task < tr.end; task += tr.stride) {
-    const BTask tk = b.tasks[task];
+    if (PF) {
+      if (!nact) {
+        if (task + tr.stride < tr.end) stage(task + tr.stride);
+        continue;
+      }
+    } else {
+      ntk = b.tasks[task];
+      if (b.inst[ntk.slot].status != SLOT_ACTIVE) continue;
+      stage(task);
+    }
+    const BTask tk = ntk;
     const BInst &in = b.inst[tk.slot];
-    if (in.status != SLOT_ACTIVE) continue;
     const int32_t t = in.t;
     double p = 1.025 * (double)t / (double)in.T;
     p = p < 1.0 ? p : 1.0;
@@ -335,18 +370,14 @@ __device__ __forceinline__ void batch_step_bt(const BArgs &b) {
     uint64_t *Sn = (t & 1) ? b.S0 : b.S1;
     float x[R][WT], y[R][WT];
 #pragma unroll
-    for (int r = 0; r < R; r++) {
-      if (r < tk.nrows) {
+    for (int r = 0; r < R; r++)
 #pragma unroll
-        for (int u = 0; u < WT; u++) {
-          x[r][u] = ld_state<NT>(b.X + (tk.row0 + r) * lda + 64 * u + lane);
-          y[r][u] = ld_state<NT>(b.Y + (tk.row0 + r) * lda + 64 * u + lane);
-        }
+      for (int u = 0; u < WT; u++) {
+        x[r][u] = nx[r][u];
+        y[r][u] = ny[r][u];
       }
-    }
-    const int64_t myrow = tk.row0 + (my_r < tk.nrows ? my_r : 0);
-    const int64_t mykb = b.rbeg[myrow];
-    const int mydeg = my_r < tk.nrows ? (int)(b.rend[myrow] - mykb) : 0;
+    const int64_t mykb = nkb;
+    const int mydeg = ndeg;
     int maxdeg = mydeg;
     for (int m = 32; m >= 1; m >>= 1) {
       const int o = __shfl_xor(maxdeg, m, 64);
@@ -431,6 +462,7 @@ __device__ __forceinline__ void batch_step_bt(const BArgs &b) {
         }
       }
     }
+    if (PF && task + tr.stride < tr.end) stage(task + tr.stride);
 #pragma unroll
     for (int r = 0; r < R; r++) {
       if (r >= tk.nrows) break;
@@ -449,9 +481,9 @@ __device__ __forceinline__ void batch_step_bt(const BArgs &b) {
   }
 }
 
-template <int R, int WT, int NT>
+template <int R, int WT, int NT, int PF>
 __global__ __launch_bounds__(kBlock) void k_batch_step_bt(BArgs b) {
-  batch_step_bt<R, WT, NT>(b);
+  batch_step_bt<R, WT, NT, PF>(b);
 }
 
 // After each step: one block per slot advances the instance's step count and,
@@ -806,12 +838,14 @@ namespace {
 
 // Tile: rows per task (1, 2, 4, 8) | non-temporal << 16 | vector gathers << 17
 // (vector gathers need 2, 4 or 8 rows per task) | XCD-affine task order << 18 |
-// bit-transpose integer sums << 19 (2, 4 or 8 rows, no vector-gather bit).
+// bit-transpose integer sums << 19 (2, 4 or 8 rows, no vector-gather bit) |
+// next-task prefetch << 20 (bit-transpose with non-temporal state).
 bool batch_tile_ok(int W, int tile) {
   const int r = tile & 0xffff, nt = (tile >> 16) & 1, vg = (tile >> 17) & 1;
-  const int bt = (tile >> 19) & 1;
-  if ((tile >> 20) != 0 || (nt != 0 && nt != 1)) return false;
+  const int bt = (tile >> 19) & 1, pf = (tile >> 20) & 1;
+  if ((tile >> 21) != 0 || (nt != 0 && nt != 1)) return false;
   if (!(W == 1 || W == 2 || W == 4)) return false;
+  if (pf && !(bt && nt)) return false;  // prefetch: bit-transpose, non-temporal only
   if (bt) return !vg && (r == 2 || r == 4 || r == 8);
   if (vg) return r == 2 || r == 4 || r == 8;
   return r == 1 || r == 2 || r == 4 || r == 8;
@@ -845,11 +879,12 @@ void launch_bstep_vg(const sis_batch *b, const BArgs &a, bool launch) {
   if (launch) hipLaunchKernelGGL((k_batch_step_vg<R, WT, NT>), dim3(grid), dim3(kBlock), 0, b->stream, a);
 }
 
-template <int R, int WT, int NT>
+template <int R, int WT, int NT, int PF>
 void launch_bstep_bt(const sis_batch *b, const BArgs &a, bool launch) {
   static unsigned grid = 0;  // per kernel; the engine runs on one kind of device
-  if (!grid) grid = persistent_grid(b, k_batch_step_bt<R, WT, NT>);
-  if (launch) hipLaunchKernelGGL((k_batch_step_bt<R, WT, NT>), dim3(grid), dim3(kBlock), 0, b->stream, a);
+  if (!grid) grid = persistent_grid(b, k_batch_step_bt<R, WT, NT, PF>);
+  if (launch)
+    hipLaunchKernelGGL((k_batch_step_bt<R, WT, NT, PF>), dim3(grid), dim3(kBlock), 0, b->stream, a);
 }
 
 // launch = false only resolves the kernel's grid (done before graph capture,
@@ -857,17 +892,20 @@ void launch_bstep_bt(const sis_batch *b, const BArgs &a, bool launch) {
 void launch_batch_step(sis_batch *b, bool launch = true) {
   const BArgs a = b->args();
   const int r = b->tile & 0xffff, nt = (b->tile >> 16) & 1, vg = (b->tile >> 17) & 1;
-  const int bt = (b->tile >> 19) & 1;
+  const int bt = (b->tile >> 19) & 1, pf = (b->tile >> 20) & 1;
   const int w = (int)b->W;
-#define SIS_BSTEPB(R, WT, NT) \
-  if (bt && r == R && w == WT && nt == NT) return launch_bstep_bt<R, WT, NT>(b, a, launch);
-#define SIS_BSTEPB_W(R, NT) SIS_BSTEPB(R, 1, NT) SIS_BSTEPB(R, 2, NT) SIS_BSTEPB(R, 4, NT)
-  SIS_BSTEPB_W(2, 0)
-  SIS_BSTEPB_W(4, 0)
-  SIS_BSTEPB_W(8, 0)
-  SIS_BSTEPB_W(2, 1)
-  SIS_BSTEPB_W(4, 1)
-  SIS_BSTEPB_W(8, 1)
+#define SIS_BSTEPB(R, WT, NT, PF) \
+  if (bt && r == R && w == WT && nt == NT && pf == PF) return launch_bstep_bt<R, WT, NT, PF>(b, a, launch);
+#define SIS_BSTEPB_W(R, NT, PF) SIS_BSTEPB(R, 1, NT, PF) SIS_BSTEPB(R, 2, NT, PF) SIS_BSTEPB(R, 4, NT, PF)
+  SIS_BSTEPB_W(2, 0, 0)
+  SIS_BSTEPB_W(4, 0, 0)
+  SIS_BSTEPB_W(8, 0, 0)
+  SIS_BSTEPB_W(2, 1, 0)
+  SIS_BSTEPB_W(4, 1, 0)
+  SIS_BSTEPB_W(8, 1, 0)
+  SIS_BSTEPB_W(2, 1, 1)
+  SIS_BSTEPB_W(4, 1, 1)
+  SIS_BSTEPB_W(8, 1, 1)
 #undef SIS_BSTEPB_W
 #undef SIS_BSTEPB
 #define SIS_BSTEP(R, WT, NT) \
@@ -1361,10 +1399,11 @@ int sis_batch_autotune(sis_batch *b, int64_t trial_steps, int32_t *tile, double 
     for (int nt : {0, 1})
       for (int vg : {0, 1})
         for (int xc : {0, 1})
-          for (int bt : {0, 1}) {
-            const int32_t tl = r | (nt << 16) | (vg << 17) | (xc << 18) | (bt << 19);
-            if (batch_tile_ok((int)b->W, tl)) cand.push_back(tl);
-          }
+          for (int bt : {0, 1})
+            for (int pf : {0, 1}) {
+              const int32_t tl = r | (nt << 16) | (vg << 17) | (xc << 18) | (bt << 19) | (pf << 20);
+              if (batch_tile_ok((int)b->W, tl)) cand.push_back(tl);
+            }
   for (int32_t tl : cand) {
     {
       b->tile = tl;

@@ -45,14 +45,19 @@ def _fields(line):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("fused", [1, 0])
-def test_maxcut_line_matches_oracle(tmp_path, fused):
+@pytest.mark.parametrize("fused,knobs", [
+    (1, {}), (0, {}),
+    # packed form, whole-run graph, tuned tile and row split: same results
+    (1, {"SIS_FORM": "packed", "SIS_GRAPH": "2", "SIS_SPLIT": "-1", "SIS_TUNE": "1"}),
+    (1, {"SIS_SPLIT": "8"}),
+])
+def test_maxcut_line_matches_oracle(tmp_path, fused, knobs):
     J = sis.Csr.gen_er(300, 0.05, sis.W_PM1, 3)
     mtx = tmp_path / "g.mtx"
     J.write_mtx(mtx)
     out = tmp_path / "out.csv"
     A, T, dt, seed = 128, 1500, 0.5, 99
-    r = run(["0", mtx, out, "maxcut", "1", fused, "1", dt, T, A], env={"SIS_SEED": str(seed)})
+    r = run(["0", mtx, out, "maxcut", "1", fused, "1", dt, T, A], env={"SIS_SEED": str(seed), **knobs})
     assert r.returncode == 0, r.stderr
     f = _fields(out.read_text())
     assert f[6:] == [str(mtx), "300", str(J.nnz), str(T), "0.5", str(A)]

@@ -65,6 +65,7 @@ def test_batch_results_equal_alone(agents_max, agents):
 XCD = 1 << 18  # XCD-affine task order: with 9 uneven instances both placements
 # (whole instance on one XCD, and the largest dealt to every XCD) occur
 BT = 1 << 19  # bit-transpose integer sums (instances with small integer weights)
+PF = 1 << 20  # next-task prefetch (bit-transpose, non-temporal)
 
 
 @pytest.mark.parametrize("agents_max,tile", [(64, 1), (64, 8 | 1 << 16), (64, 2 | 1 << 17),
@@ -73,7 +74,9 @@ BT = 1 << 19  # bit-transpose integer sums (instances with small integer weights
                                              (64, 8 | 1 << 16 | 1 << 17 | XCD),
                                              (256, 2 | 1 << 16 | 1 << 17 | XCD),
                                              (64, 4 | 1 << 16 | BT), (64, 8 | BT | XCD),
-                                             (64, 2 | BT), (128, 4 | BT), (256, 2 | 1 << 16 | BT)])
+                                             (64, 2 | BT), (128, 4 | BT), (256, 2 | 1 << 16 | BT),
+                                             (64, 8 | 1 << 16 | BT | PF), (64, 2 | 1 << 16 | BT | PF | XCD),
+                                             (256, 4 | 1 << 16 | BT | PF)])
 def test_batch_tiles_bit_exact(agents_max, tile):
     Js = instances(9, 21, 64, 500)
     A = min(agents_max, 100) if agents_max > 64 else 64
@@ -155,7 +158,7 @@ def _weighted(n, p, seed, kind):
     return sis.Csr.from_coo(n, rows, cols, vals.astype(np.float64), mirror=True)
 
 
-@pytest.mark.parametrize("tile", [4 | BT, 8 | 1 << 16 | BT | XCD, 2 | BT])
+@pytest.mark.parametrize("tile", [4 | BT, 8 | 1 << 16 | BT | XCD, 2 | BT, 8 | 1 << 16 | BT | PF])
 def test_batch_bit_transpose_mixed_weights(tile):
     """Small-integer instances take the bit-plane sums, others the readlane
     path, in the same launch; every result equals the solo solve."""
