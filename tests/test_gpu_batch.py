@@ -179,3 +179,26 @@ def test_batch_bit_transpose_mixed_weights(tile):
             np.testing.assert_allclose(bc, cuts, rtol=1e-12, atol=0)
         else:
             assert np.array_equal(bc, cuts), (tile, k)
+
+
+def test_batch_instances_without_edges():
+    """Instances with no stored entry (and a single node) run beside others;
+    each equals its solo solve (same xi0 given explicitly: the sample std of
+    an all-zero matrix is 0)."""
+    empty = lambda n: sis.Csr.from_coo(n, np.zeros(0, np.int64), np.zeros(0, np.int64), np.zeros(0))
+    Js = instances(3, 8, 64, 200) + [empty(5), empty(1), empty(130)]
+    b = sis.Batch(agents_max=64, max_instances=8, max_rows=2000, max_nnz=50000, chunk_steps=50)
+    ts = [b.submit(J, 64, 0.5, 400, update_window=1, seed=20 + k, xi0=0.3) for k, J in enumerate(Js)]
+    while b.advance(400)[0]:
+        pass
+    b.advance(0)
+    for k, (J, t) in enumerate(zip(Js, ts)):
+        s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=64, dt=0.5, xi0=0.3,
+                               form=sis.FORM_PACKED))
+        s.init_random(20 + k)
+        info = s.run(400, 1)
+        cuts, _ = s.cuts()
+        bc, bs = b.result(t, spins=True)
+        assert b.poll(t)["steps_executed"] == info["steps_executed"], k
+        assert np.array_equal(bc, cuts) and np.array_equal(bs, s.spins()), k
+        s.close()

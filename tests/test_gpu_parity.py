@@ -322,3 +322,46 @@ def test_maxcut_energy_matches_oracle_after_cuts(form):
     # cut = -(1/2)(H - S) with H = s^T J s / 2, on the current spins
     cuts, _ = s.cuts()
     assert np.array_equal(cuts, (J.sum - s.energy(sis.SPINS_CURRENT) / 2) / 2)
+
+
+@pytest.mark.parametrize("form", ["f32", "packed", "f64", "bf16"])
+def test_graph_without_edges(form):
+    """No stored entry at all (every node isolated, Z = 0): the force is zero,
+    the step is the free oscillator, the cut is -(1/2)(0 - 0) = 0 and the
+    energy 0; the row split and the tuner have nothing to split."""
+    n = 70
+    J = sis.Csr.from_coo(n, np.zeros(0, np.int64), np.zeros(0, np.int64), np.zeros(0), mirror=True)
+    assert J.nnz == 0
+    rp, col, val = J.arrays()
+    pb = orc.Problem(rp, col, val, S=J.sum)
+    dyn = sis.BSB if form == "bf16" else sis.DSB
+    A = 65
+    s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=A, form=sis.FORMS[form], dynamics=dyn, dt=0.4,
+                           xi0=0.5, split=-1))
+    X0, Y0 = seeded_state(n, A, 9)
+    s.set_state(X0, Y0)
+    s.autotune(2)
+    assert s.p.split == 0
+    s.set_state(X0, Y0)
+    info = s.run(300, 0)
+    prec = {"f32": "f32", "packed": "f32", "f64": "f64", "bf16": "bf16"}[form]
+    np_dt = np.float64 if prec == "f64" else np.float32
+    X, Y = X0.astype(np_dt), Y0.astype(np_dt)
+    r = orc.solve(pb, orc.MAXCUT, dyn, X, Y, 300, 0, P=100, dt=0.4, xi0=0.5, precision=prec)
+    assert info["steps_executed"] == r["steps"]
+    x, y = s.get_state()
+    np.testing.assert_array_equal(x, X.astype(np.float64))
+    np.testing.assert_array_equal(s.spins(), r["cur"])
+    cuts, best = s.cuts()
+    assert np.all(cuts == 0) and best == 0
+    assert np.all(s.energy() == 0)
+
+
+def test_single_node():
+    J = sis.Csr.from_coo(1, np.zeros(0, np.int64), np.zeros(0, np.int64), np.zeros(0), mirror=True)
+    s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=3, form=sis.FORMS["packed"], xi0=1.0))
+    s.init_random(1)
+    info = s.run(250, 1)
+    assert info["steps_executed"] == 300
+    cuts, best = s.cuts()
+    assert cuts.shape == (3,) and np.all(cuts == 0)
