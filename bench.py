@@ -201,6 +201,32 @@ def small_configs(a):
                               **{k: v for k, v in r.items() if k != "info"},
                               ms_per_step=r["wall"] * 1e3 / r["info"]["steps_executed"],
                               step_kernel_ms=r["info"]["step_kernel_ms"]))
+    elif a.workload == "tsp_struct":
+        # TSP bSB FP32 with the QUBO's CSR against the structured product (no
+        # matrix): n = 48 (the 10,000-step configuration), 128 and 256 cities.
+        for n, A, T in ((48, 1024, 10000), (128, 256, 1000), (256, 256, 1000)):
+            t = sis.Tsp.generate(n, 1000, 1, a.seed)
+            for structured in (1, 0):
+                params = dict(problem=sis.TSP, agents=A, dt=0.1, form=sis.FORM_F32,
+                              dynamics=sis.BSB, structured=structured)
+                t0 = time.perf_counter()
+                s = sis.Solver.from_tsp(t, params)
+                build = time.perf_counter() - t0
+                s.init_random(a.seed)
+                s.run(min(T, 200), 0)
+                s.init_random(a.seed)
+                t0 = time.perf_counter()
+                info = s.run(T, 0)
+                wall = time.perf_counter() - t0
+                lens, _ = s.tsp_lengths(t)
+                ok = lens[lens >= 0]
+                lines.append(dict(workload=f"tsp bSB f32, n={n} ({n * n} spins), {A} agents, {T} steps",
+                                  structured=structured, create_s=build, wall=wall,
+                                  ms_per_step=wall * 1e3 / info["steps_executed"],
+                                  agent_steps_per_s=info["steps_executed"] * A / wall,
+                                  valid=int(len(ok)), best=float(ok.min()) if len(ok) else None,
+                                  energy_min=float(s.energy(sis.SPINS_FINAL).min())))
+                s.close()
     elif a.workload == "powerlaw":
         # Chung-Lu N = 2^20, mean degree 8, exponent 2.1 (largest row ~1.2e5
         # entries), +-1 weights, 256 agents, dSB dt 0.5; without and with the
@@ -338,7 +364,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--workload", default="c3",
-                    choices=["c1", "c2", "c3", "c4", "powerlaw", "serving"])
+                    choices=["c1", "c2", "c3", "c4", "powerlaw", "tsp_struct", "serving"])
     ap.add_argument("--instances", type=int, default=4096)
     ap.add_argument("--chunk", type=int, default=100)
     ap.add_argument("--rate", type=float, default=0.0)
@@ -356,7 +382,7 @@ def main():
             a.steps = 5000
         serving(a)
         return
-    if a.workload in ("c1", "c2", "c4", "powerlaw"):
+    if a.workload in ("c1", "c2", "c4", "powerlaw", "tsp_struct"):
         small_configs(a)
         return
 

@@ -198,7 +198,9 @@ typedef struct sis_params {
                                sis_solver_autotune.  Bit-identical to the unsplit sum
                                when every partial sum is exact (dSB with integer
                                weights); otherwise within FP rounding of it. */
-  int32_t reserved;
+  int32_t structured;       /* TSP from sis_solver_create_tsp: 1 evaluates J x from row
+                               sums, column sums and the distance table instead of the
+                               QUBO's CSR (n >= 3; f32 n <= 256, f64 n <= 128) */
 } sis_params;
 
 typedef struct sis_run_info {
@@ -225,6 +227,17 @@ int sis_params_default(int problem, sis_params *p);
  * resolved here.  field may be NULL for MaxCut. */
 int sis_solver_create(const sis_csr *J, const double *field, const sis_params *p,
                       sis_solver **out);
+
+/* Replaces TSP_GPU(TSP_CPU&) + construct_matrix + CSR(full, n^2) + Ising_TSP
+ * (src/main.cu:229-240, src/tsp_gpu.cu:76-112, src/ising_tsp.cu:5-48) in one
+ * call: the solver for instance t, with p->ncities and the field set from t.
+ * p->structured = 0 builds the direct-to-CSR QUBO (sis_tsp_qubo); 1 keeps no
+ * matrix at all and takes J x from the QUBO's structure,
+ *   (J g)(i,k) = C/4 sum_j g(j,k) + B/4 sum_l g(i,l)
+ *              + sum_l (A d_kl / 8) (g(i-1,l) + g(i+1,l)),
+ * which is exact algebra but adds in another order than the CSR rows (dSB:
+ * identical results; bSB: within FP rounding).  Plain forms only. */
+int sis_solver_create_tsp(const sis_tsp *t, const sis_params *p, sis_solver **out);
 
 /* Replaces Ising::~Ising (src/ising.cu:303-339). */
 int sis_solver_destroy(sis_solver *s);

@@ -455,6 +455,182 @@ __global__ __launch_bounds__(kBlock) void k_step_split(StepArgs a, SplitArgs sp)
   split_finish<FORM, PROB>(a, e.row, tile, lane, G);
 }
 
+// ------------------------------------------------------- structured TSP
+//
+// (J g)(i,k) = C/4 colsum_k + B/4 rowsum_i + sum_l Dq_kl (g(i-1,l) + g(i+1,l)),
+// from the one-hot QUBO's entries (src/tsp_gpu.cu:85-106): same city at any
+// other position C/4, same position any other city B/4, itself (B+C)/4, any
+// other city at the two cyclically adjacent positions A d_kl / 8.  The
+// (B+C)/4 diagonal cancels the C/4 and B/4 self terms that colsum and rowsum
+// include; d_kk = 0.  Needs n >= 3 (two distinct adjacent positions).
+
+template <typename T, int GATHER>
+__device__ __forceinline__ T tsp_g(T x, T coupling) {
+  if (GATHER == G_DSB) return signbit(x) ? (T)-1 : (T)1;
+  return x * coupling;
+}
+
+// Row sums (position i) and column sums (city k) of g over the snapshot: one
+// wavefront per (sum, 64-agent tile), terms in index order.
+template <typename T, int GATHER>
+__global__ __launch_bounds__(kBlock) void k_tsp_sums(StepArgs a, TspArgs ts) {
+  if (a.ctrl->done) return;
+  const int64_t n = ts.n, lda = a.lda, tiles = lda / 64;
+  const int64_t g = wave_index();
+  if (g >= 2 * n * tiles) return;
+  const int lane = threadIdx.x & 63;
+  const int64_t which = g / (n * tiles), rem = g - which * n * tiles;
+  const int64_t idx = rem / tiles, a0 = (rem - idx * tiles) * 64 + lane;
+  const int64_t base = which == 0 ? idx * n : idx, stride = which == 0 ? 1 : n;
+  const T *__restrict__ Xs = (const T *)a.gsrc;
+  const T c = (T)a.coupling;
+  T acc = (T)0;
+  int64_t m = 0;
+  for (; m + 4 <= n; m += 4) {
+    const T x0 = Xs[(base + m * stride) * lda + a0];
+    const T x1 = Xs[(base + (m + 1) * stride) * lda + a0];
+    const T x2 = Xs[(base + (m + 2) * stride) * lda + a0];
+    const T x3 = Xs[(base + (m + 3) * stride) * lda + a0];
+    acc = acc + tsp_g<T, GATHER>(x0, c);
+    acc = acc + tsp_g<T, GATHER>(x1, c);
+    acc = acc + tsp_g<T, GATHER>(x2, c);
+    acc = acc + tsp_g<T, GATHER>(x3, c);
+  }
+  for (; m < n; m++) acc = acc + tsp_g<T, GATHER>(Xs[(base + m * stride) * lda + a0], c);
+  ((T *)(which == 0 ? ts.rsum : ts.csum))[idx * lda + a0] = acc;
+}
+
+// One workgroup per (position i, 64-agent tile): v_l = g(i-1,l) + g(i+1,l) is
+// staged in LDS (n x 64), then wavefront w updates cities w, w + 4, ... with
+// F = -(C/4 colsum_k + B/4 rowsum_i + sum_l Dq_kl v_l) and the TSP element
+// update of the row kernels.
+template <typename T, int GATHER>
+__global__ __launch_bounds__(kBlock) void k_tsp_step_struct(StepArgs a, TspArgs ts) {
+  extern __shared__ __align__(16) unsigned char s_raw[];
+  T *v = reinterpret_cast<T *>(s_raw);
+  if (a.ctrl->done) return;
+  const int64_t n = ts.n, lda = a.lda, tiles = lda / 64;
+  const int64_t i = blockIdx.x / tiles, tile = blockIdx.x - i * tiles;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int64_t a0 = tile * 64 + lane;
+  const T *__restrict__ Xs = (const T *)a.gsrc;
+  const T c = (T)a.coupling;
+  const int64_t im = (i + n - 1) % n, ip = (i + 1) % n;
+  for (int64_t l = w; l < n; l += kWaves)
+    v[l * 64 + lane] = tsp_g<T, GATHER>(Xs[(im * n + l) * lda + a0], c) +
+                       tsp_g<T, GATHER>(Xs[(ip * n + l) * lda + a0], c);
+  __syncthreads();
+  const int64_t t = a.ctrl->base + a.offset;
+  const T *sc = (const T *)a.sched + 4 * t;
+  const T c0 = sc[0], c1 = sc[1], c2 = sc[2], c3 = sc[3];
+  const T qB = (T)ts.qB, qC = (T)ts.qC;
+  const T rs = ((const T *)ts.rsum)[i * lda + a0];
+  const T *__restrict__ cs = (const T *)ts.csum;
+  const T *__restrict__ dq = (const T *)ts.dq;
+  const T *__restrict__ h = (const T *)a.field;
+  T *Y = (T *)a.Y;
+  for (int64_t k = w; k < n; k += kWaves) {
+    const T *dk = dq + k * n;
+    T acc = fmaT(qB, rs, qC * cs[k * lda + a0]);
+    int64_t l = 0;
+    for (; l + 4 <= n; l += 4) {
+      acc = fmaT(dk[l], v[l * 64 + lane], acc);
+      acc = fmaT(dk[l + 1], v[(l + 1) * 64 + lane], acc);
+      acc = fmaT(dk[l + 2], v[(l + 2) * 64 + lane], acc);
+      acc = fmaT(dk[l + 3], v[(l + 3) * 64 + lane], acc);
+    }
+    for (; l < n; l++) acc = fmaT(dk[l], v[l * 64 + lane], acc);
+    const int64_t e = (i * n + k) * lda + a0;
+    T x = Xs[e], y = Y[e];
+    sb_update<T, PROB_TSP>(x, y, -acc, c0, c1, c2, c3, h[i * n + k]);
+    ((T *)a.gdst)[e] = x;
+    Y[e] = y;
+  }
+}
+
+__device__ __forceinline__ double spin_of(const uint64_t *spins, int64_t e, int64_t W, int64_t a) {
+  return ((spins[e * W + (a >> 6)] >> (a & 63)) & 1) ? -1.0 : 1.0;
+}
+
+// Row / column sums of the spins (FP64, exact): as k_tsp_sums.
+__global__ __launch_bounds__(kBlock) void k_tsp_spin_sums(const uint64_t *spins, double *rsum,
+                                                          double *csum, int64_t n, int64_t W,
+                                                          int64_t lda) {
+  const int64_t tiles = lda / 64;
+  const int64_t g = wave_index();
+  if (g >= 2 * n * tiles) return;
+  const int lane = threadIdx.x & 63;
+  const int64_t which = g / (n * tiles), rem = g - which * n * tiles;
+  const int64_t idx = rem / tiles, a0 = (rem - idx * tiles) * 64 + lane;
+  double acc = 0.0;
+  for (int64_t m = 0; m < n; m++)
+    acc += spin_of(spins, which == 0 ? idx * n + m : m * n + idx, W, a0);
+  (which == 0 ? rsum : csum)[idx * lda + a0] = acc;
+}
+
+// Per-position energy partials: part[i][a] = sum_k s_ik ((J s)_ik - h_ik).
+// Every term is a multiple of 1/8 of an integer, so the sum is exact.
+__global__ __launch_bounds__(kBlock) void k_tsp_energy_struct(const uint64_t *spins,
+                                                              const double *field, TspArgs ts,
+                                                              int64_t W, int64_t lda,
+                                                              double *part) {
+  __shared__ double red[kWaves * 64];
+  const int64_t n = ts.n, tiles = lda / 64;
+  const int64_t i = blockIdx.x / tiles, tile = blockIdx.x - i * tiles;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int64_t a0 = tile * 64 + lane;
+  const int64_t im = (i + n - 1) % n, ip = (i + 1) % n;
+  const double *rsum = (const double *)ts.rsum, *csum = (const double *)ts.csum;
+  const double *dq = (const double *)ts.dq;
+  const double rs = rsum[i * lda + a0];
+  double e = 0.0;
+  for (int64_t k = w; k < n; k += kWaves) {
+    double acc = ts.qC * csum[k * lda + a0] + ts.qB * rs;
+    for (int64_t l = 0; l < n; l++)
+      acc += dq[k * n + l] * (spin_of(spins, im * n + l, W, a0) + spin_of(spins, ip * n + l, W, a0));
+    const double s = spin_of(spins, i * n + k, W, a0);
+    e += s * acc - field[i * n + k] * s;
+  }
+  red[w * 64 + lane] = e;
+  __syncthreads();
+  if (w == 0)
+    part[i * lda + a0] = ((red[lane] + red[64 + lane]) + red[128 + lane]) + red[192 + lane];
+}
+
+bool tsp_struct_supported(int form, int gather) {
+  return (form == 0 || form == 1) && (gather == G_DSB || gather == G_BSB);
+}
+
+void launch_tsp_struct_step(hipStream_t st, int form, int gather, const StepArgs &a,
+                            const TspArgs &ts) {
+  const int64_t tiles = a.lda / 64;
+  const unsigned sums_grid = grid_for(2 * ts.n * tiles, kWaves, INT32_MAX);
+  const dim3 grid((unsigned)(ts.n * tiles));
+#define SIS_TSPS(T, G)                                                                       \
+  if (form == (sizeof(T) == 8 ? 1 : 0) && gather == G) {                                    \
+    hipLaunchKernelGGL((k_tsp_sums<T, G>), dim3(sums_grid), dim3(kBlock), 0, st, a, ts);    \
+    hipLaunchKernelGGL((k_tsp_step_struct<T, G>), grid, dim3(kBlock),                       \
+                       (size_t)(ts.n * 64 * sizeof(T)), st, a, ts);                         \
+    return;                                                                                 \
+  }
+  SIS_TSPS(float, G_BSB)
+  SIS_TSPS(float, G_DSB)
+  SIS_TSPS(double, G_BSB)
+  SIS_TSPS(double, G_DSB)
+#undef SIS_TSPS
+}
+
+void launch_tsp_struct_energy(hipStream_t st, const uint64_t *spins, const double *field,
+                              const TspArgs &ts, int64_t W, int64_t lda, double *part) {
+  const int64_t tiles = lda / 64;
+  hipLaunchKernelGGL(k_tsp_spin_sums, dim3(grid_for(2 * ts.n * tiles, kWaves, INT32_MAX)),
+                     dim3(kBlock), 0, st, spins, (double *)ts.rsum, (double *)ts.csum, ts.n, W, lda);
+  hipLaunchKernelGGL(k_tsp_energy_struct, dim3((unsigned)(ts.n * tiles)), dim3(kBlock), 0, st,
+                     spins, field, ts, W, lda, part);
+}
+
 // -------------------------------------------------------------- windows
 
 // Sign scan of the window (src/kernels.cu:37-44, 84-90), race-free: each

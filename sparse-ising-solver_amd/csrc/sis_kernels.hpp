@@ -88,6 +88,26 @@ struct SplitArgs {
   int64_t chunk;        // entries per wavefront
 };
 
+// Structured TSP product (n >= 3): with g the gathered value of spin
+// (position i, city k), the one-hot QUBO gives
+//   (J g)(i,k) = C/4 colsum_k + B/4 rowsum_i + sum_l Dq_kl (g(i-1,l) + g(i+1,l))
+// with Dq_kl = A d_kl / 8 (the (B+C)/4 diagonal cancels the i = j and k = l
+// terms), so no CSR is needed.  rsum / csum are n x lda per step.
+struct TspArgs {
+  const void *dq;  // n x n, A d_kl / 8, form precision
+  void *rsum, *csum;
+  double qB, qC;   // B/4, C/4
+  int64_t n;
+};
+
+bool tsp_struct_supported(int form, int gather);
+void launch_tsp_struct_step(hipStream_t st, int form, int gather, const StepArgs &a,
+                            const TspArgs &ts);
+// Energy s^T J s - h.s of bit-packed spins with the structured product, FP64,
+// as per-position partials part[i][agent] (reduced by launch_cut_reduce).
+void launch_tsp_struct_energy(hipStream_t st, const uint64_t *spins, const double *field,
+                              const TspArgs &ts, int64_t W, int64_t lda, double *part);
+
 bool split_supported(int form, int prob, int gather);
 void launch_step_split(hipStream_t st, int form, int prob, int gather, int64_t nent,
                        const StepArgs &a, const SplitArgs &sp);

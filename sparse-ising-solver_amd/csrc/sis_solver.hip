@@ -69,6 +69,26 @@ struct sis_solver {
   DevBuf<char> split_part;
   DevBuf<int32_t> split_cnt;
 
+  // Structured TSP product (sis_solver_create_tsp with structured = 1).
+  bool structured = false;
+  int64_t tsp_n = 0;
+  double qB = 0.0, qC = 0.0;           // B/4, C/4
+  DevBuf<double> dqd;                  // A d_kl / 8, n x n
+  DevBuf<float> dqf;
+  DevBuf<char> tsum_r, tsum_c;         // per-step row / column sums, form precision
+  DevBuf<double> esum_r, esum_c;       // the same for the energy, FP64
+
+  TspArgs tsp_args(bool energy) {
+    TspArgs ts{};
+    ts.dq = energy || f64 ? (const void *)dqd.p : (const void *)dqf.p;
+    ts.rsum = energy ? (void *)esum_r.p : (void *)tsum_r.p;
+    ts.csum = energy ? (void *)esum_c.p : (void *)tsum_c.p;
+    ts.qB = qB;
+    ts.qC = qC;
+    ts.n = tsp_n;
+    return ts;
+  }
+
   std::vector<hipEvent_t> events;
   hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
   int64_t graph_P = -1;
@@ -249,6 +269,8 @@ int sis_solver_create(const sis_csr *J, const double *field, const sis_params *p
     return fail(SIS_ERR_ARG, "sis_solver_create: agents must be in [1, 262144]");
   if (!(p.dt == p.dt)) return fail(SIS_ERR_ARG, "sis_solver_create: dt is NaN");
   if (p.split < -1) return fail(SIS_ERR_ARG, "sis_solver_create: split must be >= -1");
+  if (p.structured)
+    return fail(SIS_ERR_ARG, "sis_solver_create: structured TSP solvers come from sis_solver_create_tsp");
   if (p.window_threshold < 1) p.window_threshold = p.problem == SIS_MAXCUT ? 25 : 50;
   s->N = J->n;
   s->Z = (int64_t)J->col.size();
@@ -347,6 +369,68 @@ int sis_solver_create(const sis_csr *J, const double *field, const sis_params *p
       (st = s->outv.alloc((size_t)s->lda)) || (st = s->best.alloc(1)))
     return st;
   if (p.split > 0 && (st = build_split(s.get(), p.split))) return st;
+  *out = s.release();
+  return SIS_OK;
+}
+
+int sis_solver_create_tsp(const sis_tsp *t, const sis_params *pin, sis_solver **out) {
+  if (!t || !pin || !out) return fail(SIS_ERR_ARG, "sis_solver_create_tsp: null argument");
+  *out = nullptr;
+  sis_params p = *pin;
+  p.problem = SIS_TSP;
+  p.ncities = (int32_t)t->n;
+  const int64_t n = t->n, N = n * n;
+  const bool structured = p.structured != 0;
+  p.structured = 0;
+  std::vector<double> h((size_t)N);
+  if (!structured) {
+    sis_csr *J = nullptr;
+    int st = sis_tsp_qubo(t, &J, h.data());
+    if (st) return st;
+    st = sis_solver_create(J, h.data(), &p, out);
+    sis_csr_free(J);
+    return st;
+  }
+  if (n < 3) return fail(SIS_ERR_ARG, "sis_solver_create_tsp: the structured product needs n >= 3");
+  if (p.form != SIS_FORM_F32 && p.form != SIS_FORM_F64)
+    return fail(SIS_ERR_ARG, "sis_solver_create_tsp: the structured product runs the plain forms");
+  if (n > (p.form == SIS_FORM_F32 ? 256 : 128))
+    return fail(SIS_ERR_ARG, "sis_solver_create_tsp: structured n must be <= 256 (f32) / 128 (f64)");
+  if (p.split != 0) return fail(SIS_ERR_ARG, "sis_solver_create_tsp: no row split without a matrix");
+  // the QUBO's constants and field (src/tsp_gpu.cu:81-83, src/kernels.cu:591-592)
+  const double A = 1, B = t->maxd, C = t->maxd;
+  for (int64_t i = 0; i < n; i++)
+    for (int64_t k = 0; k < n; k++)
+      h[(size_t)(i * n + k)] = -1 * (A * t->totals[(size_t)k] + (double)(n - 2) * (B + C)) / 2;
+  sis_csr E;  // no stored entry: the product is structured
+  E.n = N;
+  E.rp.assign((size_t)N + 1, 0);
+  E.sum = 0.0;
+  sis_solver *sp = nullptr;
+  int st = sis_solver_create(&E, h.data(), &p, &sp);
+  if (st) return st;
+  std::unique_ptr<sis_solver> s(sp);
+  s->structured = true;
+  s->p.structured = 1;
+  s->tsp_n = n;
+  s->qB = B / 4;
+  s->qC = C / 4;
+  std::vector<double> dq((size_t)(n * n), 0.0);
+  for (int64_t k = 0; k < n; k++)
+    for (int64_t l = 0; l < n; l++)
+      if (k != l)
+        dq[(size_t)(k * n + l)] = A * t->dist[(size_t)tri(std::min(k, l), std::max(k, l), n)] / 8;
+  const size_t esz = s->f64 ? sizeof(double) : sizeof(float);
+  if ((st = s->dqd.alloc(dq.size())) || (st = s->tsum_r.alloc((size_t)(n * s->lda) * esz)) ||
+      (st = s->tsum_c.alloc((size_t)(n * s->lda) * esz)) ||
+      (st = s->esum_r.alloc((size_t)(n * s->lda))) || (st = s->esum_c.alloc((size_t)(n * s->lda))))
+    return st;
+  HIPCHK(hipMemcpy(s->dqd.p, dq.data(), sizeof(double) * dq.size(), hipMemcpyHostToDevice));
+  if (!s->f64) {
+    std::vector<float> dqf(dq.begin(), dq.end());
+    if ((st = s->dqf.alloc(dqf.size()))) return st;
+    HIPCHK(hipMemcpy(s->dqf.p, dqf.data(), sizeof(float) * dqf.size(), hipMemcpyHostToDevice));
+  }
   *out = s.release();
   return SIS_OK;
 }
@@ -478,6 +562,11 @@ static void enqueue_step(sis_solver *s, int32_t off) {
   }
   if (s->prob == PROB_UNFUSED)
     launch_xy_update(s->stream, s->f64, s->Xbuf(cur), a.Y, s->N * s->lda, a.sched, a.ctrl, off);
+  if (s->structured) {
+    launch_tsp_struct_step(s->stream, s->p.form, s->gather, a, s->tsp_args(false));
+    s->par = nxt;
+    return;
+  }
   a.split = s->split_nent > 0 ? s->split : 0;
   if (a.split) {
     SplitArgs sp{s->split_ent.p, s->split_part.p, s->split_cnt.p, s->split};
@@ -876,6 +965,18 @@ int sis_solver_maxcut_cuts(sis_solver *s, double *cut, int64_t *best_agent) {
 int sis_solver_energy(sis_solver *s, int which, double *energy) {
   if (!s || !energy || (which != SIS_SPINS_CURRENT && which != SIS_SPINS_FINAL))
     return fail(SIS_ERR_ARG, "sis_solver_energy: bad argument");
+  if (s->structured) {
+    // per-position partials of s^T J s - h.s, reduced in position order
+    if ((int64_t)s->cutpart.n < s->tsp_n * s->lda)
+      return fail(SIS_ERR_STATE, "sis_solver_energy: partial buffer too small");
+    launch_tsp_struct_energy(s->stream, which == SIS_SPINS_CURRENT ? s->cur.p : s->fin.p, s->hd.p,
+                             s->tsp_args(true), s->W, s->lda, s->cutpart.p);
+    launch_cut_reduce(s->stream, s->cutpart.p, s->tsp_n, s->A, s->lda, 1, 0.0, s->outv.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));
+    HIPCHK(hipMemcpy(energy, s->outv.p, sizeof(double) * s->A, hipMemcpyDeviceToHost));
+    return SIS_OK;
+  }
   if (!s->has_field) {
     // Without a field: s^T J s = 2 H, from the cut-mode row sums.
     std::vector<double> c((size_t)s->A);

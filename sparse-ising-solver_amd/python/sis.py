@@ -32,7 +32,7 @@ class Params(C.Structure):
         ("coupling", C.c_double), ("window_period", C.c_int32),
         ("window_threshold", C.c_int32), ("ncities", C.c_int32),
         ("use_graph", C.c_int32), ("tile", C.c_int32), ("fused", C.c_int32),
-        ("split", C.c_int32), ("reserved", C.c_int32),
+        ("split", C.c_int32), ("structured", C.c_int32),
     ]
 
 
@@ -103,6 +103,7 @@ _SIGS = {
     "sis_gen_tsp": (None, [C.c_int64, C.c_int64, C.c_int, C.c_uint64, C.POINTER(_P)]),
     "sis_params_default": (None, [C.c_int, C.POINTER(Params)]),
     "sis_solver_create": (None, [_P, _DP, C.POINTER(Params), C.POINTER(_P)]),
+    "sis_solver_create_tsp": (None, [_P, C.POINTER(Params), C.POINTER(_P)]),
     "sis_solver_destroy": (None, [_P]),
     "sis_solver_params": (None, [_P, C.POINTER(Params)]),
     "sis_solver_init_random": (None, [_P, C.c_uint64]),
@@ -321,24 +322,44 @@ def default_params(problem):
     return p
 
 
+def _params(params):
+    if isinstance(params, dict):
+        p = default_params(params.get("problem", MAXCUT))
+        for k, v in params.items():
+            setattr(p, k, v)
+        return p
+    return params
+
+
 class Solver:
     """A device solver.  ``params`` is a Params or a dict of Params fields."""
 
     def __init__(self, J, params, field=None):
-        if isinstance(params, dict):
-            p = default_params(params.get("problem", MAXCUT))
-            for k, v in params.items():
-                setattr(p, k, v)
-            params = p
+        params = _params(params)
         self.J = J
         self.field = None if field is None else np.ascontiguousarray(field, dtype=np.float64)
         h = C.c_void_p()
         _ck(lib().sis_solver_create(J.h, _ptr(self.field, C.c_double), C.byref(params),
                                     C.byref(h)), "sis_solver_create")
+        self._attach(h, J.n)
+
+    @classmethod
+    def from_tsp(cls, tsp, params):
+        """Solver for a TSP instance (sis_solver_create_tsp); params may set
+        structured=1 to take J x from the QUBO's structure (no matrix)."""
+        params = _params(dict(params, problem=TSP) if isinstance(params, dict) else params)
+        h = C.c_void_p()
+        _ck(lib().sis_solver_create_tsp(tsp.h, C.byref(params), C.byref(h)), "sis_solver_create_tsp")
+        self = cls.__new__(cls)
+        self.J, self.field, self.tsp = None, None, tsp
+        self._attach(h, tsp.n * tsp.n)
+        return self
+
+    def _attach(self, h, N):
         self.h = h
         self.p = Params()
         _ck(lib().sis_solver_params(self.h, C.byref(self.p)), "sis_solver_params")
-        self.N, self.A = J.n, self.p.agents
+        self.N, self.A = N, self.p.agents
 
     def init_random(self, seed):
         _ck(lib().sis_solver_init_random(self.h, seed), "sis_solver_init_random")

@@ -1,0 +1,147 @@
+"""GPU parity of the structured TSP product (sis_solver_create_tsp with
+structured = 1): J x from row sums, column sums and the distance table instead
+of the one-hot QUBO's CSR, against the oracle, which evaluates the CSR.
+
+Bars:
+- dSB: every term is a multiple of 1/8 of an integer, so both the CSR and the
+  structured sums are exact and the trajectories are bit-identical.
+- bSB: the two evaluate the same algebra in a different order.
+  - One step from the same state is within 1e-6 relative (FP64: 1e-15).
+  - 1000 steps of a 20000-step ramp are within 1e-5, or within twice the
+    distance between the oracle in the same precision and the FP64 oracle
+    where the dynamics amplify rounding beyond that.
+- The energy s^T J s - h.s of the returned spins is exact.
+"""
+import numpy as np
+import pytest
+
+import oracle as orc
+import sis
+
+pytestmark = pytest.mark.gpu
+
+
+def instance(n, seed=3, span=100):
+    t = sis.Tsp.generate(n, span, 0, seed)
+    J, h = t.qubo()
+    rp, col, val = J.arrays()
+    return t, J, h, orc.Problem(rp, col, val, field=h, ncities=n)
+
+
+def seeded_state(n, A, seed):
+    rng = np.random.default_rng(seed)
+    X = rng.uniform(-0.1, 0.1, (n, A))  # X != 0 so the first step's gathers differ
+    Y = rng.uniform(-0.1, 0.1, (n, A))
+    return X.astype(np.float32).astype(np.float64), Y.astype(np.float32).astype(np.float64)
+
+
+def max_rel(x, X):
+    return float(np.abs(x - X).max() / max(np.abs(X).max(), 1e-30))
+
+
+@pytest.mark.parametrize("n", [3, 7, 12])
+@pytest.mark.parametrize("form,prec", [("f32", "f32"), ("f64", "f64")])
+def test_structured_dsb_bit_exact(n, form, prec):
+    t, J, h, pb = instance(n)
+    A = 96
+    X0, Y0 = seeded_state(n * n, A, 40 + n)
+    T, dt = 800, 0.1
+    s = sis.Solver.from_tsp(t, dict(agents=A, form=sis.FORMS[form], dynamics=sis.DSB, dt=dt,
+                                    structured=1))
+    assert s.p.structured == 1 and s.p.ncities == n
+    s.set_state(X0, Y0)
+    np_dt = np.float64 if prec == "f64" else np.float32
+    X, Y = X0.astype(np_dt), Y0.astype(np_dt)
+    for t0, k in ((0, 300), (300, 500)):
+        s.advance(t0, k, T)
+        orc.advance(pb, orc.TSP, orc.DSB, X, Y, t0, k, T, dt, 0.002, 1.005, prec)
+        x, y = s.get_state()
+        np.testing.assert_array_equal(x, X.astype(np.float64), err_msg=f"X after {t0 + k}")
+        np.testing.assert_array_equal(y, Y.astype(np.float64), err_msg=f"Y after {t0 + k}")
+
+
+@pytest.mark.parametrize("form,prec,bar1,bar", [("f32", "f32", 1e-6, 1e-5), ("f64", "f64", 1e-15, 1e-12)])
+@pytest.mark.parametrize("n", [7, 16])
+def test_structured_bsb_within_bound(form, prec, bar1, bar, n):
+    t, J, h, pb = instance(n, seed=5)
+    A = 128
+    X0, Y0 = seeded_state(n * n, A, 50 + n)
+    dt, T = 0.1, 20000
+    s = sis.Solver.from_tsp(t, dict(agents=A, form=sis.FORMS[form], dt=dt, structured=1))
+    s.set_state(X0, Y0)
+    np_dt = np.float64 if prec == "f64" else np.float32
+    X, Y = X0.astype(np_dt), Y0.astype(np_dt)
+    X64, Y64 = X0.copy(), Y0.copy()
+    errs, t0 = [], 0
+    for t1 in (1, 10, 100, 1000):
+        s.advance(t0, t1 - t0, T)
+        orc.advance(pb, orc.TSP, orc.BSB, X, Y, t0, t1 - t0, T, dt, 0.002, 1.005, prec)
+        orc.advance(pb, orc.TSP, orc.BSB, X64, Y64, t0, t1 - t0, T, dt, 0.002, 1.005, "f64")
+        x, y = s.get_state()
+        Xd, Yd = X.astype(np.float64), Y.astype(np.float64)
+        errs.append((t1, max(max_rel(x, Xd), max_rel(y, Yd)), max(max_rel(Xd, X64), max_rel(Yd, Y64))))
+        t0 = t1
+    print(f"structured bSB n={n} {form}: (step, vs oracle {prec}, oracle {prec} vs f64) {errs}")
+    assert errs[0][1] <= bar1, errs
+    for t1, e, pe in errs[1:]:
+        assert e <= max(bar, 2.0 * pe), (t1, e, pe)
+
+
+@pytest.mark.parametrize("dyn", [sis.DSB, sis.BSB])
+def test_structured_full_run_and_energy(dyn):
+    n = 6
+    t, J, h, pb = instance(n, seed=11)
+    A, T, dt = 256, 3000, 0.1
+    s = sis.Solver.from_tsp(t, dict(agents=A, form=sis.FORM_F32, dynamics=dyn, dt=dt, structured=1))
+    s.init_random(8)
+    info = s.run(T, 0)
+    fin = s.spins(sis.SPINS_FINAL)
+    cur = s.spins(sis.SPINS_CURRENT)
+    # exact energies of whatever spins came back
+    for which, sp in ((sis.SPINS_FINAL, fin), (sis.SPINS_CURRENT, cur)):
+        assert np.array_equal(s.energy(which), orc.energy_field(pb, sp))
+    lens, best = s.tsp_lengths(t)
+    _, _, d, _ = t.arrays()
+    st, lk, valid = s.agent_state()
+    for a in range(A):
+        if valid[a] > 0:
+            assert lens[a] == orc.tsp_tour_length(n, d, fin, a)[0]
+        else:
+            assert lens[a] == -1
+    if dyn == sis.DSB:  # bit-identical to the oracle's CSR evaluation
+        X, Y = orc.init_state(8, orc.TSP, n * n, A)
+        r = orc.solve(pb, orc.TSP, orc.DSB, X, Y, T, 0, P=T // 10, K=50, dt=dt, xi0=0.002,
+                      coupling=1.005)
+        assert info["steps_executed"] == r["steps"]
+        assert np.array_equal(fin, r["fin"]) and np.array_equal(valid, r["valid"])
+
+
+def test_create_tsp_matrix_mode_equals_create():
+    n = 8
+    t, J, h, pb = instance(n, seed=2)
+    A = 64
+    a = sis.Solver.from_tsp(t, dict(agents=A, form=sis.FORM_F32, dt=0.1))
+    b = sis.Solver(J, dict(problem=sis.TSP, agents=A, form=sis.FORM_F32, dt=0.1, ncities=n), field=h)
+    assert a.p.structured == 0
+    for s in (a, b):
+        s.init_random(4)
+        s.run(1000, 0)
+    assert np.array_equal(a.get_state()[0], b.get_state()[0])
+    assert np.array_equal(a.energy(sis.SPINS_FINAL), b.energy(sis.SPINS_FINAL))
+
+
+def test_structured_argument_errors():
+    t2 = sis.Tsp.generate(2, 50, 0, 1)
+    with pytest.raises(sis.SisError):
+        sis.Solver.from_tsp(t2, dict(agents=64, structured=1))
+    t = sis.Tsp.generate(8, 50, 0, 1)
+    with pytest.raises(sis.SisError):
+        sis.Solver.from_tsp(t, dict(agents=64, form=sis.FORM_BF16, structured=1))
+    with pytest.raises(sis.SisError):
+        sis.Solver.from_tsp(t, dict(agents=64, split=16, structured=1))
+    big = sis.Tsp.generate(129, 500, 0, 1)
+    with pytest.raises(sis.SisError):
+        sis.Solver.from_tsp(big, dict(agents=64, form=sis.FORM_F64, structured=1))
+    J, h = t.qubo()
+    with pytest.raises(sis.SisError):
+        sis.Solver(J, dict(problem=sis.TSP, agents=64, ncities=8, structured=1), field=h)

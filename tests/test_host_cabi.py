@@ -172,3 +172,27 @@ def test_tour_length_and_random_tour():
     assert t.random_length(7) in (14.0, 16.0, 18.0) and t.random_length(7) == t.random_length(7)
     with pytest.raises(sis.SisError):
         t.tour_length([0, 0, 1, 2])
+
+
+@pytest.mark.parametrize("n,seed", [(3, 1), (4, 2), (7, 3), (12, 4)])
+def test_tsp_structured_identity(n, seed):
+    """The structured product used by sis_solver_create_tsp(structured=1):
+    (J g)(i,k) = C/4 sum_j g(j,k) + B/4 sum_l g(i,l) + sum_l d_kl/8 (g(i-1,l) + g(i+1,l))
+    equals the direct-to-CSR QUBO times g (n >= 3)."""
+    t = sis.Tsp.generate(n, 100, 0, seed)
+    J, h = t.qubo()
+    rp, col, val = J.arrays()
+    M = sp.csr_matrix((val, col, rp), shape=(n * n, n * n))
+    _, _, d, _ = t.arrays()
+    D = np.zeros((n, n))
+    iu = np.triu_indices(n, 1)
+    D[iu] = d
+    D = D + D.T
+    B = C = t.maxd
+    rng = np.random.default_rng(seed)
+    for _ in range(3):
+        g = rng.uniform(-1, 1, n * n)
+        G = g.reshape(n, n)  # G[i, k]: position i, city k
+        v = np.roll(G, 1, axis=0) + np.roll(G, -1, axis=0)  # g(i-1, .) + g(i+1, .)
+        S = C / 4 * G.sum(axis=0)[None, :] + B / 4 * G.sum(axis=1)[:, None] + (v @ D.T) / 8
+        np.testing.assert_allclose(S.ravel(), M @ g, rtol=1e-12, atol=1e-9)
