@@ -549,6 +549,71 @@ __global__ __launch_bounds__(kBlock) void k_tsp_step_struct(StepArgs a, TspArgs 
   }
 }
 
+// FP32 form on the matrix cores: per (position i, 64-agent tile) the product
+// sum_l Dq_kl v_l is the (n x n) . (n x 64) GEMM Dq V.  Wavefront w takes the
+// 32 x 32 output tiles (cities kt*32.., agents at*32..) w, w + 4, ...; each
+// v_mfma_f32_32x32x2f32 adds two l columns: A[row][kk] = Dq[kt*32+row][l0+kk]
+// in lane row + 32 kk, B[kk][col] = v[l0+kk][at*32+col] in lane col + 32 kk,
+// and C[row][col] sits in lane col + 32 ((row / 4) % 2), register
+// 4 (row / 8) + row % 4.  K is padded to even with zeros.
+typedef float sis_floatx16 __attribute__((ext_vector_type(16)));
+
+template <int GATHER>
+__global__ __launch_bounds__(kBlock) void k_tsp_step_mfma(StepArgs a, TspArgs ts) {
+  extern __shared__ __align__(16) unsigned char s_raw[];
+  float *v = reinterpret_cast<float *>(s_raw);  // (n rounded up to even) x 64
+  if (a.ctrl->done) return;
+  const int64_t n = ts.n, lda = a.lda, tiles = lda / 64;
+  const int64_t ne = (n + 1) & ~(int64_t)1;
+  const int64_t i = blockIdx.x / tiles, tile = blockIdx.x - i * tiles;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const float *__restrict__ Xs = (const float *)a.gsrc;
+  const float c = (float)a.coupling;
+  const int64_t im = (i + n - 1) % n, ip = (i + 1) % n;
+  for (int64_t l = w; l < ne; l += kWaves)
+    v[l * 64 + lane] = l < n ? tsp_g<float, GATHER>(Xs[(im * n + l) * lda + tile * 64 + lane], c) +
+                                   tsp_g<float, GATHER>(Xs[(ip * n + l) * lda + tile * 64 + lane], c)
+                             : 0.0f;
+  __syncthreads();
+  const int64_t t = a.ctrl->base + a.offset;
+  const float *sc = (const float *)a.sched + 4 * t;
+  const float c0 = sc[0], c1 = sc[1], c2 = sc[2], c3 = sc[3];
+  const float qB = (float)ts.qB, qC = (float)ts.qC;
+  const float *__restrict__ rsum = (const float *)ts.rsum;
+  const float *__restrict__ cs = (const float *)ts.csum;
+  const float *__restrict__ dq = (const float *)ts.dq;
+  const float *__restrict__ h = (const float *)a.field;
+  float *Y = (float *)a.Y;
+  const int half = lane >> 5, col = lane & 31;
+  const int64_t ntile = ((n + 31) / 32) * 2;
+  for (int64_t tt = w; tt < ntile; tt += kWaves) {
+    const int64_t kt = tt >> 1, at = tt & 1;
+    const int64_t krow = kt * 32 + col;
+    sis_floatx16 acc = {0};
+    for (int64_t l0 = 0; l0 < ne; l0 += 2) {
+      const int64_t l = l0 + half;
+      const float av = (krow < n && l < n) ? dq[krow * n + l] : 0.0f;
+      const float bv = v[l * 64 + at * 32 + col];
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
+    }
+    const int64_t ag = tile * 64 + at * 32 + col;
+    const float rs = rsum[i * lda + ag];
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int64_t k = kt * 32 + 8 * (q >> 2) + 4 * half + (q & 3);
+      if (k < n) {
+        const float S = fmaT(qB, rs, qC * cs[k * lda + ag]) + acc[q];
+        const int64_t e = (i * n + k) * lda + ag;
+        float x = Xs[e], y = Y[e];
+        sb_update<float, PROB_TSP>(x, y, -S, c0, c1, c2, c3, h[i * n + k]);
+        ((float *)a.gdst)[e] = x;
+        Y[e] = y;
+      }
+    }
+  }
+}
+
 __device__ __forceinline__ double spin_of(const uint64_t *spins, int64_t e, int64_t W, int64_t a) {
   return ((spins[e * W + (a >> 6)] >> (a & 63)) & 1) ? -1.0 : 1.0;
 }
@@ -603,11 +668,22 @@ bool tsp_struct_supported(int form, int gather) {
   return (form == 0 || form == 1) && (gather == G_DSB || gather == G_BSB);
 }
 
-void launch_tsp_struct_step(hipStream_t st, int form, int gather, const StepArgs &a,
+void launch_tsp_struct_step(hipStream_t st, int form, int gather, bool mfma, const StepArgs &a,
                             const TspArgs &ts) {
   const int64_t tiles = a.lda / 64;
   const unsigned sums_grid = grid_for(2 * ts.n * tiles, kWaves, INT32_MAX);
   const dim3 grid((unsigned)(ts.n * tiles));
+  if (mfma && form == 0) {
+    const size_t shm = (size_t)(((ts.n + 1) & ~(int64_t)1) * 64 * sizeof(float));
+    if (gather == G_BSB) {
+      hipLaunchKernelGGL((k_tsp_sums<float, G_BSB>), dim3(sums_grid), dim3(kBlock), 0, st, a, ts);
+      hipLaunchKernelGGL((k_tsp_step_mfma<G_BSB>), grid, dim3(kBlock), shm, st, a, ts);
+    } else {
+      hipLaunchKernelGGL((k_tsp_sums<float, G_DSB>), dim3(sums_grid), dim3(kBlock), 0, st, a, ts);
+      hipLaunchKernelGGL((k_tsp_step_mfma<G_DSB>), grid, dim3(kBlock), shm, st, a, ts);
+    }
+    return;
+  }
 #define SIS_TSPS(T, G)                                                                       \
   if (form == (sizeof(T) == 8 ? 1 : 0) && gather == G) {                                    \
     hipLaunchKernelGGL((k_tsp_sums<T, G>), dim3(sums_grid), dim3(kBlock), 0, st, a, ts);    \

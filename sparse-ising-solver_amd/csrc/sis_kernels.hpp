@@ -101,7 +101,9 @@ struct TspArgs {
 };
 
 bool tsp_struct_supported(int form, int gather);
-void launch_tsp_struct_step(hipStream_t st, int form, int gather, const StepArgs &a,
+// mfma: the FP32 form's distance product on the matrix cores
+// (v_mfma_f32_32x32x2f32) instead of VALU FMAs.
+void launch_tsp_struct_step(hipStream_t st, int form, int gather, bool mfma, const StepArgs &a,
                             const TspArgs &ts);
 // Energy s^T J s - h.s of bit-packed spins with the structured product, FP64,
 // as per-position partials part[i][agent] (reduced by launch_cut_reduce).

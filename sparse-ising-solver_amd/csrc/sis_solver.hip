@@ -145,6 +145,8 @@ static int default_tile(const sis_solver *s) {
 
 static std::vector<int> candidate_tiles(const sis_solver *s) {
   std::vector<int> c;
+  // structured TSP: 1 = VALU distance product, 2 = matrix cores (FP32)
+  if (s->structured) return s->f64 ? std::vector<int>{1} : std::vector<int>{1, 2};
   if (s->p.form == SIS_FORM_PACKED) {
     for (int wt : {1, 2, 4, 8})
       for (int r : {1, 2, 4})
@@ -412,6 +414,10 @@ int sis_solver_create_tsp(const sis_tsp *t, const sis_params *pin, sis_solver **
   std::unique_ptr<sis_solver> s(sp);
   s->structured = true;
   s->p.structured = 1;
+  // tile: 1 = VALU distance product, 2 = matrix cores (default for FP32)
+  if (pin->tile == 0) s->vec = s->f64 ? 1 : 2;
+  else if (pin->tile != 1 && !(pin->tile == 2 && !s->f64))
+    return fail(SIS_ERR_ARG, "sis_solver_create_tsp: structured tile must be 1 (VALU) or 2 (MFMA, f32)");
   s->tsp_n = n;
   s->qB = B / 4;
   s->qC = C / 4;
@@ -563,7 +569,8 @@ static void enqueue_step(sis_solver *s, int32_t off) {
   if (s->prob == PROB_UNFUSED)
     launch_xy_update(s->stream, s->f64, s->Xbuf(cur), a.Y, s->N * s->lda, a.sched, a.ctrl, off);
   if (s->structured) {
-    launch_tsp_struct_step(s->stream, s->p.form, s->gather, a, s->tsp_args(false));
+    launch_tsp_struct_step(s->stream, s->p.form, s->gather, (s->vec & 0xffff) == 2, a,
+                           s->tsp_args(false));
     s->par = nxt;
     return;
   }

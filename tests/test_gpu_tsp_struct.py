@@ -39,16 +39,20 @@ def max_rel(x, X):
     return float(np.abs(x - X).max() / max(np.abs(X).max(), 1e-30))
 
 
-@pytest.mark.parametrize("n", [3, 7, 12])
-@pytest.mark.parametrize("form,prec", [("f32", "f32"), ("f64", "f64")])
-def test_structured_dsb_bit_exact(n, form, prec):
+# tile 1: distance product on VALU; 2: on the matrix cores (FP32, default)
+FORMS_T = [("f32", "f32", 1), ("f32", "f32", 2), ("f64", "f64", 1)]
+
+
+@pytest.mark.parametrize("n", [3, 7, 12, 33])
+@pytest.mark.parametrize("form,prec,tile", FORMS_T)
+def test_structured_dsb_bit_exact(n, form, prec, tile):
     t, J, h, pb = instance(n)
     A = 96
     X0, Y0 = seeded_state(n * n, A, 40 + n)
     T, dt = 800, 0.1
     s = sis.Solver.from_tsp(t, dict(agents=A, form=sis.FORMS[form], dynamics=sis.DSB, dt=dt,
-                                    structured=1))
-    assert s.p.structured == 1 and s.p.ncities == n
+                                    structured=1, tile=tile))
+    assert s.p.structured == 1 and s.p.ncities == n and s.p.tile == tile
     s.set_state(X0, Y0)
     np_dt = np.float64 if prec == "f64" else np.float32
     X, Y = X0.astype(np_dt), Y0.astype(np_dt)
@@ -60,14 +64,16 @@ def test_structured_dsb_bit_exact(n, form, prec):
         np.testing.assert_array_equal(y, Y.astype(np.float64), err_msg=f"Y after {t0 + k}")
 
 
-@pytest.mark.parametrize("form,prec,bar1,bar", [("f32", "f32", 1e-6, 1e-5), ("f64", "f64", 1e-15, 1e-12)])
-@pytest.mark.parametrize("n", [7, 16])
-def test_structured_bsb_within_bound(form, prec, bar1, bar, n):
+@pytest.mark.parametrize("form,prec,bar1,bar,tile", [("f32", "f32", 1e-6, 1e-5, 1),
+                                                     ("f32", "f32", 1e-6, 1e-5, 2),
+                                                     ("f64", "f64", 1e-15, 1e-12, 1)])
+@pytest.mark.parametrize("n", [7, 16, 40])
+def test_structured_bsb_within_bound(form, prec, bar1, bar, tile, n):
     t, J, h, pb = instance(n, seed=5)
     A = 128
     X0, Y0 = seeded_state(n * n, A, 50 + n)
     dt, T = 0.1, 20000
-    s = sis.Solver.from_tsp(t, dict(agents=A, form=sis.FORMS[form], dt=dt, structured=1))
+    s = sis.Solver.from_tsp(t, dict(agents=A, form=sis.FORMS[form], dt=dt, structured=1, tile=tile))
     s.set_state(X0, Y0)
     np_dt = np.float64 if prec == "f64" else np.float32
     X, Y = X0.astype(np_dt), Y0.astype(np_dt)
@@ -81,18 +87,20 @@ def test_structured_bsb_within_bound(form, prec, bar1, bar, n):
         Xd, Yd = X.astype(np.float64), Y.astype(np.float64)
         errs.append((t1, max(max_rel(x, Xd), max_rel(y, Yd)), max(max_rel(Xd, X64), max_rel(Yd, Y64))))
         t0 = t1
-    print(f"structured bSB n={n} {form}: (step, vs oracle {prec}, oracle {prec} vs f64) {errs}")
+    print(f"structured bSB n={n} {form} tile {tile}: (step, vs oracle {prec}, oracle {prec} vs f64) {errs}")
     assert errs[0][1] <= bar1, errs
     for t1, e, pe in errs[1:]:
         assert e <= max(bar, 2.0 * pe), (t1, e, pe)
 
 
 @pytest.mark.parametrize("dyn", [sis.DSB, sis.BSB])
-def test_structured_full_run_and_energy(dyn):
+@pytest.mark.parametrize("tile", [1, 2])
+def test_structured_full_run_and_energy(dyn, tile):
     n = 6
     t, J, h, pb = instance(n, seed=11)
     A, T, dt = 256, 3000, 0.1
-    s = sis.Solver.from_tsp(t, dict(agents=A, form=sis.FORM_F32, dynamics=dyn, dt=dt, structured=1))
+    s = sis.Solver.from_tsp(t, dict(agents=A, form=sis.FORM_F32, dynamics=dyn, dt=dt, structured=1,
+                                    tile=tile))
     s.init_random(8)
     info = s.run(T, 0)
     fin = s.spins(sis.SPINS_FINAL)
@@ -145,3 +153,18 @@ def test_structured_argument_errors():
     J, h = t.qubo()
     with pytest.raises(sis.SisError):
         sis.Solver(J, dict(problem=sis.TSP, agents=64, ncities=8, structured=1), field=h)
+
+
+def test_structured_default_tile_and_autotune():
+    t = sis.Tsp.generate(40, 100, 0, 3)
+    s = sis.Solver.from_tsp(t, dict(agents=128, form=sis.FORM_F32, structured=1))
+    assert s.p.tile == 2  # matrix cores by default in FP32
+    s.init_random(1)
+    x0 = s.get_state()[0]
+    tile, ms = s.autotune(5)
+    assert tile in (1, 2) and ms > 0
+    assert np.array_equal(s.get_state()[0], x0)
+    s64 = sis.Solver.from_tsp(t, dict(agents=128, form=sis.FORM_F64, structured=1))
+    assert s64.p.tile == 1
+    with pytest.raises(sis.SisError):
+        sis.Solver.from_tsp(t, dict(agents=128, form=sis.FORM_F64, structured=1, tile=2))
