@@ -20,13 +20,14 @@
 //   - the seed comes from SIS_SEED (default: the clock, as srand(time(0))).
 // Extra knobs (environment, so the argv layout is unchanged):
 //   SIS_FORM = f32 | f64 | packed | bf16   precision / gather form (f32)
-//   SIS_DYNAMICS = dsb | bsb               MaxCut dynamics (dsb)
+//   SIS_DYNAMICS = dsb | bsb               dynamics (MaxCut dsb, TSP bsb)
 //   SIS_EUC_ROUND = 0 | 1                  EUC_2D truncation (0, reference) or nint
 //   SIS_GRAPH = 0 | 1 | 2                  hipGraph per window period / whole run
 //   SIS_XI0 = <real>                       override the coupling scale
 //   SIS_WINDOW = <int>                     override the window period
 //   SIS_SPLIT = <int>                      row-split threshold (0 off, -1 tuned)
 //   SIS_TUNE = 0 | 1                       autotune the tile before the run
+//   SIS_STRUCTURED = 0 | 1                 TSP: structured J x, no QUBO matrix
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -85,7 +86,7 @@ struct Opts {
   uint64_t seed = 0;
 };
 
-void apply_env(sis_params &p, bool maxcut) {
+void apply_env(sis_params &p) {
   if (const char *f = env("SIS_FORM")) {
     std::string s = f;
     if (s == "f64") p.form = SIS_FORM_F64;
@@ -93,8 +94,7 @@ void apply_env(sis_params &p, bool maxcut) {
     else if (s == "bf16") p.form = SIS_FORM_BF16;
     else p.form = SIS_FORM_F32;
   }
-  if (maxcut)
-    if (const char *d = env("SIS_DYNAMICS")) p.dynamics = std::string(d) == "bsb" ? SIS_BSB : SIS_DSB;
+  if (const char *d = env("SIS_DYNAMICS")) p.dynamics = std::string(d) == "bsb" ? SIS_BSB : SIS_DSB;
   if (const char *g = env("SIS_GRAPH")) p.use_graph = std::min(2, std::max(0, std::atoi(g)));
   if (const char *r = env("SIS_SPLIT")) p.split = std::max(-1, std::atoi(r));
   if (const char *x = env("SIS_XI0")) p.xi0 = std::atof(x);
@@ -113,7 +113,7 @@ int max_cut(const Opts &o, const std::string &ifile, const std::string &ofile) {
   p.agents = o.agents;
   p.dt = o.dt;
   p.fused = o.fused ? 1 : 0;
-  apply_env(p, true);
+  apply_env(p);
   sis_solver *s = nullptr;
   if (sis_solver_create(J, nullptr, &p, &s)) return die("creating the solver");
   std::cout << "file: " << ifile << ", ";
@@ -156,17 +156,14 @@ int traveling_salesman(const Opts &o, const std::string &ifile, const std::strin
   int kind = SIS_TSP_EUC_2D;
   sis_tsp_kind_of(t, &kind);
   std::cout << n << "\n" << (kind == SIS_TSP_GEO ? "GEO" : "EUC_2D") << "\n\n" << fmt(maxd) << "\n";
-  sis_csr *J = nullptr;
-  std::vector<double> h((size_t)(n * n));
-  if (sis_tsp_qubo(t, &J, h.data())) return die("building the QUBO");
   sis_params p;
   sis_params_default(SIS_TSP, &p);
-  p.ncities = (int32_t)n;
   p.agents = o.agents;
   p.dt = o.dt;
-  apply_env(p, false);
-  sis_solver *s = nullptr;
-  if (sis_solver_create(J, h.data(), &p, &s)) return die("creating the solver");
+  apply_env(p);
+  if (const char *st = env("SIS_STRUCTURED")) p.structured = std::atoi(st) != 0;
+  sis_solver *s = nullptr;  // builds the QUBO (or, structured, no matrix) and the field
+  if (sis_solver_create_tsp(t, &p, &s)) return die("creating the solver");
   sis_run_info info;
   std::vector<double> len((size_t)o.agents);
   if (sis_solver_init_random(s, o.seed) || tune(s) || sis_solver_run(s, o.steps, o.uw, &info) ||
@@ -197,7 +194,6 @@ int traveling_salesman(const Opts &o, const std::string &ifile, const std::strin
   sis_tsp_random_length(t, o.seed ^ 0x5DEECE66Dull, &rl);
   std::cout << "\ntraveling salesman distance from random spin assignment: " << fmt(rl) << "\n";
   sis_solver_destroy(s);
-  sis_csr_free(J);
   sis_tsp_free(t);
   return 0;
 }

@@ -832,27 +832,44 @@ __global__ void k_window_merge(const uint64_t *cur, uint64_t *fin, const uint64_
 }
 
 // TSP validity (src/kernels.cu:475-500): exactly one +1 per position and per
-// city.  One wavefront per 64-agent word; spin words are wave-uniform loads.
-__global__ __launch_bounds__(kBlock) void k_tsp_valid(const uint64_t *__restrict__ cur, int64_t n,
+// city.  Stage 1, one wavefront per (line, word), line = position i (< n) or
+// city k (>= n): over the line's n spin words y = ~word (bit set = +1),
+// ones |= y after twos |= ones & y, so ones & ~twos marks the agents with
+// exactly one +1 on the line.  Stage 2 ANDs the 2n line masks of each word.
+__global__ __launch_bounds__(kBlock) void k_tsp_lines(const uint64_t *__restrict__ cur, int64_t n,
+                                                      int64_t W, uint64_t *lines,
+                                                      const Ctrl *ctrl) {
+  (void)ctrl;
+  const int64_t g = wave_index();
+  if (g >= 2 * n * W) return;
+  const int lane = threadIdx.x & 63;
+  const int64_t line = g / W, w = g - line * W;
+  const int64_t base = line < n ? line * n : line - n, stride = line < n ? 1 : n;
+  uint64_t ones = 0, twos = 0;
+  for (int64_t m = 0; m < n; m++) {
+    const uint64_t y = ~cur[(base + m * stride) * W + w];
+    twos |= ones & y;
+    ones |= y;
+  }
+  if (lane == 0) lines[line * W + w] = ones & ~twos;
+}
+
+__global__ __launch_bounds__(kBlock) void k_tsp_valid(const uint64_t *__restrict__ lines, int64_t n,
                                                       int64_t W, int64_t A, int32_t *valid_list,
                                                       uint64_t *newmask, int64_t P,
                                                       const Ctrl *ctrl) {
   const int64_t w = wave_index();
   if (w >= W) return;
   const int lane = threadIdx.x & 63;
-  int valid = 1;
-  for (int64_t i = 0; i < n; i++) {
-    int c = 0;
-    for (int64_t k = 0; k < n; k++) c += !((cur[(i * n + k) * W + w] >> lane) & 1);
-    valid &= c == 1;
-  }
-  for (int64_t k = 0; k < n; k++) {
-    int c = 0;
-    for (int64_t i = 0; i < n; i++) c += !((cur[(i * n + k) * W + w] >> lane) & 1);
-    valid &= c == 1;
+  uint64_t m = ~0ull;
+  for (int64_t line = lane; line < 2 * n; line += 64) m &= lines[line * W + w];
+  for (int s = 32; s >= 1; s >>= 1) {
+    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)m, s, 64);
+    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(m >> 32), s, 64);
+    m &= ((uint64_t)hi << 32) | lo;
   }
   const int64_t ag = 64 * w + lane;
-  valid &= ag < A;
+  const int valid = ((m >> lane) & 1) && ag < A;
   const uint64_t nm = __ballot(valid);
   if (lane == 0) newmask[w] = nm;
   if (valid) valid_list[ag] = (int32_t)(ctrl->base + P);
@@ -1243,8 +1260,11 @@ void launch_window_merge(hipStream_t st, const uint64_t *cur, uint64_t *fin,
 }
 
 void launch_tsp_valid(hipStream_t st, const uint64_t *cur, int64_t n, int64_t W, int64_t A,
-                      int32_t *valid_list, uint64_t *newmask, int64_t P, Ctrl *ctrl) {
-  hipLaunchKernelGGL(k_tsp_valid, dim3(grid_for(W, kWaves)), dim3(kBlock), 0, st, cur, n, W, A,
+                      int32_t *valid_list, uint64_t *newmask, uint64_t *lines, int64_t P,
+                      Ctrl *ctrl) {
+  hipLaunchKernelGGL(k_tsp_lines, dim3(grid_for(2 * n * W, kWaves, INT32_MAX)), dim3(kBlock), 0, st,
+                     cur, n, W, lines, ctrl);
+  hipLaunchKernelGGL(k_tsp_valid, dim3(grid_for(W, kWaves)), dim3(kBlock), 0, st, lines, n, W, A,
                      valid_list, newmask, P, ctrl);
 }
 

@@ -131,8 +131,10 @@ void launch_window_finalize(hipStream_t st, const uint64_t *part, int64_t nblk, 
                             int64_t P, Ctrl *ctrl);
 void launch_window_merge(hipStream_t st, const uint64_t *cur, uint64_t *fin,
                          const uint64_t *newmask, int64_t N, int64_t W, const Ctrl *ctrl);
+// lines: 2 n x W words of scratch (one mask per position / city line).
 void launch_tsp_valid(hipStream_t st, const uint64_t *cur, int64_t n, int64_t W, int64_t A,
-                      int32_t *valid_list, uint64_t *newmask, int64_t P, Ctrl *ctrl);
+                      int32_t *valid_list, uint64_t *newmask, uint64_t *lines, int64_t P,
+                      Ctrl *ctrl);
 void launch_ctrl_advance(hipStream_t st, Ctrl *ctrl, int64_t P);
 
 // Cut / energy: partial row sums per block, then an ordered reduction.

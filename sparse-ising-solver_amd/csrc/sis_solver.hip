@@ -43,6 +43,7 @@ struct sis_solver {
   int par = 0;                   // index of the current snapshot buffer
 
   DevBuf<uint64_t> cur, fin, newmask, part;
+  DevBuf<uint64_t> tsp_lines;    // TSP window: one-hot mask per position / city line
   DevBuf<int32_t> stab, bif, valid;
   DevBuf<Ctrl> ctrl;
   Ctrl *h_ctrl = nullptr;        // pinned mirror
@@ -370,6 +371,7 @@ int sis_solver_create(const sis_csr *J, const double *field, const sis_params *p
       (st = s->cutpart.alloc((size_t)(s->cut_blocks * s->lda))) ||
       (st = s->outv.alloc((size_t)s->lda)) || (st = s->best.alloc(1)))
     return st;
+  if (p.problem == SIS_TSP && (st = s->tsp_lines.alloc((size_t)(2 * p.ncities * s->W)))) return st;
   if (p.split > 0 && (st = build_split(s.get(), p.split))) return st;
   *out = s.release();
   return SIS_OK;
@@ -599,7 +601,8 @@ static void enqueue_window(sis_solver *s, int64_t P, int uw) {
                            s->A, s->W, s->p.window_threshold, uw, P, s->ctrl.p);
     launch_window_merge(s->stream, s->cur.p, s->fin.p, s->newmask.p, s->N, s->W, s->ctrl.p);
   } else {
-    launch_tsp_valid(s->stream, s->cur.p, s->p.ncities, s->W, s->A, s->valid.p, s->newmask.p, P,
+    launch_tsp_valid(s->stream, s->cur.p, s->p.ncities, s->W, s->A, s->valid.p, s->newmask.p,
+                     s->tsp_lines.p, P,
                      s->ctrl.p);
     launch_window_merge(s->stream, s->cur.p, s->fin.p, s->newmask.p, s->N, s->W, s->ctrl.p);
     launch_ctrl_advance(s->stream, s->ctrl.p, P);

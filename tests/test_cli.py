@@ -73,11 +73,14 @@ def test_maxcut_line_matches_oracle(tmp_path, fused, knobs):
 
 
 @pytest.mark.gpu
-def test_tsp_line_matches_oracle(tmp_path):
+@pytest.mark.parametrize("knobs", [{}, {"SIS_STRUCTURED": "1", "SIS_DYNAMICS": "dsb"}])
+def test_tsp_line_matches_oracle(tmp_path, knobs):
+    """bSB through the QUBO's CSR (the reference's TSP); and dSB through the
+    structured product, whose trajectory is bit-identical to the CSR's."""
     src = os.path.join(INPUTS, "euc8.tsp")
     out = tmp_path / "out.csv"
     A, T, dt, seed = 256, 2000, 0.1, 5
-    r = run(["0", src, out, "tsp", "0", "1", "0", dt, T, A], env={"SIS_SEED": str(seed)})
+    r = run(["0", src, out, "tsp", "0", "1", "0", dt, T, A], env={"SIS_SEED": str(seed), **knobs})
     assert r.returncode == 0, r.stderr
     f = _fields(out.read_text())
     t = sis.Tsp.read(src)
@@ -85,7 +88,8 @@ def test_tsp_line_matches_oracle(tmp_path):
     rp, col, val = J.arrays()
     pb = orc.Problem(rp, col, val, field=h, ncities=8)
     X, Y = orc.init_state(seed, orc.TSP, 64, A)
-    res = orc.solve(pb, orc.TSP, orc.BSB, X, Y, T, 0, P=T // 10, K=50, dt=dt, xi0=0.002,
+    dyn = orc.DSB if knobs.get("SIS_DYNAMICS") == "dsb" else orc.BSB
+    res = orc.solve(pb, orc.TSP, dyn, X, Y, T, 0, P=T // 10, K=50, dt=dt, xi0=0.002,
                     coupling=1.005)
     _, _, d, _ = t.arrays()
     lens = np.array([orc.tsp_tour_length(8, d, res["fin"], a)[0] for a in range(A) if res["valid"][a] > 0])
