@@ -551,8 +551,8 @@ __global__ __launch_bounds__(kBlock) void k_tsp_step_struct(StepArgs a, TspArgs 
 
 // FP32 form on the matrix cores: per (position i, 64-agent tile) the product
 // sum_l Dq_kl v_l is the (n x n) . (n x 64) GEMM Dq V.  Wavefront w takes the
-// 32 x 32 output tiles (cities kt*32.., agents at*32..) w, w + 4, ...; each
-// v_mfma_f32_32x32x2f32 adds two l columns: A[row][kk] = Dq[kt*32+row][l0+kk]
+// 32-city tiles kt = w, w + 4, ..., each as two 32 x 32 output tiles (agents
+// 0..31 and 32..63 of the tile); each v_mfma_f32_32x32x2f32 adds two l columns: A[row][kk] = Dq[kt*32+row][l0+kk]
 // in lane row + 32 kk, B[kk][col] = v[l0+kk][at*32+col] in lane col + 32 kk,
 // and C[row][col] sits in lane col + 32 ((row / 4) % 2), register
 // 4 (row / 8) + row % 4.  K is padded to even with zeros.
@@ -586,17 +586,9 @@ __global__ __launch_bounds__(kBlock) void k_tsp_step_mfma(StepArgs a, TspArgs ts
   const float *__restrict__ h = (const float *)a.field;
   float *Y = (float *)a.Y;
   const int half = lane >> 5, col = lane & 31;
-  const int64_t ntile = ((n + 31) / 32) * 2;
-  for (int64_t tt = w; tt < ntile; tt += kWaves) {
-    const int64_t kt = tt >> 1, at = tt & 1;
-    const int64_t krow = kt * 32 + col;
-    sis_floatx16 acc = {0};
-    for (int64_t l0 = 0; l0 < ne; l0 += 2) {
-      const int64_t l = l0 + half;
-      const float av = (krow < n && l < n) ? dq[krow * n + l] : 0.0f;
-      const float bv = v[l * 64 + at * 32 + col];
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
-    }
+  const int64_t ktiles = (n + 31) / 32;
+  // element update of the output tile (cities kt*32.., agents at*32..)
+  auto finish = [&](int64_t kt, int at, const sis_floatx16 &acc) {
     const int64_t ag = tile * 64 + at * 32 + col;
     const float rs = rsum[i * lda + ag];
 #pragma unroll
@@ -610,6 +602,57 @@ __global__ __launch_bounds__(kBlock) void k_tsp_step_mfma(StepArgs a, TspArgs ts
         ((float *)a.gdst)[e] = x;
         Y[e] = y;
       }
+    }
+  };
+  // U K-steps of loads go out before their MFMAs
+  constexpr int U = 4;
+  if (ktiles >= kWaves) {
+    // a wavefront takes a 32-city tile for both 32-agent halves: one A
+    // operand feeds two accumulators
+    for (int64_t kt = w; kt < ktiles; kt += kWaves) {
+      const int64_t krow = kt * 32 + col;
+      const float *drow = dq + (krow < n ? krow : 0) * n;
+      sis_floatx16 acc0 = {0}, acc1 = {0};
+      for (int64_t l0 = 0; l0 < ne; l0 += 2 * U) {
+        float av[U], b0[U], b1[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const int64_t l = l0 + 2 * u + half;
+          const bool in = l < ne;
+          av[u] = (krow < n && l < n) ? drow[l] : 0.0f;
+          b0[u] = in ? v[l * 64 + col] : 0.0f;
+          b1[u] = in ? v[l * 64 + 32 + col] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u], b0[u], acc0, 0, 0, 0);
+          acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u], b1[u], acc1, 0, 0, 0);
+        }
+      }
+      finish(kt, 0, acc0);
+      finish(kt, 1, acc1);
+    }
+  } else {
+    // few city tiles: one (city tile, agent half) per wavefront
+    for (int64_t tt = w; tt < ktiles * 2; tt += kWaves) {
+      const int64_t kt = tt >> 1;
+      const int at = (int)(tt & 1);
+      const int64_t krow = kt * 32 + col;
+      const float *drow = dq + (krow < n ? krow : 0) * n;
+      sis_floatx16 acc = {0};
+      for (int64_t l0 = 0; l0 < ne; l0 += 2 * U) {
+        float av[U], bv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const int64_t l = l0 + 2 * u + half;
+          av[u] = (krow < n && l < n) ? drow[l] : 0.0f;
+          bv[u] = l < ne ? v[l * 64 + at * 32 + col] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u], bv[u], acc, 0, 0, 0);
+      }
+      finish(kt, at, acc);
     }
   }
 }
