@@ -166,7 +166,7 @@ def run_solver(J, params, T, uw, seed, field=None, autotune=True, tsp=None):
 def small_configs(a):
     """The remaining configurations, one JSON line each (not the headline)."""
     lines = []
-    if a.workload == "c1":
+    if a.workload == "er800":
         # ER G(800, 0.06), +1 weights, 128 agents, bSB, dt 0.1, 2000 steps, no
         # window exit; c0 = 0.5 / (sqrt(mean degree) * std(J)) as configured.
         J = sis.Csr.gen_er(800, 0.06, sis.W_ONE, a.seed)
@@ -176,27 +176,27 @@ def small_configs(a):
         for g in (0, 1, 2):
             r = run_solver(J, dict(problem=sis.MAXCUT, agents=128, dt=0.1, dynamics=sis.BSB,
                                    form=sis.FORM_F32, xi0=c0, use_graph=g), 2000, 0, a.seed)
-            lines.append(dict(workload="c1 ER N=800 p=0.06 bSB plain FP32, 128 agents, 2000 steps",
+            lines.append(dict(workload="ER N=800 p=0.06 bSB plain FP32, 128 agents, 2000 steps",
                               use_graph=g, nnz=J.nnz, xi0=c0, **{k: v for k, v in r.items() if k != "info"},
                               ms_per_step=r["wall"] * 1e3 / r["info"]["steps_executed"]))
-    elif a.workload == "c2":
+    elif a.workload == "torus100":
         J = sis.Csr.gen_torus(100, 100, sis.W_PM1, a.seed)
         for form in ("f32", "packed"):
             r = run_solver(J, dict(problem=sis.MAXCUT, agents=1024, dt=0.5, form=sis.FORMS[form],
                                    use_graph=1), a.steps if a.steps != 20000 else 10000, 1, a.seed)
             ms = r["info"]["step_kernel_ms"] / max(r["info"]["steps_executed"], 1)
-            lines.append(dict(workload="c2 torus 100x100 +-1 dSB, 1024 agents, 10000 steps, uw=1",
+            lines.append(dict(workload="torus 100x100 +-1 dSB, 1024 agents, 10000 steps, uw=1",
                               form=form, steps_executed=r["info"]["steps_executed"],
                               **{k: v for k, v in r.items() if k != "info"},
                               ms_per_step=r["wall"] * 1e3 / max(r["info"]["steps_executed"], 1)))
-    elif a.workload == "c4":
+    elif a.workload == "tsp48":
         t = sis.Tsp.generate(48, 1000, 1, a.seed)
         J, h = t.qubo()
         for form, split in (("f32", 0), ("bf16", 0), ("f32", -1), ("bf16", -1)):
             r = run_solver(J, dict(problem=sis.TSP, agents=1024, dt=0.1, form=sis.FORMS[form],
                                    dynamics=sis.BSB, ncities=48, use_graph=1, split=split),
                            a.steps if a.steps != 20000 else 10000, 0, a.seed, field=h, tsp=t)
-            lines.append(dict(workload="c4 TSP n=48 (2304 spins) bSB, 1024 agents, 10000 steps",
+            lines.append(dict(workload="TSP n=48 (2304 spins) bSB, 1024 agents, 10000 steps",
                               form=form, split_requested=split, nnz=J.nnz,
                               **{k: v for k, v in r.items() if k != "info"},
                               ms_per_step=r["wall"] * 1e3 / r["info"]["steps_executed"],
@@ -363,8 +363,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--workload", default="c3",
-                    choices=["c1", "c2", "c3", "c4", "powerlaw", "tsp_struct", "serving"])
+    ap.add_argument("--workload", default="regular2p20",
+                    choices=["regular2p20", "er800", "torus100", "tsp48", "powerlaw", "tsp_struct",
+                             "serving"],
+                    help="regular2p20: the headline (random 4-regular graph, N = 2^20); the "
+                         "others print one JSON line per configuration")
     ap.add_argument("--instances", type=int, default=4096)
     ap.add_argument("--chunk", type=int, default=100)
     ap.add_argument("--rate", type=float, default=0.0)
@@ -374,7 +377,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: the solver runs on a single device (agents shard "
-              "across GPUs with no per-step communication, see DESIGN.md); use --gpus 1.",
+              "across GPUs with no per-step communication); use --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
     if a.workload == "serving":
@@ -382,7 +385,7 @@ def main():
             a.steps = 5000
         serving(a)
         return
-    if a.workload in ("c1", "c2", "c4", "powerlaw", "tsp_struct"):
+    if a.workload in ("er800", "torus100", "tsp48", "powerlaw", "tsp_struct"):
         small_configs(a)
         return
 
