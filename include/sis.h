@@ -323,7 +323,12 @@ typedef struct sis_batch_params {
   int32_t tile;          /* rows per task (1, 2, 4, 8) | non-temporal << 16 |
                             vector gathers << 17 (2, 4 or 8 rows) | XCD-affine
                             task order << 18 (each instance's tasks on one XCD,
-                            so its sign words stay in that XCD's L2); 0 = default */
+                            so its sign words stay in that XCD's L2) |
+                            bit-transpose integer sums << 19 | next-task
+                            prefetch << 20 | sign words staged in LDS << 21
+                            (2, 4 or 8 rows) | LDS segment code s << 22
+                            (segments of 512 << s rows, s = 0..3);
+                            0 = default */
   int32_t eager;         /* 1: launch each chunk's kernels directly instead of
                             replaying the captured hipGraph */
 } sis_batch_params;

@@ -486,6 +486,170 @@ __global__ __launch_bounds__(kBlock) void k_batch_step_bt(BArgs b) {
   batch_step_bt<R, WT, NT, PF>(b);
 }
 
+// Same step as k_batch_step_bt with each instance's sign words staged in LDS.
+// The task table holds segments of up to 512 << s rows of one instance (s from
+// the tile, 0..3); a
+// workgroup takes a segment, copies the instance's whole sign array (n x W
+// words, coalesced) into LDS, and its wavefronts run the segment's rows R at a
+// time with every neighbour sign word read from LDS, so the random gathers no
+// longer miss L2.  Instances larger than the staging buffer read the words
+// from global memory as k_batch_step_bt does.  Same arithmetic, same order:
+// bit-identical.
+constexpr int kSegRows = 512;     // rows per segment at segment code 0
+constexpr int kLdsWords = 4096;   // staged sign words per workgroup (32 KB)
+
+template <int R, int WT, int NT>
+__global__ __launch_bounds__(kBlock) void k_batch_step_lds(BArgs b) {
+  __shared__ uint64_t s_sign[kLdsWords];
+  constexpr int SL = 64 / R;
+  constexpr int kPlanes = 2;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  // segment range of the workgroup: the whole table, or its XCD's slice
+  int first = blockIdx.x, end = *b.ntasks, stride = gridDim.x;
+  if (b.xcd) {
+    first = b.ntasks[1 + blockIdx.x % kXcds] + (int)(blockIdx.x / kXcds);
+    end = b.ntasks[2 + blockIdx.x % kXcds];
+    stride = gridDim.x / kXcds;
+  }
+  const int64_t lda = b.lda, W = WT;
+  const int my_r = lane / SL, my_i = lane % SL;
+  for (int seg = first; seg < end; seg += stride) {
+    const BTask sg = b.tasks[seg];
+    const BInst &in = b.inst[sg.slot];
+    if (in.status != SLOT_ACTIVE) continue;  // the same for the whole workgroup
+    const int32_t t = in.t;
+    const uint64_t *S = (t & 1) ? b.S1 : b.S0;
+    uint64_t *Sn = (t & 1) ? b.S0 : b.S1;
+    const int64_t off = in.row_off, nw = (int64_t)in.n * W;
+    const bool lds = nw <= kLdsWords;
+    __syncthreads();  // the previous segment is done with the staged words
+    if (lds)
+      for (int64_t q = threadIdx.x; q < nw; q += blockDim.x) s_sign[q] = S[off * W + q];
+    __syncthreads();
+    double p = 1.025 * (double)t / (double)in.T;
+    p = p < 1.0 ? p : 1.0;
+    const float c0 = (float)(in.dt * (p + 1)), c1 = (float)(in.dt * (p - 1));
+    const float c2 = (float)(in.dt * in.xi0);
+    for (int64_t r0 = (int64_t)w * R; r0 < sg.nrows; r0 += (int64_t)kWaves * R) {
+      const int64_t row0 = sg.row0 + r0;
+      const int nrows = (int)(sg.nrows - r0 < R ? sg.nrows - r0 : R);
+      float x[R][WT], y[R][WT];
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        if (r < nrows) {
+#pragma unroll
+          for (int u = 0; u < WT; u++) {
+            x[r][u] = ld_state<NT>(b.X + (row0 + r) * lda + 64 * u + lane);
+            y[r][u] = ld_state<NT>(b.Y + (row0 + r) * lda + 64 * u + lane);
+          }
+        }
+      }
+      const int64_t myrow = row0 + (my_r < nrows ? my_r : 0);
+      const int64_t mykb = b.rbeg[myrow];
+      const int mydeg = my_r < nrows ? (int)(b.rend[myrow] - mykb) : 0;
+      int maxdeg = mydeg;
+      for (int m = 32; m >= 1; m >>= 1) {
+        const int o = __shfl_xor(maxdeg, m, 64);
+        maxdeg = o > maxdeg ? o : maxdeg;
+      }
+      maxdeg = __builtin_amdgcn_readfirstlane(maxdeg);
+      float F[R][WT];
+      if (in.intw) {
+        int Sa[R][WT], Vr[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          Vr[r] = 0;
+#pragma unroll
+          for (int u = 0; u < WT; u++) Sa[r][u] = 0;
+        }
+        for (int d0 = 0; d0 < maxdeg; d0 += SL) {
+          const bool ok = d0 + my_i < mydeg;
+          const int64_t k = ok ? mykb + d0 + my_i : 0;
+          const int64_t j = ok ? (int64_t)b.col[k] : row0;
+          const int vi = ok ? (int)b.val[k] : 0;
+          const int av = vi < 0 ? -vi : vi;
+          uint64_t wd[WT];
+#pragma unroll
+          for (int u = 0; u < WT; u++) wd[u] = lds ? s_sign[(j - off) * W + u] : S[j * W + u];
+          uint64_t Pm[kPlanes], Nm[kPlanes];
+#pragma unroll
+          for (int q = 0; q < kPlanes; q++) {
+            Pm[q] = __ballot(vi > 0 && ((av >> q) & 1));
+            Nm[q] = __ballot(vi < 0 && ((av >> q) & 1));
+          }
+#pragma unroll
+          for (int r = 0; r < R; r++) {
+            const uint64_t rm = (SL == 64 ? ~0ull : ((1ull << SL) - 1)) << (r * SL);
+#pragma unroll
+            for (int q = 0; q < kPlanes; q++)
+              Vr[r] += (__popcll(Pm[q] & rm) - __popcll(Nm[q] & rm)) * (1 << q);
+          }
+#pragma unroll
+          for (int u = 0; u < WT; u++) {
+            const uint64_t m = bit_transpose64(wd[u], lane);
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+              const uint64_t rm = (SL == 64 ? ~0ull : ((1ull << SL) - 1)) << (r * SL);
+              int acc = 0;
+#pragma unroll
+              for (int q = 0; q < kPlanes; q++)
+                acc += (__popcll(m & Pm[q] & rm) - __popcll(m & Nm[q] & rm)) * (1 << q);
+              Sa[r][u] += acc;
+            }
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+          for (int u = 0; u < WT; u++) F[r][u] = (float)(2 * Sa[r][u] - Vr[r]);
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+          for (int u = 0; u < WT; u++) F[r][u] = 0.0f;
+        for (int d0 = 0; d0 < maxdeg; d0 += SL) {
+          const bool ok = d0 + my_i < mydeg;
+          const int64_t k = ok ? mykb + d0 + my_i : 0;
+          const int64_t j = ok ? (int64_t)b.col[k] : row0;
+          const float v = ok ? b.val[k] : 0.0f;
+          uint64_t wd[WT];
+#pragma unroll
+          for (int u = 0; u < WT; u++) wd[u] = lds ? s_sign[(j - off) * W + u] : S[j * W + u];
+#pragma unroll
+          for (int r = 0; r < R; r++) {
+#pragma unroll
+            for (int i = 0; i < SL; i++) {
+              const int l = r * SL + i;
+              const float vv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+#pragma unroll
+              for (int u = 0; u < WT; u++) {
+                const uint64_t ww = readlane64(wd[u], l);
+                F[r][u] = ((ww >> lane) & 1) ? F[r][u] + vv : F[r][u] - vv;
+              }
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        if (r >= nrows) break;
+        const int64_t row = row0 + r;
+        uint64_t mine = 0;
+#pragma unroll
+        for (int u = 0; u < WT; u++) {
+          sb_update<float, PROB_MAXCUT>(x[r][u], y[r][u], F[r][u], c0, c1, c2, 0.0f, 0.0f);
+          st_state<NT>(b.X + row * lda + 64 * u + lane, x[r][u]);
+          st_state<NT>(b.Y + row * lda + 64 * u + lane, y[r][u]);
+          const uint64_t bits = __ballot(signbit(x[r][u]));
+          if (lane == u) mine = bits;
+        }
+        if (lane < WT) Sn[row * W + lane] = mine;
+      }
+    }
+  }
+}
+
 // After each step: one block per slot advances the instance's step count and,
 // at a window boundary, runs its window (src/kernels.cu:24-91, race-free) and
 // decides whether the instance is finished.
@@ -839,12 +1003,17 @@ namespace {
 // Tile: rows per task (1, 2, 4, 8) | non-temporal << 16 | vector gathers << 17
 // (vector gathers need 2, 4 or 8 rows per task) | XCD-affine task order << 18 |
 // bit-transpose integer sums << 19 (2, 4 or 8 rows, no vector-gather bit) |
-// next-task prefetch << 20 (bit-transpose with non-temporal state).
+// next-task prefetch << 20 (bit-transpose with non-temporal state) |
+// sign words staged in LDS << 21 (2, 4 or 8 rows per pass, no other bit but
+// non-temporal and XCD-affine) | segment code s << 22 (LDS staging: segments
+// of 512 << s rows, s = 0..3).
 bool batch_tile_ok(int W, int tile) {
   const int r = tile & 0xffff, nt = (tile >> 16) & 1, vg = (tile >> 17) & 1;
-  const int bt = (tile >> 19) & 1, pf = (tile >> 20) & 1;
-  if ((tile >> 21) != 0 || (nt != 0 && nt != 1)) return false;
+  const int bt = (tile >> 19) & 1, pf = (tile >> 20) & 1, ls = (tile >> 21) & 1;
+  if ((tile >> 24) != 0 || (nt != 0 && nt != 1)) return false;
+  if (!ls && (tile >> 22) != 0) return false;  // segment code only with LDS staging
   if (!(W == 1 || W == 2 || W == 4)) return false;
+  if (ls) return !vg && !bt && !pf && (r == 2 || r == 4 || r == 8);
   if (pf && !(bt && nt)) return false;  // prefetch: bit-transpose, non-temporal only
   if (bt) return !vg && (r == 2 || r == 4 || r == 8);
   if (vg) return r == 2 || r == 4 || r == 8;
@@ -887,13 +1056,32 @@ void launch_bstep_bt(const sis_batch *b, const BArgs &a, bool launch) {
     hipLaunchKernelGGL((k_batch_step_bt<R, WT, NT, PF>), dim3(grid), dim3(kBlock), 0, b->stream, a);
 }
 
+template <int R, int WT, int NT>
+void launch_bstep_lds(const sis_batch *b, const BArgs &a, bool launch) {
+  static unsigned grid = 0;  // per kernel; the engine runs on one kind of device
+  if (!grid) grid = persistent_grid(b, k_batch_step_lds<R, WT, NT>);
+  if (launch)
+    hipLaunchKernelGGL((k_batch_step_lds<R, WT, NT>), dim3(grid), dim3(kBlock), 0, b->stream, a);
+}
+
 // launch = false only resolves the kernel's grid (done before graph capture,
 // where the occupancy query should not run).
 void launch_batch_step(sis_batch *b, bool launch = true) {
   const BArgs a = b->args();
   const int r = b->tile & 0xffff, nt = (b->tile >> 16) & 1, vg = (b->tile >> 17) & 1;
-  const int bt = (b->tile >> 19) & 1, pf = (b->tile >> 20) & 1;
+  const int bt = (b->tile >> 19) & 1, pf = (b->tile >> 20) & 1, ls = (b->tile >> 21) & 1;
   const int w = (int)b->W;
+#define SIS_BSTEPL(R, WT, NT) \
+  if (ls && r == R && w == WT && nt == NT) return launch_bstep_lds<R, WT, NT>(b, a, launch);
+#define SIS_BSTEPL_W(R, NT) SIS_BSTEPL(R, 1, NT) SIS_BSTEPL(R, 2, NT) SIS_BSTEPL(R, 4, NT)
+  SIS_BSTEPL_W(2, 0)
+  SIS_BSTEPL_W(4, 0)
+  SIS_BSTEPL_W(8, 0)
+  SIS_BSTEPL_W(2, 1)
+  SIS_BSTEPL_W(4, 1)
+  SIS_BSTEPL_W(8, 1)
+#undef SIS_BSTEPL_W
+#undef SIS_BSTEPL
 #define SIS_BSTEPB(R, WT, NT, PF) \
   if (bt && r == R && w == WT && nt == NT && pf == PF) return launch_bstep_bt<R, WT, NT, PF>(b, a, launch);
 #define SIS_BSTEPB_W(R, NT, PF) SIS_BSTEPB(R, 1, NT, PF) SIS_BSTEPB(R, 2, NT, PF) SIS_BSTEPB(R, 4, NT, PF)
@@ -942,10 +1130,11 @@ void launch_batch_post(sis_batch *b) {
 // Task table.  Plain order: instance after instance.  XCD-affine order:
 // instances are placed whole on the least-loaded XCD (largest first, load =
 // state bytes + gathers); one too large for a fair share (over a sixteenth of
-// the total) has its tasks dealt round-robin to all XCDs instead.
+// the total) has its tasks dealt round-robin to all XCDs instead.  With
+// LDS-staged sign words a task is a segment of up to kSegRows << s rows.
 int rebuild_tasks(sis_batch *b) {
   if (!b->tasks_dirty) return SIS_OK;
-  const int R = b->tile & 0xffff;
+  const int R = ((b->tile >> 21) & 1) ? kSegRows << ((b->tile >> 22) & 3) : (b->tile & 0xffff);
   const bool xcd = (b->tile >> 18) & 1;
   std::vector<std::vector<BTask>> per(xcd ? kXcds : 1);
   std::vector<const Live *> order;
@@ -1400,10 +1589,13 @@ int sis_batch_autotune(sis_batch *b, int64_t trial_steps, int32_t *tile, double 
       for (int vg : {0, 1})
         for (int xc : {0, 1})
           for (int bt : {0, 1})
-            for (int pf : {0, 1}) {
-              const int32_t tl = r | (nt << 16) | (vg << 17) | (xc << 18) | (bt << 19) | (pf << 20);
-              if (batch_tile_ok((int)b->W, tl)) cand.push_back(tl);
-            }
+            for (int pf : {0, 1})
+              for (int ls : {0, 1})
+                for (int sg = 0; sg < (ls ? 4 : 1); sg++) {
+                  const int32_t tl = r | (nt << 16) | (vg << 17) | (xc << 18) | (bt << 19) |
+                                     (pf << 20) | (ls << 21) | (sg << 22);
+                  if (batch_tile_ok((int)b->W, tl)) cand.push_back(tl);
+                }
   for (int32_t tl : cand) {
     {
       b->tile = tl;

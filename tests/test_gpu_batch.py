@@ -66,6 +66,7 @@ XCD = 1 << 18  # XCD-affine task order: with 9 uneven instances both placements
 # (whole instance on one XCD, and the largest dealt to every XCD) occur
 BT = 1 << 19  # bit-transpose integer sums (instances with small integer weights)
 PF = 1 << 20  # next-task prefetch (bit-transpose, non-temporal)
+LS = 1 << 21  # sign words staged in LDS, segments of 512 rows
 
 
 @pytest.mark.parametrize("agents_max,tile", [(64, 1), (64, 8 | 1 << 16), (64, 2 | 1 << 17),
@@ -76,7 +77,9 @@ PF = 1 << 20  # next-task prefetch (bit-transpose, non-temporal)
                                              (64, 4 | 1 << 16 | BT), (64, 8 | BT | XCD),
                                              (64, 2 | BT), (128, 4 | BT), (256, 2 | 1 << 16 | BT),
                                              (64, 8 | 1 << 16 | BT | PF), (64, 2 | 1 << 16 | BT | PF | XCD),
-                                             (256, 4 | 1 << 16 | BT | PF)])
+                                             (256, 4 | 1 << 16 | BT | PF), (64, 4 | LS),
+                                             (64, 8 | 1 << 16 | LS | XCD), (128, 2 | LS),
+                                             (256, 4 | 1 << 16 | LS)])
 def test_batch_tiles_bit_exact(agents_max, tile):
     Js = instances(9, 21, 64, 500)
     A = min(agents_max, 100) if agents_max > 64 else 64
@@ -176,6 +179,36 @@ def test_batch_bit_transpose_mixed_weights(tile):
         assert b.poll(t)["steps_executed"] == steps, k
         assert np.array_equal(bs, spins), (tile, k)
         if k == 5:  # real weights: the two cut kernels add FP64 terms in different orders
+            np.testing.assert_allclose(bc, cuts, rtol=1e-12, atol=0)
+        else:
+            assert np.array_equal(bc, cuts), (tile, k)
+
+
+@pytest.mark.parametrize("agents_max,tile", [(64, 4 | LS), (64, 2 | 1 << 16 | LS | XCD),
+                                             (64, 8 | LS | 1 << 22), (64, 4 | LS | 3 << 22),
+                                             (256, 4 | 1 << 16 | LS | 2 << 22)])
+def test_batch_lds_staging_segments_and_fallback(agents_max, tile):
+    """Instances of one segment, of several (segments of 512 << s rows for
+    segment code s; 700 and 1500 rows give partial last segments) and too
+    large for the 4096 staged words (5000 rows at 64 agents; above 1024 rows
+    at 256 agents), with small integer and real weights, in one launch: each
+    equals its solo solve."""
+    sizes = [100, 700, 1500, 5000]
+    Js = [sis.Csr.gen_er(n, min(1.0, 8.0 / (n - 1)), sis.W_INT3, 70 + k) for k, n in enumerate(sizes)]
+    Js.append(_weighted(900, 0.01, 5, "real"))
+    A = 64 if agents_max == 64 else 200
+    b = sis.Batch(agents_max=agents_max, max_instances=5, max_rows=9000, max_nnz=200000,
+                  chunk_steps=100, tile=tile)
+    tickets = [b.submit(J, A, 0.5, 500, update_window=1, seed=80 + k) for k, J in enumerate(Js)]
+    while b.advance(500)[0]:
+        pass
+    b.advance(0)
+    for k, (J, t) in enumerate(zip(Js, tickets)):
+        steps, cuts, spins = solve_alone(J, A, 0.5, 500, 1, 80 + k)
+        bc, bs = b.result(t, spins=True)
+        assert b.poll(t)["steps_executed"] == steps, (tile, k)
+        assert np.array_equal(bs, spins), (tile, k)
+        if k == 4:  # real weights: the two cut kernels add FP64 terms in different orders
             np.testing.assert_allclose(bc, cuts, rtol=1e-12, atol=0)
         else:
             assert np.array_equal(bc, cuts), (tile, k)
