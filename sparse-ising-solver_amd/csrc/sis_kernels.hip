@@ -471,7 +471,8 @@ __device__ __forceinline__ T tsp_g(T x, T coupling) {
 }
 
 // Row sums (position i) and column sums (city k) of g over the snapshot: one
-// wavefront per (sum, 64-agent tile), terms in index order.
+// wavefront per (sum, 64-agent tile), terms in index order, loads issued 16
+// at a time (offsets below n^2 lda <= 2^34 elements stay 64-bit).
 template <typename T, int GATHER>
 __global__ __launch_bounds__(kBlock) void k_tsp_sums(StepArgs a, TspArgs ts) {
   if (a.ctrl->done) return;
@@ -482,21 +483,20 @@ __global__ __launch_bounds__(kBlock) void k_tsp_sums(StepArgs a, TspArgs ts) {
   const int64_t which = g / (n * tiles), rem = g - which * n * tiles;
   const int64_t idx = rem / tiles, a0 = (rem - idx * tiles) * 64 + lane;
   const int64_t base = which == 0 ? idx * n : idx, stride = which == 0 ? 1 : n;
-  const T *__restrict__ Xs = (const T *)a.gsrc;
+  const T *__restrict__ Xs = (const T *)a.gsrc + base * lda + a0;
+  const int64_t step = stride * lda;
   const T c = (T)a.coupling;
+  constexpr int B = 16;
   T acc = (T)0;
   int64_t m = 0;
-  for (; m + 4 <= n; m += 4) {
-    const T x0 = Xs[(base + m * stride) * lda + a0];
-    const T x1 = Xs[(base + (m + 1) * stride) * lda + a0];
-    const T x2 = Xs[(base + (m + 2) * stride) * lda + a0];
-    const T x3 = Xs[(base + (m + 3) * stride) * lda + a0];
-    acc = acc + tsp_g<T, GATHER>(x0, c);
-    acc = acc + tsp_g<T, GATHER>(x1, c);
-    acc = acc + tsp_g<T, GATHER>(x2, c);
-    acc = acc + tsp_g<T, GATHER>(x3, c);
+  for (; m + B <= n; m += B) {
+    T xv[B];
+#pragma unroll
+    for (int u = 0; u < B; u++) xv[u] = Xs[(m + u) * step];
+#pragma unroll
+    for (int u = 0; u < B; u++) acc = acc + tsp_g<T, GATHER>(xv[u], c);
   }
-  for (; m < n; m++) acc = acc + tsp_g<T, GATHER>(Xs[(base + m * stride) * lda + a0], c);
+  for (; m < n; m++) acc = acc + tsp_g<T, GATHER>(Xs[m * step], c);
   ((T *)(which == 0 ? ts.rsum : ts.csum))[idx * lda + a0] = acc;
 }
 
@@ -555,52 +555,95 @@ __global__ __launch_bounds__(kBlock) void k_tsp_step_struct(StepArgs a, TspArgs 
 // 0..31 and 32..63 of the tile); each v_mfma_f32_32x32x2f32 adds two l columns: A[row][kk] = Dq[kt*32+row][l0+kk]
 // in lane row + 32 kk, B[kk][col] = v[l0+kk][at*32+col] in lane col + 32 kk,
 // and C[row][col] sits in lane col + 32 ((row / 4) % 2), register
-// 4 (row / 8) + row % 4.  K is padded to even with zeros.
+// 4 (row / 8) + row % 4.  K is padded to a multiple of 8 with zero terms
+// (zero distance columns, zero v rows); cities past n have zero distance rows.
+//
+// The kernel is bound by memory latency, not by the matrix cores: every
+// phase issues its global loads in batches (indices clamped into range, the
+// values of padding slots discarded afterwards) before it uses them.
 typedef float sis_floatx16 __attribute__((ext_vector_type(16)));
 
 template <int GATHER>
 __global__ __launch_bounds__(kBlock) void k_tsp_step_mfma(StepArgs a, TspArgs ts) {
   extern __shared__ __align__(16) unsigned char s_raw[];
-  float *v = reinterpret_cast<float *>(s_raw);  // (n rounded up to even) x 64
+  float *v = reinterpret_cast<float *>(s_raw);  // nep x 64, rows n.. zero
   if (a.ctrl->done) return;
   const int64_t n = ts.n, lda = a.lda, tiles = lda / 64;
-  const int64_t ne = (n + 1) & ~(int64_t)1;
+  const int64_t ne = ts.nep;  // K padded to a multiple of 2 U with zero terms
   const int64_t i = blockIdx.x / tiles, tile = blockIdx.x - i * tiles;
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  const float *__restrict__ Xs = (const float *)a.gsrc;
   const float c = (float)a.coupling;
   const int64_t im = (i + n - 1) % n, ip = (i + 1) % n;
-  for (int64_t l = w; l < ne; l += kWaves)
-    v[l * 64 + lane] = l < n ? tsp_g<float, GATHER>(Xs[(im * n + l) * lda + tile * 64 + lane], c) +
-                                   tsp_g<float, GATHER>(Xs[(ip * n + l) * lda + tile * 64 + lane], c)
-                             : 0.0f;
+  // Addresses: a uniform base per position block (n x lda elements) plus a
+  // 32-bit lane offset below n lda <= 2^26 (n <= 256, lda <= 2^18).
+  const uint32_t ld = (uint32_t)lda;
+  const float *__restrict__ xm_b = (const float *)a.gsrc + im * n * lda + tile * 64;
+  const float *__restrict__ xp_b = (const float *)a.gsrc + ip * n * lda + tile * 64;
+  // v_l = g(i-1, l) + g(i+1, l), SU rows per batch of loads
+  constexpr int SU = 8;
+  for (int l0 = w; l0 < (int)ne; l0 += kWaves * SU) {
+    float xm[SU], xp[SU];
+#pragma unroll
+    for (int u = 0; u < SU; u++) {
+      const int l = l0 + u * kWaves;
+      const uint32_t o = (uint32_t)(l < (int)n ? l : (int)n - 1) * ld + (uint32_t)lane;
+      xm[u] = xm_b[o];
+      xp[u] = xp_b[o];
+    }
+#pragma unroll
+    for (int u = 0; u < SU; u++) {
+      const int l = l0 + u * kWaves;
+      if (l < (int)ne)
+        v[l * 64 + lane] =
+            l < (int)n ? tsp_g<float, GATHER>(xm[u], c) + tsp_g<float, GATHER>(xp[u], c) : 0.0f;
+    }
+  }
   __syncthreads();
   const int64_t t = a.ctrl->base + a.offset;
   const float *sc = (const float *)a.sched + 4 * t;
   const float c0 = sc[0], c1 = sc[1], c2 = sc[2], c3 = sc[3];
   const float qB = (float)ts.qB, qC = (float)ts.qC;
-  const float *__restrict__ rsum = (const float *)ts.rsum;
-  const float *__restrict__ cs = (const float *)ts.csum;
-  const float *__restrict__ dq = (const float *)ts.dq;
-  const float *__restrict__ h = (const float *)a.field;
-  float *Y = (float *)a.Y;
+  const float *__restrict__ rs_b = (const float *)ts.rsum + i * lda + tile * 64;
+  const float *__restrict__ cs_b = (const float *)ts.csum + tile * 64;
+  const float *__restrict__ dq = ts.dqm;
+  const float *__restrict__ h_b = (const float *)a.field + i * n;
+  const float *__restrict__ x_b = (const float *)a.gsrc + i * n * lda + tile * 64;
+  float *__restrict__ xn_b = (float *)a.gdst + i * n * lda + tile * 64;
+  float *__restrict__ y_b = (float *)a.Y + i * n * lda + tile * 64;
   const int half = lane >> 5, col = lane & 31;
   const int64_t ktiles = (n + 31) / 32;
-  // element update of the output tile (cities kt*32.., agents at*32..)
+  // element update of the output tile (cities kt*32.., agents at*32..), 4
+  // elements per batch of loads (122 VGPRs: 3 wavefronts per SIMD)
   auto finish = [&](int64_t kt, int at, const sis_floatx16 &acc) {
-    const int64_t ag = tile * 64 + at * 32 + col;
-    const float rs = rsum[i * lda + ag];
+    const uint32_t ag = (uint32_t)(at * 32 + col);
+    const float rs = rs_b[ag];
 #pragma unroll
-    for (int q = 0; q < 16; q++) {
-      const int64_t k = kt * 32 + 8 * (q >> 2) + 4 * half + (q & 3);
-      if (k < n) {
-        const float S = fmaT(qB, rs, qC * cs[k * lda + ag]) + acc[q];
-        const int64_t e = (i * n + k) * lda + ag;
-        float x = Xs[e], y = Y[e];
-        sb_update<float, PROB_TSP>(x, y, -S, c0, c1, c2, c3, h[i * n + k]);
-        ((float *)a.gdst)[e] = x;
-        Y[e] = y;
+    for (int q0 = 0; q0 < 16; q0 += 4) {
+      float cv[4], xv[4], yv[4], hv[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int q = q0 + u;
+        const int k = (int)kt * 32 + 8 * (q >> 2) + 4 * half + (q & 3);
+        const uint32_t kc = (uint32_t)(k < (int)n ? k : (int)n - 1);
+        const uint32_t e = kc * ld + ag;
+        cv[u] = cs_b[e];
+        xv[u] = x_b[e];
+        yv[u] = y_b[e];
+        hv[u] = h_b[kc];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int q = q0 + u;
+        const int k = (int)kt * 32 + 8 * (q >> 2) + 4 * half + (q & 3);
+        if (k < (int)n) {
+          const float S = fmaT(qB, rs, qC * cv[u]) + acc[q];
+          const uint32_t e = (uint32_t)k * ld + ag;
+          float x = xv[u], y = yv[u];
+          sb_update<float, PROB_TSP>(x, y, -S, c0, c1, c2, c3, hv[u]);
+          xn_b[e] = x;
+          y_b[e] = y;
+        }
       }
     }
   };
@@ -610,18 +653,25 @@ __global__ __launch_bounds__(kBlock) void k_tsp_step_mfma(StepArgs a, TspArgs ts
     // a wavefront takes a 32-city tile for both 32-agent halves: one A
     // operand feeds two accumulators
     for (int64_t kt = w; kt < ktiles; kt += kWaves) {
-      const int64_t krow = kt * 32 + col;
-      const float *drow = dq + (krow < n ? krow : 0) * n;
+      const float *drow = dq + kt * 32 * ne;  // rows of the tile; zero rows past n
+      const uint32_t dro = (uint32_t)(col * ne);
       sis_floatx16 acc0 = {0}, acc1 = {0};
+      float an[U];  // A operands of the next K block, loaded one block ahead
+#pragma unroll
+      for (int u = 0; u < U; u++) an[u] = drow[dro + (uint32_t)(2 * u + half)];
       for (int64_t l0 = 0; l0 < ne; l0 += 2 * U) {
         float av[U], b0[U], b1[U];
 #pragma unroll
+        for (int u = 0; u < U; u++) av[u] = an[u];
+        if (l0 + 2 * U < ne) {
+#pragma unroll
+          for (int u = 0; u < U; u++) an[u] = drow[dro + (uint32_t)(l0 + 2 * U + 2 * u + half)];
+        }
+#pragma unroll
         for (int u = 0; u < U; u++) {
           const int64_t l = l0 + 2 * u + half;
-          const bool in = l < ne;
-          av[u] = (krow < n && l < n) ? drow[l] : 0.0f;
-          b0[u] = in ? v[l * 64 + col] : 0.0f;
-          b1[u] = in ? v[l * 64 + 32 + col] : 0.0f;
+          b0[u] = v[l * 64 + col];
+          b1[u] = v[l * 64 + 32 + col];
         }
 #pragma unroll
         for (int u = 0; u < U; u++) {
@@ -637,16 +687,24 @@ __global__ __launch_bounds__(kBlock) void k_tsp_step_mfma(StepArgs a, TspArgs ts
     for (int64_t tt = w; tt < ktiles * 2; tt += kWaves) {
       const int64_t kt = tt >> 1;
       const int at = (int)(tt & 1);
-      const int64_t krow = kt * 32 + col;
-      const float *drow = dq + (krow < n ? krow : 0) * n;
+      const float *drow = dq + kt * 32 * ne;  // rows of the tile; zero rows past n
+      const uint32_t dro = (uint32_t)(col * ne);
       sis_floatx16 acc = {0};
+      float an[U];  // A operands of the next K block, loaded one block ahead
+#pragma unroll
+      for (int u = 0; u < U; u++) an[u] = drow[dro + (uint32_t)(2 * u + half)];
       for (int64_t l0 = 0; l0 < ne; l0 += 2 * U) {
         float av[U], bv[U];
 #pragma unroll
+        for (int u = 0; u < U; u++) av[u] = an[u];
+        if (l0 + 2 * U < ne) {
+#pragma unroll
+          for (int u = 0; u < U; u++) an[u] = drow[dro + (uint32_t)(l0 + 2 * U + 2 * u + half)];
+        }
+#pragma unroll
         for (int u = 0; u < U; u++) {
           const int64_t l = l0 + 2 * u + half;
-          av[u] = (krow < n && l < n) ? drow[l] : 0.0f;
-          bv[u] = l < ne ? v[l * 64 + at * 32 + col] : 0.0f;
+          bv[u] = v[l * 64 + at * 32 + col];
         }
 #pragma unroll
         for (int u = 0; u < U; u++)
@@ -717,7 +775,7 @@ void launch_tsp_struct_step(hipStream_t st, int form, int gather, bool mfma, con
   const unsigned sums_grid = grid_for(2 * ts.n * tiles, kWaves, INT32_MAX);
   const dim3 grid((unsigned)(ts.n * tiles));
   if (mfma && form == 0) {
-    const size_t shm = (size_t)(((ts.n + 1) & ~(int64_t)1) * 64 * sizeof(float));
+    const size_t shm = (size_t)(ts.nep * 64 * sizeof(float));
     if (gather == G_BSB) {
       hipLaunchKernelGGL((k_tsp_sums<float, G_BSB>), dim3(sums_grid), dim3(kBlock), 0, st, a, ts);
       hipLaunchKernelGGL((k_tsp_step_mfma<G_BSB>), grid, dim3(kBlock), shm, st, a, ts);

@@ -95,9 +95,12 @@ struct SplitArgs {
 // terms), so no CSR is needed.  rsum / csum are n x lda per step.
 struct TspArgs {
   const void *dq;  // n x n, A d_kl / 8, form precision
+  // FP32 A d_kl / 8 zero-padded to (n rounded up to 32) x nep rows / columns
+  // (nep = n rounded up to 8), for the matrix-core kernel
+  const float *dqm;
   void *rsum, *csum;
   double qB, qC;   // B/4, C/4
-  int64_t n;
+  int64_t n, nep;
 };
 
 bool tsp_struct_supported(int form, int gather);

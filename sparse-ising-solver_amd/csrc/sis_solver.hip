@@ -76,6 +76,8 @@ struct sis_solver {
   double qB = 0.0, qC = 0.0;           // B/4, C/4
   DevBuf<double> dqd;                  // A d_kl / 8, n x n
   DevBuf<float> dqf;
+  DevBuf<float> dqm;                   // FP32, zero-padded for the matrix cores
+  int64_t tsp_nep = 0;                 // n rounded up to 8
   DevBuf<char> tsum_r, tsum_c;         // per-step row / column sums, form precision
   DevBuf<double> esum_r, esum_c;       // the same for the energy, FP64
 
@@ -84,9 +86,11 @@ struct sis_solver {
     ts.dq = energy || f64 ? (const void *)dqd.p : (const void *)dqf.p;
     ts.rsum = energy ? (void *)esum_r.p : (void *)tsum_r.p;
     ts.csum = energy ? (void *)esum_c.p : (void *)tsum_c.p;
+    ts.dqm = dqm.p;
     ts.qB = qB;
     ts.qC = qC;
     ts.n = tsp_n;
+    ts.nep = tsp_nep;
     return ts;
   }
 
@@ -438,6 +442,14 @@ int sis_solver_create_tsp(const sis_tsp *t, const sis_params *pin, sis_solver **
     std::vector<float> dqf(dq.begin(), dq.end());
     if ((st = s->dqf.alloc(dqf.size()))) return st;
     HIPCHK(hipMemcpy(s->dqf.p, dqf.data(), sizeof(float) * dqf.size(), hipMemcpyHostToDevice));
+    // (n rounded up to 32) x nep, zero outside n x n
+    const int64_t nep = (n + 7) & ~(int64_t)7, rows = (n + 31) / 32 * 32;
+    std::vector<float> dm((size_t)(rows * nep), 0.0f);
+    for (int64_t k = 0; k < n; k++)
+      for (int64_t l = 0; l < n; l++) dm[(size_t)(k * nep + l)] = dqf[(size_t)(k * n + l)];
+    if ((st = s->dqm.alloc(dm.size()))) return st;
+    HIPCHK(hipMemcpy(s->dqm.p, dm.data(), sizeof(float) * dm.size(), hipMemcpyHostToDevice));
+    s->tsp_nep = nep;
   }
   *out = s.release();
   return SIS_OK;
