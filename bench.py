@@ -135,8 +135,13 @@ def cpu_baseline(J, A, dt, xi0, seconds, seed):
 
 
 def run_solver(J, params, T, uw, seed, field=None, autotune=True, tsp=None):
-    """Timed sis_solver_run (pre-step + T steps + windows) of one configuration."""
-    s = sis.Solver(J, params, field=field)
+    """Timed sis_solver_run (pre-step + T steps + windows) of one configuration.
+    With params["structured"] the TSP solver is built from `tsp` directly (no
+    matrix)."""
+    if params.get("structured"):
+        s = sis.Solver.from_tsp(tsp, params)
+    else:
+        s = sis.Solver(J, params, field=field)
     tune = None
     if autotune:
         s.init_random(seed + 1)
@@ -192,12 +197,16 @@ def small_configs(a):
     elif a.workload == "tsp48":
         t = sis.Tsp.generate(48, 1000, 1, a.seed)
         J, h = t.qubo()
-        for form, split in (("f32", 0), ("bf16", 0), ("f32", -1), ("bf16", -1)):
+        # CSR forms with and without the row split, then the structured
+        # product (no matrix, distance GEMM on the matrix cores)
+        for form, split, st in (("f32", 0, 0), ("bf16", 0, 0), ("f32", -1, 0), ("bf16", -1, 0),
+                                ("f32", 0, 1)):
             r = run_solver(J, dict(problem=sis.TSP, agents=1024, dt=0.1, form=sis.FORMS[form],
-                                   dynamics=sis.BSB, ncities=48, use_graph=1, split=split),
+                                   dynamics=sis.BSB, ncities=48, use_graph=1, split=split,
+                                   structured=st),
                            a.steps if a.steps != 20000 else 10000, 0, a.seed, field=h, tsp=t)
             lines.append(dict(workload="TSP n=48 (2304 spins) bSB, 1024 agents, 10000 steps",
-                              form=form, split_requested=split, nnz=J.nnz,
+                              form=form, split_requested=split, structured=st, nnz=J.nnz,
                               **{k: v for k, v in r.items() if k != "info"},
                               ms_per_step=r["wall"] * 1e3 / r["info"]["steps_executed"],
                               step_kernel_ms=r["info"]["step_kernel_ms"]))

@@ -1,0 +1,51 @@
+"""bench.py's output contract: one JSON line with the metric, the roofline of
+the dominant kernel and the CPU baseline; a multi-GPU request exits with a
+message (the solver runs on one device)."""
+import json
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
+
+
+def run_bench(*args, timeout=300):
+    return subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), *args], cwd=ROOT,
+                          capture_output=True, text=True, timeout=timeout)
+
+
+def test_more_than_one_gpu_exits_with_message():
+    r = run_bench("--gpus", "2")
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
+
+
+@pytest.mark.gpu
+def test_headline_line_has_the_contract_fields():
+    # a small instance of the headline workload: random 4-regular graph,
+    # packed dSB against plain FP32 dSB, CPU baseline on a 1 s sample
+    r = run_bench("--nodes-log2", "14", "--agents", "128", "--steps", "300", "--warmup", "10",
+                  "--cpu-seconds", "1")
+    assert r.returncode == 0, r.stderr[-2000:]
+    lines = [l for l in r.stdout.splitlines() if l.strip()]
+    assert len(lines) == 1
+    d = json.loads(lines[0])
+    for k in ("metric", "value", "unit", "n_gpus", "steps", "warmup", "ms_per_step",
+              "higher_is_better", "scaling", "vs_baseline", "dtype", "data", "config", "roofline",
+              "cpu_baseline"):
+        assert k in d, k
+    assert d["n_gpus"] == 1 and d["steps"] == 300 and d["warmup"] == 10
+    assert d["value"] > 0 and d["unit"] == "agent-steps/s" and d["higher_is_better"] is True
+    assert "workload" in d["config"]
+    rf = d["roofline"]
+    assert rf["bound"] == "hbm" and rf["unit"] == "GB/s" and rf["peak"] == 8000.0
+    assert rf["achieved"] > 0 and abs(rf["frac"] - rf["achieved"] / rf["peak"]) < 1e-12
+    assert "traffic" in rf  # null away from the profiled size
+    cb = d["cpu_baseline"]
+    assert cb["value"] > 0 and cb["cores"] >= 1 and cb["kind"] in ("port", "reference")
+    assert cb["sample"]
+    # packed and plain dSB follow the same trajectory: identical best cut
+    assert d["best_cut_rel_diff_vs_plain"] == 0.0
+    assert d["energy_check"]["max_abs_err_cut"] == 0.0
