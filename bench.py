@@ -77,9 +77,21 @@ def time_form(J, form, A, dt, K, W, seed, dyn, autotune=True):
     wall = time.perf_counter() - t0
     cuts, best = s.cuts()
     check = energy_check(J, s, cuts, best)
+    cont = continuous_energy(J, s) if dyn == sis.BSB else None
     s.close()
     return dict(info=info, wall=wall, best_cut=float(cuts.max()), best_agent=int(best),
-                mean_cut=float(cuts.mean()), tune=tune, energy_check=check)
+                mean_cut=float(cuts.mean()), tune=tune, energy_check=check, cont_energy=cont)
+
+
+def continuous_energy(J, s, nagents=16):
+    """bSB: energy (1/2) x^T J x of the continuous state X after the run, for
+    the first agents, evaluated in FP64 on the host (scipy)."""
+    import scipy.sparse as sp
+
+    rp, col, val = J.arrays()
+    M = sp.csr_matrix((val, col, rp), shape=(J.n, J.n))
+    x = s.get_state()[0][:, :min(nagents, s.A)]
+    return 0.5 * np.einsum("ia,ia->a", x, M @ x)
 
 
 def energy_check(J, s, cuts, best, nagents=16):
@@ -178,11 +190,14 @@ def small_configs(a):
         mean_deg = J.nnz / J.n
         sigma = 0.7 / (J.xi0() * np.sqrt(J.n))  # sample std of all N^2 entries
         c0 = 0.5 / (np.sqrt(mean_deg) * sigma)
-        for g in (0, 1, 2):
+        # the configured c0, then the reference's own xi0 = 0.7 / (sigma sqrt N)
+        for g, xi0, kind in ((0, c0, "configured"), (1, c0, "configured"), (2, c0, "configured"),
+                             (1, J.xi0(), "reference")):
             r = run_solver(J, dict(problem=sis.MAXCUT, agents=128, dt=0.1, dynamics=sis.BSB,
-                                   form=sis.FORM_F32, xi0=c0, use_graph=g), 2000, 0, a.seed)
+                                   form=sis.FORM_F32, xi0=xi0, use_graph=g), 2000, 0, a.seed)
             lines.append(dict(workload="ER N=800 p=0.06 bSB plain FP32, 128 agents, 2000 steps",
-                              use_graph=g, nnz=J.nnz, xi0=c0, **{k: v for k, v in r.items() if k != "info"},
+                              use_graph=g, nnz=J.nnz, xi0=xi0, xi0_kind=kind,
+                              **{k: v for k, v in r.items() if k != "info"},
                               ms_per_step=r["wall"] * 1e3 / r["info"]["steps_executed"]))
     elif a.workload == "torus100":
         J = sis.Csr.gen_torus(100, 100, sis.W_PM1, a.seed)
@@ -430,7 +445,8 @@ def main():
         "dtype": "f32" if a.form != "f64" else "f64",
         "data": "synthetic (seeded random 4-regular graph, +-1 weights, seeded initial state)",
         "config": {
-            "workload": f"maxcut dSB, random 4-regular N=2^{a.nodes_log2}, Z={Z}, +-1 weights, "
+            "workload": f"maxcut {'bSB' if dyn == sis.BSB else 'dSB'}, random 4-regular "
+                        f"N=2^{a.nodes_log2}, Z={Z}, +-1 weights, "
                         f"{A} agents, dt {dt}, {K} steps, window every 100 steps",
             "form": a.form,
             "nodes": N,
@@ -475,6 +491,14 @@ def main():
         }
         out["speedup_vs_plain"] = value / bvalue
         out["best_cut_rel_diff_vs_plain"] = (head["best_cut"] - base["best_cut"]) / abs(base["best_cut"])
+        if head["cont_energy"] is not None and base["cont_energy"] is not None:
+            # bSB: how far the form's rounding moves the continuous-state energy
+            # from the plain FP32 trajectory at equal steps (first 16 agents)
+            eh, eb = head["cont_energy"], base["cont_energy"]
+            out["continuous_energy_vs_plain"] = dict(
+                agents=len(eb), max_rel_diff=float(np.max(np.abs(eh - eb) / np.abs(eb))),
+                mean_rel_diff=float(np.mean(np.abs(eh - eb) / np.abs(eb))),
+                host="scipy FP64 (1/2) x^T J x of each run's final FP32 state")
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(J, A, dt, xi0, a.cpu_seconds, a.seed)
     print(json.dumps(out), flush=True)
