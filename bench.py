@@ -227,10 +227,11 @@ def small_configs(a):
                               step_kernel_ms=r["info"]["step_kernel_ms"]))
     elif a.workload == "tsp_struct":
         # TSP bSB FP32 with the QUBO's CSR against the structured product (no
-        # matrix): n = 48 (the 10,000-step configuration), 128 and 256 cities.
-        for n, A, T in ((48, 1024, 10000), (128, 256, 1000), (256, 256, 1000)):
+        # matrix): n = 48 (the 10,000-step configuration), 128 and 256 cities;
+        # n = 512 structured only (its CSR would hold 537M entries).
+        for n, A, T in ((48, 1024, 10000), (128, 256, 1000), (256, 256, 1000), (512, 256, 500)):
             t = sis.Tsp.generate(n, 1000, 1, a.seed)
-            for structured in (1, 0):
+            for structured in ((1, 0) if n <= 256 else (1,)):
                 params = dict(problem=sis.TSP, agents=A, dt=0.1, form=sis.FORM_F32,
                               dynamics=sis.BSB, structured=structured)
                 t0 = time.perf_counter()

@@ -200,7 +200,7 @@ typedef struct sis_params {
                                weights); otherwise within FP rounding of it. */
   int32_t structured;       /* TSP from sis_solver_create_tsp: 1 evaluates J x from row
                                sums, column sums and the distance table instead of the
-                               QUBO's CSR (n >= 3; f32 n <= 256, f64 n <= 128) */
+                               QUBO's CSR (n >= 3; f32 n <= 512, f64 n <= 256) */
 } sis_params;
 
 typedef struct sis_run_info {

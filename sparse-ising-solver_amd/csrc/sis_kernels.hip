@@ -769,6 +769,18 @@ bool tsp_struct_supported(int form, int gather) {
   return (form == 0 || form == 1) && (gather == G_DSB || gather == G_BSB);
 }
 
+// Dynamic LDS above 64 KB (the staged v at n > 256 in FP32, n > 128 in FP64):
+// raise the kernel's limit once to the 160 KB of a gfx950 CU.
+template <auto K>
+static void allow_big_lds() {
+  static bool done = false;  // once per kernel
+  if (!done) {
+    (void)hipFuncSetAttribute((const void *)K, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              160 * 1024);
+    done = true;
+  }
+}
+
 void launch_tsp_struct_step(hipStream_t st, int form, int gather, bool mfma, const StepArgs &a,
                             const TspArgs &ts) {
   const int64_t tiles = a.lda / 64;
@@ -777,9 +789,11 @@ void launch_tsp_struct_step(hipStream_t st, int form, int gather, bool mfma, con
   if (mfma && form == 0) {
     const size_t shm = (size_t)(ts.nep * 64 * sizeof(float));
     if (gather == G_BSB) {
+      allow_big_lds<k_tsp_step_mfma<G_BSB>>();
       hipLaunchKernelGGL((k_tsp_sums<float, G_BSB>), dim3(sums_grid), dim3(kBlock), 0, st, a, ts);
       hipLaunchKernelGGL((k_tsp_step_mfma<G_BSB>), grid, dim3(kBlock), shm, st, a, ts);
     } else {
+      allow_big_lds<k_tsp_step_mfma<G_DSB>>();
       hipLaunchKernelGGL((k_tsp_sums<float, G_DSB>), dim3(sums_grid), dim3(kBlock), 0, st, a, ts);
       hipLaunchKernelGGL((k_tsp_step_mfma<G_DSB>), grid, dim3(kBlock), shm, st, a, ts);
     }
@@ -787,6 +801,7 @@ void launch_tsp_struct_step(hipStream_t st, int form, int gather, bool mfma, con
   }
 #define SIS_TSPS(T, G)                                                                       \
   if (form == (sizeof(T) == 8 ? 1 : 0) && gather == G) {                                    \
+    allow_big_lds<k_tsp_step_struct<T, G>>();                                               \
     hipLaunchKernelGGL((k_tsp_sums<T, G>), dim3(sums_grid), dim3(kBlock), 0, st, a, ts);    \
     hipLaunchKernelGGL((k_tsp_step_struct<T, G>), grid, dim3(kBlock),                       \
                        (size_t)(ts.n * 64 * sizeof(T)), st, a, ts);                         \

@@ -402,8 +402,9 @@ int sis_solver_create_tsp(const sis_tsp *t, const sis_params *pin, sis_solver **
   if (n < 3) return fail(SIS_ERR_ARG, "sis_solver_create_tsp: the structured product needs n >= 3");
   if (p.form != SIS_FORM_F32 && p.form != SIS_FORM_F64)
     return fail(SIS_ERR_ARG, "sis_solver_create_tsp: the structured product runs the plain forms");
-  if (n > (p.form == SIS_FORM_F32 ? 256 : 128))
-    return fail(SIS_ERR_ARG, "sis_solver_create_tsp: structured n must be <= 256 (f32) / 128 (f64)");
+  // v (n x 64 values of the form) is staged in LDS: at most 128 KB
+  if (n > (p.form == SIS_FORM_F32 ? 512 : 256))
+    return fail(SIS_ERR_ARG, "sis_solver_create_tsp: structured n must be <= 512 (f32) / 256 (f64)");
   if (p.split != 0) return fail(SIS_ERR_ARG, "sis_solver_create_tsp: no row split without a matrix");
   // the QUBO's constants and field (src/tsp_gpu.cu:81-83, src/kernels.cu:591-592)
   const double A = 1, B = t->maxd, C = t->maxd;
@@ -414,6 +415,7 @@ int sis_solver_create_tsp(const sis_tsp *t, const sis_params *pin, sis_solver **
   E.n = N;
   E.rp.assign((size_t)N + 1, 0);
   E.sum = 0.0;
+  p.tile = 0;  // the structured tile (checked below) is not a row-kernel tile
   sis_solver *sp = nullptr;
   int st = sis_solver_create(&E, h.data(), &p, &sp);
   if (st) return st;
@@ -424,6 +426,7 @@ int sis_solver_create_tsp(const sis_tsp *t, const sis_params *pin, sis_solver **
   if (pin->tile == 0) s->vec = s->f64 ? 1 : 2;
   else if (pin->tile != 1 && !(pin->tile == 2 && !s->f64))
     return fail(SIS_ERR_ARG, "sis_solver_create_tsp: structured tile must be 1 (VALU) or 2 (MFMA, f32)");
+  else s->vec = pin->tile;
   s->tsp_n = n;
   s->qB = B / 4;
   s->qC = C / 4;

@@ -138,6 +138,44 @@ def test_create_tsp_matrix_mode_equals_create():
     assert np.array_equal(a.energy(sis.SPINS_FINAL), b.energy(sis.SPINS_FINAL))
 
 
+@pytest.mark.parametrize("n,form,tile", [(300, "f32", 2), (300, "f32", 1), (200, "f64", 1)])
+def test_structured_large_n_dsb_equals_csr(n, form, tile):
+    """Above 64 KB of staged v (n > 256 in FP32, n > 128 in FP64): dSB through
+    the structured product is bit-identical to dSB through the QUBO's CSR
+    (both sums are exact multiples of 1/8)."""
+    t = sis.Tsp.generate(n, 1000, 1, 7)
+    A, T = 64, 1000
+    runs = []
+    for structured in (1, 0):
+        params = dict(agents=A, form=sis.FORMS[form], dynamics=sis.DSB, dt=0.1,
+                      structured=structured)
+        if structured:
+            params["tile"] = tile
+        s = sis.Solver.from_tsp(t, params)
+        s.init_random(5)
+        s.advance(0, 60, T)
+        runs.append(s.get_state())
+        s.close()
+    (xs, ys), (xc, yc) = runs
+    np.testing.assert_array_equal(xs, xc)
+    np.testing.assert_array_equal(ys, yc)
+
+
+def test_structured_n512_tiles_agree():
+    """The largest structured FP32 instance (n = 512, 128 KB of staged v):
+    matrix cores and VALU give the same dSB trajectory."""
+    t = sis.Tsp.generate(512, 1000, 1, 8)
+    out = []
+    for tile in (1, 2):
+        s = sis.Solver.from_tsp(t, dict(agents=64, form=sis.FORM_F32, dynamics=sis.DSB, dt=0.1,
+                                        structured=1, tile=tile))
+        s.init_random(6)
+        s.advance(0, 30, 1000)
+        out.append(s.get_state()[0])
+        s.close()
+    np.testing.assert_array_equal(out[0], out[1])
+
+
 def test_structured_argument_errors():
     t2 = sis.Tsp.generate(2, 50, 0, 1)
     with pytest.raises(sis.SisError):
@@ -147,9 +185,12 @@ def test_structured_argument_errors():
         sis.Solver.from_tsp(t, dict(agents=64, form=sis.FORM_BF16, structured=1))
     with pytest.raises(sis.SisError):
         sis.Solver.from_tsp(t, dict(agents=64, split=16, structured=1))
-    big = sis.Tsp.generate(129, 500, 0, 1)
+    big = sis.Tsp.generate(257, 500, 0, 1)
     with pytest.raises(sis.SisError):
         sis.Solver.from_tsp(big, dict(agents=64, form=sis.FORM_F64, structured=1))
+    with pytest.raises(sis.SisError):
+        sis.Solver.from_tsp(sis.Tsp.generate(513, 500, 0, 1),
+                            dict(agents=64, form=sis.FORM_F32, structured=1))
     J, h = t.qubo()
     with pytest.raises(sis.SisError):
         sis.Solver(J, dict(problem=sis.TSP, agents=64, ncities=8, structured=1), field=h)
