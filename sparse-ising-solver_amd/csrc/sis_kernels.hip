@@ -517,9 +517,23 @@ __global__ __launch_bounds__(kBlock) void k_tsp_step_struct(StepArgs a, TspArgs 
   const T *__restrict__ Xs = (const T *)a.gsrc;
   const T c = (T)a.coupling;
   const int64_t im = (i + n - 1) % n, ip = (i + 1) % n;
-  for (int64_t l = w; l < n; l += kWaves)
-    v[l * 64 + lane] = tsp_g<T, GATHER>(Xs[(im * n + l) * lda + a0], c) +
-                       tsp_g<T, GATHER>(Xs[(ip * n + l) * lda + a0], c);
+  // v_l = g(i-1, l) + g(i+1, l), SU rows per batch of loads
+  constexpr int SU = 8;
+  for (int64_t l0 = w; l0 < n; l0 += (int64_t)kWaves * SU) {
+    T xm[SU], xp[SU];
+#pragma unroll
+    for (int u = 0; u < SU; u++) {
+      const int64_t l = l0 + (int64_t)u * kWaves;
+      const int64_t lc = l < n ? l : n - 1;
+      xm[u] = Xs[(im * n + lc) * lda + a0];
+      xp[u] = Xs[(ip * n + lc) * lda + a0];
+    }
+#pragma unroll
+    for (int u = 0; u < SU; u++) {
+      const int64_t l = l0 + (int64_t)u * kWaves;
+      if (l < n) v[l * 64 + lane] = tsp_g<T, GATHER>(xm[u], c) + tsp_g<T, GATHER>(xp[u], c);
+    }
+  }
   __syncthreads();
   const int64_t t = a.ctrl->base + a.offset;
   const T *sc = (const T *)a.sched + 4 * t;
@@ -532,6 +546,10 @@ __global__ __launch_bounds__(kBlock) void k_tsp_step_struct(StepArgs a, TspArgs 
   T *Y = (T *)a.Y;
   for (int64_t k = w; k < n; k += kWaves) {
     const T *dk = dq + k * n;
+    // the element's loads go out before the distance product
+    const int64_t e = (i * n + k) * lda + a0;
+    T x = Xs[e], y = Y[e];
+    const T hk = h[i * n + k];
     T acc = fmaT(qB, rs, qC * cs[k * lda + a0]);
     int64_t l = 0;
     for (; l + 4 <= n; l += 4) {
@@ -541,9 +559,7 @@ __global__ __launch_bounds__(kBlock) void k_tsp_step_struct(StepArgs a, TspArgs 
       acc = fmaT(dk[l + 3], v[(l + 3) * 64 + lane], acc);
     }
     for (; l < n; l++) acc = fmaT(dk[l], v[l * 64 + lane], acc);
-    const int64_t e = (i * n + k) * lda + a0;
-    T x = Xs[e], y = Y[e];
-    sb_update<T, PROB_TSP>(x, y, -acc, c0, c1, c2, c3, h[i * n + k]);
+    sb_update<T, PROB_TSP>(x, y, -acc, c0, c1, c2, c3, hk);
     ((T *)a.gdst)[e] = x;
     Y[e] = y;
   }
