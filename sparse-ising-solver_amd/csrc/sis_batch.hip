@@ -298,7 +298,10 @@ __device__ __forceinline__ uint64_t bit_transpose64(uint64_t w, int lane) {
     const int s = 32 >> st;
     const uint64_t m = kMask[st];
     const uint64_t p = shfl_xor64(w, s);
-    w = (lane & s) ? (((p >> s) & m) | (w & ~m)) : ((w & m) | ((p & m) << s));
+    // both halves, then a bitwise select (no divergent branch per stage)
+    const uint64_t hi = ((p >> s) & m) | (w & ~m), lo = (w & m) | ((p & m) << s);
+    const uint64_t sel = 0ull - (uint64_t)((lane & s) != 0);
+    w = (hi & sel) | (lo & ~sel);
   }
   return w;
 }
