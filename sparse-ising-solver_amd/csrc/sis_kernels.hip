@@ -1054,18 +1054,43 @@ __global__ __launch_bounds__(kBlock) void k_cut_rows(const int64_t *__restrict__
   for (int64_t w = 0; w < W; w++) {
     double acc = 0.0;
     if (ka < kb) {
-      int64_t i = i0, iend = rp[i0 + 1];
-      double si = ((spins[i * W + w] >> lane) & 1) ? -1.0 : 1.0;
-      for (int64_t k = ka; k < kb; k++) {
-        if (k >= iend) {
-          do {
-            i++;
-            iend = rp[i + 1];
-          } while (k >= iend);
-          si = ((spins[i * W + w] >> lane) & 1) ? -1.0 : 1.0;
+      // 64 entries per pass: lane l loads entry kc + l's weight and column
+      // sign word, and the end and sign word of row ib + l (a block of the
+      // rows the pass walks through); the terms are then added one entry at a
+      // time in CSR order from readlane broadcasts.
+      int64_t i = i0, ib = i0;
+      auto load_rows = [&](int64_t base, int64_t &bnd, uint64_t &wi) {
+        const int64_t rr = base + lane;
+        bnd = rr < N ? rp[rr + 1] : INT64_MAX;
+        wi = rr < N ? spins[rr * W + w] : 0;
+      };
+      int64_t bnd;
+      uint64_t wi;
+      load_rows(ib, bnd, wi);
+      int64_t iend = (int64_t)readlane64((uint64_t)bnd, 0);
+      double si = ((readlane64(wi, 0) >> lane) & 1) ? -1.0 : 1.0;
+      for (int64_t kc = ka; kc < kb; kc += 64) {
+        const int cnt = kb - kc < 64 ? (int)(kb - kc) : 64;
+        const int64_t kk = lane < cnt ? kc + lane : kc;
+        const double vk = val[kk];
+        const uint64_t wj = spins[(int64_t)col[kk] * W + w];
+        for (int e = 0; e < cnt; e++) {
+          const int64_t k = kc + e;
+          if (k >= iend) {
+            do {
+              i++;
+              if (i - ib == 64) {  // past the loaded block of rows
+                ib = i;
+                load_rows(ib, bnd, wi);
+              }
+              iend = (int64_t)readlane64((uint64_t)bnd, (int)(i - ib));
+            } while (k >= iend);
+            si = ((readlane64(wi, (int)(i - ib)) >> lane) & 1) ? -1.0 : 1.0;
+          }
+          const double sj = ((readlane64(wj, e) >> lane) & 1) ? -1.0 : 1.0;
+          const double v = __longlong_as_double((long long)readlane64((uint64_t)__double_as_longlong(vk), e));
+          acc = __builtin_fma(v, si * sj, acc);
         }
-        const double sj = ((spins[(int64_t)col[k] * W + w] >> lane) & 1) ? -1.0 : 1.0;
-        acc = __builtin_fma(val[k], si * sj, acc);
       }
       if (MODE == 0) acc *= 0.5;
     }
