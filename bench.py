@@ -9,8 +9,9 @@ same trajectory (bit-identical) beside it for the speed-up and the
 equal-steps best-cut comparison.
 
 The timed region is one sis_solver_run (pre-step + K steps + K/100 windows),
-with the state resident in HBM; update_window is off inside the timed region
-so that exactly K steps run.  The step-kernel time comes from HIP events the
+with the state resident in HBM; the update window's early exit is off inside
+the timed region so that exactly K steps run (on this graph no agent locks
+within 20,000 steps, so the exit would not fire either).  The step-kernel time comes from HIP events the
 library records around the step launches on the solver's stream.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--form packed|f32|bf16|f64]
@@ -448,7 +449,8 @@ def main():
         "config": {
             "workload": f"maxcut {'bSB' if dyn == sis.BSB else 'dSB'}, random 4-regular "
                         f"N=2^{a.nodes_log2}, Z={Z}, +-1 weights, "
-                        f"{A} agents, dt {dt}, {K} steps, window every 100 steps",
+                        f"{A} agents, dt {dt}, {K} steps, update window every 100 steps "
+                        f"(locks counted, early exit off: exactly {K} steps)",
             "form": a.form,
             "nodes": N,
             "nnz": Z,
