@@ -141,10 +141,16 @@ static int default_tile(const sis_solver *s) {
   int v = 1;
   if (s->p.form == SIS_FORM_PACKED) {
     while (v < 4 && s->W % (v * 2) == 0) v *= 2;
-    return v | (1 << 8);
+    v |= 1 << 8;
+  } else {
+    const int vmax = s->f64 ? 2 : 4;
+    while (v < vmax && s->lda % (64 * v * 2) == 0) v *= 2;
   }
-  const int vmax = s->f64 ? 2 : 4;
-  while (v < vmax && s->lda % (64 * v * 2) == 0) v *= 2;
+  // Non-temporal own-row X / Y once the state (X and Y) is larger than the
+  // 256 MB Infinity Cache: streamed once per step, it then does not displace
+  // the gathered neighbour data (the autotuner's pick at N = 2^20, 256 agents).
+  const double state = (double)s->N * (double)s->lda * (s->f64 ? 16.0 : 8.0);
+  if (state > 256.0 * 1024 * 1024 && tile_ok(s, v | (1 << 16))) v |= 1 << 16;
   return v;
 }
 
