@@ -328,7 +328,7 @@ typedef struct sis_batch_params {
                             prefetch << 20 | sign words staged in LDS << 21
                             (2, 4 or 8 rows) | LDS segment code s << 22
                             (segments of 512 << s rows, s = 0..3);
-                            0 = default */
+                            0 = default (8 rows, non-temporal, bit-transpose) */
   int32_t eager;         /* 1: launch each chunk's kernels directly instead of
                             replaying the captured hipGraph */
 } sis_batch_params;

@@ -1408,7 +1408,9 @@ int sis_batch_create(const sis_batch_params *pin, sis_batch **out) {
   if (!b) return fail(SIS_ERR_NOMEM, "out of host memory");
   b->W = p.agents_max / 64;
   b->lda = p.agents_max;
-  if (p.tile == 0) p.tile = 2 | (1 << 16);
+  // default: bit-transpose sums (other weights take its readlane path), 8 rows
+  // per task, non-temporal state -- the autotuner's pick on mixed serving sets
+  if (p.tile == 0) p.tile = 8 | (1 << 16) | (1 << 19);
   if (!batch_tile_ok((int)b->W, p.tile)) return fail(SIS_ERR_ARG, "sis_batch_create: unsupported tile");
   b->tile = p.tile;
   b->p = p;
