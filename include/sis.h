@@ -189,7 +189,11 @@ typedef struct sis_params {
   int32_t tile;             /* wavefront tile, 0 = default: agents per lane (1, 2, 4)
                                for the plain and BF16 forms; for the packed form
                                words (1, 2, 4, 8) | rows (1, 2, 4) << 8 |
-                               non-temporal state (0, 1) << 16 */
+                               non-temporal state (0, 1) << 16; or resident
+                               1 << 20 | log2 G << 21 (one workgroup per G = 1..8
+                               agents, state in registers / LDS for a window
+                               period; fused MaxCut FP32, weights +-1,
+                               N <= 32767); structured TSP: 1 VALU, 2 MFMA */
   int32_t fused;            /* MaxCut: 1 fused step (src/kernels.cu:211-269), 0 the
                                four-kernel path (src/ising.cu:167-172); plain forms */
   int32_t split;            /* row split: rows with more than `split` stored entries

@@ -839,6 +839,155 @@ void launch_tsp_struct_energy(hipStream_t st, const uint64_t *spins, const doubl
                      spins, field, ts, W, lda, part);
 }
 
+// ------------------------------------------------- resident small problems
+//
+// Agents are independent, so one workgroup can own G agents of a small
+// problem outright: their X and Y stay in registers and the gather source (the
+// sign bits, or X itself for bSB) in LDS for `steps` consecutive steps, with
+// one workgroup barrier per step and no HBM traffic in between.  The CSR is
+// copied into LDS once per launch as row offsets (uint32) and 16-bit entries:
+// column in bits 0..14, weight sign in bit 15 (weights +-1, N <= 32767).
+// Pair q = row * G + agent is owned by thread q mod 1024 (register slot
+// q / 1024), so the G agents of a row sit in consecutive lanes of one
+// wavefront.  Every term is added in CSR order with the row kernels'
+// operations: results are bit-identical to k_step_plain / k_step_packed.
+constexpr int kResBlock = 1024;
+
+template <int GATHER, int M>
+__global__ __launch_bounds__(kResBlock) void k_resident(StepArgs a, ResArgs r) {
+  extern __shared__ __align__(16) unsigned char s_raw[];
+  if (a.ctrl->done) return;
+  const int N = (int)a.N, G = r.G, lg = r.lgG;
+  const int NG = N * G;
+  const int64_t lda = a.lda;
+  const int t = threadIdx.x, lane = t & 63;
+  const int64_t g0 = (int64_t)blockIdx.x * G;
+  const int Z = (int)a.rp[N];
+  uint32_t *srp = reinterpret_cast<uint32_t *>(s_raw);
+  uint16_t *sent = reinterpret_cast<uint16_t *>(srp + N + 1);
+  unsigned char *snap = s_raw + r.snap_off;  // two buffers of r.snap_bytes
+  for (int i = t; i <= N; i += kResBlock) srp[i] = (uint32_t)a.rp[i];
+  const float *__restrict__ val = (const float *)a.val;
+  for (int k = t; k < Z; k += kResBlock)
+    sent[k] = (uint16_t)(a.col[k] | (signbit(val[k]) ? 0x8000 : 0));
+  // own pairs, and the snapshot of step 0
+  float x[M], y[M];
+  const float *__restrict__ Xin = r.Xin;
+  float *Y = (float *)a.Y;
+#pragma unroll
+  for (int m = 0; m < M; m++) {
+    const int q = t + kResBlock * m;
+    x[m] = y[m] = 0.0f;
+    if (q < NG) {
+      const int64_t e = (int64_t)(q >> lg) * lda + g0 + (q & (G - 1));
+      x[m] = Xin[e];
+      y[m] = Y[e];
+    }
+  }
+  auto put_snapshot = [&](unsigned char *buf) {
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+      const int q = t + kResBlock * m;
+      if (GATHER == G_DSB) {
+        // the G sign bits of a row are G consecutive lanes: one byte per row
+        const uint64_t bits = __ballot(q < NG && signbit(x[m]));
+        if (q < NG && (q & (G - 1)) == 0)
+          buf[q >> lg] = (unsigned char)((bits >> (lane & ~(G - 1))) & ((1u << G) - 1));
+      } else if (q < NG) {
+        reinterpret_cast<float *>(buf)[q] = x[m];
+      }
+    }
+  };
+  put_snapshot(snap);
+  __syncthreads();
+  const float coupling = (float)a.coupling;
+  // step coefficients, loaded one step ahead so no step waits on them
+  const float *sc = (const float *)a.sched + 4 * (a.ctrl->base + a.offset);
+  float n0 = sc[0], n1 = sc[1], n2 = sc[2];
+  for (int st = 0; st < r.steps; st++) {
+    const float c0 = n0, c1 = n1, c2 = n2;
+    if (st + 1 < r.steps) {
+      n0 = sc[4 * (st + 1)];
+      n1 = sc[4 * (st + 1) + 1];
+      n2 = sc[4 * (st + 1) + 2];
+    }
+    const unsigned char *cur = snap + (st & 1) * r.snap_bytes;
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+      const int q = t + kResBlock * m;
+      if (q < NG) {
+        const int row = q >> lg, ag = q & (G - 1);
+        float F = 0.0f;
+        uint32_t k = srp[row];
+        const uint32_t k1 = srp[row + 1];
+        // gather term of entry e (weight +-1 from bit 15)
+        auto term = [&](float Fin, uint32_t e, float g) {
+          const float v = (e & 0x8000) ? -1.0f : 1.0f;
+          if (GATHER == G_DSB) return g != 0.0f ? Fin + v : Fin - v;
+          return sb_accum<float, G_BSB>(Fin, v, g, coupling);
+        };
+        auto gather = [&](uint32_t e) {
+          const int j = (int)(e & 0x7fff);
+          if (GATHER == G_DSB) return (float)((cur[j] >> ag) & 1);
+          return reinterpret_cast<const float *>(cur)[j * G + ag];
+        };
+        // four entries' LDS loads in flight, then the terms in CSR order
+        for (; k + 4 <= k1; k += 4) {
+          uint32_t e[4];
+          float g[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) e[u] = sent[k + u];
+#pragma unroll
+          for (int u = 0; u < 4; u++) g[u] = gather(e[u]);
+#pragma unroll
+          for (int u = 0; u < 4; u++) F = term(F, e[u], g[u]);
+        }
+        for (; k < k1; k++) {
+          const uint32_t e = sent[k];
+          F = term(F, e, gather(e));
+        }
+        sb_update<float, PROB_MAXCUT>(x[m], y[m], F, c0, c1, c2, 0.0f, 0.0f);
+      }
+    }
+    put_snapshot(snap + ((st + 1) & 1) * r.snap_bytes);
+    __syncthreads();
+  }
+  float *Xout = r.Xout;
+#pragma unroll
+  for (int m = 0; m < M; m++) {
+    const int q = t + kResBlock * m;
+    if (q < NG) {
+      const int64_t e = (int64_t)(q >> lg) * lda + g0 + (q & (G - 1));
+      Xout[e] = x[m];
+      Y[e] = y[m];
+    }
+  }
+}
+
+int resident_pairs_max() { return kResBlock * 12; }  // the largest instance below: M = 12
+
+bool launch_resident(hipStream_t st, int gather, const StepArgs &a, const ResArgs &r) {
+  const int64_t NG = a.N * r.G;
+  const dim3 grid((unsigned)(a.lda / r.G));
+  const size_t shm = (size_t)r.snap_off + 2 * (size_t)r.snap_bytes;
+#define SIS_RES(GA, MM)                                                              \
+  if (gather == GA && NG <= (int64_t)kResBlock * MM) {                              \
+    allow_big_lds<k_resident<GA, MM>>();                                            \
+    hipLaunchKernelGGL((k_resident<GA, MM>), grid, dim3(kResBlock), shm, st, a, r); \
+    return true;                                                                    \
+  }
+  SIS_RES(G_DSB, 4)
+  SIS_RES(G_DSB, 8)
+  SIS_RES(G_DSB, 10)
+  SIS_RES(G_DSB, 12)
+  SIS_RES(G_BSB, 4)
+  SIS_RES(G_BSB, 8)
+  SIS_RES(G_BSB, 10)
+  SIS_RES(G_BSB, 12)
+#undef SIS_RES
+  return false;
+}
+
 // -------------------------------------------------------------- windows
 
 // Sign scan of the window (src/kernels.cu:37-44, 84-90), race-free: each

@@ -113,6 +113,23 @@ void launch_tsp_struct_step(hipStream_t st, int form, int gather, bool mfma, con
 void launch_tsp_struct_energy(hipStream_t st, const uint64_t *spins, const double *field,
                               const TspArgs &ts, int64_t W, int64_t lda, double *part);
 
+// Resident small problems (k_resident): one 1024-thread workgroup per G
+// agents runs `steps` steps with X, Y in registers and the CSR and gather
+// source in LDS.  MaxCut fused step, FP32, weights +-1, N <= 32767.
+// LDS: (N+1) uint32 row offsets, Z uint16 entries, then two snapshot buffers
+// of snap_bytes at snap_off (dSB: one byte of G <= 8 sign bits per row; bSB:
+// N x G floats).
+struct ResArgs {
+  const float *Xin;  // X at the first step (plain: the snapshot buffer)
+  float *Xout;       // X after the last step
+  int32_t G, lgG;    // agents per workgroup, log2 G
+  int32_t steps;
+  int32_t snap_off, snap_bytes;
+};
+int resident_pairs_max();  // N x G limit of one workgroup
+// false: no kernel instance for these sizes (nothing launched)
+bool launch_resident(hipStream_t st, int gather, const StepArgs &a, const ResArgs &r);
+
 bool split_supported(int form, int prob, int gather);
 void launch_step_split(hipStream_t st, int form, int prob, int gather, int64_t nent,
                        const StepArgs &a, const SplitArgs &sp);

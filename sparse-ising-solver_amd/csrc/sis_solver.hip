@@ -61,6 +61,8 @@ struct sis_solver {
   int64_t dist_n = -1;
   std::vector<double> dist_host;
 
+  bool pm1 = false;               // every stored weight is exactly +1 or -1
+
   // Row split (k_step_split): rows with more than `split` stored entries.
   std::vector<int64_t> rp_host;  // row pointers, for building the split tables
   int64_t max_deg = 0;
@@ -129,9 +131,37 @@ struct sis_solver {
   void *Xcur() { return p.form == SIS_FORM_F32 || p.form == SIS_FORM_F64 ? Xbuf(par) : Xbuf(0); }
 };
 
+// Resident tiles (k_resident): bit 20, log2 of the agents per workgroup in
+// bits 21-22.  Fused MaxCut in FP32 (plain dSB / bSB, packed dSB), weights
+// +-1, N <= 32767, N x G pairs per 1024-thread workgroup within the kernel's
+// register slots, CSR and two snapshots within 160 KB of LDS.
+constexpr int kResidentBit = 1 << 20;
+
+static bool resident_layout(const sis_solver *s, int vec, ResArgs *r) {
+  if (!(vec & kResidentBit) || (vec & ~(kResidentBit | (3 << 21))) != 0) return false;
+  const int lg = (vec >> 21) & 3, G = 1 << lg;
+  if (s->prob != PROB_MAXCUT || s->structured || !s->pm1) return false;
+  if (!(s->p.form == SIS_FORM_F32 || (s->p.form == SIS_FORM_PACKED && s->gather == G_DSB)))
+    return false;
+  if (s->N > 32767 || s->N * G > resident_pairs_max() || s->lda % G != 0) return false;
+  auto al16 = [](int64_t b) { return (b + 15) / 16 * 16; };
+  const int64_t off = al16(4 * (s->N + 1) + 2 * s->Z);
+  const int64_t snap = al16(s->gather == G_DSB ? s->N : 4 * s->N * G);
+  if (off + 2 * snap > 160 * 1024) return false;
+  if (r) {
+    r->G = G;
+    r->lgG = lg;
+    r->snap_off = (int32_t)off;
+    r->snap_bytes = (int32_t)snap;
+  }
+  return true;
+}
+
 // Tile encoding: plain / BF16 = agents per lane | non-temporal << 16;
-// packed = words | rows << 8 | non-temporal << 16 (see sis_kernels.hpp).
+// packed = words | rows << 8 | non-temporal << 16 (see sis_kernels.hpp);
+// resident tiles as above.
 static bool tile_ok(const sis_solver *s, int vec) {
+  if (vec & kResidentBit) return resident_layout(s, vec, nullptr);
   if (!step_supported(s->p.form, s->prob, s->gather, vec)) return false;
   if (s->p.form == SIS_FORM_PACKED) return s->W % (vec & 0xff) == 0;
   return s->lda % (64 * (vec & 0xffff)) == 0;
@@ -166,6 +196,7 @@ static std::vector<int> candidate_tiles(const sis_solver *s) {
     for (int v : {1, 2, 4})
       for (int nt : {0, 1}) c.push_back(v | (nt << 16));
   }
+  for (int lg = 0; lg < 4; lg++) c.push_back(kResidentBit | (lg << 21));
   std::vector<int> ok;
   for (int v : c)
     if (tile_ok(s, v)) ok.push_back(v);
@@ -288,6 +319,12 @@ int sis_solver_create(const sis_csr *J, const double *field, const sis_params *p
   s->N = J->n;
   s->Z = (int64_t)J->col.size();
   s->S = J->sum;
+  s->pm1 = true;
+  for (double v : J->val)
+    if (v != 1.0 && v != -1.0) {
+      s->pm1 = false;
+      break;
+    }
   if (s->N < 1) return fail(SIS_ERR_ARG, "sis_solver_create: empty matrix");
   if (p.problem == SIS_TSP) {
     if (p.ncities < 1 || (int64_t)p.ncities * p.ncities != s->N)
@@ -334,6 +371,7 @@ int sis_solver_create(const sis_csr *J, const double *field, const sis_params *p
   HIPCHK(hipMemcpy(s->rp.p, J->rp.data(), sizeof(int64_t) * (N + 1), hipMemcpyHostToDevice));
   s->rp_host.assign(J->rp.begin(), J->rp.begin() + N + 1);
   for (int64_t i = 0; i < N; i++) s->max_deg = std::max(s->max_deg, J->rp[i + 1] - J->rp[i]);
+
   if (Z) {
     HIPCHK(hipMemcpy(s->col.p, J->col.data(), sizeof(int32_t) * Z, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(s->vald.p, J->val.data(), sizeof(double) * Z, hipMemcpyHostToDevice));
@@ -606,6 +644,41 @@ static void enqueue_step(sis_solver *s, int32_t off) {
   s->par = nxt;
 }
 
+// Steps off0 .. off0 + n - 1 of the current window period: n fused steps, or
+// with a resident tile one k_resident launch per 1000 steps (then, for the
+// packed form, the sign words of the final X for the window and the next
+// step).
+static void enqueue_steps(sis_solver *s, int32_t off0, int64_t n) {
+  ResArgs r{};
+  if (!resident_layout(s, s->vec, &r)) {
+    for (int64_t k = 0; k < n; k++) enqueue_step(s, (int32_t)(off0 + k));
+    return;
+  }
+  StepArgs a{};
+  a.rp = s->rp.p;
+  a.col = s->col.p;
+  a.val = s->valf.p;
+  a.sched = s->schedf.p;
+  a.ctrl = s->ctrl.p;
+  a.N = s->N;
+  a.lda = s->lda;
+  a.W = s->W;
+  a.coupling = s->p.coupling;
+  a.Y = s->Yf.p;
+  const bool packed = s->p.form == SIS_FORM_PACKED;
+  for (int64_t k0 = 0; k0 < n; k0 += 1000) {
+    const int32_t m = (int32_t)std::min<int64_t>(1000, n - k0);
+    a.offset = (int32_t)(off0 + k0);
+    r.steps = m;
+    r.Xin = packed ? s->Xf[0].p : s->Xf[s->par].p;
+    r.Xout = packed ? s->Xf[0].p : s->Xf[s->par ^ (m & 1)].p;
+    (void)launch_resident(s->stream, s->gather, a, r);
+    s->par ^= (m & 1);
+  }
+  if (packed)
+    launch_pack_signs(s->stream, SRC_F32, s->Xf[0].p, s->sg[s->par].p, s->N, s->W, s->lda);
+}
+
 // Enqueue the window kernels that follow a period of P steps.
 static void enqueue_window(sis_solver *s, int64_t P, int uw) {
   int src;
@@ -652,7 +725,7 @@ static int build_graphs(sis_solver *s, int64_t P, int uw) {
     s->par = ph;
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-    for (int64_t k = 0; k < P; k++) enqueue_step(s, (int32_t)k);
+    enqueue_steps(s, 0, P);
     enqueue_window(s, P, uw);
     HIPCHK(hipStreamEndCapture(s->stream, &g));
     hipError_t e = hipGraphInstantiate(&s->graph_exec[ph], g, nullptr, nullptr, 0);
@@ -679,7 +752,7 @@ static int build_run_graph(sis_solver *s, int64_t P, int64_t nwin, int uw) {
   hipGraph_t g = nullptr;
   HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
   for (int64_t w = 0; w < nwin; w++) {
-    for (int64_t k = 0; k < P; k++) enqueue_step(s, (int32_t)k);
+    enqueue_steps(s, 0, P);
     enqueue_window(s, P, uw);
   }
   HIPCHK(hipStreamEndCapture(s->stream, &g));
@@ -788,7 +861,7 @@ int sis_solver_advance(sis_solver *s, int64_t t0, int64_t nsteps, int64_t T) {
   c.base = t0;
   c.merge = s->p.problem == SIS_TSP ? 1 : 0;
   HIPCHK(hipMemcpyAsync(s->ctrl.p, &c, sizeof(Ctrl), hipMemcpyHostToDevice, s->stream));
-  for (int64_t k = 0; k < nsteps; k++) enqueue_step(s, (int32_t)k);
+  enqueue_steps(s, 0, nsteps);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s->stream));
   return SIS_OK;
@@ -834,9 +907,9 @@ int sis_solver_autotune(sis_solver *s, int64_t trial_steps, int32_t *tile, doubl
   };
   auto trial = [&](double *per) -> int {
     if (int e = restore()) return e;
-    for (int64_t k = 0; k < warm; k++) enqueue_step(s, (int32_t)k);
+    enqueue_steps(s, 0, warm);
     HIPCHK(hipEventRecord(s->events[0], s->stream));
-    for (int64_t k = 0; k < trial_steps; k++) enqueue_step(s, (int32_t)(warm + k));
+    enqueue_steps(s, (int32_t)warm, trial_steps);
     HIPCHK(hipEventRecord(s->events[1], s->stream));
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s->stream));
@@ -919,7 +992,7 @@ int sis_solver_run(sis_solver *s, int64_t max_steps, int update_window, sis_run_
         if (P & 1) s->par ^= 1;
       } else {
         HIPCHK(hipEventRecord(s->events[2 + 2 * enq], s->stream));
-        for (int64_t k = 0; k < P; k++) enqueue_step(s, (int32_t)k);
+        enqueue_steps(s, 0, P);
         HIPCHK(hipEventRecord(s->events[3 + 2 * enq], s->stream));
         enqueue_window(s, P, uw);
       }

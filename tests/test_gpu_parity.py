@@ -254,6 +254,68 @@ def test_plain_and_bf16_tiles_bit_exact(form, dyn, tile):
     assert np.array_equal(x, X.astype(np.float64)) and np.array_equal(y, Y.astype(np.float64))
 
 
+RES = 1 << 20  # resident tile: one workgroup per 2^lg agents, state in registers / LDS
+
+
+@pytest.mark.parametrize("form,dyn,lg", [("f32", sis.DSB, 0), ("f32", sis.DSB, 3), ("f32", sis.BSB, 0),
+                                         ("f32", sis.BSB, 2), ("packed", sis.DSB, 1),
+                                         ("packed", sis.DSB, 3)])
+def test_resident_tiles_bit_exact(form, dyn, lg):
+    # +-1 weights (the resident CSR encoding), 203 rows; two launches of odd
+    # length so the snapshot parity flips between them
+    J, pb = maxcut_problem(203, 0.04, sis.W_PM1, 21)
+    A, T, dt = 128, 300, 0.5
+    tile = RES | lg << 21
+    s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=A, dt=dt, form=sis.FORMS[form], dynamics=dyn,
+                           tile=tile))
+    assert s.p.tile == tile
+    X0, Y0 = seeded_state(J.n, A, 22)
+    s.set_state(X0, Y0)
+    s.advance(0, 137, T)
+    s.advance(137, 63, T)
+    X, Y = X0.astype(np.float32), Y0.astype(np.float32)
+    orc.advance(pb, orc.MAXCUT, dyn, X, Y, 0, 200, T, dt, s.p.xi0, 1.0, "f32")
+    x, y = s.get_state()
+    assert np.array_equal(x, X.astype(np.float64)) and np.array_equal(y, Y.astype(np.float64))
+
+
+@pytest.mark.parametrize("use_graph", [0, 1, 2])
+@pytest.mark.parametrize("form,lg", [("packed", 2), ("f32", 0)])
+def test_resident_full_runs_with_windows(use_graph, form, lg):
+    # torus 100 x 100 (N = 10^4: ten register slots per thread at G = 1),
+    # update window with early exit, graph replay: steps, spins, locks and
+    # cuts equal the oracle's
+    J = sis.Csr.gen_torus(100, 100, sis.W_PM1, 5) if lg == 0 else sis.Csr.gen_torus(40, 40, sis.W_PM1, 5)
+    rp, col, val = J.arrays()
+    pb = orc.Problem(rp, col, val, S=J.sum)
+    A, T, dt = 64, 1500, 0.5
+    tile = RES | lg << 21
+    s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=A, dt=dt, form=sis.FORMS[form], tile=tile,
+                           use_graph=use_graph, window_threshold=5))
+    assert s.p.tile == tile
+    s.init_random(7)
+    info = s.run(T, 1)
+    X, Y = orc.init_state(7, orc.MAXCUT, J.n, A)
+    r = orc.solve(pb, orc.MAXCUT, orc.DSB, X, Y, T, 1, P=100, K=5, dt=dt, xi0=s.p.xi0)
+    assert info["steps_executed"] == r["steps"]
+    assert np.array_equal(s.get_state()[0], X.astype(np.float64))
+    assert np.array_equal(s.spins(sis.SPINS_FINAL), r["fin"])
+    assert np.array_equal(s.cuts()[0], orc.cut(pb, r["cur"]))
+
+
+def test_resident_tile_rejected_where_it_does_not_apply():
+    J, _ = maxcut_problem(203, 0.04, sis.W_INT3, 21)  # weights outside +-1
+    with pytest.raises(sis.SisError):
+        sis.Solver(J, dict(problem=sis.MAXCUT, agents=128, form=sis.FORM_PACKED, tile=RES))
+    Jp, _ = maxcut_problem(203, 0.04, sis.W_PM1, 21)
+    with pytest.raises(sis.SisError):  # BF16 shadow form
+        sis.Solver(Jp, dict(problem=sis.MAXCUT, agents=128, form=sis.FORM_BF16, dynamics=sis.BSB,
+                            tile=RES))
+    big = sis.Csr.gen_torus(200, 200, sis.W_PM1, 1)  # N = 40,000 > 32,767
+    with pytest.raises(sis.SisError):
+        sis.Solver(big, dict(problem=sis.MAXCUT, agents=64, form=sis.FORM_PACKED, tile=RES))
+
+
 @pytest.mark.parametrize("form", ["packed", "f32", "bf16"])
 def test_autotune_keeps_state_and_results(form):
     J, pb = maxcut_problem(300, 0.03, sis.W_PM1, 14)
