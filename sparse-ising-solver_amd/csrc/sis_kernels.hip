@@ -781,10 +781,6 @@ __global__ __launch_bounds__(kBlock) void k_tsp_energy_struct(const uint64_t *sp
     part[i * lda + a0] = ((red[lane] + red[64 + lane]) + red[128 + lane]) + red[192 + lane];
 }
 
-bool tsp_struct_supported(int form, int gather) {
-  return (form == 0 || form == 1) && (gather == G_DSB || gather == G_BSB);
-}
-
 // Dynamic LDS above 64 KB (the staged v at n > 256 in FP32, n > 128 in FP64):
 // raise the kernel's limit once to the 160 KB of a gfx950 CU.
 template <auto K>
@@ -796,6 +792,177 @@ static void allow_big_lds() {
     done = true;
   }
 }
+
+constexpr int kResBlock = 1024;  // threads of a resident workgroup
+
+// Resident structured TSP (FP32, n <= 64): one 1024-thread workgroup owns G
+// agents for a whole launch of `steps` steps.  Per step it needs, for every
+// agent, the row and column sums of g and the GEMM Dq . V with
+// V[l][(i, a)] = g(i-1, l, a) + g(i+1, l, a): (n x n) . (n x nG), computed
+// with v_mfma_f32_16x16x4f32 (A[r][k] in lane r + 16 k, B[k][c] in lane
+// c + 16 k, C[r][c] in lane c + 16 (r / 4), register r % 4, checked on the
+// device).  Wavefront w owns the output tiles w, w + 16, ... (cities x
+// columns), and the X / Y / h of exactly those elements stay in registers in
+// the accumulator layout, so the GEMM result needs no exchange.  LDS holds g
+// of the current and next step (n^2 G floats each), Dq and the sums.  Two
+// workgroup barriers per step.
+constexpr int kTspResTiles = 3;  // output tiles per wavefront (48 tiles of 16 x 16)
+
+template <int GATHER>
+__global__ __launch_bounds__(kResBlock) void k_tsp_resident(StepArgs a, TspArgs ts, ResArgs r) {
+  extern __shared__ __align__(16) unsigned char s_raw[];
+  if (a.ctrl->done) return;
+  const int n = (int)ts.n, G = r.G, lg = r.lgG, NC = n * G, nG = n * G;
+  const int nep = (int)ts.nep, CT = (n + 15) / 16, NT = (NC + 15) / 16;
+  const int64_t lda = a.lda;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int64_t g0 = (int64_t)blockIdx.x * G;
+  float *snap = reinterpret_cast<float *>(s_raw);  // 2 x n^2 G
+  float *dqs = snap + 2 * n * nG;                  // CT*16 x nep
+  float *rs = dqs + CT * 16 * nep;                 // n x G row sums
+  float *cs = rs + nG;                             // n x G column sums
+  for (int e = t; e < CT * 16 * nep; e += kResBlock) dqs[e] = ts.dqm[e];
+  // owned elements: tile u of this wavefront, register q
+  float x[kTspResTiles][4], y[kTspResTiles][4], hv[kTspResTiles][4];
+  const float *__restrict__ Xin = r.Xin;
+  float *Y = (float *)a.Y;
+  const float *__restrict__ hf = (const float *)a.field;
+  const float c = (float)a.coupling;
+  auto elem = [&](int u, int q, int &k, int &col) {
+    const int tile = w + 16 * u, ct = tile / NT, nt = tile - ct * NT;
+    k = ct * 16 + 4 * (lane >> 4) + q;
+    col = nt * 16 + (lane & 15);
+    return tile < CT * NT && k < n && col < NC;
+  };
+#pragma unroll
+  for (int u = 0; u < kTspResTiles; u++)
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      int k, col;
+      x[u][q] = y[u][q] = hv[u][q] = 0.0f;
+      if (elem(u, q, k, col)) {
+        const int i = col >> lg, ag = col & (G - 1);
+        const int64_t e = (int64_t)(i * n + k) * lda + g0 + ag;
+        x[u][q] = Xin[e];
+        y[u][q] = Y[e];
+        hv[u][q] = hf[i * n + k];
+      }
+    }
+  auto put = [&](float *buf) {
+#pragma unroll
+    for (int u = 0; u < kTspResTiles; u++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        int k, col;
+        if (elem(u, q, k, col)) {
+          const int i = col >> lg, ag = col & (G - 1);
+          buf[(i * n + k) * G + ag] = tsp_g<float, GATHER>(x[u][q], c);
+        }
+      }
+  };
+  put(snap);
+  __syncthreads();
+  const float qB = (float)ts.qB, qC = (float)ts.qC;
+  const float *sc = (const float *)a.sched + 4 * (a.ctrl->base + a.offset);
+  float n0 = sc[0], n1 = sc[1], n2 = sc[2], n3 = sc[3];
+  for (int st = 0; st < r.steps; st++) {
+    const float c0 = n0, c1 = n1, c2 = n2, c3 = n3;
+    if (st + 1 < r.steps) {
+      n0 = sc[4 * (st + 1)];
+      n1 = sc[4 * (st + 1) + 1];
+      n2 = sc[4 * (st + 1) + 2];
+      n3 = sc[4 * (st + 1) + 3];
+    }
+    const float *S = snap + (st & 1) * n * nG;
+    // row sums (position i, agent a) and column sums (city k, agent a): 8
+    // lanes per sum, each over every 8th term, then a butterfly over the 8
+    for (int item = t; item < 2 * nG * 8; item += kResBlock) {
+      const int sum = item >> 3, part = item & 7;
+      const bool row = sum < nG;
+      const int idx = (row ? sum : sum - nG) >> lg, ag = sum & (G - 1);
+      float acc = 0.0f;
+      for (int m = part; m < n; m += 8)
+        acc += row ? S[(idx * n + m) * G + ag] : S[(m * n + idx) * G + ag];
+      acc += __shfl_xor(acc, 1, 64);
+      acc += __shfl_xor(acc, 2, 64);
+      acc += __shfl_xor(acc, 4, 64);
+      if (part == 0) (row ? rs : cs)[row ? sum : sum - nG] = acc;
+    }
+    __syncthreads();
+    float *Sn = snap + ((st + 1) & 1) * n * nG;
+#pragma unroll
+    for (int u = 0; u < kTspResTiles; u++) {
+      const int tile = w + 16 * u;
+      if (tile >= CT * NT) break;  // wavefront-uniform
+      const int ct = tile / NT, nt = tile - ct * NT;
+      // B operand column of this lane: (position i, agent a)
+      const int col = nt * 16 + (lane & 15);
+      const bool cok = col < NC;
+      const int i = cok ? col >> lg : 0, ag = col & (G - 1);
+      const int im = (i + n - 1) % n, ip = (i + 1) % n;
+      const float *drow = dqs + (ct * 16 + (lane & 15)) * nep;
+      typedef float f4 __attribute__((ext_vector_type(4)));
+      f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+      for (int l0 = 0; l0 < n; l0 += 4) {
+        const int l = l0 + (lane >> 4);
+        const float av = drow[l];  // zero past n (padded table)
+        const float bv = (cok && l < n) ? S[(im * n + l) * G + ag] + S[(ip * n + l) * G + ag] : 0.0f;
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int k = ct * 16 + 4 * (lane >> 4) + q;
+        if (cok && k < n) {
+          const float Sv = fmaT(qB, rs[i * G + ag], qC * cs[k * G + ag]) + acc[q];
+          sb_update<float, PROB_TSP>(x[u][q], y[u][q], -Sv, c0, c1, c2, c3, hv[u][q]);
+          Sn[(i * n + k) * G + ag] = tsp_g<float, GATHER>(x[u][q], c);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  float *Xout = r.Xout;
+#pragma unroll
+  for (int u = 0; u < kTspResTiles; u++)
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      int k, col;
+      if (elem(u, q, k, col)) {
+        const int i = col >> lg, ag = col & (G - 1);
+        const int64_t e = (int64_t)(i * n + k) * lda + g0 + ag;
+        Xout[e] = x[u][q];
+        Y[e] = y[u][q];
+      }
+    }
+}
+
+size_t tsp_resident_lds(int64_t n, int G, int64_t nep) {
+  return sizeof(float) * (size_t)(2 * n * n * G + (n + 15) / 16 * 16 * nep + 2 * n * G);
+}
+
+bool tsp_resident_ok(int64_t n, int G, int64_t nep) {
+  const int64_t tiles = (n + 15) / 16 * ((n * G + 15) / 16);
+  return n >= 3 && n <= 64 && (G == 1 || G == 2 || G == 4) && tiles <= 16 * kTspResTiles &&
+         tsp_resident_lds(n, G, nep) <= 160 * 1024;
+}
+
+void launch_tsp_resident(hipStream_t st, int gather, const StepArgs &a, const TspArgs &ts,
+                         const ResArgs &r) {
+  const dim3 grid((unsigned)(a.lda / r.G));
+  const size_t shm = tsp_resident_lds(ts.n, r.G, ts.nep);
+  if (gather == G_BSB) {
+    allow_big_lds<k_tsp_resident<G_BSB>>();
+    hipLaunchKernelGGL((k_tsp_resident<G_BSB>), grid, dim3(kResBlock), shm, st, a, ts, r);
+  } else {
+    allow_big_lds<k_tsp_resident<G_DSB>>();
+    hipLaunchKernelGGL((k_tsp_resident<G_DSB>), grid, dim3(kResBlock), shm, st, a, ts, r);
+  }
+}
+
+bool tsp_struct_supported(int form, int gather) {
+  return (form == 0 || form == 1) && (gather == G_DSB || gather == G_BSB);
+}
+
 
 void launch_tsp_struct_step(hipStream_t st, int form, int gather, bool mfma, const StepArgs &a,
                             const TspArgs &ts) {
@@ -851,7 +1018,6 @@ void launch_tsp_struct_energy(hipStream_t st, const uint64_t *spins, const doubl
 // q / 1024), so the G agents of a row sit in consecutive lanes of one
 // wavefront.  Every term is added in CSR order with the row kernels'
 // operations: results are bit-identical to k_step_plain / k_step_packed.
-constexpr int kResBlock = 1024;
 
 template <int GATHER, int M>
 __global__ __launch_bounds__(kResBlock) void k_resident(StepArgs a, ResArgs r) {

@@ -130,6 +130,14 @@ int resident_pairs_max();  // N x G limit of one workgroup
 // false: no kernel instance for these sizes (nothing launched)
 bool launch_resident(hipStream_t st, int gather, const StepArgs &a, const ResArgs &r);
 
+// Resident structured TSP (FP32, 3 <= n <= 64, G = 1, 2 or 4 agents per
+// workgroup): `steps` steps per launch with the elements' X / Y in registers
+// and g, the distance table and the sums in LDS.
+size_t tsp_resident_lds(int64_t n, int G, int64_t nep);
+bool tsp_resident_ok(int64_t n, int G, int64_t nep);
+void launch_tsp_resident(hipStream_t st, int gather, const StepArgs &a, const TspArgs &ts,
+                         const ResArgs &r);
+
 bool split_supported(int form, int prob, int gather);
 void launch_step_split(hipStream_t st, int form, int prob, int gather, int64_t nent,
                        const StepArgs &a, const SplitArgs &sp);

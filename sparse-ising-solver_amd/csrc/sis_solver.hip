@@ -80,6 +80,7 @@ struct sis_solver {
   DevBuf<float> dqf;
   DevBuf<float> dqm;                   // FP32, zero-padded for the matrix cores
   int64_t tsp_nep = 0;                 // n rounded up to 8
+  int tsp_res_G = 0;                   // agents per workgroup of tile 3 (0: not available)
   DevBuf<char> tsum_r, tsum_c;         // per-step row / column sums, form precision
   DevBuf<double> esum_r, esum_c;       // the same for the energy, FP64
 
@@ -187,7 +188,11 @@ static int default_tile(const sis_solver *s) {
 static std::vector<int> candidate_tiles(const sis_solver *s) {
   std::vector<int> c;
   // structured TSP: 1 = VALU distance product, 2 = matrix cores (FP32)
-  if (s->structured) return s->f64 ? std::vector<int>{1} : std::vector<int>{1, 2};
+  if (s->structured) {
+    if (s->f64) return {1};
+    if (s->tsp_res_G) return {1, 2, 3};
+    return {1, 2};
+  }
   if (s->p.form == SIS_FORM_PACKED) {
     for (int wt : {1, 2, 4, 8})
       for (int r : {1, 2, 4})
@@ -466,11 +471,7 @@ int sis_solver_create_tsp(const sis_tsp *t, const sis_params *pin, sis_solver **
   std::unique_ptr<sis_solver> s(sp);
   s->structured = true;
   s->p.structured = 1;
-  // tile: 1 = VALU distance product, 2 = matrix cores (default for FP32)
-  if (pin->tile == 0) s->vec = s->f64 ? 1 : 2;
-  else if (pin->tile != 1 && !(pin->tile == 2 && !s->f64))
-    return fail(SIS_ERR_ARG, "sis_solver_create_tsp: structured tile must be 1 (VALU) or 2 (MFMA, f32)");
-  else s->vec = pin->tile;
+
   s->tsp_n = n;
   s->qB = B / 4;
   s->qC = C / 4;
@@ -497,7 +498,21 @@ int sis_solver_create_tsp(const sis_tsp *t, const sis_params *pin, sis_solver **
     if ((st = s->dqm.alloc(dm.size()))) return st;
     HIPCHK(hipMemcpy(s->dqm.p, dm.data(), sizeof(float) * dm.size(), hipMemcpyHostToDevice));
     s->tsp_nep = nep;
+    // resident tile: the most agents per workgroup that fit (4, 2, 1)
+    for (int G : {4, 2, 1})
+      if (s->lda % G == 0 && tsp_resident_ok(n, G, nep)) {
+        s->tsp_res_G = G;
+        break;
+      }
   }
+  // tile: 1 = VALU distance product, 2 = matrix cores (default for FP32),
+  // 3 = resident workgroups (FP32, n <= 64)
+  if (pin->tile == 0) s->vec = s->f64 ? 1 : 2;
+  else if (pin->tile == 1 || (pin->tile == 2 && !s->f64) || (pin->tile == 3 && s->tsp_res_G))
+    s->vec = pin->tile;
+  else
+    return fail(SIS_ERR_ARG, "sis_solver_create_tsp: structured tile must be 1 (VALU), 2 (MFMA, "
+                             "f32) or 3 (resident, f32, n <= 64)");
   *out = s.release();
   return SIS_OK;
 }
@@ -650,6 +665,29 @@ static void enqueue_step(sis_solver *s, int32_t off) {
 // step).
 static void enqueue_steps(sis_solver *s, int32_t off0, int64_t n) {
   ResArgs r{};
+  if (s->structured && s->vec == 3) {
+    StepArgs a{};
+    a.field = s->hf.p;
+    a.sched = s->schedf.p;
+    a.ctrl = s->ctrl.p;
+    a.N = s->N;
+    a.lda = s->lda;
+    a.W = s->W;
+    a.coupling = s->p.coupling;
+    a.Y = s->Yf.p;
+    r.G = s->tsp_res_G;
+    r.lgG = s->tsp_res_G == 4 ? 2 : s->tsp_res_G == 2 ? 1 : 0;
+    for (int64_t k0 = 0; k0 < n; k0 += 1000) {
+      const int32_t m = (int32_t)std::min<int64_t>(1000, n - k0);
+      a.offset = (int32_t)(off0 + k0);
+      r.steps = m;
+      r.Xin = s->Xf[s->par].p;
+      r.Xout = s->Xf[s->par ^ (m & 1)].p;
+      launch_tsp_resident(s->stream, s->gather, a, s->tsp_args(false), r);
+      s->par ^= (m & 1);
+    }
+    return;
+  }
   if (!resident_layout(s, s->vec, &r)) {
     for (int64_t k = 0; k < n; k++) enqueue_step(s, (int32_t)(off0 + k));
     return;

@@ -39,8 +39,9 @@ def max_rel(x, X):
     return float(np.abs(x - X).max() / max(np.abs(X).max(), 1e-30))
 
 
-# tile 1: distance product on VALU; 2: on the matrix cores (FP32, default)
-FORMS_T = [("f32", "f32", 1), ("f32", "f32", 2), ("f64", "f64", 1)]
+# tile 1: distance product on VALU; 2: on the matrix cores (FP32, default);
+# 3: resident workgroups (FP32, n <= 64: state in registers, g in LDS)
+FORMS_T = [("f32", "f32", 1), ("f32", "f32", 2), ("f32", "f32", 3), ("f64", "f64", 1)]
 
 
 @pytest.mark.parametrize("n", [3, 7, 12, 33])
@@ -66,6 +67,7 @@ def test_structured_dsb_bit_exact(n, form, prec, tile):
 
 @pytest.mark.parametrize("form,prec,bar1,bar,tile", [("f32", "f32", 1e-6, 1e-5, 1),
                                                      ("f32", "f32", 1e-6, 1e-5, 2),
+                                                     ("f32", "f32", 1e-6, 1e-5, 3),
                                                      ("f64", "f64", 1e-15, 1e-12, 1)])
 @pytest.mark.parametrize("n", [7, 16, 40])
 def test_structured_bsb_within_bound(form, prec, bar1, bar, tile, n):
@@ -94,7 +96,7 @@ def test_structured_bsb_within_bound(form, prec, bar1, bar, tile, n):
 
 
 @pytest.mark.parametrize("dyn", [sis.DSB, sis.BSB])
-@pytest.mark.parametrize("tile", [1, 2])
+@pytest.mark.parametrize("tile", [1, 2, 3])
 def test_structured_full_run_and_energy(dyn, tile):
     n = 6
     t, J, h, pb = instance(n, seed=11)
@@ -176,6 +178,25 @@ def test_structured_n512_tiles_agree():
     np.testing.assert_array_equal(out[0], out[1])
 
 
+@pytest.mark.parametrize("agents", [64, 100, 1024])
+def test_structured_resident_equals_mfma_dsb(agents):
+    """Tile 3 (resident workgroups of 1, 2 or 4 agents) against tile 2 at the
+    10,000-step TSP size (n = 48): dSB sums are exact, so the trajectories
+    are bit-identical, including agent counts that leave padding lanes."""
+    t = sis.Tsp.generate(48, 1000, 1, 9)
+    out = []
+    for tile in (2, 3):
+        s = sis.Solver.from_tsp(t, dict(agents=agents, form=sis.FORM_F32, dynamics=sis.DSB, dt=0.1,
+                                        structured=1, tile=tile))
+        assert s.p.tile == tile
+        s.init_random(4)
+        s.advance(0, 777, 10000)
+        out.append(s.get_state())
+        s.close()
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+
+
 def test_structured_argument_errors():
     t2 = sis.Tsp.generate(2, 50, 0, 1)
     with pytest.raises(sis.SisError):
@@ -203,9 +224,14 @@ def test_structured_default_tile_and_autotune():
     s.init_random(1)
     x0 = s.get_state()[0]
     tile, ms = s.autotune(5)
-    assert tile in (1, 2) and ms > 0
+    assert tile in (1, 2, 3) and ms > 0
     assert np.array_equal(s.get_state()[0], x0)
     s64 = sis.Solver.from_tsp(t, dict(agents=128, form=sis.FORM_F64, structured=1))
     assert s64.p.tile == 1
     with pytest.raises(sis.SisError):
         sis.Solver.from_tsp(t, dict(agents=128, form=sis.FORM_F64, structured=1, tile=2))
+    with pytest.raises(sis.SisError):  # resident tile: FP32 only
+        sis.Solver.from_tsp(t, dict(agents=128, form=sis.FORM_F64, structured=1, tile=3))
+    with pytest.raises(sis.SisError):  # resident tile: n <= 64
+        sis.Solver.from_tsp(sis.Tsp.generate(65, 100, 0, 3),
+                            dict(agents=128, form=sis.FORM_F32, structured=1, tile=3))
