@@ -818,15 +818,19 @@ __global__ __launch_bounds__(kResBlock) void k_tsp_resident(StepArgs a, TspArgs 
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int64_t g0 = (int64_t)blockIdx.x * G;
   float *snap = reinterpret_cast<float *>(s_raw);  // 2 x n^2 G
-  float *dqs = snap + 2 * n * nG;                  // CT*16 x nep
-  float *rs = dqs + CT * 16 * nep;                 // n x G row sums
+  // Dq rows at an odd stride (nep + 1): the 16 rows one MFMA A operand reads
+  // fall into distinct LDS banks
+  const int dst = nep + 1;
+  float *dqs = snap + 2 * n * nG;                  // CT*16 x dst
+  float *rs = dqs + CT * 16 * dst;                 // n x G row sums
   float *cs = rs + nG;                             // n x G column sums
-  for (int e = t; e < CT * 16 * nep; e += kResBlock) dqs[e] = ts.dqm[e];
+  float *hs = cs + nG;                             // n x n field
+  for (int e = t; e < CT * 16 * nep; e += kResBlock) dqs[(e / nep) * dst + e % nep] = ts.dqm[e];
+  for (int e = t; e < n * n; e += kResBlock) hs[e] = ((const float *)a.field)[e];
   // owned elements: tile u of this wavefront, register q
-  float x[kTspResTiles][4], y[kTspResTiles][4], hv[kTspResTiles][4];
+  float x[kTspResTiles][4], y[kTspResTiles][4];
   const float *__restrict__ Xin = r.Xin;
   float *Y = (float *)a.Y;
-  const float *__restrict__ hf = (const float *)a.field;
   const float c = (float)a.coupling;
   auto elem = [&](int u, int q, int &k, int &col) {
     const int tile = w + 16 * u, ct = tile / NT, nt = tile - ct * NT;
@@ -839,13 +843,12 @@ __global__ __launch_bounds__(kResBlock) void k_tsp_resident(StepArgs a, TspArgs 
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       int k, col;
-      x[u][q] = y[u][q] = hv[u][q] = 0.0f;
+      x[u][q] = y[u][q] = 0.0f;
       if (elem(u, q, k, col)) {
         const int i = col >> lg, ag = col & (G - 1);
         const int64_t e = (int64_t)(i * n + k) * lda + g0 + ag;
         x[u][q] = Xin[e];
         y[u][q] = Y[e];
-        hv[u][q] = hf[i * n + k];
       }
     }
   auto put = [&](float *buf) {
@@ -890,31 +893,52 @@ __global__ __launch_bounds__(kResBlock) void k_tsp_resident(StepArgs a, TspArgs 
     }
     __syncthreads();
     float *Sn = snap + ((st + 1) & 1) * n * nG;
+    // the wavefront's tiles advance through K together: independent
+    // accumulators, so consecutive MFMAs do not wait on each other
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    f4 acc[kTspResTiles];
+    int ti[kTspResTiles], tag[kTspResTiles], tim[kTspResTiles], tip[kTspResTiles];
+    bool tok[kTspResTiles], tcok[kTspResTiles];
+    const float *drow[kTspResTiles];
 #pragma unroll
     for (int u = 0; u < kTspResTiles; u++) {
       const int tile = w + 16 * u;
-      if (tile >= CT * NT) break;  // wavefront-uniform
-      const int ct = tile / NT, nt = tile - ct * NT;
-      // B operand column of this lane: (position i, agent a)
+      tok[u] = tile < CT * NT;  // wavefront-uniform
+      const int ct = tok[u] ? tile / NT : 0, nt = tok[u] ? tile - ct * NT : 0;
       const int col = nt * 16 + (lane & 15);
-      const bool cok = col < NC;
-      const int i = cok ? col >> lg : 0, ag = col & (G - 1);
-      const int im = (i + n - 1) % n, ip = (i + 1) % n;
-      const float *drow = dqs + (ct * 16 + (lane & 15)) * nep;
-      typedef float f4 __attribute__((ext_vector_type(4)));
-      f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-      for (int l0 = 0; l0 < n; l0 += 4) {
-        const int l = l0 + (lane >> 4);
-        const float av = drow[l];  // zero past n (padded table)
-        const float bv = (cok && l < n) ? S[(im * n + l) * G + ag] + S[(ip * n + l) * G + ag] : 0.0f;
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
+      tcok[u] = tok[u] && col < NC;
+      ti[u] = tcok[u] ? col >> lg : 0;
+      tag[u] = col & (G - 1);
+      tim[u] = (ti[u] + n - 1) % n;
+      tip[u] = (ti[u] + 1) % n;
+      drow[u] = dqs + (ct * 16 + (lane & 15)) * dst;
+      acc[u] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+    for (int l0 = 0; l0 < n; l0 += 4) {
+      const int l = l0 + (lane >> 4);
+      float av[kTspResTiles], bv[kTspResTiles];
+#pragma unroll
+      for (int u = 0; u < kTspResTiles; u++) {
+        av[u] = drow[u][l];  // zero past n (padded table)
+        bv[u] = (tcok[u] && l < n)
+                    ? S[(tim[u] * n + l) * G + tag[u]] + S[(tip[u] * n + l) * G + tag[u]]
+                    : 0.0f;
       }
+#pragma unroll
+      for (int u = 0; u < kTspResTiles; u++)
+        if (tok[u]) acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u], bv[u], acc[u], 0, 0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < kTspResTiles; u++) {
+      if (!tok[u]) continue;
+      const int tile = w + 16 * u, ct = tile / NT;
+      const int i = ti[u], ag = tag[u];
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         const int k = ct * 16 + 4 * (lane >> 4) + q;
-        if (cok && k < n) {
-          const float Sv = fmaT(qB, rs[i * G + ag], qC * cs[k * G + ag]) + acc[q];
-          sb_update<float, PROB_TSP>(x[u][q], y[u][q], -Sv, c0, c1, c2, c3, hv[u][q]);
+        if (tcok[u] && k < n) {
+          const float Sv = fmaT(qB, rs[i * G + ag], qC * cs[k * G + ag]) + acc[u][q];
+          sb_update<float, PROB_TSP>(x[u][q], y[u][q], -Sv, c0, c1, c2, c3, hs[i * n + k]);
           Sn[(i * n + k) * G + ag] = tsp_g<float, GATHER>(x[u][q], c);
         }
       }
@@ -937,7 +961,7 @@ __global__ __launch_bounds__(kResBlock) void k_tsp_resident(StepArgs a, TspArgs 
 }
 
 size_t tsp_resident_lds(int64_t n, int G, int64_t nep) {
-  return sizeof(float) * (size_t)(2 * n * n * G + (n + 15) / 16 * 16 * nep + 2 * n * G);
+  return sizeof(float) * (size_t)(2 * n * n * G + (n + 15) / 16 * 16 * (nep + 1) + 2 * n * G + n * n);
 }
 
 bool tsp_resident_ok(int64_t n, int G, int64_t nep) {
