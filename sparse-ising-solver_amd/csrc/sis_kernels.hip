@@ -1481,14 +1481,24 @@ __global__ void k_tsp_len(const uint64_t *__restrict__ fin, const int32_t *valid
   if (w >= W) return;
   int64_t m = 0, first = -1, prev = -1;
   double sum = 0.0;
-  for (int64_t o = 0; o < n; o++)
-    for (int64_t c = 0; c < n; c++) {
-      if ((fin[(o * n + c) * W + w] >> lane) & 1) continue;
+  // spins in (position, city) order; the words (one per spin, shared by the
+  // wavefront's 64 agents) are loaded 16 at a time before they are walked
+  constexpr int B = 16;
+  const int64_t nn = n * n;
+  for (int64_t s0 = 0; s0 < nn; s0 += B) {
+    uint64_t wd[B];
+#pragma unroll
+    for (int u = 0; u < B; u++) wd[u] = fin[(s0 + u < nn ? s0 + u : nn - 1) * W + w];
+#pragma unroll
+    for (int u = 0; u < B; u++) {
+      if (s0 + u >= nn || ((wd[u] >> lane) & 1)) continue;
+      const int64_t c = (s0 + u) % n;
       if (m == 0) first = c;
       else if (prev != c) sum += dist[d_tri(prev < c ? prev : c, prev < c ? c : prev, n)];
       prev = c;
       m++;
     }
+  }
   if (ag >= A) return;
   if (valid_list[ag] <= 0 || m != n) {
     len[ag] = -1.0;
