@@ -710,7 +710,17 @@ static void enqueue_steps(sis_solver *s, int32_t off0, int64_t n) {
     r.steps = m;
     r.Xin = packed ? s->Xf[0].p : s->Xf[s->par].p;
     r.Xout = packed ? s->Xf[0].p : s->Xf[s->par ^ (m & 1)].p;
-    (void)launch_resident(s->stream, s->gather, a, r);
+    if (!launch_resident(s->stream, s->gather, a, r)) {
+      // no kernel instance for this size (resident_layout admits none such):
+      // the remaining steps go through the row kernels
+      if (packed)
+        launch_pack_signs(s->stream, SRC_F32, s->Xf[0].p, s->sg[s->par].p, s->N, s->W, s->lda);
+      const int keep = s->vec;
+      s->vec = default_tile(s);
+      for (int64_t k = k0; k < n; k++) enqueue_step(s, (int32_t)(off0 + k));
+      s->vec = keep;
+      return;
+    }
     s->par ^= (m & 1);
   }
   if (packed)
