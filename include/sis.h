@@ -194,7 +194,7 @@ typedef struct sis_params {
                                agents, state in registers / LDS for a window
                                period; fused MaxCut FP32, weights +-1,
                                N <= 32767); structured TSP: 1 VALU, 2 MFMA, 3
-                               resident workgroups (FP32, n <= 64) */
+                               resident workgroups (FP32, n <= 96) */
   int32_t fused;            /* MaxCut: 1 fused step (src/kernels.cu:211-269), 0 the
                                four-kernel path (src/ising.cu:167-172); plain forms */
   int32_t split;            /* row split: rows with more than `split` stored entries

@@ -795,7 +795,8 @@ static void allow_big_lds() {
 
 constexpr int kResBlock = 1024;  // threads of a resident workgroup
 
-// Resident structured TSP (FP32, n <= 64): one 1024-thread workgroup owns G
+// Resident structured TSP (FP32, up to 48 output tiles of 16 x 16 and 160 KB
+// of LDS: n <= 64 at G = 2, n <= 96 at G = 1): one 1024-thread workgroup owns G
 // agents for a whole launch of `steps` steps.  Per step it needs, for every
 // agent, the row and column sums of g and the GEMM Dq . V with
 // V[l][(i, a)] = g(i-1, l, a) + g(i+1, l, a): (n x n) . (n x nG), computed
@@ -966,7 +967,7 @@ size_t tsp_resident_lds(int64_t n, int G, int64_t nep) {
 
 bool tsp_resident_ok(int64_t n, int G, int64_t nep) {
   const int64_t tiles = (n + 15) / 16 * ((n * G + 15) / 16);
-  return n >= 3 && n <= 64 && (G == 1 || G == 2 || G == 4) && tiles <= 16 * kTspResTiles &&
+  return n >= 3 && (G == 1 || G == 2 || G == 4) && tiles <= 16 * kTspResTiles &&
          tsp_resident_lds(n, G, nep) <= 160 * 1024;
 }
 

@@ -130,8 +130,8 @@ int resident_pairs_max();  // N x G limit of one workgroup
 // false: no kernel instance for these sizes (nothing launched)
 bool launch_resident(hipStream_t st, int gather, const StepArgs &a, const ResArgs &r);
 
-// Resident structured TSP (FP32, 3 <= n <= 64, G = 1, 2 or 4 agents per
-// workgroup): `steps` steps per launch with the elements' X / Y in registers
+// Resident structured TSP (FP32, n >= 3 up to 48 output tiles and 160 KB of
+// LDS, G = 1, 2 or 4 agents per workgroup): `steps` steps per launch with the elements' X / Y in registers
 // and g, the distance table and the sums in LDS.
 size_t tsp_resident_lds(int64_t n, int G, int64_t nep);
 bool tsp_resident_ok(int64_t n, int G, int64_t nep);

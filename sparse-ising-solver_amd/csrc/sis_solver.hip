@@ -506,13 +506,13 @@ int sis_solver_create_tsp(const sis_tsp *t, const sis_params *pin, sis_solver **
       }
   }
   // tile: 1 = VALU distance product, 2 = matrix cores (default for FP32),
-  // 3 = resident workgroups (FP32, n <= 64)
+  // 3 = resident workgroups (FP32; n <= 48 at 4 agents per workgroup, <= 96 at 1)
   if (pin->tile == 0) s->vec = s->f64 ? 1 : 2;
   else if (pin->tile == 1 || (pin->tile == 2 && !s->f64) || (pin->tile == 3 && s->tsp_res_G))
     s->vec = pin->tile;
   else
     return fail(SIS_ERR_ARG, "sis_solver_create_tsp: structured tile must be 1 (VALU), 2 (MFMA, "
-                             "f32) or 3 (resident, f32, n <= 64)");
+                             "f32) or 3 (resident, f32, n <= 96)");
   *out = s.release();
   return SIS_OK;
 }

@@ -40,7 +40,7 @@ def max_rel(x, X):
 
 
 # tile 1: distance product on VALU; 2: on the matrix cores (FP32, default);
-# 3: resident workgroups (FP32, n <= 64: state in registers, g in LDS)
+# 3: resident workgroups (FP32, n <= 96: state in registers, g in LDS)
 FORMS_T = [("f32", "f32", 1), ("f32", "f32", 2), ("f32", "f32", 3), ("f64", "f64", 1)]
 
 
@@ -178,12 +178,13 @@ def test_structured_n512_tiles_agree():
     np.testing.assert_array_equal(out[0], out[1])
 
 
-@pytest.mark.parametrize("agents", [64, 100, 1024])
-def test_structured_resident_equals_mfma_dsb(agents):
-    """Tile 3 (resident workgroups of 1, 2 or 4 agents) against tile 2 at the
-    10,000-step TSP size (n = 48): dSB sums are exact, so the trajectories
-    are bit-identical, including agent counts that leave padding lanes."""
-    t = sis.Tsp.generate(48, 1000, 1, 9)
+@pytest.mark.parametrize("n,agents", [(48, 64), (48, 100), (48, 1024), (64, 256), (80, 128), (96, 64)])
+def test_structured_resident_equals_mfma_dsb(n, agents):
+    """Tile 3 (resident workgroups of 1, 2 or 4 agents) against tile 2, at the
+    10,000-step TSP size (n = 48) and up to the largest resident n: dSB sums
+    are exact, so the trajectories are bit-identical, including agent counts
+    that leave padding lanes."""
+    t = sis.Tsp.generate(n, 1000, 1, 9)
     out = []
     for tile in (2, 3):
         s = sis.Solver.from_tsp(t, dict(agents=agents, form=sis.FORM_F32, dynamics=sis.DSB, dt=0.1,
@@ -232,6 +233,6 @@ def test_structured_default_tile_and_autotune():
         sis.Solver.from_tsp(t, dict(agents=128, form=sis.FORM_F64, structured=1, tile=2))
     with pytest.raises(sis.SisError):  # resident tile: FP32 only
         sis.Solver.from_tsp(t, dict(agents=128, form=sis.FORM_F64, structured=1, tile=3))
-    with pytest.raises(sis.SisError):  # resident tile: n <= 64
-        sis.Solver.from_tsp(sis.Tsp.generate(65, 100, 0, 3),
+    with pytest.raises(sis.SisError):  # resident tile: n <= 96
+        sis.Solver.from_tsp(sis.Tsp.generate(97, 100, 0, 3),
                             dict(agents=128, form=sis.FORM_F32, structured=1, tile=3))
