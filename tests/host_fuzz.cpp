@@ -1,4 +1,4 @@
-// Host-code robustness driver, built by tests/test_host_fuzz.py with g++
+// Host-code robustness program, built by tests/test_host_fuzz.py with g++
 // -fsanitize=address,undefined: the golden reader inputs, seeded mutations of
 // them (byte flips, truncations, inserted numbers, comments and huge values)
 // and the generators at their edges.  Every call must return a status; the
