@@ -1588,6 +1588,14 @@ int sis_batch_autotune(sis_batch *b, int64_t trial_steps, int32_t *tile, double 
   double best_ms = 1e300;
   int32_t best = b->tile;
   const int32_t saved = b->tile;
+  // LDS-staged signs pay only when the gathers outweigh copying each
+  // instance's sign array per segment: tried from a mean degree of 32 up
+  int64_t rows = 0, nnz = 0;
+  for (auto &kv : b->live) {
+    rows += kv.second.req.n;
+    nnz += (int64_t)kv.second.req.col.size();
+  }
+  const bool try_lds = rows > 0 && nnz >= 32 * rows;
   std::vector<int32_t> cand;
   for (int r : {1, 2, 4, 8})
     for (int nt : {0, 1})
@@ -1595,7 +1603,7 @@ int sis_batch_autotune(sis_batch *b, int64_t trial_steps, int32_t *tile, double 
         for (int xc : {0, 1})
           for (int bt : {0, 1})
             for (int pf : {0, 1})
-              for (int ls : {0, 1})
+              for (int ls = 0; ls < (try_lds ? 2 : 1); ls++)
                 for (int sg = 0; sg < (ls ? 4 : 1); sg++) {
                   const int32_t tl = r | (nt << 16) | (vg << 17) | (xc << 18) | (bt << 19) |
                                      (pf << 20) | (ls << 21) | (sg << 22);
