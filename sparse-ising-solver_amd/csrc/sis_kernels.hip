@@ -1053,14 +1053,21 @@ __global__ __launch_bounds__(kResBlock) void k_resident(StepArgs a, ResArgs r) {
   const int64_t lda = a.lda;
   const int t = threadIdx.x, lane = t & 63;
   const int64_t g0 = (int64_t)blockIdx.x * G;
-  const int Z = (int)a.rp[N];
-  uint32_t *srp = reinterpret_cast<uint32_t *>(s_raw);
-  uint16_t *sent = reinterpret_cast<uint16_t *>(srp + N + 1);
+  uint16_t *sent = reinterpret_cast<uint16_t *>(s_raw);  // ELL entries
   unsigned char *snap = s_raw + r.snap_off;  // two buffers of r.snap_bytes
-  for (int i = t; i <= N; i += kResBlock) srp[i] = (uint32_t)a.rp[i];
-  const float *__restrict__ val = (const float *)a.val;
-  for (int k = t; k < Z; k += kResBlock)
-    sent[k] = (uint16_t)(a.col[k] | (signbit(val[k]) ? 0x8000 : 0));
+  for (int k = t; k < r.ell_words; k += kResBlock) sent[k] = r.ell[k];
+  // the rows of register slot m of this wavefront form ELL group wave + 16 m:
+  // its entries start at gbase[m], `gwid[m]` per row, row-interleaved
+  const int R = 64 >> lg, rloc = lane >> lg;
+  const int wv = __builtin_amdgcn_readfirstlane(t >> 6);  // wavefront-uniform: scalar loads
+  uint32_t gbase[M], gwid[M];
+#pragma unroll
+  for (int m = 0; m < M; m++) {
+    const int grp = wv + 16 * m;
+    const bool has = grp < r.ngroups;
+    gbase[m] = has ? r.gofs[grp] : 0;
+    gwid[m] = has ? (r.gofs[grp + 1] - gbase[m]) / (uint32_t)R : 0;
+  }
   // own pairs, and the snapshot of step 0
   float x[M], y[M];
   const float *__restrict__ Xin = r.Xin;
@@ -1107,34 +1114,38 @@ __global__ __launch_bounds__(kResBlock) void k_resident(StepArgs a, ResArgs r) {
     for (int m = 0; m < M; m++) {
       const int q = t + kResBlock * m;
       if (q < NG) {
-        const int row = q >> lg, ag = q & (G - 1);
+        const int ag = q & (G - 1);
         float F = 0.0f;
-        uint32_t k = srp[row];
-        const uint32_t k1 = srp[row + 1];
-        // gather term of entry e (weight +-1 from bit 15)
+        // gather term of entry e (weight +-1 from bit 15; 0xFFFF pads a row)
         auto term = [&](float Fin, uint32_t e, float g) {
+          if (e == 0xFFFFu) return Fin;
           const float v = (e & 0x8000) ? -1.0f : 1.0f;
           if (GATHER == G_DSB) return g != 0.0f ? Fin + v : Fin - v;
           return sb_accum<float, G_BSB>(Fin, v, g, coupling);
         };
         auto gather = [&](uint32_t e) {
-          const int j = (int)(e & 0x7fff);
+          const int j = e == 0xFFFFu ? 0 : (int)(e & 0x7fff);
           if (GATHER == G_DSB) return (float)((cur[j] >> ag) & 1);
           return reinterpret_cast<const float *>(cur)[j * G + ag];
         };
-        // four entries' LDS loads in flight, then the terms in CSR order
-        for (; k + 4 <= k1; k += 4) {
+        // entry k of the row at gbase + k R + rloc: the wavefront's rows read
+        // consecutive LDS words; four entries' loads in flight, then the
+        // terms in CSR order
+        const uint32_t w = gwid[m];
+        const uint16_t *ent = sent + gbase[m] + rloc;
+        uint32_t k = 0;
+        for (; k + 4 <= w; k += 4) {
           uint32_t e[4];
           float g[4];
 #pragma unroll
-          for (int u = 0; u < 4; u++) e[u] = sent[k + u];
+          for (int u = 0; u < 4; u++) e[u] = ent[(k + u) * R];
 #pragma unroll
           for (int u = 0; u < 4; u++) g[u] = gather(e[u]);
 #pragma unroll
           for (int u = 0; u < 4; u++) F = term(F, e[u], g[u]);
         }
-        for (; k < k1; k++) {
-          const uint32_t e = sent[k];
+        for (; k < w; k++) {
+          const uint32_t e = ent[k * R];
           F = term(F, e, gather(e));
         }
         sb_update<float, PROB_MAXCUT>(x[m], y[m], F, c0, c1, c2, 0.0f, 0.0f);

@@ -114,14 +114,21 @@ void launch_tsp_struct_energy(hipStream_t st, const uint64_t *spins, const doubl
                               const TspArgs &ts, int64_t W, int64_t lda, double *part);
 
 // Resident small problems (k_resident): one 1024-thread workgroup per G
-// agents runs `steps` steps with X, Y in registers and the CSR and gather
+// agents runs `steps` steps with X, Y in registers and the matrix and gather
 // source in LDS.  MaxCut fused step, FP32, weights +-1, N <= 32767.
-// LDS: (N+1) uint32 row offsets, Z uint16 entries, then two snapshot buffers
-// of snap_bytes at snap_off (dSB: one byte of G <= 8 sign bits per row; bSB:
-// N x G floats).
+// The matrix is in ELL groups of R = 64 / G rows (the rows of one wavefront
+// register slot): group g holds width_g = max degree entries per row, entry k
+// of row g R + l at gofs[g] + k R + l, as uint16 (column in bits 0..14,
+// weight sign in bit 15, 0xFFFF past the row's end), so a wavefront's rows
+// read consecutive LDS words.  LDS: the ell_words entries, then two snapshot
+// buffers of snap_bytes at snap_off (dSB: one byte of G <= 8 sign bits per
+// row; bSB: N x G floats).
 struct ResArgs {
   const float *Xin;  // X at the first step (plain: the snapshot buffer)
   float *Xout;       // X after the last step
+  const uint16_t *ell;
+  const uint32_t *gofs;  // ngroups + 1 entry offsets
+  int32_t ngroups, ell_words;
   int32_t G, lgG;    // agents per workgroup, log2 G
   int32_t steps;
   int32_t snap_off, snap_bytes;

@@ -62,6 +62,11 @@ struct sis_solver {
   std::vector<double> dist_host;
 
   bool pm1 = false;               // every stored weight is exactly +1 or -1
+  // ELL copies of the matrix for the resident tiles, per log2 G
+  bool ell_ok[4] = {false, false, false, false};
+  int32_t ell_words[4] = {0, 0, 0, 0}, ell_groups[4] = {0, 0, 0, 0};
+  DevBuf<uint16_t> ell[4];
+  DevBuf<uint32_t> gofs[4];
 
   // Row split (k_step_split): rows with more than `split` stored entries.
   std::vector<int64_t> rp_host;  // row pointers, for building the split tables
@@ -144,9 +149,10 @@ static bool resident_layout(const sis_solver *s, int vec, ResArgs *r) {
   if (s->prob != PROB_MAXCUT || s->structured || !s->pm1) return false;
   if (!(s->p.form == SIS_FORM_F32 || (s->p.form == SIS_FORM_PACKED && s->gather == G_DSB)))
     return false;
-  if (s->N > 32767 || s->N * G > resident_pairs_max() || s->lda % G != 0) return false;
+  if (s->N > 32767 || s->N * G > resident_pairs_max() || s->lda % G != 0 || !s->ell_ok[lg])
+    return false;
   auto al16 = [](int64_t b) { return (b + 15) / 16 * 16; };
-  const int64_t off = al16(4 * (s->N + 1) + 2 * s->Z);
+  const int64_t off = al16(2 * (int64_t)s->ell_words[lg]);
   const int64_t snap = al16(s->gather == G_DSB ? s->N : 4 * s->N * G);
   if (off + 2 * snap > 160 * 1024) return false;
   if (r) {
@@ -154,8 +160,46 @@ static bool resident_layout(const sis_solver *s, int vec, ResArgs *r) {
     r->lgG = lg;
     r->snap_off = (int32_t)off;
     r->snap_bytes = (int32_t)snap;
+    r->ell = s->ell[lg].p;
+    r->gofs = s->gofs[lg].p;
+    r->ngroups = s->ell_groups[lg];
+    r->ell_words = s->ell_words[lg];
   }
   return true;
+}
+
+// ELL groups of R = 64 / G rows for the resident tiles (only for matrices
+// whose entries fit the kernel's LDS at all): entry k of row g R + l at
+// gofs[g] + k R + l, column | weight sign << 15, 0xFFFF past a row's end.
+static int build_ell(sis_solver *s, const sis_csr *J) {
+  if (!s->pm1 || s->N > 32767 || s->prob != PROB_MAXCUT || 2 * s->Z > 160 * 1024) return SIS_OK;
+  for (int lg = 0; lg < 4; lg++) {
+    const int64_t R = 64 >> lg, ng = (s->N + R - 1) / R;
+    std::vector<uint32_t> go((size_t)ng + 1, 0);
+    for (int64_t g = 0; g < ng; g++) {
+      int64_t wmax = 0;
+      for (int64_t r = g * R; r < std::min(s->N, (g + 1) * R); r++)
+        wmax = std::max(wmax, J->rp[r + 1] - J->rp[r]);
+      go[(size_t)g + 1] = go[(size_t)g] + (uint32_t)(wmax * R);
+    }
+    const int64_t words = go[(size_t)ng];
+    if (2 * words > 160 * 1024) continue;
+    std::vector<uint16_t> e((size_t)std::max<int64_t>(1, words), 0xFFFF);
+    for (int64_t row = 0; row < s->N; row++) {
+      const int64_t g = row / R, l = row % R;
+      for (int64_t k = J->rp[row]; k < J->rp[row + 1]; k++)
+        e[(size_t)(go[(size_t)g] + (k - J->rp[row]) * R + l)] =
+            (uint16_t)(J->col[(size_t)k] | (J->val[(size_t)k] < 0 ? 0x8000 : 0));
+    }
+    int st;
+    if ((st = s->ell[lg].alloc(e.size())) || (st = s->gofs[lg].alloc(go.size()))) return st;
+    HIPCHK(hipMemcpy(s->ell[lg].p, e.data(), sizeof(uint16_t) * e.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(s->gofs[lg].p, go.data(), sizeof(uint32_t) * go.size(), hipMemcpyHostToDevice));
+    s->ell_words[lg] = (int32_t)words;
+    s->ell_groups[lg] = (int32_t)ng;
+    s->ell_ok[lg] = true;
+  }
+  return SIS_OK;
 }
 
 // Tile encoding: plain / BF16 = agents per lane | non-temporal << 16;
@@ -356,6 +400,7 @@ int sis_solver_create(const sis_csr *J, const double *field, const sis_params *p
     return fail(SIS_ERR_ARG, "sis_solver_create: the unfused step is plain-form dSB only");
   s->prob = prob;
   s->p = p;
+  if (int st0 = build_ell(s.get(), J)) return st0;  // resident tiles' matrix copies
   const int vec = p.tile > 0 ? p.tile : default_tile(s.get());
   if (!tile_ok(s.get(), vec))
     return fail(SIS_ERR_ARG, "sis_solver_create: unsupported tile " + std::to_string(vec) +
