@@ -271,6 +271,8 @@ int sis_solver_run(sis_solver *s, int64_t max_steps, int update_window, sis_run_
 
 /* Tile autotuner (no reference counterpart): times trial_steps fused steps
  * for every tile the form supports on this problem and keeps the fastest.
+ * Resident tiles, which run a window period per launch, are timed over one
+ * period (window_period, else 100 steps; at most 1000) when that is longer.
  * All tiles compute bit-identical results; the state is restored afterwards.
  * tile (may be NULL) receives the chosen tile, ms_per_step (may be NULL) its
  * time per step. */
