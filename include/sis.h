@@ -193,7 +193,7 @@ typedef struct sis_params {
                                1 << 20 | log2 G << 21 (one workgroup per G = 1..8
                                agents, state in registers / LDS for a window
                                period; fused MaxCut FP32, weights +-1,
-                               N <= 32767); structured TSP: 1 VALU, 2 MFMA, 3
+                               N x G <= 12288); structured TSP: 1 VALU, 2 MFMA, 3
                                resident workgroups (FP32, n <= 96) */
   int32_t fused;            /* MaxCut: 1 fused step (src/kernels.cu:211-269), 0 the
                                four-kernel path (src/ising.cu:167-172); plain forms */

@@ -1034,15 +1034,23 @@ void launch_tsp_struct_energy(hipStream_t st, const uint64_t *spins, const doubl
 // ------------------------------------------------- resident small problems
 //
 // Agents are independent, so one workgroup can own G agents of a small
-// problem outright: their X and Y stay in registers and the gather source (the
-// sign bits, or X itself for bSB) in LDS for `steps` consecutive steps, with
-// one workgroup barrier per step and no HBM traffic in between.  The CSR is
-// copied into LDS once per launch as row offsets (uint32) and 16-bit entries:
-// column in bits 0..14, weight sign in bit 15 (weights +-1, N <= 32767).
+// problem outright: their X and Y stay in registers and the gather source in
+// LDS for `steps` consecutive steps, with one workgroup barrier per step and
+// no HBM traffic in between.  Weights are +-1.
 // Pair q = row * G + agent is owned by thread q mod 1024 (register slot
 // q / 1024), so the G agents of a row sit in consecutive lanes of one
-// wavefront.  Every term is added in CSR order with the row kernels'
-// operations: results are bit-identical to k_step_plain / k_step_packed.
+// wavefront.
+// The gather source holds each term ready to add: slot q the term of a +1
+// weight on pair q, slot q + N G that of a -1 weight, and G zero slots at
+// 2 N G for the padding.  dSB: int8 -sgn(x) (+1 where x has its sign bit set),
+// summed as integers; bSB: float -(x * coupling), so F + slot equals the row
+// kernels' fma(-v, x * coupling, F).  The matrix is in LDS in ELL groups of the
+// R = 64 / G rows of one wavefront register slot: entry k of row g R + l is
+// u16 number 4 (c R + l) + (k - 4 c) past the group's start, c = k / 4, so a
+// lane reads four entries with one 8-byte load and a wavefront's loads are
+// contiguous.  An entry is the slot of its (column, weight) for agent 0.
+// Terms are added in CSR order (bSB) or exactly (dSB integer sums):
+// results are bit-identical to k_step_plain / k_step_packed.
 
 template <int GATHER, int M>
 __global__ __launch_bounds__(kResBlock) void k_resident(StepArgs a, ResArgs r) {
@@ -1053,11 +1061,21 @@ __global__ __launch_bounds__(kResBlock) void k_resident(StepArgs a, ResArgs r) {
   const int64_t lda = a.lda;
   const int t = threadIdx.x, lane = t & 63;
   const int64_t g0 = (int64_t)blockIdx.x * G;
-  uint16_t *sent = reinterpret_cast<uint16_t *>(s_raw);  // ELL entries
-  unsigned char *snap = s_raw + r.snap_off;  // two buffers of r.snap_bytes
-  for (int k = t; k < r.ell_words; k += kResBlock) sent[k] = r.ell[k];
-  // the rows of register slot m of this wavefront form ELL group wave + 16 m:
-  // its entries start at gbase[m], `gwid[m]` per row, row-interleaved
+  uint2 *sent = reinterpret_cast<uint2 *>(s_raw);  // ELL, four entries per uint2
+  unsigned char *snap = s_raw + r.snap_off;         // two buffers of r.snap_bytes
+  const uint2 *__restrict__ gell = reinterpret_cast<const uint2 *>(r.ell);
+  for (int k = t; k < (r.ell_words >> 2); k += kResBlock) sent[k] = gell[k];
+  if (t < G) {  // padding slots of both buffers
+    if (GATHER == G_DSB) {
+      reinterpret_cast<int8_t *>(snap)[2 * NG + t] = 0;
+      reinterpret_cast<int8_t *>(snap + r.snap_bytes)[2 * NG + t] = 0;
+    } else {
+      reinterpret_cast<float *>(snap)[2 * NG + t] = 0.0f;
+      reinterpret_cast<float *>(snap + r.snap_bytes)[2 * NG + t] = 0.0f;
+    }
+  }
+  // register slot m of this wavefront holds ELL group wave + 16 m: its chunks
+  // start at gbase[m] (in uint2), gwid[m] chunks per row
   const int R = 64 >> lg, rloc = lane >> lg;
   const int wv = __builtin_amdgcn_readfirstlane(t >> 6);  // wavefront-uniform: scalar loads
   uint32_t gbase[M], gwid[M];
@@ -1065,10 +1083,10 @@ __global__ __launch_bounds__(kResBlock) void k_resident(StepArgs a, ResArgs r) {
   for (int m = 0; m < M; m++) {
     const int grp = wv + 16 * m;
     const bool has = grp < r.ngroups;
-    gbase[m] = has ? r.gofs[grp] : 0;
-    gwid[m] = has ? (r.gofs[grp + 1] - gbase[m]) / (uint32_t)R : 0;
+    gbase[m] = has ? r.gofs[grp] >> 2 : 0;
+    gwid[m] = has ? ((r.gofs[grp + 1] >> 2) - gbase[m]) / (uint32_t)R : 0;
   }
-  // own pairs, and the snapshot of step 0
+  // own pairs, and the gather source of step 0
   float x[M], y[M];
   const float *__restrict__ Xin = r.Xin;
   float *Y = (float *)a.Y;
@@ -1082,23 +1100,26 @@ __global__ __launch_bounds__(kResBlock) void k_resident(StepArgs a, ResArgs r) {
       y[m] = Y[e];
     }
   }
+  const float coupling = (float)a.coupling;
   auto put_snapshot = [&](unsigned char *buf) {
 #pragma unroll
     for (int m = 0; m < M; m++) {
       const int q = t + kResBlock * m;
-      if (GATHER == G_DSB) {
-        // the G sign bits of a row are G consecutive lanes: one byte per row
-        const uint64_t bits = __ballot(q < NG && signbit(x[m]));
-        if (q < NG && (q & (G - 1)) == 0)
-          buf[q >> lg] = (unsigned char)((bits >> (lane & ~(G - 1))) & ((1u << G) - 1));
-      } else if (q < NG) {
-        reinterpret_cast<float *>(buf)[q] = x[m];
+      if (q < NG) {
+        if (GATHER == G_DSB) {
+          const int8_t sg = signbit(x[m]) ? (int8_t)1 : (int8_t)-1;
+          reinterpret_cast<int8_t *>(buf)[q] = sg;
+          reinterpret_cast<int8_t *>(buf)[q + NG] = (int8_t)-sg;
+        } else {
+          const float p = x[m] * coupling;
+          reinterpret_cast<float *>(buf)[q] = -p;
+          reinterpret_cast<float *>(buf)[q + NG] = p;
+        }
       }
     }
   };
   put_snapshot(snap);
   __syncthreads();
-  const float coupling = (float)a.coupling;
   // step coefficients, loaded one step ahead so no step waits on them
   const float *sc = (const float *)a.sched + 4 * (a.ctrl->base + a.offset);
   float n0 = sc[0], n1 = sc[1], n2 = sc[2];
@@ -1115,38 +1136,32 @@ __global__ __launch_bounds__(kResBlock) void k_resident(StepArgs a, ResArgs r) {
       const int q = t + kResBlock * m;
       if (q < NG) {
         const int ag = q & (G - 1);
-        float F = 0.0f;
-        // gather term of entry e (weight +-1 from bit 15; 0xFFFF pads a row)
-        auto term = [&](float Fin, uint32_t e, float g) {
-          if (e == 0xFFFFu) return Fin;
-          const float v = (e & 0x8000) ? -1.0f : 1.0f;
-          if (GATHER == G_DSB) return g != 0.0f ? Fin + v : Fin - v;
-          return sb_accum<float, G_BSB>(Fin, v, g, coupling);
-        };
-        auto gather = [&](uint32_t e) {
-          const int j = e == 0xFFFFu ? 0 : (int)(e & 0x7fff);
-          if (GATHER == G_DSB) return (float)((cur[j] >> ag) & 1);
-          return reinterpret_cast<const float *>(cur)[j * G + ag];
-        };
-        // entry k of the row at gbase + k R + rloc: the wavefront's rows read
-        // consecutive LDS words; four entries' loads in flight, then the
-        // terms in CSR order
+        const uint2 *ent = sent + gbase[m] + rloc;
         const uint32_t w = gwid[m];
-        const uint16_t *ent = sent + gbase[m] + rloc;
-        uint32_t k = 0;
-        for (; k + 4 <= w; k += 4) {
-          uint32_t e[4];
-          float g[4];
-#pragma unroll
-          for (int u = 0; u < 4; u++) e[u] = ent[(k + u) * R];
-#pragma unroll
-          for (int u = 0; u < 4; u++) g[u] = gather(e[u]);
-#pragma unroll
-          for (int u = 0; u < 4; u++) F = term(F, e[u], g[u]);
-        }
-        for (; k < w; k++) {
-          const uint32_t e = ent[k * R];
-          F = term(F, e, gather(e));
+        float F;
+        if (GATHER == G_DSB) {
+          const int8_t *src = reinterpret_cast<const int8_t *>(cur) + ag;
+          int Fi = 0;
+          for (uint32_t c = 0; c < w; c++) {
+            const uint2 e = ent[c * R];
+            Fi += src[e.x & 0xffff];
+            Fi += src[e.x >> 16];
+            Fi += src[e.y & 0xffff];
+            Fi += src[e.y >> 16];
+          }
+          F = (float)Fi;
+        } else {
+          const float *src = reinterpret_cast<const float *>(cur) + ag;
+          F = 0.0f;
+          for (uint32_t c = 0; c < w; c++) {
+            const uint2 e = ent[c * R];
+            const float t0 = src[e.x & 0xffff], t1 = src[e.x >> 16];
+            const float t2 = src[e.y & 0xffff], t3 = src[e.y >> 16];
+            F = F + t0;
+            F = F + t1;
+            F = F + t2;
+            F = F + t3;
+          }
         }
         sb_update<float, PROB_MAXCUT>(x[m], y[m], F, c0, c1, c2, 0.0f, 0.0f);
       }

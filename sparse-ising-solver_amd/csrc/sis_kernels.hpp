@@ -115,14 +115,15 @@ void launch_tsp_struct_energy(hipStream_t st, const uint64_t *spins, const doubl
 
 // Resident small problems (k_resident): one 1024-thread workgroup per G
 // agents runs `steps` steps with X, Y in registers and the matrix and gather
-// source in LDS.  MaxCut fused step, FP32, weights +-1, N <= 32767.
-// The matrix is in ELL groups of R = 64 / G rows (the rows of one wavefront
-// register slot): group g holds width_g = max degree entries per row, entry k
-// of row g R + l at gofs[g] + k R + l, as uint16 (column in bits 0..14,
-// weight sign in bit 15, 0xFFFF past the row's end), so a wavefront's rows
-// read consecutive LDS words.  LDS: the ell_words entries, then two snapshot
-// buffers of snap_bytes at snap_off (dSB: one byte of G <= 8 sign bits per
-// row; bSB: N x G floats).
+// source in LDS.  MaxCut fused step, FP32, weights +-1, N x G <= 12288.
+// The gather source has 2 N G + G slots: slot q (pair q = row G + agent) the
+// term of a +1 weight on pair q, q + N G that of a -1 weight, zeros from 2 N G
+// (dSB int8 -sgn x; bSB float -(x * coupling)).  The matrix is in ELL groups
+// of R = 64 / G rows (the rows of one wavefront register slot), padded to
+// whole chunks of four entries: entry k of row g R + l is uint16 number
+// gofs[g] + 4 ((k / 4) R + l) + k % 4, the gather slot of its column and
+// weight sign for agent 0 (2 N G past the row's end).  LDS: the ell_words
+// entries, then two gather-slot buffers of snap_bytes at snap_off.
 struct ResArgs {
   const float *Xin;  // X at the first step (plain: the snapshot buffer)
   float *Xout;       // X after the last step

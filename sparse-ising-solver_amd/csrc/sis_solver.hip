@@ -139,8 +139,9 @@ struct sis_solver {
 
 // Resident tiles (k_resident): bit 20, log2 of the agents per workgroup in
 // bits 21-22.  Fused MaxCut in FP32 (plain dSB / bSB, packed dSB), weights
-// +-1, N <= 32767, N x G pairs per 1024-thread workgroup within the kernel's
-// register slots, CSR and two snapshots within 160 KB of LDS.
+// +-1, N x G pairs per 1024-thread workgroup within the kernel's register
+// slots (12,288), the ELL matrix and two buffers of gather slots within 160 KB
+// of LDS.
 constexpr int kResidentBit = 1 << 20;
 
 static bool resident_layout(const sis_solver *s, int vec, ResArgs *r) {
@@ -149,11 +150,13 @@ static bool resident_layout(const sis_solver *s, int vec, ResArgs *r) {
   if (s->prob != PROB_MAXCUT || s->structured || !s->pm1) return false;
   if (!(s->p.form == SIS_FORM_F32 || (s->p.form == SIS_FORM_PACKED && s->gather == G_DSB)))
     return false;
-  if (s->N > 32767 || s->N * G > resident_pairs_max() || s->lda % G != 0 || !s->ell_ok[lg])
+  if (s->N * G > resident_pairs_max() || s->lda % G != 0 || !s->ell_ok[lg])
     return false;
   auto al16 = [](int64_t b) { return (b + 15) / 16 * 16; };
   const int64_t off = al16(2 * (int64_t)s->ell_words[lg]);
-  const int64_t snap = al16(s->gather == G_DSB ? s->N : 4 * s->N * G);
+  // gather slots: +1 and -1 weight terms of every pair, then G zeros
+  const int64_t slots = 2 * s->N * G + G;
+  const int64_t snap = al16(s->gather == G_DSB ? slots : 4 * slots);
   if (off + 2 * snap > 160 * 1024) return false;
   if (r) {
     r->G = G;
@@ -169,27 +172,32 @@ static bool resident_layout(const sis_solver *s, int vec, ResArgs *r) {
 }
 
 // ELL groups of R = 64 / G rows for the resident tiles (only for matrices
-// whose entries fit the kernel's LDS at all): entry k of row g R + l at
-// gofs[g] + k R + l, column | weight sign << 15, 0xFFFF past a row's end.
+// whose entries fit the kernel's LDS at all), padded to whole chunks of four
+// entries: entry k of row g R + l at gofs[g] + 4 ((k / 4) R + l) + k % 4.  An
+// entry is the gather slot of agent 0: column G, plus N G for a -1 weight;
+// 2 N G (a zero slot) past a row's end.
 static int build_ell(sis_solver *s, const sis_csr *J) {
-  if (!s->pm1 || s->N > 32767 || s->prob != PROB_MAXCUT || 2 * s->Z > 160 * 1024) return SIS_OK;
+  if (!s->pm1 || s->N > resident_pairs_max() || s->prob != PROB_MAXCUT || 2 * s->Z > 160 * 1024) return SIS_OK;
   for (int lg = 0; lg < 4; lg++) {
-    const int64_t R = 64 >> lg, ng = (s->N + R - 1) / R;
+    const int64_t G = int64_t(1) << lg, R = 64 >> lg, ng = (s->N + R - 1) / R;
+    if (2 * s->N * G + G > 65535) continue;  // slots must fit the 16-bit entries
     std::vector<uint32_t> go((size_t)ng + 1, 0);
     for (int64_t g = 0; g < ng; g++) {
       int64_t wmax = 0;
       for (int64_t r = g * R; r < std::min(s->N, (g + 1) * R); r++)
         wmax = std::max(wmax, J->rp[r + 1] - J->rp[r]);
-      go[(size_t)g + 1] = go[(size_t)g] + (uint32_t)(wmax * R);
+      go[(size_t)g + 1] = go[(size_t)g] + (uint32_t)((wmax + 3) / 4 * 4 * R);
     }
     const int64_t words = go[(size_t)ng];
     if (2 * words > 160 * 1024) continue;
-    std::vector<uint16_t> e((size_t)std::max<int64_t>(1, words), 0xFFFF);
+    std::vector<uint16_t> e((size_t)std::max<int64_t>(4, words), (uint16_t)(2 * s->N * G));
     for (int64_t row = 0; row < s->N; row++) {
       const int64_t g = row / R, l = row % R;
-      for (int64_t k = J->rp[row]; k < J->rp[row + 1]; k++)
-        e[(size_t)(go[(size_t)g] + (k - J->rp[row]) * R + l)] =
-            (uint16_t)(J->col[(size_t)k] | (J->val[(size_t)k] < 0 ? 0x8000 : 0));
+      for (int64_t k = J->rp[row]; k < J->rp[row + 1]; k++) {
+        const int64_t kk = k - J->rp[row];
+        e[(size_t)(go[(size_t)g] + 4 * ((kk / 4) * R + l) + kk % 4)] =
+            (uint16_t)(J->col[(size_t)k] * G + (J->val[(size_t)k] < 0 ? s->N * G : 0));
+      }
     }
     int st;
     if ((st = s->ell[lg].alloc(e.size())) || (st = s->gofs[lg].alloc(go.size()))) return st;

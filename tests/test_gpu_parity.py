@@ -279,6 +279,33 @@ def test_resident_tiles_bit_exact(form, dyn, lg):
     assert np.array_equal(x, X.astype(np.float64)) and np.array_equal(y, Y.astype(np.float64))
 
 
+@pytest.mark.parametrize("form,dyn,lg", [("f32", sis.DSB, 0), ("f32", sis.BSB, 1), ("packed", sis.DSB, 3)])
+def test_resident_hub_and_empty_rows_bit_exact(form, dyn, lg):
+    # 150 rows (not a multiple of 64): a hub joined to rows 1..100, a sparse
+    # +-1 tail, rows 120..149 without entries.  ELL groups of very different
+    # widths, zero-slot padding and empty rows.
+    rng = np.random.default_rng(31)
+    rows = [0] * 100 + list(rng.integers(1, 120, 60))
+    cols = list(range(1, 101)) + list(rng.integers(1, 120, 60))
+    keep = [(r, c) for r, c in zip(rows, cols) if r != c]
+    vals = rng.choice([-1.0, 1.0], len(keep))
+    J = sis.Csr.from_coo(150, [r for r, _ in keep], [c for _, c in keep], vals)
+    rp, col, val = J.arrays()
+    pb = orc.Problem(rp, col, val, S=J.sum)
+    A, T, dt = 64, 300, 0.5
+    tile = RES | lg << 21
+    s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=A, dt=dt, form=sis.FORMS[form], dynamics=dyn,
+                           tile=tile))
+    assert s.p.tile == tile
+    X0, Y0 = seeded_state(J.n, A, 32)
+    s.set_state(X0, Y0)
+    s.advance(0, 151, T)
+    X, Y = X0.astype(np.float32), Y0.astype(np.float32)
+    orc.advance(pb, orc.MAXCUT, dyn, X, Y, 0, 151, T, dt, s.p.xi0, 1.0, "f32")
+    x, y = s.get_state()
+    assert np.array_equal(x, X.astype(np.float64)) and np.array_equal(y, Y.astype(np.float64))
+
+
 @pytest.mark.parametrize("use_graph", [0, 1, 2])
 @pytest.mark.parametrize("form,lg", [("packed", 2), ("f32", 0)])
 def test_resident_full_runs_with_windows(use_graph, form, lg):
@@ -311,7 +338,7 @@ def test_resident_tile_rejected_where_it_does_not_apply():
     with pytest.raises(sis.SisError):  # BF16 shadow form
         sis.Solver(Jp, dict(problem=sis.MAXCUT, agents=128, form=sis.FORM_BF16, dynamics=sis.BSB,
                             tile=RES))
-    big = sis.Csr.gen_torus(200, 200, sis.W_PM1, 1)  # N = 40,000 > 32,767
+    big = sis.Csr.gen_torus(200, 200, sis.W_PM1, 1)  # N = 40,000 > 12,288 pairs
     with pytest.raises(sis.SisError):
         sis.Solver(big, dict(problem=sis.MAXCUT, agents=64, form=sis.FORM_PACKED, tile=RES))
 
