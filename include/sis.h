@@ -334,8 +334,15 @@ typedef struct sis_batch_params {
                             bit-transpose integer sums << 19 | next-task
                             prefetch << 20 | sign words staged in LDS << 21
                             (2, 4 or 8 rows) | LDS segment code s << 22
-                            (segments of 512 << s rows, s = 0..3);
-                            0 = default (8 rows, non-temporal, bit-transpose) */
+                            (segments of 512 << s rows, s = 0..3) | resident
+                            chunks << 24 (one workgroup per instance and group
+                            of agents runs a whole chunk with its state on
+                            chip, while every resident instance has integer
+                            weights in [-3, 3], N <= 8192, its matrix within
+                            LDS and a window period that chunk_steps divides;
+                            otherwise the chunk runs the streaming tile);
+                            0 = default (resident chunks; 8 rows,
+                            non-temporal, bit-transpose otherwise) */
   int32_t eager;         /* 1: launch each chunk's kernels directly instead of
                             replaying the captured hipGraph */
 } sis_batch_params;
@@ -375,7 +382,8 @@ int sis_batch_result(sis_batch *b, int64_t ticket, double *cuts, int32_t *spins)
 int sis_batch_release(sis_batch *b, int64_t ticket);
 
 /* Times each tile on the resident instances (state restored afterwards) and
- * keeps the fastest; all tiles are bit-identical. */
+ * keeps the fastest; all tiles are bit-identical.  Resident chunks, on top of
+ * the fastest streaming tile, are timed over one chunk. */
 int sis_batch_autotune(sis_batch *b, int64_t trial_steps, int32_t *tile, double *ms_per_step);
 
 #ifdef __cplusplus

@@ -653,18 +653,258 @@ __global__ __launch_bounds__(kBlock) void k_batch_step_lds(BArgs b) {
   }
 }
 
-// After each step: one block per slot advances the instance's step count and,
-// at a window boundary, runs its window (src/kernels.cu:24-91, race-free) and
-// decides whether the instance is finished.
-__global__ __launch_bounds__(kBlock) void k_batch_post(BArgs b, int32_t nslots) {
+// Resident chunks (tile bit 24): one 1024-thread workgroup per (instance,
+// group of G agents) runs a whole chunk of K steps with the pairs' X and Y in
+// registers and the instance's matrix and gather slots in LDS, as the
+// single-problem resident kernel does.  Instances need integer weights in
+// [-3, 3] and n G <= 8192 pairs; the F of dSB is then an exact integer sum,
+// so it equals the sequential sums of the step kernels bit for bit, whichever
+// thread computes a row.
+// Set-up per task: the rows are counting-sorted by degree (keys capped at
+// 255), so the 64 / G rows of a wavefront register slot have nearly equal
+// degrees; the matrix is laid out as ELL groups of those rows, padded to the
+// group's longest row in chunks of four entries (entry k of sorted row
+// g R + l is u16 number 4 (gofs[g] + (k / 4) R + l) + k % 4, gofs in chunks).
+// Gather slots (int8): q = row G + agent holds -sgn x of pair q (+1 where x has
+// its sign bit set), q + n G its negation.  Entry: the slot of its column for
+// agent 0, plus n G for a negative weight, | |v| << 14; padding is 0 (weight
+// 0).  LDS bytes: histogram 1024, group offsets 528, sorted rows and degrees
+// 2 al16(2 n), two slot buffers 2 al16(2 n G), then the ELL.  The grid is
+// persistent: workgroups walk the task list (largest instances first).
+struct BResTask {
+  int32_t slot, g0, lg, pad;
+};
+constexpr int kBResBlock = 1024, kBResSlots = 8, kBResPairs = kBResBlock * kBResSlots;
+constexpr int kBResLds = 160 * 1024, kBResHead = 1024 + 528;
+
+__host__ __device__ constexpr int64_t bres_al16(int64_t v) { return (v + 15) / 16 * 16; }
+
+__device__ __forceinline__ uint32_t bres_scan64(uint32_t v, int lane) {  // inclusive
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t u = __shfl_up(v, off, 64);
+    if (lane >= off) v += u;
+  }
+  return v;
+}
+
+__global__ __launch_bounds__(kBResBlock) void k_batch_resident(BArgs b, const BResTask *rt,
+                                                               const int32_t *nrt, int32_t K) {
+  extern __shared__ __align__(16) unsigned char s_raw[];
+  constexpr int M = kBResSlots;
+  const int ntask = *nrt, t = threadIdx.x, lane = t & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int64_t lda = b.lda;
+  uint32_t *hist = reinterpret_cast<uint32_t *>(s_raw);
+  uint32_t *gofs = hist + 256;
+  for (int task = blockIdx.x; task < ntask; task += gridDim.x) {
+    const BResTask tk = rt[task];
+    const BInst &in = b.inst[tk.slot];
+    if (in.status != SLOT_ACTIVE) continue;  // the same for the whole workgroup
+    const int n = in.n, lg = tk.lg, G = 1 << lg, nG = n << lg, R = 64 >> lg;
+    const int ngr = (n + R - 1) / R;  // <= 128
+    const int64_t r0 = in.row_off, g0 = tk.g0;
+    const int64_t kbase = b.rbeg[r0];
+    uint16_t *perm = reinterpret_cast<uint16_t *>(s_raw + kBResHead);
+    uint16_t *degs = reinterpret_cast<uint16_t *>(s_raw + kBResHead + bres_al16(2 * (int64_t)n));
+    int8_t *slots = reinterpret_cast<int8_t *>(s_raw + kBResHead + 2 * bres_al16(2 * (int64_t)n));
+    const int64_t sbytes = bres_al16(2 * (int64_t)nG);
+    uint2 *ell = reinterpret_cast<uint2 *>(reinterpret_cast<unsigned char *>(slots) + 2 * sbytes);
+    // counting sort of the rows by degree
+    if (t < 256) hist[t] = 0;
+    __syncthreads();
+    for (int i = t; i < n; i += kBResBlock) {
+      const int d = (int)(b.rend[r0 + i] - b.rbeg[r0 + i]);
+      degs[i] = (uint16_t)d;
+      atomicAdd(&hist[d < 255 ? d : 255], 1u);
+    }
+    __syncthreads();
+    if (t < 64) {
+      uint32_t a[4], sum = 0;
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        a[u] = hist[4 * t + u];
+        sum += a[u];
+      }
+      uint32_t ex = bres_scan64(sum, lane) - sum;
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        hist[4 * t + u] = ex;
+        ex += a[u];
+      }
+    }
+    __syncthreads();
+    for (int i = t; i < n; i += kBResBlock) {
+      const int d = degs[i];
+      perm[atomicAdd(&hist[d < 255 ? d : 255], 1u)] = (uint16_t)i;
+    }
+    __syncthreads();
+    // group widths in chunks of four, and their offsets (in uint2)
+    if (t < 64) {
+      uint32_t c[2];
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        const int g = 2 * t + u;
+        uint32_t w = 0;
+        if (g < ngr)
+          for (int l = 0; l < R && g * R + l < n; l++) {
+            const uint32_t d = degs[perm[g * R + l]];
+            w = d > w ? d : w;
+          }
+        c[u] = (w + 3) / 4 * (uint32_t)R;
+      }
+      const uint32_t inc = bres_scan64(c[0] + c[1], lane);
+      gofs[2 * t] = inc - c[0] - c[1];
+      gofs[2 * t + 1] = inc - c[1];
+      if (t == 63) gofs[128] = inc;
+    }
+    __syncthreads();
+    // the ELL, one sorted row per thread
+    for (int sr = t; sr < n; sr += kBResBlock) {
+      const int row = perm[sr], g = sr >> (6 - lg), l = sr & (R - 1);
+      const int d = degs[row];
+      const int64_t k0 = b.rbeg[r0 + row];
+      const int nk = (int)(gofs[g + 1] - gofs[g]) / R * 4;
+      uint16_t *dst = reinterpret_cast<uint16_t *>(ell + gofs[g] + l);
+      for (int k = 0; k < nk; k++) {
+        uint16_t e = 0;
+        if (k < d) {
+          const int j = b.col[k0 + k] - (int)r0;
+          const float v = b.val[k0 + k];
+          e = (uint16_t)((((uint32_t)j << lg) + (v < 0.0f ? (uint32_t)nG : 0u)) |
+                         ((uint32_t)fabsf(v) << 14));
+        }
+        dst[4 * (k >> 2) * R + (k & 3)] = e;
+      }
+    }
+    // own pairs: register slot m of this wavefront is ELL group wv + 16 m
+    float x[M], y[M];
+    int rowm[M];
+    uint32_t gb[M], gw[M];
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+      const int q = t + kBResBlock * m, grp = wv + 16 * m;
+      x[m] = y[m] = 0.0f;
+      rowm[m] = 0;
+      const bool has = grp < ngr;
+      gb[m] = __builtin_amdgcn_readfirstlane(has ? gofs[grp] : 0u);
+      gw[m] = __builtin_amdgcn_readfirstlane(has ? (gofs[grp + 1] - gofs[grp]) / (uint32_t)R : 0u);
+      if (q < nG) {
+        rowm[m] = perm[q >> lg];
+        const int64_t e = (r0 + rowm[m]) * lda + g0 + (q & (G - 1));
+        x[m] = b.X[e];
+        y[m] = b.Y[e];
+      }
+    }
+    auto put_slots = [&](int8_t *buf) {
+#pragma unroll
+      for (int m = 0; m < M; m++) {
+        const int q = t + kBResBlock * m;
+        if (q < nG) {
+          const int8_t sg = signbit(x[m]) ? (int8_t)1 : (int8_t)-1;
+          const int qs = (rowm[m] << lg) + (q & (G - 1));
+          buf[qs] = sg;
+          buf[qs + nG] = (int8_t)-sg;
+        }
+      }
+    };
+    put_slots(slots);
+    __syncthreads();
+    const int32_t t0 = in.t, T = in.T;
+    const double dt = in.dt, xi0 = in.xi0;
+    const float c2 = (float)(dt * xi0);
+    const int rl = lane >> lg;
+    for (int st = 0; st < K; st++) {
+      // src/kernels.cu:225-226, 258-262, evaluated as the host schedule does
+      double p = 1.025 * (double)(t0 + st) / (double)T;
+      p = p < 1.0 ? p : 1.0;
+      const float c0 = (float)(dt * (p + 1)), c1 = (float)(dt * (p - 1));
+      const int8_t *cur = slots + (st & 1) * sbytes;
+#pragma unroll
+      for (int m = 0; m < M; m++) {
+        const int q = t + kBResBlock * m;
+        if (q < nG) {
+          const int8_t *src = cur + (q & (G - 1));
+          const uint2 *e4 = ell + gb[m] + rl;
+          // two chunks' entries and their eight slots loaded before the adds
+          auto term4 = [&](const uint2 e, const int (&g)[4]) {
+            return (int)((e.x >> 14) & 3) * g[0] + (int)(e.x >> 30) * g[1] +
+                   (int)((e.y >> 14) & 3) * g[2] + (int)(e.y >> 30) * g[3];
+          };
+          auto load4 = [&](const uint2 e, int (&g)[4]) {
+            g[0] = src[e.x & 0x3fff];
+            g[1] = src[(e.x >> 16) & 0x3fff];
+            g[2] = src[e.y & 0x3fff];
+            g[3] = src[(e.y >> 16) & 0x3fff];
+          };
+          int Fi = 0;
+          uint32_t c = 0;
+          for (; c + 2 <= gw[m]; c += 2) {
+            const uint2 e = e4[c * R], f = e4[(c + 1) * R];
+            int ge[4], gf[4];
+            load4(e, ge);
+            load4(f, gf);
+            Fi += term4(e, ge) + term4(f, gf);
+          }
+          if (c < gw[m]) {
+            const uint2 e = e4[c * R];
+            int ge[4];
+            load4(e, ge);
+            Fi += term4(e, ge);
+          }
+          sb_update<float, PROB_MAXCUT>(x[m], y[m], (float)Fi, c0, c1, c2, 0.0f, 0.0f);
+        }
+      }
+      put_slots(slots + ((st + 1) & 1) * sbytes);
+      __syncthreads();
+    }
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+      const int q = t + kBResBlock * m;
+      if (q < nG) {
+        const int64_t e = (r0 + rowm[m]) * lda + g0 + (q & (G - 1));
+        b.X[e] = x[m];
+        b.Y[e] = y[m];
+      }
+    }
+  }
+}
+
+// After each step (K = 1), or after a resident chunk of K steps that ends at
+// or before the instance's next window boundary: one block per slot advances
+// the instance's step count and, at a window boundary, runs its window
+// (src/kernels.cu:24-91, race-free) and decides whether the instance is
+// finished.  With `pack` it first writes the sign words of X into the buffer
+// of step t + K, as the step kernels do, for the window and the next step.
+__global__ __launch_bounds__(kBlock) void k_batch_post(BArgs b, int32_t nslots, int32_t K,
+                                                       int32_t pack) {
   __shared__ uint64_t s_mism[4], s_new[4];
   __shared__ int s_nbif;
   const int slot = blockIdx.x;
   if (slot >= nslots) return;
   BInst &in = b.inst[slot];
   if (in.status != SLOT_ACTIVE) return;
-  const int32_t t1 = in.t + 1;
+  const int32_t t1 = in.t + K;
   const int64_t W = b.W, r0 = in.row_off, n = in.n;
+  if (pack) {
+    uint64_t *Sn = (t1 & 1) ? b.S1 : b.S0;
+    const int lane = threadIdx.x & 63;
+    const int64_t nw = blockDim.x >> 6, lda = b.lda;
+    // one wavefront per (row, word), four in flight
+    for (int64_t p0 = (threadIdx.x >> 6) * 4; p0 < n * W; p0 += nw * 4) {
+      float xv[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int64_t p = p0 + u;
+        xv[u] = p < n * W ? b.X[(r0 + p / W) * lda + 64 * (p % W) + lane] : 0.0f;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const uint64_t bits = __ballot(signbit(xv[u]));
+        if (lane == 0 && p0 + u < n * W) Sn[r0 * W + p0 + u] = bits;
+      }
+    }
+  }
   __syncthreads();
   if (t1 % in.P == 0) {
     if (threadIdx.x < 4) {
@@ -922,6 +1162,7 @@ struct Live {
   int32_t slot;
   int64_t row_off, nnz_off;
   double admit_s;
+  int32_t res_lg;  // log2 agents per resident-chunk workgroup, -1: not eligible
 };
 
 struct Result {
@@ -968,6 +1209,12 @@ struct sis_batch {
   bool tasks_dirty = true;
   hipGraphExec_t graph = nullptr;
   int32_t graph_tile = -1;
+  // resident chunks (tile bit 24): task list, and whether every live
+  // instance qualifies (otherwise chunks run the streaming tile)
+  DevBuf<BResTask> rtasks;
+  DevBuf<int32_t> nrtasks;
+  bool res_ok = false, graph_res = false;
+  int32_t res_grid = 0;
 
   ~sis_batch() {
     if (graph) (void)hipGraphExecDestroy(graph);
@@ -1009,8 +1256,12 @@ namespace {
 // next-task prefetch << 20 (bit-transpose with non-temporal state) |
 // sign words staged in LDS << 21 (2, 4 or 8 rows per pass, no other bit but
 // non-temporal and XCD-affine) | segment code s << 22 (LDS staging: segments
-// of 512 << s rows, s = 0..3).
+// of 512 << s rows, s = 0..3) | resident chunks << 24 (with any streaming tile,
+// which runs the chunks whenever a live instance does not qualify).
+constexpr int kBatchResBit = 1 << 24;
+
 bool batch_tile_ok(int W, int tile) {
+  tile &= ~kBatchResBit;
   const int r = tile & 0xffff, nt = (tile >> 16) & 1, vg = (tile >> 17) & 1;
   const int bt = (tile >> 19) & 1, pf = (tile >> 20) & 1, ls = (tile >> 21) & 1;
   if ((tile >> 24) != 0 || (nt != 0 && nt != 1)) return false;
@@ -1125,9 +1376,81 @@ void launch_batch_step(sis_batch *b, bool launch = true) {
 #undef SIS_BSTEPV
 }
 
-void launch_batch_post(sis_batch *b) {
+void launch_batch_post(sis_batch *b, int32_t K = 1, int32_t pack = 0) {
   hipLaunchKernelGGL(k_batch_post, dim3(b->p.max_instances), dim3(kBlock), 0, b->stream, b->args(),
-                     b->p.max_instances);
+                     b->p.max_instances, K, pack);
+}
+
+template <typename T>
+int ensure_cap(DevBuf<T> &buf, size_t n) {
+  if (buf.n >= n && buf.p) return SIS_OK;
+  return buf.alloc(std::max<size_t>(n, 2 * buf.n));
+}
+
+// LDS bytes of one resident task (an upper bound of what the kernel lays
+// out): rows sorted by degree with keys capped at 255, every row of degree
+// 255 or more counted at the largest such degree (the kernel's order among
+// them is arbitrary), groups of 64 / G rows padded to chunks of four.
+int64_t bres_lds(const Request &q, int lg) {
+  const int64_t n = q.n, R = 64 >> lg;
+  std::vector<int64_t> d((size_t)n);
+  int64_t dcap = 0;
+  for (int64_t i = 0; i < n; i++) {
+    d[(size_t)i] = q.rp[(size_t)i + 1] - q.rp[(size_t)i];
+    if (d[(size_t)i] > 65535) return INT64_MAX;  // degrees are kept as uint16
+    if (d[(size_t)i] >= 255) dcap = std::max(dcap, d[(size_t)i]);
+  }
+  for (auto &v : d)
+    if (v >= 255) v = dcap;
+  std::sort(d.begin(), d.end());
+  int64_t chunks = 0;
+  for (int64_t g = 0; g * R < n; g++) chunks += (d[(size_t)std::min(n, (g + 1) * R) - 1] + 3) / 4 * R;
+  return kBResHead + 2 * bres_al16(2 * n) + 2 * bres_al16(2 * (n << lg)) + 8 * chunks;
+}
+
+// Agents per resident workgroup: the largest power of two G <= 64 with
+// n G <= kBResPairs (-1: the instance does not qualify).
+int bres_lg(const Live &l, int64_t chunk) {
+  const Request &q = l.req;
+  if (q.n < 1 || q.n > kBResPairs || q.P < 1 || q.P % chunk != 0) return -1;
+  for (double v : q.val)
+    if (!(v == std::floor(v) && v >= -3.0 && v <= 3.0)) return -1;
+  int lg = 6;
+  while (lg > 0 && (q.n << lg) > kBResPairs) lg--;
+  if (bres_lds(q, lg) > kBResLds) return -1;
+  return lg;
+}
+
+// Lets k_batch_resident use 160 KB of dynamic LDS (once per process; call
+// outside graph capture).
+void bres_attr() {
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute((const void *)k_batch_resident,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, kBResLds);
+    attr = true;
+  }
+}
+
+void launch_batch_resident(sis_batch *b, int32_t K) {
+  bres_attr();
+  hipLaunchKernelGGL(k_batch_resident, dim3((unsigned)b->res_grid), dim3(kBResBlock), kBResLds,
+                     b->stream, b->args(), b->rtasks.p, b->nrtasks.p, K);
+}
+
+// One chunk: with resident chunks on and every live instance qualifying, one
+// resident launch of chunk_steps steps and one post; else chunk_steps
+// streaming steps, each followed by its post.
+void enqueue_chunk(sis_batch *b) {
+  if ((b->tile & kBatchResBit) && b->res_ok) {
+    launch_batch_resident(b, b->p.chunk_steps);
+    launch_batch_post(b, b->p.chunk_steps, 1);
+    return;
+  }
+  for (int32_t k = 0; k < b->p.chunk_steps; k++) {
+    launch_batch_step(b);
+    launch_batch_post(b);
+  }
 }
 
 // Task table.  Plain order: instance after instance.  XCD-affine order:
@@ -1179,15 +1502,47 @@ int rebuild_tasks(sis_batch *b) {
     HIPCHK(hipMemcpyAsync(b->tasks.p, t.data(), sizeof(BTask) * t.size(), hipMemcpyHostToDevice,
                           b->stream));
   HIPCHK(hipMemcpyAsync(b->ntasks.p, hdr, sizeof(hdr), hipMemcpyHostToDevice, b->stream));
+  // resident chunks: (slot, first agent, log2 G) per workgroup, largest
+  // instances first so the persistent grid ends evenly
+  b->res_ok = false;
+  if (b->tile & kBatchResBit) {
+    std::vector<std::pair<int64_t, const Live *>> res;
+    bool ok = !b->live.empty();
+    for (auto &kv : b->live) {
+      const int lg = kv.second.res_lg;
+      if (lg < 0) {
+        ok = false;
+        break;
+      }
+      res.push_back({kv.second.req.n << lg, &kv.second});
+    }
+    if (ok) {
+      std::stable_sort(res.begin(), res.end(),
+                       [](const std::pair<int64_t, const Live *> &x,
+                          const std::pair<int64_t, const Live *> &y) { return x.first > y.first; });
+      std::vector<BResTask> rt;
+      for (auto &pr : res) {
+        const int lg = pr.second->res_lg;
+        for (int64_t g0 = 0; g0 < b->lda; g0 += (int64_t)1 << lg)
+          rt.push_back(BResTask{pr.second->slot, (int32_t)g0, lg, 0});
+      }
+      int st;
+      if (b->rtasks.n < rt.size() && b->graph) {  // the captured launch holds the old list
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+      }
+      if ((st = ensure_cap(b->rtasks, rt.size())) || (st = ensure_cap(b->nrtasks, 1))) return st;
+      const int32_t nrt = (int32_t)rt.size();
+      HIPCHK(hipMemcpyAsync(b->rtasks.p, rt.data(), sizeof(BResTask) * rt.size(),
+                            hipMemcpyHostToDevice, b->stream));
+      HIPCHK(hipMemcpyAsync(b->nrtasks.p, &nrt, sizeof(nrt), hipMemcpyHostToDevice, b->stream));
+      b->res_grid = b->cus;  // one 1024-thread workgroup per CU (registers, LDS)
+      b->res_ok = true;
+    }
+  }
   HIPCHK(hipStreamSynchronize(b->stream));
   b->tasks_dirty = false;
   return SIS_OK;
-}
-
-template <typename T>
-int ensure_cap(DevBuf<T> &buf, size_t n) {
-  if (buf.n >= n && buf.p) return SIS_OK;
-  return buf.alloc(std::max<size_t>(n, 2 * buf.n));
 }
 
 // Copies the results of finished slots (one cut kernel, one gather kernel and
@@ -1320,6 +1675,7 @@ int admit(sis_batch *b) {
     l.row_off = r0;
     l.nnz_off = z0;
     l.admit_s = t_admit;
+    l.res_lg = bres_lg(l, b->p.chunk_steps);
     Result &res = b->results[l.req.ticket];
     res.state = 1;
     res.admit_s = t_admit;
@@ -1352,32 +1708,28 @@ int admit(sis_batch *b) {
 }
 
 int build_graph(sis_batch *b) {
-  if (b->graph && b->graph_tile == b->tile) return SIS_OK;
+  if (b->graph && b->graph_tile == b->tile && b->graph_res == b->res_ok) return SIS_OK;
   if (b->graph) {
     (void)hipGraphExecDestroy(b->graph);
     b->graph = nullptr;
   }
   hipGraph_t g = nullptr;
   launch_batch_step(b, false);  // resolve the grid outside the capture
+  bres_attr();
   HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  for (int32_t k = 0; k < b->p.chunk_steps; k++) {
-    launch_batch_step(b);
-    launch_batch_post(b);
-  }
+  enqueue_chunk(b);
   HIPCHK(hipStreamEndCapture(b->stream, &g));
   hipError_t e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) return fail(SIS_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
   b->graph_tile = b->tile;
+  b->graph_res = b->res_ok;
   return SIS_OK;
 }
 
 int run_chunks(sis_batch *b, int64_t chunks) {
   if (b->p.eager) {
-    for (int64_t c = 0; c < chunks * b->p.chunk_steps; c++) {
-      launch_batch_step(b);
-      launch_batch_post(b);
-    }
+    for (int64_t c = 0; c < chunks; c++) enqueue_chunk(b);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(b->stream));
     return SIS_OK;
@@ -1408,9 +1760,10 @@ int sis_batch_create(const sis_batch_params *pin, sis_batch **out) {
   if (!b) return fail(SIS_ERR_NOMEM, "out of host memory");
   b->W = p.agents_max / 64;
   b->lda = p.agents_max;
-  // default: bit-transpose sums (other weights take its readlane path), 8 rows
-  // per task, non-temporal state -- the autotuner's pick on mixed serving sets
-  if (p.tile == 0) p.tile = 8 | (1 << 16) | (1 << 19);
+  // default: resident chunks while every instance qualifies, else
+  // bit-transpose sums (other weights take its readlane path), 8 rows per
+  // task, non-temporal state -- the autotuner's pick on mixed serving sets
+  if (p.tile == 0) p.tile = 8 | (1 << 16) | (1 << 19) | kBatchResBit;
   if (!batch_tile_ok((int)b->W, p.tile)) return fail(SIS_ERR_ARG, "sis_batch_create: unsupported tile");
   b->tile = p.tile;
   b->p = p;
@@ -1631,6 +1984,28 @@ int sis_batch_autotune(sis_batch *b, int64_t trial_steps, int32_t *tile, double 
       if (ms / trial_steps < best_ms) {
         best_ms = ms / trial_steps;
         best = b->tile;
+      }
+    }
+  }
+  // Resident chunks on top of the best streaming tile, timed over one chunk
+  // (their matrix loads are paid once per chunk), when every instance qualifies.
+  if (!st) {
+    b->tile = best | kBatchResBit;
+    b->tasks_dirty = true;
+    if (!(st = copy_all(false)) && !(st = rebuild_tasks(b)) && b->res_ok) {
+      enqueue_chunk(b);
+      (void)hipEventRecord(e0, b->stream);
+      enqueue_chunk(b);
+      (void)hipEventRecord(e1, b->stream);
+      if (hipStreamSynchronize(b->stream) != hipSuccess) {
+        st = fail(SIS_ERR_HIP, "sis_batch_autotune: trial failed");
+      } else {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (ms / b->p.chunk_steps < best_ms) {
+          best_ms = ms / b->p.chunk_steps;
+          best = b->tile;
+        }
       }
     }
   }

@@ -235,3 +235,45 @@ def test_batch_instances_without_edges():
         assert b.poll(t)["steps_executed"] == info["steps_executed"], k
         assert np.array_equal(bc, cuts) and np.array_equal(bs, s.spins()), k
         s.close()
+
+
+RES = 1 << 24  # resident chunks (with the streaming tile as fallback)
+
+
+@pytest.mark.parametrize("agents_max,tile,chunk,eager", [(64, 8 | 1 << 16 | BT | RES, 100, False),
+                                                         (64, 4 | RES, 50, True),
+                                                         (128, 2 | 1 << 16 | RES, 100, False),
+                                                         (256, 4 | BT | RES, 50, False),
+                                                         (64, 8 | BT | RES, 30, False)])
+def test_batch_resident_chunks(agents_max, tile, chunk, eager):
+    """Resident chunks against solo solves: instances of 1 to 6000 rows (one
+    to 64 agents per workgroup), early exits, admission in waves, and a
+    real-weight instance that keeps the batch on the streaming tile while it
+    is live (it retires first, then the chunks go resident).  Chunks of 30
+    steps do not divide the window period: streaming throughout."""
+    empty = sis.Csr.from_coo(1, np.zeros(0, np.int64), np.zeros(0, np.int64), np.zeros(0))
+    Js = instances(8, 91, 64, 3000) + [sis.Csr.gen_er(6000, 8.0 / 5999, sis.W_INT3, 92),
+                                       _weighted(200, 0.05, 7, "real"), empty]
+    T = [600, 900, 500, 800, 700, 900, 600, 800, 700, 300, 500]
+    A = 64 if agents_max == 64 else 100
+    total = sum(J.n for J in Js)
+    b = sis.Batch(agents_max=agents_max, max_instances=8, max_rows=max(total // 2, 6000),
+                  max_nnz=sum(J.nnz for J in Js), chunk_steps=chunk, tile=tile, eager=eager)
+    ts = [b.submit(J, A, 0.5, T[k], update_window=k % 2, seed=50 + k, xi0=0.3)
+          for k, J in enumerate(Js)]
+    while sum(b.advance(300)):
+        pass
+    b.advance(0)
+    for k, (J, t) in enumerate(zip(Js, ts)):
+        s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=A, dt=0.5, xi0=0.3, form=sis.FORM_PACKED))
+        s.init_random(50 + k)
+        info = s.run(T[k], k % 2)
+        cuts, _ = s.cuts()
+        bc, bs = b.result(t, spins=True)
+        assert b.poll(t)["steps_executed"] == info["steps_executed"], (tile, k)
+        assert np.array_equal(bs, s.spins()), (tile, k)
+        if k == 9:  # real weights: the two cut kernels add FP64 terms in different orders
+            np.testing.assert_allclose(bc, cuts, rtol=1e-12, atol=0)
+        else:
+            assert np.array_equal(bc, cuts), (tile, k)
+        s.close()
