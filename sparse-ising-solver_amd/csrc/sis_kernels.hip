@@ -795,6 +795,12 @@ static void allow_big_lds() {
 
 constexpr int kResBlock = 1024;  // threads of a resident workgroup
 
+// Position stride of the resident TSP kernel's g buffers: the smallest
+// PS >= n G with PS = G (mod 32).
+__host__ __device__ constexpr int tsp_res_stride(int64_t n, int G) {
+  return (int)(n * G + ((G - n * G) % 32 + 32) % 32);
+}
+
 // Resident structured TSP (FP32, up to 48 output tiles of 16 x 16 and 160 KB
 // of LDS: n <= 64 at G = 2, n <= 96 at G = 1): one 1024-thread workgroup owns G
 // agents for a whole launch of `steps` steps.  Per step it needs, for every
@@ -803,10 +809,15 @@ constexpr int kResBlock = 1024;  // threads of a resident workgroup
 // with v_mfma_f32_16x16x4f32 (A[r][k] in lane r + 16 k, B[k][c] in lane
 // c + 16 k, C[r][c] in lane c + 16 (r / 4), register r % 4, checked on the
 // device).  Wavefront w owns the output tiles w, w + 16, ... (cities x
-// columns), and the X / Y / h of exactly those elements stay in registers in
+// columns), or, when there are at most 3 city tiles and 16 column tiles, the
+// city tiles 0 .. 2 of column tile w, whose B operands are then the same (one
+// load per K step for all three).  The X / Y / h of exactly the owned
+// elements stay in registers in
 // the accumulator layout, so the GEMM result needs no exchange.  LDS holds g
-// of the current and next step (n^2 G floats each), Dq and the sums.  Two
-// workgroup barriers per step.
+// of the current and next step (n positions each, element (i, k, agent) at
+// i PS + k G + agent, with the position stride PS >= n G padded to PS = G
+// mod 32 so that the positions a wavefront reads together fall into distinct
+// banks), Dq and the sums.  Two workgroup barriers per step.
 constexpr int kTspResTiles = 3;  // output tiles per wavefront (48 tiles of 16 x 16)
 
 template <int GATHER>
@@ -818,11 +829,12 @@ __global__ __launch_bounds__(kResBlock) void k_tsp_resident(StepArgs a, TspArgs 
   const int64_t lda = a.lda;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int64_t g0 = (int64_t)blockIdx.x * G;
-  float *snap = reinterpret_cast<float *>(s_raw);  // 2 x n^2 G
+  const int PS = tsp_res_stride(n, G);
+  float *snap = reinterpret_cast<float *>(s_raw);  // 2 x n PS
   // Dq rows at an odd stride (nep + 1): the 16 rows one MFMA A operand reads
   // fall into distinct LDS banks
   const int dst = nep + 1;
-  float *dqs = snap + 2 * n * nG;                  // CT*16 x dst
+  float *dqs = snap + 2 * n * PS;                  // CT*16 x dst
   float *rs = dqs + CT * 16 * dst;                 // n x G row sums
   float *cs = rs + nG;                             // n x G column sums
   float *hs = cs + nG;                             // n x n field
@@ -833,11 +845,25 @@ __global__ __launch_bounds__(kResBlock) void k_tsp_resident(StepArgs a, TspArgs 
   const float *__restrict__ Xin = r.Xin;
   float *Y = (float *)a.Y;
   const float c = (float)a.coupling;
+  // tile u of this wavefront: city tile ct, column tile nt (wavefront-uniform)
+  const bool colown = CT <= kTspResTiles && NT <= 16;
+  auto tile_of = [&](int u, int &ct, int &nt) {
+    if (colown) {
+      ct = u;
+      nt = w;
+      return u < CT && w < NT;
+    }
+    const int tile = w + 16 * u;
+    ct = tile / NT;
+    nt = tile - ct * NT;
+    return tile < CT * NT;
+  };
   auto elem = [&](int u, int q, int &k, int &col) {
-    const int tile = w + 16 * u, ct = tile / NT, nt = tile - ct * NT;
+    int ct, nt;
+    const bool ok = tile_of(u, ct, nt);
     k = ct * 16 + 4 * (lane >> 4) + q;
     col = nt * 16 + (lane & 15);
-    return tile < CT * NT && k < n && col < NC;
+    return ok && k < n && col < NC;
   };
 #pragma unroll
   for (int u = 0; u < kTspResTiles; u++)
@@ -860,7 +886,7 @@ __global__ __launch_bounds__(kResBlock) void k_tsp_resident(StepArgs a, TspArgs 
         int k, col;
         if (elem(u, q, k, col)) {
           const int i = col >> lg, ag = col & (G - 1);
-          buf[(i * n + k) * G + ag] = tsp_g<float, GATHER>(x[u][q], c);
+          buf[i * PS + k * G + ag] = tsp_g<float, GATHER>(x[u][q], c);
         }
       }
   };
@@ -877,7 +903,7 @@ __global__ __launch_bounds__(kResBlock) void k_tsp_resident(StepArgs a, TspArgs 
       n2 = sc[4 * (st + 1) + 2];
       n3 = sc[4 * (st + 1) + 3];
     }
-    const float *S = snap + (st & 1) * n * nG;
+    const float *S = snap + (st & 1) * n * PS;
     // row sums (position i, agent a) and column sums (city k, agent a): 8
     // lanes per sum, each over every 8th term, then a butterfly over the 8
     for (int item = t; item < 2 * nG * 8; item += kResBlock) {
@@ -886,14 +912,14 @@ __global__ __launch_bounds__(kResBlock) void k_tsp_resident(StepArgs a, TspArgs 
       const int idx = (row ? sum : sum - nG) >> lg, ag = sum & (G - 1);
       float acc = 0.0f;
       for (int m = part; m < n; m += 8)
-        acc += row ? S[(idx * n + m) * G + ag] : S[(m * n + idx) * G + ag];
+        acc += row ? S[idx * PS + m * G + ag] : S[m * PS + idx * G + ag];
       acc += __shfl_xor(acc, 1, 64);
       acc += __shfl_xor(acc, 2, 64);
       acc += __shfl_xor(acc, 4, 64);
       if (part == 0) (row ? rs : cs)[row ? sum : sum - nG] = acc;
     }
     __syncthreads();
-    float *Sn = snap + ((st + 1) & 1) * n * nG;
+    float *Sn = snap + ((st + 1) & 1) * n * PS;
     // the wavefront's tiles advance through K together: independent
     // accumulators, so consecutive MFMAs do not wait on each other
     typedef float f4 __attribute__((ext_vector_type(4)));
@@ -901,11 +927,13 @@ __global__ __launch_bounds__(kResBlock) void k_tsp_resident(StepArgs a, TspArgs 
     int ti[kTspResTiles], tag[kTspResTiles], tim[kTspResTiles], tip[kTspResTiles];
     bool tok[kTspResTiles], tcok[kTspResTiles];
     const float *drow[kTspResTiles];
+    int tct[kTspResTiles];
 #pragma unroll
     for (int u = 0; u < kTspResTiles; u++) {
-      const int tile = w + 16 * u;
-      tok[u] = tile < CT * NT;  // wavefront-uniform
-      const int ct = tok[u] ? tile / NT : 0, nt = tok[u] ? tile - ct * NT : 0;
+      int ct, nt;
+      tok[u] = tile_of(u, ct, nt);  // wavefront-uniform
+      if (!tok[u]) ct = nt = 0;
+      tct[u] = ct;
       const int col = nt * 16 + (lane & 15);
       tcok[u] = tok[u] && col < NC;
       ti[u] = tcok[u] ? col >> lg : 0;
@@ -915,24 +943,40 @@ __global__ __launch_bounds__(kResBlock) void k_tsp_resident(StepArgs a, TspArgs 
       drow[u] = dqs + (ct * 16 + (lane & 15)) * dst;
       acc[u] = f4{0.0f, 0.0f, 0.0f, 0.0f};
     }
-    for (int l0 = 0; l0 < n; l0 += 4) {
-      const int l = l0 + (lane >> 4);
-      float av[kTspResTiles], bv[kTspResTiles];
+    if (colown) {
+      // one column tile: the B operand is shared by the wavefront's tiles
+      for (int l0 = 0; l0 < n; l0 += 4) {
+        const int l = l0 + (lane >> 4);
+        float av[kTspResTiles];
 #pragma unroll
-      for (int u = 0; u < kTspResTiles; u++) {
-        av[u] = drow[u][l];  // zero past n (padded table)
-        bv[u] = (tcok[u] && l < n)
-                    ? S[(tim[u] * n + l) * G + tag[u]] + S[(tip[u] * n + l) * G + tag[u]]
-                    : 0.0f;
+        for (int u = 0; u < kTspResTiles; u++) av[u] = drow[u][l];  // zero past n
+        const float bv = (tcok[0] && l < n)
+                             ? S[tim[0] * PS + l * G + tag[0]] + S[tip[0] * PS + l * G + tag[0]]
+                             : 0.0f;
+#pragma unroll
+        for (int u = 0; u < kTspResTiles; u++)
+          if (tok[u]) acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u], bv, acc[u], 0, 0, 0);
       }
+    } else {
+      for (int l0 = 0; l0 < n; l0 += 4) {
+        const int l = l0 + (lane >> 4);
+        float av[kTspResTiles], bv[kTspResTiles];
 #pragma unroll
-      for (int u = 0; u < kTspResTiles; u++)
-        if (tok[u]) acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u], bv[u], acc[u], 0, 0, 0);
+        for (int u = 0; u < kTspResTiles; u++) {
+          av[u] = drow[u][l];  // zero past n (padded table)
+          bv[u] = (tcok[u] && l < n)
+                      ? S[tim[u] * PS + l * G + tag[u]] + S[tip[u] * PS + l * G + tag[u]]
+                      : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < kTspResTiles; u++)
+          if (tok[u]) acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u], bv[u], acc[u], 0, 0, 0);
+      }
     }
 #pragma unroll
     for (int u = 0; u < kTspResTiles; u++) {
       if (!tok[u]) continue;
-      const int tile = w + 16 * u, ct = tile / NT;
+      const int ct = tct[u];
       const int i = ti[u], ag = tag[u];
 #pragma unroll
       for (int q = 0; q < 4; q++) {
@@ -940,7 +984,7 @@ __global__ __launch_bounds__(kResBlock) void k_tsp_resident(StepArgs a, TspArgs 
         if (tcok[u] && k < n) {
           const float Sv = fmaT(qB, rs[i * G + ag], qC * cs[k * G + ag]) + acc[u][q];
           sb_update<float, PROB_TSP>(x[u][q], y[u][q], -Sv, c0, c1, c2, c3, hs[i * n + k]);
-          Sn[(i * n + k) * G + ag] = tsp_g<float, GATHER>(x[u][q], c);
+          Sn[i * PS + k * G + ag] = tsp_g<float, GATHER>(x[u][q], c);
         }
       }
     }
@@ -962,7 +1006,8 @@ __global__ __launch_bounds__(kResBlock) void k_tsp_resident(StepArgs a, TspArgs 
 }
 
 size_t tsp_resident_lds(int64_t n, int G, int64_t nep) {
-  return sizeof(float) * (size_t)(2 * n * n * G + (n + 15) / 16 * 16 * (nep + 1) + 2 * n * G + n * n);
+  return sizeof(float) *
+         (size_t)(2 * n * tsp_res_stride(n, G) + (n + 15) / 16 * 16 * (nep + 1) + 2 * n * G + n * n);
 }
 
 bool tsp_resident_ok(int64_t n, int G, int64_t nep) {
