@@ -710,13 +710,32 @@ __global__ __launch_bounds__(kBResBlock) void k_batch_resident(BArgs b, const BR
     int8_t *slots = reinterpret_cast<int8_t *>(s_raw + kBResHead + 2 * bres_al16(2 * (int64_t)n));
     const int64_t sbytes = bres_al16(2 * (int64_t)nG);
     uint2 *ell = reinterpret_cast<uint2 *>(reinterpret_cast<unsigned char *>(slots) + 2 * sbytes);
-    // counting sort of the rows by degree
+    // counting sort of the rows by degree.  Thread t owns rows t + 1024 j
+    // (j < 8, as n <= 8192): their start and degree stay in registers, and
+    // their sorted positions come back from the scatter.
+    constexpr int RJ = kBResPairs / kBResBlock;
+    uint32_t rb[RJ];
+    int dg[RJ], ps[RJ];
+#pragma unroll
+    for (int j = 0; j < RJ; j++) {
+      const int i = t + kBResBlock * j;
+      rb[j] = 0;
+      dg[j] = 0;
+      if (i < n) {
+        const int64_t kb0 = b.rbeg[r0 + i];
+        rb[j] = (uint32_t)(kb0 - kbase);
+        dg[j] = (int)(b.rend[r0 + i] - kb0);
+      }
+    }
     if (t < 256) hist[t] = 0;
     __syncthreads();
-    for (int i = t; i < n; i += kBResBlock) {
-      const int d = (int)(b.rend[r0 + i] - b.rbeg[r0 + i]);
-      degs[i] = (uint16_t)d;
-      atomicAdd(&hist[d < 255 ? d : 255], 1u);
+#pragma unroll
+    for (int j = 0; j < RJ; j++) {
+      const int i = t + kBResBlock * j;
+      if (i < n) {
+        degs[i] = (uint16_t)dg[j];
+        atomicAdd(&hist[dg[j] < 255 ? dg[j] : 255], 1u);
+      }
     }
     __syncthreads();
     if (t < 64) {
@@ -734,9 +753,14 @@ __global__ __launch_bounds__(kBResBlock) void k_batch_resident(BArgs b, const BR
       }
     }
     __syncthreads();
-    for (int i = t; i < n; i += kBResBlock) {
-      const int d = degs[i];
-      perm[atomicAdd(&hist[d < 255 ? d : 255], 1u)] = (uint16_t)i;
+#pragma unroll
+    for (int j = 0; j < RJ; j++) {
+      const int i = t + kBResBlock * j;
+      ps[j] = 0;
+      if (i < n) {
+        ps[j] = (int)atomicAdd(&hist[dg[j] < 255 ? dg[j] : 255], 1u);
+        perm[ps[j]] = (uint16_t)i;
+      }
     }
     __syncthreads();
     // group widths in chunks of four, and their offsets (in uint2)
@@ -759,22 +783,35 @@ __global__ __launch_bounds__(kBResBlock) void k_batch_resident(BArgs b, const BR
       if (t == 63) gofs[128] = inc;
     }
     __syncthreads();
-    // the ELL, one sorted row per thread
-    for (int sr = t; sr < n; sr += kBResBlock) {
-      const int row = perm[sr], g = sr >> (6 - lg), l = sr & (R - 1);
-      const int d = degs[row];
-      const int64_t k0 = b.rbeg[r0 + row];
+    // the ELL of the thread's own rows, eight entries' loads in flight
+#pragma unroll
+    for (int j = 0; j < RJ; j++) {
+      if (t + kBResBlock * j >= n) continue;
+      const int sr = ps[j], g = sr >> (6 - lg), l = sr & (R - 1), d = dg[j];
       const int nk = (int)(gofs[g + 1] - gofs[g]) / R * 4;
       uint16_t *dst = reinterpret_cast<uint16_t *>(ell + gofs[g] + l);
-      for (int k = 0; k < nk; k++) {
-        uint16_t e = 0;
-        if (k < d) {
-          const int j = b.col[k0 + k] - (int)r0;
-          const float v = b.val[k0 + k];
-          e = (uint16_t)((((uint32_t)j << lg) + (v < 0.0f ? (uint32_t)nG : 0u)) |
-                         ((uint32_t)fabsf(v) << 14));
+      const int32_t *cp = b.col + kbase + rb[j];
+      const float *vp = b.val + kbase + rb[j];
+      for (int k0 = 0; k0 < nk; k0 += 8) {
+        int jc[8];
+        float vv[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const bool in = k0 + u < d;
+          jc[u] = in ? cp[k0 + u] : 0;
+          vv[u] = in ? vp[k0 + u] : 0.0f;
         }
-        dst[4 * (k >> 2) * R + (k & 3)] = e;
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const int k = k0 + u;
+          if (k < nk) {
+            uint16_t e = 0;
+            if (k < d)
+              e = (uint16_t)((((uint32_t)(jc[u] - (int)r0) << lg) + (vv[u] < 0.0f ? (uint32_t)nG : 0u)) |
+                             ((uint32_t)fabsf(vv[u]) << 14));
+            dst[4 * (k >> 2) * R + (k & 3)] = e;
+          }
+        }
       }
     }
     // own pairs: register slot m of this wavefront is ELL group wv + 16 m
