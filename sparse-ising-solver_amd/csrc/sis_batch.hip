@@ -816,19 +816,20 @@ __global__ __launch_bounds__(kBResBlock) void k_batch_resident(BArgs b, const BR
     }
     // own pairs: register slot m of this wavefront is ELL group wv + 16 m
     float x[M], y[M];
-    int rowm[M];
+    int qsm[M];  // gather slot of the pair: its row G + agent
     uint32_t gb[M], gw[M];
 #pragma unroll
     for (int m = 0; m < M; m++) {
       const int q = t + kBResBlock * m, grp = wv + 16 * m;
       x[m] = y[m] = 0.0f;
-      rowm[m] = 0;
+      qsm[m] = 0;
       const bool has = grp < ngr;
       gb[m] = __builtin_amdgcn_readfirstlane(has ? gofs[grp] : 0u);
       gw[m] = __builtin_amdgcn_readfirstlane(has ? (gofs[grp + 1] - gofs[grp]) / (uint32_t)R : 0u);
       if (q < nG) {
-        rowm[m] = perm[q >> lg];
-        const int64_t e = (r0 + rowm[m]) * lda + g0 + (q & (G - 1));
+        const int row = perm[q >> lg];
+        qsm[m] = (row << lg) + (q & (G - 1));
+        const int64_t e = (r0 + row) * lda + g0 + (q & (G - 1));
         x[m] = b.X[e];
         y[m] = b.Y[e];
       }
@@ -839,9 +840,8 @@ __global__ __launch_bounds__(kBResBlock) void k_batch_resident(BArgs b, const BR
         const int q = t + kBResBlock * m;
         if (q < nG) {
           const int8_t sg = signbit(x[m]) ? (int8_t)1 : (int8_t)-1;
-          const int qs = (rowm[m] << lg) + (q & (G - 1));
-          buf[qs] = sg;
-          buf[qs + nG] = (int8_t)-sg;
+          buf[qsm[m]] = sg;
+          buf[qsm[m] + nG] = (int8_t)-sg;
         }
       }
     };
@@ -851,11 +851,21 @@ __global__ __launch_bounds__(kBResBlock) void k_batch_resident(BArgs b, const BR
     const double dt = in.dt, xi0 = in.xi0;
     const float c2 = (float)(dt * xi0);
     const int rl = lane >> lg;
+    // step coefficients, 128 steps at a time, in the histogram's LDS (free
+    // after the set-up)
+    float *coef = reinterpret_cast<float *>(hist);
     for (int st = 0; st < K; st++) {
-      // src/kernels.cu:225-226, 258-262, evaluated as the host schedule does
-      double p = 1.025 * (double)(t0 + st) / (double)T;
-      p = p < 1.0 ? p : 1.0;
-      const float c0 = (float)(dt * (p + 1)), c1 = (float)(dt * (p - 1));
+      if ((st & 127) == 0) {
+        if (t < 128 && st + t < K) {
+          // src/kernels.cu:225-226, 258-262, evaluated as the host schedule does
+          double p = 1.025 * (double)(t0 + st + t) / (double)T;
+          p = p < 1.0 ? p : 1.0;
+          coef[2 * t] = (float)(dt * (p + 1));
+          coef[2 * t + 1] = (float)(dt * (p - 1));
+        }
+        __syncthreads();
+      }
+      const float c0 = coef[2 * (st & 127)], c1 = coef[2 * (st & 127) + 1];
       const int8_t *cur = slots + (st & 1) * sbytes;
 #pragma unroll
       for (int m = 0; m < M; m++) {
@@ -899,7 +909,7 @@ __global__ __launch_bounds__(kBResBlock) void k_batch_resident(BArgs b, const BR
     for (int m = 0; m < M; m++) {
       const int q = t + kBResBlock * m;
       if (q < nG) {
-        const int64_t e = (r0 + rowm[m]) * lda + g0 + (q & (G - 1));
+        const int64_t e = (r0 + (qsm[m] >> lg)) * lda + g0 + (q & (G - 1));
         b.X[e] = x[m];
         b.Y[e] = y[m];
       }
