@@ -277,3 +277,41 @@ def test_batch_resident_chunks(agents_max, tile, chunk, eager):
         else:
             assert np.array_equal(bc, cuts), (tile, k)
         s.close()
+
+
+@pytest.mark.parametrize("seed", [101, 202, 303])
+def test_batch_resident_random_mixes(seed):
+    """Seeded random mixes on the default tile (resident chunks): sizes from 1
+    to 3000 rows, +-1 and small-integer weights, agents below agents_max,
+    random step counts and update-window flags, chunks of 25, 50 or 100
+    steps; every instance equals its solo solve."""
+    rng = np.random.default_rng(seed)
+    chunk = int(rng.choice([25, 50, 100]))
+    agents_max = int(rng.choice([64, 128]))
+    Js, params = [], []
+    for k in range(12):
+        n = int(np.exp(rng.uniform(0, np.log(3000))))
+        if n < 2:
+            J = sis.Csr.from_coo(1, np.zeros(0, np.int64), np.zeros(0, np.int64), np.zeros(0))
+        else:
+            J = sis.Csr.gen_er(n, min(1.0, 6.0 / (n - 1)), sis.W_PM1 if k % 3 == 0 else sis.W_INT3,
+                               seed * 100 + k)
+        Js.append(J)
+        params.append((int(rng.integers(1, agents_max + 1)), int(rng.integers(1, 900)),
+                       int(rng.integers(0, 2))))
+    b = sis.Batch(agents_max=agents_max, max_instances=6, max_rows=4000, max_nnz=60000,
+                  chunk_steps=chunk)
+    ts = [b.submit(J, A, 0.5, T, update_window=uw, seed=seed + k, xi0=0.3)
+          for k, (J, (A, T, uw)) in enumerate(zip(Js, params))]
+    while sum(b.advance(200)):
+        pass
+    b.advance(0)
+    for k, (J, t, (A, T, uw)) in enumerate(zip(Js, ts, params)):
+        s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=A, dt=0.5, xi0=0.3, form=sis.FORM_PACKED))
+        s.init_random(seed + k)
+        info = s.run(T, uw)
+        cuts, _ = s.cuts()
+        bc, bs = b.result(t, spins=True)
+        assert b.poll(t)["steps_executed"] == info["steps_executed"], (seed, k)
+        assert np.array_equal(bs, s.spins()) and np.array_equal(bc, cuts), (seed, k, J.n, A, T, uw)
+        s.close()
