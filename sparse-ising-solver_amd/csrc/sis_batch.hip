@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -1468,14 +1469,17 @@ int bres_lg(const Live &l, int64_t chunk) {
   return lg;
 }
 
-// Lets k_batch_resident use 160 KB of dynamic LDS (once per process; call
-// outside graph capture).
+// Lets k_batch_resident use 160 KB of dynamic LDS (once per device 0..63;
+// call outside graph capture).
 void bres_attr() {
-  static bool attr = false;
-  if (!attr) {
+  static std::atomic<uint64_t> done{0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+  const uint64_t bit = uint64_t(1) << dev;
+  if (!(done.load() & bit)) {
     (void)hipFuncSetAttribute((const void *)k_batch_resident,
                               hipFuncAttributeMaxDynamicSharedMemorySize, kBResLds);
-    attr = true;
+    done.fetch_or(bit);
   }
 }
 

@@ -4,6 +4,8 @@
 // Every `a += b * c` of the reference step is an explicit fused multiply-add
 // here and nothing else is contracted (-ffp-contract=off), so the FP32 and
 // FP64 forms are bit-exact with a CPU evaluation in the same order.
+#include <atomic>
+
 #include "sis_device.hpp"
 #include "sis_kernels.hpp"
 
@@ -782,14 +784,18 @@ __global__ __launch_bounds__(kBlock) void k_tsp_energy_struct(const uint64_t *sp
 }
 
 // Dynamic LDS above 64 KB (the staged v at n > 256 in FP32, n > 128 in FP64):
-// raise the kernel's limit once to the 160 KB of a gfx950 CU.
+// raise the kernel's limit to the 160 KB of a gfx950 CU, once per kernel and
+// device (devices 0..63; the attribute belongs to the current device).
 template <auto K>
 static void allow_big_lds() {
-  static bool done = false;  // once per kernel
-  if (!done) {
+  static std::atomic<uint64_t> done{0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+  const uint64_t bit = uint64_t(1) << dev;
+  if (!(done.load() & bit)) {
     (void)hipFuncSetAttribute((const void *)K, hipFuncAttributeMaxDynamicSharedMemorySize,
                               160 * 1024);
-    done = true;
+    done.fetch_or(bit);
   }
 }
 
