@@ -669,9 +669,10 @@ __global__ __launch_bounds__(kBlock) void k_batch_step_lds(BArgs b) {
 // Gather slots (int8): q = row G + agent holds -sgn x of pair q (+1 where x has
 // its sign bit set), q + n G its negation.  Entry: the slot of its column for
 // agent 0, plus n G for a negative weight, | |v| << 14; padding is 0 (weight
-// 0).  LDS bytes: histogram 1024, group offsets 528, sorted rows and degrees
-// 2 al16(2 n), two slot buffers 2 al16(2 n G), then the ELL.  The grid is
-// persistent: workgroups walk the task list (largest instances first).
+// 0).  LDS bytes: histogram 1024 (after the set-up: the coefficients of 128
+// steps), group offsets 528, sorted rows and degrees 2 al16(2 n), two slot
+// buffers 2 al16(2 n G), then the ELL.  The grid is persistent: workgroups
+// walk the task list (largest instances first).
 struct BResTask {
   int32_t slot, g0, lg, pad;
 };
