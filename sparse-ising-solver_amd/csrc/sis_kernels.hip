@@ -899,6 +899,47 @@ __global__ __launch_bounds__(kResBlock) void k_tsp_resident(StepArgs a, TspArgs 
   put(snap);
   __syncthreads();
   const float qB = (float)ts.qB, qC = (float)ts.qC;
+  // Everything below that does not change between steps, computed once:
+  // the wavefront's tiles, the B-operand rows of each tile, and per owned
+  // element its field value and its row-sum, column-sum and g offsets.
+  int tim[kTspResTiles], tip[kTspResTiles], rsi[kTspResTiles], pbase[kTspResTiles];
+  bool tok[kTspResTiles], tcok[kTspResTiles];
+  const float *drow[kTspResTiles];
+  int kb[kTspResTiles], hb[kTspResTiles];  // element (u, q): cs at kb + q G, h at hb + q
+  uint32_t eok = 0;  // bit 4 u + q: element (u, q) is owned
+#pragma unroll
+  for (int u = 0; u < kTspResTiles; u++) {
+    int ct, nt;
+    tok[u] = tile_of(u, ct, nt);  // wavefront-uniform
+    if (!tok[u]) ct = nt = 0;
+    const int col = nt * 16 + (lane & 15);
+    tcok[u] = tok[u] && col < NC;
+    const int i = tcok[u] ? col >> lg : 0, ag = col & (G - 1);
+    tim[u] = ((i + n - 1) % n) * PS + ag;
+    tip[u] = ((i + 1) % n) * PS + ag;
+    rsi[u] = i * G + ag;
+    pbase[u] = i * PS;
+    drow[u] = dqs + (ct * 16 + (lane & 15)) * dst;
+    const int k0 = ct * 16 + 4 * (lane >> 4);
+    kb[u] = k0 * G + ag;
+    hb[u] = i * n + k0;
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      if (tcok[u] && k0 + q < n) eok |= 1u << (4 * u + q);
+  }
+  // sums: thread t takes items t, t + 1024, ... (at most 4: 16 n G <= 4096);
+  // item = 8 sum + part sums every 8th term of row or column sum `sum`
+  // (sums nG .. 2 nG - 1 are the column sums: cs follows rs)
+  constexpr int kSumItems = 4;
+  int sb[kSumItems];
+#pragma unroll
+  for (int j = 0; j < kSumItems; j++) {
+    const int item = t + kResBlock * j;
+    const int sum = item >> 3, part = item & 7;
+    const bool row = sum < nG;
+    const int idx = (row ? sum : sum - nG) >> lg, ag = sum & (G - 1);
+    sb[j] = row ? idx * PS + part * G + ag : part * PS + idx * G + ag;
+  }
   const float *sc = (const float *)a.sched + 4 * (a.ctrl->base + a.offset);
   float n0 = sc[0], n1 = sc[1], n2 = sc[2], n3 = sc[3];
   for (int st = 0; st < r.steps; st++) {
@@ -912,17 +953,19 @@ __global__ __launch_bounds__(kResBlock) void k_tsp_resident(StepArgs a, TspArgs 
     const float *S = snap + (st & 1) * n * PS;
     // row sums (position i, agent a) and column sums (city k, agent a): 8
     // lanes per sum, each over every 8th term, then a butterfly over the 8
-    for (int item = t; item < 2 * nG * 8; item += kResBlock) {
-      const int sum = item >> 3, part = item & 7;
-      const bool row = sum < nG;
-      const int idx = (row ? sum : sum - nG) >> lg, ag = sum & (G - 1);
+#pragma unroll
+    for (int j = 0; j < kSumItems; j++) {
+      const int item = t + kResBlock * j;
+      if (item >= 2 * nG * 8) break;  // the same for the whole wavefront
+      const int part = item & 7;
+      const int stride = (item >> 3) < nG ? 8 * G : 8 * PS;
       float acc = 0.0f;
-      for (int m = part; m < n; m += 8)
-        acc += row ? S[idx * PS + m * G + ag] : S[m * PS + idx * G + ag];
+      int o = sb[j];
+      for (int m = part; m < n; m += 8, o += stride) acc += S[o];
       acc += __shfl_xor(acc, 1, 64);
       acc += __shfl_xor(acc, 2, 64);
       acc += __shfl_xor(acc, 4, 64);
-      if (part == 0) (row ? rs : cs)[row ? sum : sum - nG] = acc;
+      if (part == 0) rs[item >> 3] = acc;
     }
     __syncthreads();
     float *Sn = snap + ((st + 1) & 1) * n * PS;
@@ -930,25 +973,8 @@ __global__ __launch_bounds__(kResBlock) void k_tsp_resident(StepArgs a, TspArgs 
     // accumulators, so consecutive MFMAs do not wait on each other
     typedef float f4 __attribute__((ext_vector_type(4)));
     f4 acc[kTspResTiles];
-    int ti[kTspResTiles], tag[kTspResTiles], tim[kTspResTiles], tip[kTspResTiles];
-    bool tok[kTspResTiles], tcok[kTspResTiles];
-    const float *drow[kTspResTiles];
-    int tct[kTspResTiles];
 #pragma unroll
-    for (int u = 0; u < kTspResTiles; u++) {
-      int ct, nt;
-      tok[u] = tile_of(u, ct, nt);  // wavefront-uniform
-      if (!tok[u]) ct = nt = 0;
-      tct[u] = ct;
-      const int col = nt * 16 + (lane & 15);
-      tcok[u] = tok[u] && col < NC;
-      ti[u] = tcok[u] ? col >> lg : 0;
-      tag[u] = col & (G - 1);
-      tim[u] = (ti[u] + n - 1) % n;
-      tip[u] = (ti[u] + 1) % n;
-      drow[u] = dqs + (ct * 16 + (lane & 15)) * dst;
-      acc[u] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-    }
+    for (int u = 0; u < kTspResTiles; u++) acc[u] = f4{0.0f, 0.0f, 0.0f, 0.0f};
     if (colown) {
       // one column tile: the B operand is shared by the wavefront's tiles
       for (int l0 = 0; l0 < n; l0 += 4) {
@@ -956,9 +982,7 @@ __global__ __launch_bounds__(kResBlock) void k_tsp_resident(StepArgs a, TspArgs 
         float av[kTspResTiles];
 #pragma unroll
         for (int u = 0; u < kTspResTiles; u++) av[u] = drow[u][l];  // zero past n
-        const float bv = (tcok[0] && l < n)
-                             ? S[tim[0] * PS + l * G + tag[0]] + S[tip[0] * PS + l * G + tag[0]]
-                             : 0.0f;
+        const float bv = (tcok[0] && l < n) ? S[tim[0] + l * G] + S[tip[0] + l * G] : 0.0f;
 #pragma unroll
         for (int u = 0; u < kTspResTiles; u++)
           if (tok[u]) acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u], bv, acc[u], 0, 0, 0);
@@ -970,9 +994,7 @@ __global__ __launch_bounds__(kResBlock) void k_tsp_resident(StepArgs a, TspArgs 
 #pragma unroll
         for (int u = 0; u < kTspResTiles; u++) {
           av[u] = drow[u][l];  // zero past n (padded table)
-          bv[u] = (tcok[u] && l < n)
-                      ? S[tim[u] * PS + l * G + tag[u]] + S[tip[u] * PS + l * G + tag[u]]
-                      : 0.0f;
+          bv[u] = (tcok[u] && l < n) ? S[tim[u] + l * G] + S[tip[u] + l * G] : 0.0f;
         }
 #pragma unroll
         for (int u = 0; u < kTspResTiles; u++)
@@ -982,15 +1004,14 @@ __global__ __launch_bounds__(kResBlock) void k_tsp_resident(StepArgs a, TspArgs 
 #pragma unroll
     for (int u = 0; u < kTspResTiles; u++) {
       if (!tok[u]) continue;
-      const int ct = tct[u];
-      const int i = ti[u], ag = tag[u];
+      const float rv = rs[rsi[u]];
 #pragma unroll
       for (int q = 0; q < 4; q++) {
-        const int k = ct * 16 + 4 * (lane >> 4) + q;
-        if (tcok[u] && k < n) {
-          const float Sv = fmaT(qB, rs[i * G + ag], qC * cs[k * G + ag]) + acc[u][q];
-          sb_update<float, PROB_TSP>(x[u][q], y[u][q], -Sv, c0, c1, c2, c3, hs[i * n + k]);
-          Sn[i * PS + k * G + ag] = tsp_g<float, GATHER>(x[u][q], c);
+        if ((eok >> (4 * u + q)) & 1) {
+          const int ci = kb[u] + q * G;
+          const float Sv = fmaT(qB, rv, qC * cs[ci]) + acc[u][q];
+          sb_update<float, PROB_TSP>(x[u][q], y[u][q], -Sv, c0, c1, c2, c3, hs[hb[u] + q]);
+          Sn[pbase[u] + ci] = tsp_g<float, GATHER>(x[u][q], c);
         }
       }
     }
@@ -1019,6 +1040,7 @@ size_t tsp_resident_lds(int64_t n, int G, int64_t nep) {
 bool tsp_resident_ok(int64_t n, int G, int64_t nep) {
   const int64_t tiles = (n + 15) / 16 * ((n * G + 15) / 16);
   return n >= 3 && (G == 1 || G == 2 || G == 4) && tiles <= 16 * kTspResTiles &&
+         16 * n * G <= 4 * kResBlock &&  // at most 4 sum items per thread
          tsp_resident_lds(n, G, nep) <= 160 * 1024;
 }
 
