@@ -1125,7 +1125,9 @@ void launch_tsp_struct_energy(hipStream_t st, const uint64_t *spins, const doubl
 // Terms are added in CSR order (bSB) or exactly (dSB integer sums):
 // results are bit-identical to k_step_plain / k_step_packed.
 
-template <int GATHER, int M>
+// SORTED: register slot pairs follow the degree order of r.perm; each slot
+// keeps its pair (row G + agent) in qp[m].
+template <int GATHER, int M, int SORTED>
 __global__ __launch_bounds__(kResBlock) void k_resident(StepArgs a, ResArgs r) {
   extern __shared__ __align__(16) unsigned char s_raw[];
   if (a.ctrl->done) return;
@@ -1161,14 +1163,17 @@ __global__ __launch_bounds__(kResBlock) void k_resident(StepArgs a, ResArgs r) {
   }
   // own pairs, and the gather source of step 0
   float x[M], y[M];
+  int qp[M];  // the slot's pair: row G + agent
   const float *__restrict__ Xin = r.Xin;
   float *Y = (float *)a.Y;
 #pragma unroll
   for (int m = 0; m < M; m++) {
     const int q = t + kResBlock * m;
     x[m] = y[m] = 0.0f;
+    qp[m] = q;
     if (q < NG) {
-      const int64_t e = (int64_t)(q >> lg) * lda + g0 + (q & (G - 1));
+      if (SORTED) qp[m] = ((int)r.perm[q >> lg] << lg) + (q & (G - 1));
+      const int64_t e = (int64_t)(qp[m] >> lg) * lda + g0 + (q & (G - 1));
       x[m] = Xin[e];
       y[m] = Y[e];
     }
@@ -1179,14 +1184,15 @@ __global__ __launch_bounds__(kResBlock) void k_resident(StepArgs a, ResArgs r) {
     for (int m = 0; m < M; m++) {
       const int q = t + kResBlock * m;
       if (q < NG) {
+        const int qs = SORTED ? qp[m] : q;
         if (GATHER == G_DSB) {
           const int8_t sg = signbit(x[m]) ? (int8_t)1 : (int8_t)-1;
-          reinterpret_cast<int8_t *>(buf)[q] = sg;
-          reinterpret_cast<int8_t *>(buf)[q + NG] = (int8_t)-sg;
+          reinterpret_cast<int8_t *>(buf)[qs] = sg;
+          reinterpret_cast<int8_t *>(buf)[qs + NG] = (int8_t)-sg;
         } else {
           const float p = x[m] * coupling;
-          reinterpret_cast<float *>(buf)[q] = -p;
-          reinterpret_cast<float *>(buf)[q + NG] = p;
+          reinterpret_cast<float *>(buf)[qs] = -p;
+          reinterpret_cast<float *>(buf)[qs + NG] = p;
         }
       }
     }
@@ -1247,7 +1253,7 @@ __global__ __launch_bounds__(kResBlock) void k_resident(StepArgs a, ResArgs r) {
   for (int m = 0; m < M; m++) {
     const int q = t + kResBlock * m;
     if (q < NG) {
-      const int64_t e = (int64_t)(q >> lg) * lda + g0 + (q & (G - 1));
+      const int64_t e = (int64_t)((SORTED ? qp[m] : q) >> lg) * lda + g0 + (q & (G - 1));
       Xout[e] = x[m];
       Y[e] = y[m];
     }
@@ -1255,25 +1261,31 @@ __global__ __launch_bounds__(kResBlock) void k_resident(StepArgs a, ResArgs r) {
 }
 
 int resident_pairs_max() { return kResBlock * 12; }  // the largest instance below: M = 12
+int resident_sorted_pairs_max() { return kResBlock * 8; }  // sorted instances up to M = 8
 
 bool launch_resident(hipStream_t st, int gather, const StepArgs &a, const ResArgs &r) {
   const int64_t NG = a.N * r.G;
   const dim3 grid((unsigned)(a.lda / r.G));
   const size_t shm = (size_t)r.snap_off + 2 * (size_t)r.snap_bytes;
-#define SIS_RES(GA, MM)                                                              \
-  if (gather == GA && NG <= (int64_t)kResBlock * MM) {                              \
-    allow_big_lds<k_resident<GA, MM>>();                                            \
-    hipLaunchKernelGGL((k_resident<GA, MM>), grid, dim3(kResBlock), shm, st, a, r); \
-    return true;                                                                    \
+  const int sorted = r.perm != nullptr;
+#define SIS_RES(GA, MM, SO)                                                                  \
+  if (gather == GA && sorted == SO && NG <= (int64_t)kResBlock * MM) {                      \
+    allow_big_lds<k_resident<GA, MM, SO>>();                                                \
+    hipLaunchKernelGGL((k_resident<GA, MM, SO>), grid, dim3(kResBlock), shm, st, a, r);    \
+    return true;                                                                            \
   }
-  SIS_RES(G_DSB, 4)
-  SIS_RES(G_DSB, 8)
-  SIS_RES(G_DSB, 10)
-  SIS_RES(G_DSB, 12)
-  SIS_RES(G_BSB, 4)
-  SIS_RES(G_BSB, 8)
-  SIS_RES(G_BSB, 10)
-  SIS_RES(G_BSB, 12)
+  SIS_RES(G_DSB, 4, 0)
+  SIS_RES(G_DSB, 8, 0)
+  SIS_RES(G_DSB, 10, 0)
+  SIS_RES(G_DSB, 12, 0)
+  SIS_RES(G_BSB, 4, 0)
+  SIS_RES(G_BSB, 8, 0)
+  SIS_RES(G_BSB, 10, 0)
+  SIS_RES(G_BSB, 12, 0)
+  SIS_RES(G_DSB, 4, 1)
+  SIS_RES(G_DSB, 8, 1)
+  SIS_RES(G_BSB, 4, 1)
+  SIS_RES(G_BSB, 8, 1)
 #undef SIS_RES
   return false;
 }

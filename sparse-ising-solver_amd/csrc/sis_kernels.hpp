@@ -120,21 +120,25 @@ void launch_tsp_struct_energy(hipStream_t st, const uint64_t *spins, const doubl
 // term of a +1 weight on pair q, q + N G that of a -1 weight, zeros from 2 N G
 // (dSB int8 -sgn x; bSB float -(x * coupling)).  The matrix is in ELL groups
 // of R = 64 / G rows (the rows of one wavefront register slot), padded to
-// whole chunks of four entries: entry k of row g R + l is uint16 number
-// gofs[g] + 4 ((k / 4) R + l) + k % 4, the gather slot of its column and
-// weight sign for agent 0 (2 N G past the row's end).  LDS: the ell_words
-// entries, then two gather-slot buffers of snap_bytes at snap_off.
+// whole chunks of four entries: entry k of (sorted) row g R + l is uint16
+// number gofs[g] + 4 ((k / 4) R + l) + k % 4, the gather slot of its column
+// and weight sign for agent 0 (2 N G past the row's end).  With perm the rows
+// are in degree order (sorted row s is row perm[s]), so a group pads to
+// nearly equal degrees.  LDS: the ell_words entries, then two gather-slot
+// buffers of snap_bytes at snap_off.
 struct ResArgs {
   const float *Xin;  // X at the first step (plain: the snapshot buffer)
   float *Xout;       // X after the last step
   const uint16_t *ell;
   const uint32_t *gofs;  // ngroups + 1 entry offsets
+  const uint16_t *perm;  // sorted index -> row, or null (natural order)
   int32_t ngroups, ell_words;
   int32_t G, lgG;    // agents per workgroup, log2 G
   int32_t steps;
   int32_t snap_off, snap_bytes;
 };
-int resident_pairs_max();  // N x G limit of one workgroup
+int resident_pairs_max();         // N x G limit of one workgroup
+int resident_sorted_pairs_max();  // the same with rows in degree order (perm)
 // false: no kernel instance for these sizes (nothing launched)
 bool launch_resident(hipStream_t st, int gather, const StepArgs &a, const ResArgs &r);
 

@@ -67,6 +67,7 @@ struct sis_solver {
   int32_t ell_words[4] = {0, 0, 0, 0}, ell_groups[4] = {0, 0, 0, 0};
   DevBuf<uint16_t> ell[4];
   DevBuf<uint32_t> gofs[4];
+  DevBuf<uint16_t> ell_perm[4];  // rows in degree order, when the ELL uses it
 
   // Row split (k_step_split): rows with more than `split` stored entries.
   std::vector<int64_t> rp_host;  // row pointers, for building the split tables
@@ -165,6 +166,7 @@ static bool resident_layout(const sis_solver *s, int vec, ResArgs *r) {
     r->snap_bytes = (int32_t)snap;
     r->ell = s->ell[lg].p;
     r->gofs = s->gofs[lg].p;
+    r->perm = s->ell_perm[lg].p;  // null: rows in natural order
     r->ngroups = s->ell_groups[lg];
     r->ell_words = s->ell_words[lg];
   }
@@ -178,21 +180,37 @@ static bool resident_layout(const sis_solver *s, int vec, ResArgs *r) {
 // 2 N G (a zero slot) past a row's end.
 static int build_ell(sis_solver *s, const sis_csr *J) {
   if (!s->pm1 || s->N > resident_pairs_max() || s->prob != PROB_MAXCUT || 2 * s->Z > 160 * 1024) return SIS_OK;
+  std::vector<int64_t> byd((size_t)s->N);  // rows by descending degree (stable)
+  for (int64_t i = 0; i < s->N; i++) byd[(size_t)i] = i;
+  auto deg = [&](int64_t r) { return J->rp[r + 1] - J->rp[r]; };
+  std::stable_sort(byd.begin(), byd.end(), [&](int64_t x, int64_t y) { return deg(x) > deg(y); });
   for (int lg = 0; lg < 4; lg++) {
     const int64_t G = int64_t(1) << lg, R = 64 >> lg, ng = (s->N + R - 1) / R;
     if (2 * s->N * G + G > 65535) continue;  // slots must fit the 16-bit entries
-    std::vector<uint32_t> go((size_t)ng + 1, 0);
-    for (int64_t g = 0; g < ng; g++) {
-      int64_t wmax = 0;
-      for (int64_t r = g * R; r < std::min(s->N, (g + 1) * R); r++)
-        wmax = std::max(wmax, J->rp[r + 1] - J->rp[r]);
-      go[(size_t)g + 1] = go[(size_t)g] + (uint32_t)((wmax + 3) / 4 * 4 * R);
+    // group widths for rows in natural order and in degree order; the sorted
+    // layout (kernel instances up to 8 register slots) when 10% smaller
+    auto layout = [&](bool sorted, std::vector<uint32_t> &go) {
+      go.assign((size_t)ng + 1, 0);
+      for (int64_t g = 0; g < ng; g++) {
+        int64_t wmax = 0;
+        for (int64_t i = g * R; i < std::min(s->N, (g + 1) * R); i++)
+          wmax = std::max(wmax, deg(sorted ? byd[(size_t)i] : i));
+        go[(size_t)g + 1] = go[(size_t)g] + (uint32_t)((wmax + 3) / 4 * 4 * R);
+      }
+      return (int64_t)go[(size_t)ng];
+    };
+    std::vector<uint32_t> go, gos;
+    int64_t words = layout(false, go);
+    const int64_t words_s = layout(true, gos);
+    const bool sorted = s->N * G <= resident_sorted_pairs_max() && 10 * words_s <= 9 * words;
+    if (sorted) {
+      go.swap(gos);
+      words = words_s;
     }
-    const int64_t words = go[(size_t)ng];
     if (2 * words > 160 * 1024) continue;
     std::vector<uint16_t> e((size_t)std::max<int64_t>(4, words), (uint16_t)(2 * s->N * G));
-    for (int64_t row = 0; row < s->N; row++) {
-      const int64_t g = row / R, l = row % R;
+    for (int64_t si = 0; si < s->N; si++) {
+      const int64_t row = sorted ? byd[(size_t)si] : si, g = si / R, l = si % R;
       for (int64_t k = J->rp[row]; k < J->rp[row + 1]; k++) {
         const int64_t kk = k - J->rp[row];
         e[(size_t)(go[(size_t)g] + 4 * ((kk / 4) * R + l) + kk % 4)] =
@@ -203,6 +221,12 @@ static int build_ell(sis_solver *s, const sis_csr *J) {
     if ((st = s->ell[lg].alloc(e.size())) || (st = s->gofs[lg].alloc(go.size()))) return st;
     HIPCHK(hipMemcpy(s->ell[lg].p, e.data(), sizeof(uint16_t) * e.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(s->gofs[lg].p, go.data(), sizeof(uint32_t) * go.size(), hipMemcpyHostToDevice));
+    if (sorted) {
+      std::vector<uint16_t> pm((size_t)s->N);
+      for (int64_t si = 0; si < s->N; si++) pm[(size_t)si] = (uint16_t)byd[(size_t)si];
+      if ((st = s->ell_perm[lg].alloc(pm.size()))) return st;
+      HIPCHK(hipMemcpy(s->ell_perm[lg].p, pm.data(), sizeof(uint16_t) * pm.size(), hipMemcpyHostToDevice));
+    }
     s->ell_words[lg] = (int32_t)words;
     s->ell_groups[lg] = (int32_t)ng;
     s->ell_ok[lg] = true;
