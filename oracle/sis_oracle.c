@@ -217,12 +217,50 @@ static void orc_unfused_f32(const int64_t *rp, const int32_t *col, const float *
   free(Xu);
 }
 
+/* 12-bit fixed-point state (the narrow-state dSB form).  X is stored as the
+ * integer code round(x * 2^10) in [-1024, 1024], or +-1025 for any |x| > 1
+ * (the value of an element beyond the wall does not matter: the next step
+ * confines it to sgn x with y = 0, src/kernels.cu:236-242); Y as
+ * round(y * 2^sy) clamped to [-2047, 2047] (0 beside a +-1025 X code).  Codes
+ * are rounded to nearest even and stored as integers, so a code of 0 is +0.0
+ * (sgn +1).  sy is the largest s in [0, 11] with 2047 * 2^-s >= 2/dt + dt,
+ * the bound of |y| while |x| <= 1: |x_new - x| = c0 |y'| <= 2 with
+ * c0 = dt (p + 1) >= dt gives |y'| <= 2 / dt, and |c1 x| <= dt. */
+int orc_q12_shift(double dt) {
+  if (!(dt > 0)) return -1;
+  double ymax = 2.0 / dt + dt;
+  int s = 11;
+  while (s > 0 && ldexp(2047.0, -s) < ymax) s--;
+  return s;
+}
+
+static inline void orc_q12_round(float *x, float *y, int sy) {
+  if (fabsf(*x) > 1.0f) {
+    *x = signbit(*x) ? -1025.0f / 1024.0f : 1025.0f / 1024.0f;
+    *y = 0.0f;
+    return;
+  }
+  int xq = (int)rintf(ldexpf(*x, 10));
+  float yr = rintf(ldexpf(*y, sy));
+  int yq = yr > 2047.0f ? 2047 : yr < -2047.0f ? -2047 : (int)yr;
+  *x = ldexpf((float)xq, -10);
+  *y = ldexpf((float)yq, -sy);
+}
+
+/* Rounds FP32 (x, y) pairs to the codes above in place. */
+void orc_q12_round_state(int64_t n, float *X, float *Y, int sy) {
+  for (int64_t e = 0; e < n; e++) orc_q12_round(&X[e], &Y[e], sy);
+}
+
 /* One fused step over all (i, a), FP64.
  * MaxCut (src/kernels.cu:236-262): confine |x| > 1 to sgn x and y = 0, else
  * y += (dt xi0) F with F = -sum_k J_k g(x_col); then x += dt(p+1) y and
  * y += dt(p-1) x for every element.
  * TSP (src/kernels.cu:564-596): a confined element skips the whole update;
- * otherwise y += dt (xi0 (2 I1 + b h_i)), x += dt y, y += dt(-(1-a)) x. */
+ * otherwise y += dt (xi0 (2 I1 + b h_i)), x += dt y, y += dt(-(1-a)) x.
+ * Each row's forces are accumulated for all agents at once (agent index
+ * innermost); per element the terms are still added in ascending CSR order,
+ * so the result is the same as one element at a time. */
 void orc_step_f64(int problem, int gather, const int64_t *rp, const int32_t *col, const double *val,
                   const double *h, int64_t N, int64_t A, const double *Xs, double *Xn, double *Y,
                   const double *c, double coupling) {
@@ -230,96 +268,133 @@ void orc_step_f64(int problem, int gather, const int64_t *rp, const int32_t *col
     orc_unfused_f64(rp, col, val, N, A, Xs, Xn, Y, c);
     return;
   }
-#pragma omp parallel for schedule(static)
-  for (int64_t i = 0; i < N; i++) {
-    for (int64_t a = 0; a < A; a++) {
-      int64_t e = i * A + a;
-      double x = Xs[e], y = Y[e];
-      int conf = fabs(x) > 1;
-      double F = 0.0;
-      if (!conf) {
-        for (int64_t k = rp[i]; k < rp[i + 1]; k++) {
-          double xj = Xs[(int64_t)col[k] * A + a];
-          double g = gather == ORC_DSB ? (signbit(xj) ? -1.0 : 1.0) : xj * coupling;
-          F = fma(-val[k], g, F);
-        }
+#pragma omp parallel
+  {
+    double *F = (double *)malloc(sizeof(double) * (size_t)(A > 0 ? A : 1));
+#pragma omp for schedule(static)
+    for (int64_t i = 0; i < N; i++) {
+      for (int64_t a = 0; a < A; a++) F[a] = 0.0;
+      for (int64_t k = rp[i]; k < rp[i + 1]; k++) {
+        const double *xj = Xs + (int64_t)col[k] * A;
+        const double v = val[k];
+        if (gather == ORC_DSB)
+          for (int64_t a = 0; a < A; a++) F[a] = fma(-v, copysign(1.0, xj[a]), F[a]);
+        else
+          for (int64_t a = 0; a < A; a++) F[a] = fma(-v, xj[a] * coupling, F[a]);
       }
-      if (problem == ORC_MAXCUT) {
-        if (conf) {
-          x = signbit(x) ? -1.0 : 1.0;
-          y = 0.0;
-        } else {
-          y = fma(c[2], F, y);
-        }
-        x = fma(c[0], y, x);
-        y = fma(c[1], x, y);
-      } else {
-        if (conf) {
-          x = signbit(x) ? -1.0 : 1.0;
-          y = 0.0;
-        } else {
-          double q = fma(c[3], h[i], 2 * F);
-          y = fma(c[0], c[2] * q, y);
+      for (int64_t a = 0; a < A; a++) {
+        int64_t e = i * A + a;
+        double x = Xs[e], y = Y[e];
+        int conf = fabs(x) > 1;
+        if (problem == ORC_MAXCUT) {
+          if (conf) {
+            x = signbit(x) ? -1.0 : 1.0;
+            y = 0.0;
+          } else {
+            y = fma(c[2], F[a], y);
+          }
           x = fma(c[0], y, x);
           y = fma(c[1], x, y);
+        } else {
+          if (conf) {
+            x = signbit(x) ? -1.0 : 1.0;
+            y = 0.0;
+          } else {
+            double q = fma(c[3], h[i], 2 * F[a]);
+            y = fma(c[0], c[2] * q, y);
+            x = fma(c[0], y, x);
+            y = fma(c[1], x, y);
+          }
         }
+        Xn[e] = x;
+        Y[e] = y;
       }
-      Xn[e] = x;
-      Y[e] = y;
     }
+    free(F);
   }
 }
 
-/* FP32 form of the same step.  bf16_gather != 0 rounds every gathered x to
- * bfloat16 (round to nearest even) before use: the BF16-shadow bSB form. */
-void orc_step_f32(int problem, int gather, int bf16_gather, const int64_t *rp, const int32_t *col,
-                  const float *val, const float *h, int64_t N, int64_t A, const float *Xs, float *Xn,
-                  float *Y, const double *cd, double coupling_d) {
+/* FP32 form of the same step.  precision 1 rounds every gathered x to
+ * bfloat16 (round to nearest even) before use: the BF16-shadow bSB form.
+ * precision 2 (MaxCut dSB) rounds every updated (x, y) to the 12-bit codes
+ * above with Y shift sy: the narrow-state form. */
+static void orc_step_f32_core(int problem, int gather, int precision, int sy, const int64_t *rp,
+                              const int32_t *col, const float *val, const float *h, int64_t N,
+                              int64_t A, const float *Xs, float *Xn, float *Y, const double *cd,
+                              double coupling_d) {
   if (problem == ORC_UNFUSED) {
     orc_unfused_f32(rp, col, val, N, A, Xs, Xn, Y, cd);
     return;
   }
-  float c0 = (float)cd[0], c1 = (float)cd[1], c2 = (float)cd[2], c3 = (float)cd[3];
-  float coupling = (float)coupling_d;
-#pragma omp parallel for schedule(static)
-  for (int64_t i = 0; i < N; i++) {
-    for (int64_t a = 0; a < A; a++) {
-      int64_t e = i * A + a;
-      float x = Xs[e], y = Y[e];
-      int conf = fabsf(x) > 1.0f;
-      float F = 0.0f;
-      if (!conf) {
-        for (int64_t k = rp[i]; k < rp[i + 1]; k++) {
-          float xj = Xs[(int64_t)col[k] * A + a];
-          if (bf16_gather) xj = orc_bf16_round(xj);
-          float g = gather == ORC_DSB ? (signbit(xj) ? -1.0f : 1.0f) : xj * coupling;
-          F = fmaf(-val[k], g, F);
+  const float c0 = (float)cd[0], c1 = (float)cd[1], c2 = (float)cd[2], c3 = (float)cd[3];
+  const float coupling = (float)coupling_d;
+  const int bf16_gather = precision == 1;
+#pragma omp parallel
+  {
+    float *F = (float *)malloc(sizeof(float) * (size_t)(A > 0 ? A : 1));
+#pragma omp for schedule(static)
+    for (int64_t i = 0; i < N; i++) {
+      for (int64_t a = 0; a < A; a++) F[a] = 0.0f;
+      for (int64_t k = rp[i]; k < rp[i + 1]; k++) {
+        const float *xj = Xs + (int64_t)col[k] * A;
+        const float v = val[k];
+        if (gather == ORC_DSB) {
+          if (bf16_gather)
+            for (int64_t a = 0; a < A; a++)
+              F[a] = fmaf(-v, copysignf(1.0f, orc_bf16_round(xj[a])), F[a]);
+          else
+            for (int64_t a = 0; a < A; a++) F[a] = fmaf(-v, copysignf(1.0f, xj[a]), F[a]);
+        } else {
+          if (bf16_gather)
+            for (int64_t a = 0; a < A; a++) F[a] = fmaf(-v, orc_bf16_round(xj[a]) * coupling, F[a]);
+          else
+            for (int64_t a = 0; a < A; a++) F[a] = fmaf(-v, xj[a] * coupling, F[a]);
         }
       }
-      if (problem == ORC_MAXCUT) {
-        if (conf) {
-          x = signbit(x) ? -1.0f : 1.0f;
-          y = 0.0f;
-        } else {
-          y = fmaf(c2, F, y);
-        }
-        x = fmaf(c0, y, x);
-        y = fmaf(c1, x, y);
-      } else {
-        if (conf) {
-          x = signbit(x) ? -1.0f : 1.0f;
-          y = 0.0f;
-        } else {
-          float q = fmaf(c3, h[i], 2.0f * F);
-          y = fmaf(c0, c2 * q, y);
+      for (int64_t a = 0; a < A; a++) {
+        int64_t e = i * A + a;
+        float x = Xs[e], y = Y[e];
+        int conf = fabsf(x) > 1.0f;
+        if (problem == ORC_MAXCUT) {
+          if (conf) {
+            x = signbit(x) ? -1.0f : 1.0f;
+            y = 0.0f;
+          } else {
+            y = fmaf(c2, F[a], y);
+          }
           x = fmaf(c0, y, x);
           y = fmaf(c1, x, y);
+        } else {
+          if (conf) {
+            x = signbit(x) ? -1.0f : 1.0f;
+            y = 0.0f;
+          } else {
+            float q = fmaf(c3, h[i], 2.0f * F[a]);
+            y = fmaf(c0, c2 * q, y);
+            x = fmaf(c0, y, x);
+            y = fmaf(c1, x, y);
+          }
         }
+        if (precision == 2) orc_q12_round(&x, &y, sy);
+        Xn[e] = x;
+        Y[e] = y;
       }
-      Xn[e] = x;
-      Y[e] = y;
     }
+    free(F);
   }
+}
+
+void orc_step_f32(int problem, int gather, int bf16_gather, const int64_t *rp, const int32_t *col,
+                  const float *val, const float *h, int64_t N, int64_t A, const float *Xs, float *Xn,
+                  float *Y, const double *cd, double coupling_d) {
+  orc_step_f32_core(problem, gather, bf16_gather ? 1 : 0, 0, rp, col, val, h, N, A, Xs, Xn, Y, cd,
+                    coupling_d);
+}
+
+/* Narrow-state dSB step (precision 2): Xs / Y hold decoded codes. */
+void orc_step_q12(const int64_t *rp, const int32_t *col, const float *val, int64_t N, int64_t A,
+                  const float *Xs, float *Xn, float *Y, const double *cd, int sy) {
+  orc_step_f32_core(ORC_MAXCUT, ORC_DSB, 2, sy, rp, col, val, NULL, N, A, Xs, Xn, Y, cd, 1.0);
 }
 
 /* ------------------------------------------------------------- windows */
@@ -333,25 +408,28 @@ static int64_t orc_window_maxcut_core(const void *Xv, int is_f32, int64_t N, int
                                       int32_t *cur, int32_t *fin, int32_t *stab, uint8_t *bif) {
   const float *Xf = (const float *)Xv;
   const double *Xd = (const double *)Xv;
+#define ORC_SGN(e) ((is_f32 ? signbit(Xf[e]) : signbit(Xd[e])) ? -1 : 1)
+  uint8_t *eq = (uint8_t *)malloc((size_t)(A > 0 ? A : 1));
+  uint8_t *now_locked = (uint8_t *)malloc((size_t)(A > 0 ? A : 1));
+  for (int64_t a = 0; a < A; a++) eq[a] = 1;
+  for (int64_t i = 0; i < N; i++)
+    for (int64_t a = 0; a < A; a++)
+      if (ORC_SGN(i * A + a) != cur[i * A + a]) eq[a] = 0;
   int64_t nbif = 0;
   for (int64_t a = 0; a < A; a++) {
-    int eq = 1;
-    for (int64_t i = 0; i < N; i++) {
-      int s = (is_f32 ? signbit(Xf[i * A + a]) : signbit(Xd[i * A + a])) ? -1 : 1;
-      if (s != cur[i * A + a]) { eq = 0; break; }
-    }
     uint8_t prev = bif[a];
-    if (!prev) stab[a] = eq ? stab[a] + 1 : 0;
+    if (!prev) stab[a] = eq[a] ? stab[a] + 1 : 0;
     uint8_t now = stab[a] == K - 1;
     bif[a] = now;
-    if (now && !prev) {
-      for (int64_t i = 0; i < N; i++)
-        fin[i * A + a] = (is_f32 ? signbit(Xf[i * A + a]) : signbit(Xd[i * A + a])) ? -1 : 1;
-    }
+    now_locked[a] = now && !prev;
     nbif += now;
   }
-  for (int64_t e = 0; e < N * A; e++)
-    cur[e] = (is_f32 ? signbit(Xf[e]) : signbit(Xd[e])) ? -1 : 1;
+  for (int64_t i = 0; i < N; i++)
+    for (int64_t a = 0; a < A; a++)
+      if (now_locked[a]) fin[i * A + a] = ORC_SGN(i * A + a);
+  for (int64_t e = 0; e < N * A; e++) cur[e] = ORC_SGN(e);
+  free(eq);
+  free(now_locked);
   return nbif;
 }
 
@@ -364,31 +442,61 @@ static void orc_window_tsp_core(const void *Xv, int is_f32, int64_t n, int64_t A
   const float *Xf = (const float *)Xv;
   const double *Xd = (const double *)Xv;
   int64_t N = n * n;
-  int32_t *cnt = (int32_t *)calloc((size_t)(2 * n), sizeof(int32_t));
-  for (int64_t a = 0; a < A; a++) {
-    memset(cnt, 0, sizeof(int32_t) * 2 * n);
-    for (int64_t i = 0; i < N; i++) {
-      int neg = is_f32 ? signbit(Xf[i * A + a]) : signbit(Xd[i * A + a]);
-      if (!neg) {
-        cnt[i % n] += 1;     /* city visits */
-        cnt[n + i / n] += 1; /* position visits */
+  /* visits per (line, agent): lines 0..n-1 are cities, n..2n-1 positions */
+  int32_t *cnt = (int32_t *)calloc((size_t)(2 * n * A), sizeof(int32_t));
+  uint8_t *valid = (uint8_t *)malloc((size_t)(A > 0 ? A : 1));
+  for (int64_t i = 0; i < N; i++)
+    for (int64_t a = 0; a < A; a++)
+      if (ORC_SGN(i * A + a) == 1) {
+        cnt[(i % n) * A + a] += 1;     /* city visits */
+        cnt[(n + i / n) * A + a] += 1; /* position visits */
       }
-    }
-    int valid = 1;
+  for (int64_t a = 0; a < A; a++) {
+    valid[a] = 1;
     for (int64_t k = 0; k < 2 * n; k++)
-      if (cnt[k] != 1) { valid = 0; break; }
-    if (valid) {
-      valid_list[a] = (int32_t)t;
-      for (int64_t i = 0; i < N; i++)
-        fin[i * A + a] = (is_f32 ? signbit(Xf[i * A + a]) : signbit(Xd[i * A + a])) ? -1 : 1;
-    }
+      if (cnt[k * A + a] != 1) { valid[a] = 0; break; }
+    if (valid[a]) valid_list[a] = (int32_t)t;
   }
+  for (int64_t i = 0; i < N; i++)
+    for (int64_t a = 0; a < A; a++)
+      if (valid[a]) fin[i * A + a] = ORC_SGN(i * A + a);
+  for (int64_t e = 0; e < N * A; e++) cur[e] = ORC_SGN(e);
+#undef ORC_SGN
   free(cnt);
-  for (int64_t e = 0; e < N * A; e++)
-    cur[e] = (is_f32 ? signbit(Xf[e]) : signbit(Xd[e])) ? -1 : 1;
+  free(valid);
 }
 
 /* --------------------------------------------------------------- solve */
+
+static int64_t orc_solve_f32_core(int problem, int gather, int precision, int sy,
+                                  const int64_t *rp, const int32_t *col, const float *val,
+                                  const float *h, int64_t N, int64_t A, int64_t T, int uw, int64_t P,
+                                  int K, int64_t ncities, double dt, double xi0, double coupling,
+                                  float *X, float *Y, int32_t *cur, int32_t *fin, int32_t *stab,
+                                  uint8_t *bif, int32_t *valid_list) {
+  float *Xb = (float *)malloc(sizeof(float) * (size_t)(N * A));
+  orc_prestep_f32(N * A, dt, X, Y);
+  if (precision == 2) orc_q12_round_state(N * A, X, Y, sy);
+  int64_t steps = 0;
+  double c[4];
+  while (steps < T) {
+    for (int64_t s = 0; s < P; s++) {
+      orc_sched(problem, steps + s, T, dt, xi0, c);
+      orc_step_f32_core(problem, gather, precision, sy, rp, col, val, h, N, A, X, Xb, Y, c,
+                        coupling);
+      memcpy(X, Xb, sizeof(float) * (size_t)(N * A));
+    }
+    steps += P;
+    if (problem != ORC_TSP) {
+      int64_t nb = orc_window_maxcut_core(X, 1, N, A, K, cur, fin, stab, bif);
+      if (uw && nb == A) break;
+    } else {
+      orc_window_tsp_core(X, 1, ncities, A, steps, cur, fin, valid_list);
+    }
+  }
+  free(Xb);
+  return steps;
+}
 
 /* Whole solve from a given initial state (src/ising.cu:140-226,
  * src/ising_tsp.cu:111-164): pre-step, then repeat {P steps, window} while
@@ -402,26 +510,21 @@ int64_t orc_solve_f32(int problem, int gather, int precision, const int64_t *rp,
                       int64_t P, int K, int64_t ncities, double dt, double xi0, double coupling,
                       float *X, float *Y, int32_t *cur, int32_t *fin, int32_t *stab, uint8_t *bif,
                       int32_t *valid_list) {
-  float *Xb = (float *)malloc(sizeof(float) * (size_t)(N * A));
-  orc_prestep_f32(N * A, dt, X, Y);
-  int64_t steps = 0;
-  double c[4];
-  while (steps < T) {
-    for (int64_t s = 0; s < P; s++) {
-      orc_sched(problem, steps + s, T, dt, xi0, c);
-      orc_step_f32(problem, gather, precision == 1, rp, col, val, h, N, A, X, Xb, Y, c, coupling);
-      memcpy(X, Xb, sizeof(float) * (size_t)(N * A));
-    }
-    steps += P;
-    if (problem != ORC_TSP) {
-      int64_t nb = orc_window_maxcut_core(X, 1, N, A, K, cur, fin, stab, bif);
-      if (uw && nb == A) break;
-    } else {
-      orc_window_tsp_core(X, 1, ncities, A, steps, cur, fin, valid_list);
-    }
-  }
-  free(Xb);
-  return steps;
+  return orc_solve_f32_core(problem, gather, precision == 1 ? 1 : 0, 0, rp, col, val, h, N, A, T,
+                            uw, P, K, ncities, dt, xi0, coupling, X, Y, cur, fin, stab, bif,
+                            valid_list);
+}
+
+/* The same for the narrow-state dSB MaxCut form: X, Y arrive as FP32 values
+ * and are rounded to the 12-bit codes after the pre-step (the device rounds
+ * at set_state too, which leaves values already on the code grid unchanged)
+ * and after every step. */
+int64_t orc_solve_q12(const int64_t *rp, const int32_t *col, const float *val, int64_t N,
+                      int64_t A, int64_t T, int uw, int64_t P, int K, double dt, double xi0, int sy,
+                      float *X, float *Y, int32_t *cur, int32_t *fin, int32_t *stab, uint8_t *bif) {
+  orc_q12_round_state(N * A, X, Y, sy);
+  return orc_solve_f32_core(ORC_MAXCUT, ORC_DSB, 2, sy, rp, col, val, NULL, N, A, T, uw, P, K, 0,
+                            dt, xi0, 1.0, X, Y, cur, fin, stab, bif, NULL);
 }
 
 int64_t orc_solve_f64(int problem, int gather, const int64_t *rp, const int32_t *col,
@@ -467,6 +570,26 @@ void orc_advance_f32(int problem, int gather, int precision, const int64_t *rp, 
   free(Xb);
 }
 
+/* Narrow-state dSB MaxCut steps t0 .. t0+nsteps-1 on decoded codes. */
+void orc_advance_q12(const int64_t *rp, const int32_t *col, const float *val, int64_t N, int64_t A,
+                     int64_t t0, int64_t nsteps, int64_t T, double dt, double xi0, int sy, float *X,
+                     float *Y) {
+  float *Xb = (float *)malloc(sizeof(float) * (size_t)(N * A));
+  double c[4];
+  for (int64_t s = 0; s < nsteps; s++) {
+    orc_sched(ORC_MAXCUT, t0 + s, T, dt, xi0, c);
+    orc_step_q12(rp, col, val, N, A, X, Xb, Y, c, sy);
+    memcpy(X, Xb, sizeof(float) * (size_t)(N * A));
+  }
+  free(Xb);
+}
+
+/* Pre-step (src/kernels.cu:15-20 at t = 0) on decoded codes, then rounding. */
+void orc_prestep_q12(int64_t n, double dt, int sy, float *X, float *Y) {
+  orc_prestep_f32(n, dt, X, Y);
+  orc_q12_round_state(n, X, Y, sy);
+}
+
 void orc_advance_f64(int problem, int gather, const int64_t *rp, const int32_t *col,
                      const double *val, const double *h, int64_t N, int64_t A, int64_t t0,
                      int64_t nsteps, int64_t T, double dt, double xi0, double coupling, double *X,
@@ -498,31 +621,37 @@ void orc_window_tsp_f32(const float *X, int64_t n, int64_t A, int64_t t, int32_t
  * order as src/kernels.cu:394-397 does) and cut_a = -0.5 (H_a - S). */
 void orc_cut(const int64_t *rp, const int32_t *col, const double *val, int64_t N, int64_t A,
              const int32_t *spins, double S, double *cut) {
-  for (int64_t a = 0; a < A; a++) {
-    double H = 0.0;
-    for (int64_t i = 0; i < N; i++) {
-      double tmp = 0.0;
-      for (int64_t k = rp[i]; k < rp[i + 1]; k++)
-        tmp = fma(val[k], (double)spins[(int64_t)col[k] * A + a], tmp);
-      H += tmp * spins[i * A + a] * 0.5;
+  double *H = (double *)calloc((size_t)(A > 0 ? A : 1), sizeof(double));
+  double *tmp = (double *)malloc(sizeof(double) * (size_t)(A > 0 ? A : 1));
+  for (int64_t i = 0; i < N; i++) {
+    for (int64_t a = 0; a < A; a++) tmp[a] = 0.0;
+    for (int64_t k = rp[i]; k < rp[i + 1]; k++) {
+      const int32_t *sj = spins + (int64_t)col[k] * A;
+      for (int64_t a = 0; a < A; a++) tmp[a] = fma(val[k], (double)sj[a], tmp[a]);
     }
-    cut[a] = (-0.5) * (H - S);
+    for (int64_t a = 0; a < A; a++) H[a] += tmp[a] * spins[i * A + a] * 0.5;
   }
+  for (int64_t a = 0; a < A; a++) cut[a] = (-0.5) * (H[a] - S);
+  free(H);
+  free(tmp);
 }
 
 /* Ising energy E_a = s^T J s - h.s for the TSP encoding (FP64). */
 void orc_energy_field(const int64_t *rp, const int32_t *col, const double *val, const double *h,
                       int64_t N, int64_t A, const int32_t *spins, double *E) {
-  for (int64_t a = 0; a < A; a++) {
-    double acc = 0.0;
-    for (int64_t i = 0; i < N; i++) {
-      double tmp = 0.0;
-      for (int64_t k = rp[i]; k < rp[i + 1]; k++)
-        tmp += val[k] * (double)spins[(int64_t)col[k] * A + a];
-      acc += tmp * spins[i * A + a] - h[i] * spins[i * A + a];
+  double *acc = (double *)calloc((size_t)(A > 0 ? A : 1), sizeof(double));
+  double *tmp = (double *)malloc(sizeof(double) * (size_t)(A > 0 ? A : 1));
+  for (int64_t i = 0; i < N; i++) {
+    for (int64_t a = 0; a < A; a++) tmp[a] = 0.0;
+    for (int64_t k = rp[i]; k < rp[i + 1]; k++) {
+      const int32_t *sj = spins + (int64_t)col[k] * A;
+      for (int64_t a = 0; a < A; a++) tmp[a] += val[k] * (double)sj[a];
     }
-    E[a] = acc;
+    for (int64_t a = 0; a < A; a++) acc[a] += tmp[a] * spins[i * A + a] - h[i] * spins[i * A + a];
   }
+  for (int64_t a = 0; a < A; a++) E[a] = acc[a];
+  free(acc);
+  free(tmp);
 }
 
 /* Host statistics over per-agent cuts (src/ising.cu:238-265). out = {min,
