@@ -4,18 +4,22 @@ Workload (default): MaxCut dSB on a random 4-regular graph, N = 2^20 nodes,
 Z = 4 N = 4,194,304 symmetric nonzeros, weights +-1 uniform from seed, 256
 agents, dt = 0.5, max_steps = K (20,000 by default), windows every 100 steps.
 A "step" is one fused SB step over all N x 256 node-agents.  The headline
-form is the bit-packed-sign dSB kernel; the plain FP32 dSB kernel runs the
-same trajectory (bit-identical) beside it for the speed-up and the
-equal-steps best-cut comparison.
+form (--form) runs beside the plain FP32 dSB kernel for the speed-up and the
+equal-steps best-cut comparison; --also adds more forms, and the line also
+carries bSB with BF16 gathers against plain FP32 bSB, a batched-serving
+summary (4096 small instances) and the CPU baseline.
 
-The timed region is one sis_solver_run (pre-step + K steps + K/100 windows),
-with the state resident in HBM; the update window's early exit is off inside
-the timed region so that exactly K steps run (on this graph no agent locks
-within 20,000 steps, so the exit would not fire either).  The step-kernel time comes from HIP events the
-library records around the step launches on the solver's stream.
+The timed region is one sis_solver_run (pre-step + steps + windows), with the
+state resident in HBM; the update window's early exit is off inside the timed
+region.  A run executes whole window periods (100 steps), so K is rounded up
+to a multiple of 100: the line's "steps" is the executed count and
+ms_per_step divides the timed region by it.  The step-kernel time comes from
+HIP events the library records around the step launches on the solver's
+stream.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--form packed|f32|bf16|f64]
-                    [--baseline-form f32] [--cpu-seconds S] [--no-cpu]
+                    [--baseline-form f32] [--also f1,f2] [--cpu-seconds S] [--no-cpu]
+                    [--no-bf16] [--no-serving] [--workload ...]
 """
 import argparse
 import json
@@ -114,10 +118,55 @@ def energy_check(J, s, cuts, best, nagents=16):
                 host="scipy FP64 sparse product of the device's current spins")
 
 
+def host_cpu():
+    """lscpu model, sockets and physical cores of the host."""
+    import subprocess
+
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, check=True).stdout
+        kv = dict(line.split(":", 1) for line in out.splitlines() if ":" in line)
+        kv = {k.strip(): v.strip() for k, v in kv.items()}
+        sockets = int(kv.get("Socket(s)", "1"))
+        per = int(kv.get("Core(s) per socket", "0"))
+        info = dict(model=kv.get("Model name", "?"), sockets=sockets,
+                    physical_cores=sockets * per,
+                    threads_per_core=int(kv.get("Thread(s) per core", "1")))
+    except (OSError, ValueError, subprocess.CalledProcessError):
+        pass
+    return info
+
+
+def native_oracle():
+    """The CPU step built for this host: oracle/sis_oracle.c (rows in parallel,
+    agents innermost so they vectorise) with -O3 -march=native -fopenmp,
+    compiled into a temporary directory.  Falls back to the prebuilt
+    liboracle.so (-O3 -mavx2 -mfma) when no compiler is available."""
+    import subprocess
+    import tempfile
+
+    src = os.path.join(ROOT, "oracle", "sis_oracle.c")
+    out = os.path.join(tempfile.mkdtemp(prefix="sis_cpu_"), "liboracle_native.so")
+    cmd = ["gcc", "-O3", "-march=native", "-fPIC", "-std=c11", "-ffp-contract=off", "-fopenmp",
+           "-shared", "-o", out, src, "-lm"]
+    try:
+        subprocess.run(cmd, check=True, capture_output=True)
+        return out, " ".join(cmd[:8])
+    except (OSError, subprocess.CalledProcessError):
+        return os.path.join(ROOT, "oracle", "liboracle.so"), "prebuilt -O3 -mavx2 -mfma"
+
+
 def cpu_baseline(J, A, dt, xi0, seconds, seed):
+    """FP32 dSB step (the plain form's arithmetic) on the host cores: the
+    oracle's step compiled natively, timed over a bounded sample of steps of
+    the same graph and agents (steady_clock-equivalent perf_counter around the
+    step loop only)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
 
+    path, build = native_oracle()
+    orc.LIB_PATH = path
+    orc._lib = None
     rp, col, val = J.arrays()
     pb = orc.Problem(rp, col, val, S=J.sum)
     X, Y = orc.init_state(seed, orc.MAXCUT, J.n, A)
@@ -134,17 +183,17 @@ def cpu_baseline(J, A, dt, xi0, seconds, seed):
         return n, time.perf_counter() - c
 
     n, el = timed(0, seconds)
-    # one core: a single step (the data is warm from the run above)
     orc.set_threads(1)
     c = time.perf_counter()
-    orc.advance(pb, orc.MAXCUT, orc.DSB, X, Y, n + 1, 1, T, dt, xi0, 1.0, "f32")
-    el1 = time.perf_counter() - c
+    orc.advance(pb, orc.MAXCUT, orc.DSB, X, Y, n + 1, 2, T, dt, xi0, 1.0, "f32")
+    el1 = (time.perf_counter() - c) / 2
     orc.set_threads(threads)
     return dict(value=n * A / el, unit="agent-steps/s", cores=threads, kind="port",
-                one_core=A / el1,
-                sample=f"oracle FP32 dSB step (OpenMP, {threads} threads) on the same graph and "
-                       f"agents, {n} steps after 1 warm-up step, {el:.1f} s; one_core: "
-                       f"1 step on 1 thread, {el1:.1f} s")
+                one_core=A / el1, host=host_cpu(), build=build,
+                sample=f"FP32 dSB step of the same graph and agents (rows in parallel, agents "
+                       f"innermost), {threads} OpenMP threads (the process's CPU share), "
+                       f"{n} steps after 1 warm-up step in {el:.1f} s; one_core: 2 steps on "
+                       f"1 thread, {2 * el1:.1f} s")
 
 
 def run_solver(J, params, T, uw, seed, field=None, autotune=True, tsp=None):
@@ -175,9 +224,20 @@ def run_solver(J, params, T, uw, seed, field=None, autotune=True, tsp=None):
         ok = lens[lens >= 0]
         out.update(valid=int(len(ok)), best=float(ok.min()) if len(ok) else None,
                    mean=float(ok.mean()) if len(ok) else None)
-        E = s.energy(sis.SPINS_FINAL)
-        out["energy_min"] = float(E.min())
+        out.update(tsp_energies(s, lens))
     s.close()
+    return out
+
+
+def tsp_energies(s, lens):
+    """Energies s^T J s - h.s of the spins a TSP run returns: the minimum over
+    every agent's current spins (sgn X at the last window), and over the final
+    spins of agents that reached a valid tour (agents without one keep the
+    reset value there, so they are left out)."""
+    Ec = s.energy(sis.SPINS_CURRENT)
+    out = dict(energy_min_current=float(Ec.min()))
+    if (lens >= 0).any():
+        out["energy_min_final_valid"] = float(s.energy(sis.SPINS_FINAL)[lens >= 0].min())
     return out
 
 
@@ -251,7 +311,7 @@ def small_configs(a):
                                   ms_per_step=wall * 1e3 / info["steps_executed"],
                                   agent_steps_per_s=info["steps_executed"] * A / wall,
                                   valid=int(len(ok)), best=float(ok.min()) if len(ok) else None,
-                                  energy_min=float(s.energy(sis.SPINS_FINAL).min())))
+                                  **tsp_energies(s, lens)))
                 s.close()
     elif a.workload == "powerlaw":
         # Chung-Lu N = 2^20, mean degree 8, exponent 2.1 (largest row ~1.2e5
@@ -278,20 +338,24 @@ def small_configs(a):
         print(json.dumps(ln), flush=True)
 
 
-def serving(a):
+def serving_run(seed, count, T, chunk, eager, rate, seq_sample):
     """Batched serving of many small MaxCut dSB instances: N log-uniform in
     [64, 4096], Erdos-Renyi with mean degree 8, weights uniform in
-    {-3..3}\\{0}, 64 agents, dt 0.5, 5000 steps, update window on (instances
-    retire when every agent has locked)."""
-    rng = np.random.default_rng(a.seed)
-    count, A, dt, T = a.instances, 64, 0.5, a.steps
+    {-3..3}\\{0}, 64 agents, dt 0.5, T steps, update window on (instances
+    retire when every agent has locked).  Closed loop (everything submitted at
+    once), then Poisson arrivals, then a sample solved one at a time with the
+    plain FP32 solver.  The plain FP32 dSB trajectory is bit-identical to the
+    packed one the batch runs, so each sampled instance's cuts and spins must
+    equal its result in the closed-loop mix."""
+    rng = np.random.default_rng(seed)
+    A, dt = 64, 0.5
     sizes = np.floor(np.exp(rng.uniform(np.log(64), np.log(4096), count))).astype(int)
     Js = [sis.Csr.gen_er(int(n), 8.0 / (n - 1), sis.W_INT3, 7000 + k) for k, n in enumerate(sizes)]
     rows, nnz = int(sizes.sum()), int(sum(J.nnz for J in Js))
 
     def engine(tile=0):
         return sis.Batch(agents_max=A, max_instances=count, max_rows=rows, max_nnz=nnz,
-                         chunk_steps=a.chunk, tile=tile, eager=a.eager)
+                         chunk_steps=chunk, tile=tile, eager=eager)
 
     # Tile choice on the full resident set, outside the timed regions.
     b = engine()
@@ -306,19 +370,21 @@ def serving(a):
     t0 = time.perf_counter()
     tickets = [b.submit(J, A, dt, T, 1, seed=k) for k, J in enumerate(Js)]
     while True:
-        running, queued = b.advance(a.chunk)
+        running, queued = b.advance(chunk)
         if running == 0 and queued == 0:
             break
     wall = time.perf_counter() - t0
     st = [b.poll(t) for t in tickets]
-    steps_done = sum(s["steps_executed"] for s in st)
-    closed = dict(solves_per_s=count / wall, wall_s=wall,
-                  mean_steps=steps_done / count,
-                  node_agent_steps_per_s=sum(s["steps_executed"] * s["n"] * A for s in st) / wall)
+    steps_done = sum(x["steps_executed"] for x in st)
+    closed = dict(solves_per_s=count / wall, wall_s=wall, mean_steps=steps_done / count,
+                  node_agent_steps_per_s=sum(x["steps_executed"] * x["n"] * A for x in st) / wall)
+    sample = list(range(0, count, max(1, count // seq_sample)))[:seq_sample]
+    mixed = {k: b.result(tickets[k], spins=True) for k in sample}
+    mixed_steps = {k: st[k]["steps_executed"] for k in sample}
     b.close()
 
     # Poisson arrivals at rate lam (default: 70% of the closed-loop rate).
-    lam = a.rate if a.rate > 0 else 0.7 * closed["solves_per_s"]
+    lam = rate if rate > 0 else 0.7 * closed["solves_per_s"]
     arrivals = np.cumsum(rng.exponential(1.0 / lam, count))
     b = engine(tile)
     start = time.perf_counter()
@@ -329,7 +395,7 @@ def serving(a):
             t = b.submit(Js[nxt], A, dt, T, 1, seed=nxt)
             delay[t] = (time.perf_counter() - start) - arrivals[nxt]  # arrival -> submit
             nxt += 1
-        running, queued = b.advance(a.chunk)
+        running, queued = b.advance(chunk)
         if nxt == count and running == 0 and queued == 0:
             break
         if running == 0 and queued == 0:
@@ -337,42 +403,55 @@ def serving(a):
     done = {t: b.poll(t) for t in delay}
     # latency = scheduled arrival -> observed retirement (library clock for
     # submit -> done, plus the loop's arrival -> submit delay)
-    lats = np.array([s["done_s"] - s["submit_s"] + delay[t] for t, s in done.items()])
-    span = max(s["done_s"] for s in done.values()) - min(s["submit_s"] for s in done.values())
+    lats = np.array([x["done_s"] - x["submit_s"] + delay[t] for t, x in done.items()])
+    span = max(x["done_s"] for x in done.values()) - min(x["submit_s"] for x in done.values())
     poisson = dict(rate_per_s=lam, solves_per_s=count / span, p50_s=float(np.percentile(lats, 50)),
                    p99_s=float(np.percentile(lats, 99)), mean_s=float(lats.mean()))
     b.close()
 
-    # Sequential plain FP32 solves of a sample (graph-replayed window periods).
-    sample = list(range(0, count, max(1, count // a.seq_sample)))[: a.seq_sample]
-    t0 = time.perf_counter()
+    # Sequential plain FP32 solves of the sample (graph-replayed window
+    # periods); the time covers create + init + run + cuts of each solve.
+    seq_time, same = 0.0, 0
     for k in sample:
+        t0 = time.perf_counter()
         s = sis.Solver(Js[k], dict(problem=sis.MAXCUT, agents=A, dt=dt, form=sis.FORM_F32,
-                                   use_graph=0 if a.eager else 1))
+                                   use_graph=0 if eager else 1))
         s.init_random(k)
-        s.run(T, 1)
-        s.cuts()
+        info = s.run(T, 1)
+        cuts, _ = s.cuts()
+        seq_time += time.perf_counter() - t0
+        mc, ms = mixed[k]
+        same += int(np.array_equal(cuts, mc) and np.array_equal(s.spins(), ms)
+                    and info["steps_executed"] == mixed_steps[k])
         s.close()
-    seq = len(sample) / (time.perf_counter() - t0)
+    seq = len(sample) / seq_time
+    return dict(count=count, rows=rows, nnz=nnz, tile=tile, closed=closed, poisson=poisson,
+                seq=seq, sample=len(sample), same=same)
+
+
+def serving(a):
+    r = serving_run(a.seed, a.instances, a.steps, a.chunk, a.eager, a.rate, a.seq_sample)
     out = {
         "metric": "solves/s (batched serving) vs sequential plain FP32 solves",
-        "value": closed["solves_per_s"],
+        "value": r["closed"]["solves_per_s"],
         "unit": "solves/s",
         "n_gpus": 1,
-        "steps": T,
+        "steps": a.steps,
         "warmup": 0,
         "higher_is_better": True,
         "dtype": "f32",
         "data": "synthetic (seeded Erdos-Renyi instances, seeded initial state)",
-        "config": {"workload": f"{count} MaxCut instances, N log-uniform [64,4096] (sum {rows}), "
-                               f"ER mean degree 8, weights {{-3..3}}\\{{0}}, {A} agents, dt {dt}, "
-                               f"{T} steps, update window on, chunk {a.chunk} steps",
-                   "autotuned_tile": tile, "nnz": nnz},
-        "closed_loop": closed,
-        "poisson": poisson,
-        "sequential_plain_solves_per_s": seq,
-        "sequential_sample": len(sample),
-        "speedup_vs_sequential_plain": closed["solves_per_s"] / seq,
+        "config": {"workload": f"{r['count']} MaxCut instances, N log-uniform [64,4096] (sum "
+                               f"{r['rows']}), ER mean degree 8, weights {{-3..3}}\\{{0}}, 64 "
+                               f"agents, dt 0.5, {a.steps} steps, update window on, chunk "
+                               f"{a.chunk} steps",
+                   "autotuned_tile": r["tile"], "nnz": r["nnz"]},
+        "closed_loop": r["closed"],
+        "poisson": r["poisson"],
+        "sequential_plain_solves_per_s": r["seq"],
+        "sequential_sample": r["sample"],
+        "speedup_vs_sequential_plain": r["closed"]["solves_per_s"] / r["seq"],
+        "alone_vs_mix_identical": f"{r['same']}/{r['sample']}",
     }
     print(json.dumps(out), flush=True)
 
@@ -388,6 +467,10 @@ def main():
     ap.add_argument("--agents", type=int, default=256)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-bf16", action="store_true", help="skip the BF16 bSB legs")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving sub-object")
+    ap.add_argument("--also", default="", help="comma-separated extra forms timed beside the "
+                    "headline (same dynamics)")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--workload", default="regular2p20",
                     choices=["regular2p20", "er800", "torus100", "tsp48", "powerlaw", "tsp_struct",
@@ -397,10 +480,11 @@ def main():
     ap.add_argument("--instances", type=int, default=4096)
     ap.add_argument("--chunk", type=int, default=100)
     ap.add_argument("--rate", type=float, default=0.0)
-    ap.add_argument("--seq-sample", type=int, default=16)
+    ap.add_argument("--seq-sample", type=int, default=32)
     ap.add_argument("--eager", action="store_true",
                     help="serving: launch chunk kernels directly instead of replaying a hipGraph")
     a = ap.parse_args()
+    a.also = [f for f in a.also.split(",") if f]
     if a.gpus != 1:
         print("bench.py measures one GPU: the solver runs on a single device (agents shard "
               "across GPUs with no per-step communication); use --gpus 1.",
@@ -421,26 +505,43 @@ def main():
     Z = J.nnz
     xi0 = J.xi0()
 
+    def leg(form, dyn):
+        r = time_form(J, form, A, dt, K, a.warmup, a.seed, dyn)
+        info = r["info"]
+        ex = max(info["steps_executed"], 1)
+        step_ms = info["step_kernel_ms"] / ex
+        nbytes = step_bytes(form, N, Z, A)
+        r.update(form=form, executed=info["steps_executed"], value=info["steps_executed"] * A / r["wall"],
+                 ms_per_step=r["wall"] * 1e3 / ex, avg_launch_ms=step_ms, bytes=nbytes,
+                 achieved=nbytes / (step_ms * 1e-3) / 1e9)
+        return r
+
+    def summary(r):
+        return {"form": r["form"], "value": r["value"], "ms_per_step": r["ms_per_step"],
+                "avg_launch_ms": r["avg_launch_ms"], "achieved_GBs": r["achieved"],
+                "frac": r["achieved"] / HBM_PEAK_GBS, "best_cut": r["best_cut"],
+                "mean_cut": r["mean_cut"], "autotuned_tile": r["tune"],
+                "energy_check": r["energy_check"]}
+
+    def cut_diff(r, base):
+        return (r["best_cut"] - base["best_cut"]) / abs(base["best_cut"])
+
     # The BF16 form is bSB; its plain baseline runs bSB too.
     dyn = sis.BSB if a.form == "bf16" else sis.DSB
-    head = time_form(J, a.form, A, dt, K, a.warmup, a.seed, dyn)
+    head = leg(a.form, dyn)
     base = None
     if a.baseline_form != "none" and a.baseline_form != a.form:
-        base = time_form(J, a.baseline_form, A, dt, K, a.warmup, a.seed, dyn)
-
-    hi = head["info"]
-    value = hi["steps_executed"] * A / head["wall"]
-    avg_step_ms = hi["step_kernel_ms"] / max(hi["steps_executed"], 1)
-    nbytes = step_bytes(a.form, N, Z, A)
-    achieved = nbytes / (avg_step_ms * 1e-3) / 1e9
+        base = leg(a.baseline_form, dyn)
+    executed = head["executed"]
+    traffic = measured_traffic(a.form, (head["tune"] or {}).get("tile"), N, Z, A)
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": head["value"],
         "unit": "agent-steps/s",
         "n_gpus": 1,
-        "steps": K,
+        "steps": executed,
         "warmup": a.warmup,
-        "ms_per_step": head["wall"] * 1e3 / max(hi["steps_executed"], 1),
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -448,60 +549,76 @@ def main():
         "data": "synthetic (seeded random 4-regular graph, +-1 weights, seeded initial state)",
         "config": {
             "workload": f"maxcut {'bSB' if dyn == sis.BSB else 'dSB'}, random 4-regular "
-                        f"N=2^{a.nodes_log2}, Z={Z}, +-1 weights, "
-                        f"{A} agents, dt {dt}, {K} steps, update window every 100 steps "
-                        f"(locks counted, early exit off: exactly {K} steps)",
+                        f"N=2^{a.nodes_log2}, Z={Z}, +-1 weights, {A} agents, dt {dt}, "
+                        f"{executed} steps executed ({K} requested; a run executes whole "
+                        f"window periods of 100 steps), update window every 100 steps "
+                        f"(locks counted, early exit off)",
             "form": a.form,
             "nodes": N,
             "nnz": Z,
             "agents": A,
             "xi0": xi0,
+            "steps_requested": K,
             "autotuned_tile": head["tune"],
         },
         "roofline": {
             "bound": "hbm",
-            "achieved": achieved,
+            "achieved": head["achieved"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(a.form, (head["tune"] or {}).get("tile"), N, Z, A),
+            "frac": head["achieved"] / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "traffic_source": ("profiles/pmc_traffic_maxcut_4regular_N2p20_A256.json: rocprofv3 "
+                               "FETCH_SIZE + WRITE_SIZE of this kernel tile, calibrated (not "
+                               "measured in this run)") if traffic is not None else None,
             "kernel": f"step ({a.form})",
-            "bytes_per_launch": nbytes,
-            "avg_launch_ms": avg_step_ms,
+            "bytes_per_launch": head["bytes"],
+            "avg_launch_ms": head["avg_launch_ms"],
+            "timing": "HIP events around the step launches on the solver's stream",
         },
-        "device_ms": hi["device_ms"],
-        "step_kernel_ms": hi["step_kernel_ms"],
+        "device_ms": head["info"]["device_ms"],
+        "step_kernel_ms": head["info"]["step_kernel_ms"],
         "best_cut": head["best_cut"],
         "best_agent": head["best_agent"],
         "mean_cut": head["mean_cut"],
         "energy_check": head["energy_check"],
     }
     if base is not None:
-        bi = base["info"]
-        bvalue = bi["steps_executed"] * A / base["wall"]
-        bstep = bi["step_kernel_ms"] / max(bi["steps_executed"], 1)
-        bbytes = step_bytes(a.baseline_form, N, Z, A)
-        out["plain"] = {
-            "form": a.baseline_form,
-            "value": bvalue,
-            "avg_launch_ms": bstep,
-            "achieved_GBs": bbytes / (bstep * 1e-3) / 1e9,
-            "frac": bbytes / (bstep * 1e-3) / 1e9 / HBM_PEAK_GBS,
-            "best_cut": base["best_cut"],
-            "mean_cut": base["mean_cut"],
-            "autotuned_tile": base["tune"],
-            "energy_check": base["energy_check"],
-        }
-        out["speedup_vs_plain"] = value / bvalue
-        out["best_cut_rel_diff_vs_plain"] = (head["best_cut"] - base["best_cut"]) / abs(base["best_cut"])
-        if head["cont_energy"] is not None and base["cont_energy"] is not None:
-            # bSB: how far the form's rounding moves the continuous-state energy
-            # from the plain FP32 trajectory at equal steps (first 16 agents)
-            eh, eb = head["cont_energy"], base["cont_energy"]
-            out["continuous_energy_vs_plain"] = dict(
+        out["plain"] = summary(base)
+        out["speedup_vs_plain"] = head["value"] / base["value"]
+        out["best_cut_rel_diff_vs_plain"] = cut_diff(head, base)
+    for form in a.also:
+        if form in (a.form, a.baseline_form):
+            continue
+        r = leg(form, dyn)
+        d = summary(r)
+        if base is not None:
+            d["speedup_vs_plain"] = r["value"] / base["value"]
+            d["best_cut_rel_diff_vs_plain"] = cut_diff(r, base)
+        out[form] = d
+    if not a.no_bf16 and dyn == sis.DSB:
+        # bSB with BF16 gathers against plain FP32 bSB at equal steps
+        rb, rp = leg("bf16", sis.BSB), leg("f32", sis.BSB)
+        eh, eb = rb["cont_energy"], rp["cont_energy"]
+        out["bf16_bsb"] = dict(
+            summary(rb), plain_bsb=summary(rp), speedup_vs_plain_bsb=rb["value"] / rp["value"],
+            best_cut_rel_diff_vs_plain_bsb=cut_diff(rb, rp),
+            continuous_energy_vs_plain_bsb=dict(
                 agents=len(eb), max_rel_diff=float(np.max(np.abs(eh - eb) / np.abs(eb))),
                 mean_rel_diff=float(np.mean(np.abs(eh - eb) / np.abs(eb))),
-                host="scipy FP64 (1/2) x^T J x of each run's final FP32 state")
+                host="scipy FP64 (1/2) x^T J x of each run's final FP32 state"))
+    if not a.no_serving:
+        r = serving_run(a.seed, a.instances, 5000, a.chunk, a.eager, a.rate, a.seq_sample)
+        out["serving"] = dict(
+            workload=f"{r['count']} MaxCut instances, N log-uniform [64,4096], ER mean degree 8, "
+                     f"weights {{-3..3}}\\{{0}}, 64 agents, dt 0.5, 5000 steps, update window on",
+            closed_loop_solves_per_s=r["closed"]["solves_per_s"],
+            poisson_rate_per_s=r["poisson"]["rate_per_s"],
+            poisson_solves_per_s=r["poisson"]["solves_per_s"],
+            p50_s=r["poisson"]["p50_s"], p99_s=r["poisson"]["p99_s"],
+            sequential_plain_solves_per_s=r["seq"],
+            speedup_vs_sequential_plain=r["closed"]["solves_per_s"] / r["seq"],
+            alone_vs_mix_identical=f"{r['same']}/{r['sample']}", tile=r["tile"])
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(J, A, dt, xi0, a.cpu_seconds, a.seed)
     print(json.dumps(out), flush=True)

@@ -26,8 +26,10 @@ def test_more_than_one_gpu_exits_with_message():
 def test_headline_line_has_the_contract_fields():
     # a small instance of the headline workload: random 4-regular graph,
     # packed dSB against plain FP32 dSB, CPU baseline on a 1 s sample
-    r = run_bench("--nodes-log2", "14", "--agents", "128", "--steps", "300", "--warmup", "10",
-                  "--cpu-seconds", "1")
+    # --steps 150: a run executes whole window periods, so 200 steps run and
+    # the line reports 200 (the timed region's real size)
+    r = run_bench("--nodes-log2", "14", "--agents", "128", "--steps", "150", "--warmup", "10",
+                  "--cpu-seconds", "1", "--instances", "64", "--seq-sample", "4")
     assert r.returncode == 0, r.stderr[-2000:]
     lines = [l for l in r.stdout.splitlines() if l.strip()]
     assert len(lines) == 1
@@ -36,7 +38,8 @@ def test_headline_line_has_the_contract_fields():
               "higher_is_better", "scaling", "vs_baseline", "dtype", "data", "config", "roofline",
               "cpu_baseline"):
         assert k in d, k
-    assert d["n_gpus"] == 1 and d["steps"] == 300 and d["warmup"] == 10
+    assert d["n_gpus"] == 1 and d["steps"] == 200 and d["warmup"] == 10
+    assert d["config"]["steps_requested"] == 150
     assert d["value"] > 0 and d["unit"] == "agent-steps/s" and d["higher_is_better"] is True
     assert "workload" in d["config"]
     rf = d["roofline"]
@@ -49,3 +52,8 @@ def test_headline_line_has_the_contract_fields():
     # packed and plain dSB follow the same trajectory: identical best cut
     assert d["best_cut_rel_diff_vs_plain"] == 0.0
     assert d["energy_check"]["max_abs_err_cut"] == 0.0
+    bf = d["bf16_bsb"]
+    assert bf["form"] == "bf16" and bf["plain_bsb"]["form"] == "f32" and bf["value"] > 0
+    assert abs(bf["best_cut_rel_diff_vs_plain_bsb"]) < 0.01
+    sv = d["serving"]
+    assert sv["alone_vs_mix_identical"] == "4/4" and sv["closed_loop_solves_per_s"] > 0

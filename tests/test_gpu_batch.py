@@ -306,6 +306,7 @@ def test_batch_resident_random_mixes(seed):
     while sum(b.advance(200)):
         pass
     b.advance(0)
+    seen_g = set()
     for k, (J, t, (A, T, uw)) in enumerate(zip(Js, ts, params)):
         s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=A, dt=0.5, xi0=0.3, form=sis.FORM_PACKED))
         s.init_random(seed + k)
@@ -315,3 +316,19 @@ def test_batch_resident_random_mixes(seed):
         assert b.poll(t)["steps_executed"] == info["steps_executed"], (seed, k)
         assert np.array_equal(bs, s.spins()) and np.array_equal(bc, cuts), (seed, k, J.n, A, T, uw)
         s.close()
+        # the first instance of each resident group size G (agents per
+        # workgroup: the largest power of two <= 64 with N G <= 8192) also
+        # against the oracle directly
+        G = 64
+        while G > 1 and J.n * G > 8192:
+            G //= 2
+        if G in seen_g:
+            continue
+        seen_g.add(G)
+        rp, col, val = J.arrays()
+        pb = orc.Problem(rp, col, val, S=J.sum)
+        X, Y = orc.init_state(seed + k, orc.MAXCUT, J.n, A)
+        r = orc.solve(pb, orc.MAXCUT, orc.DSB, X, Y, T, uw, P=100, K=25, dt=0.5, xi0=0.3)
+        assert b.poll(t)["steps_executed"] == r["steps"], (seed, k, G)
+        assert np.array_equal(bs, r["cur"]) and np.array_equal(bc, orc.cut(pb, r["cur"])), (seed, k, G)
+    assert len(seen_g) >= 2
