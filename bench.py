@@ -44,6 +44,8 @@ def step_bytes(form, N, Z, A):
     state = 16 * N * A                 # X, Y read + write, FP32
     if form == "packed":
         return state + N * A // 8 + Z * A // 8 + csr
+    if form == "q12":  # 12-bit X and Y codes: 3 bytes read + 3 written
+        return 6 * N * A + N * A // 8 + Z * A // 8 + csr
     if form == "bf16":
         return state + 2 * N * A + 2 * Z * A + csr
     if form == "f64":

@@ -49,7 +49,24 @@ enum sis_form {
   SIS_FORM_F32 = 0,    /* plain: X, Y FP32, gathers read FP32 X */
   SIS_FORM_F64 = 1,    /* plain FP64, the reference's own precision */
   SIS_FORM_PACKED = 2, /* dSB only: gathers read 1 sign bit per agent */
-  SIS_FORM_BF16 = 3    /* bSB only: gathers read a BF16 shadow of X */
+  SIS_FORM_BF16 = 3,   /* bSB only: gathers read a BF16 shadow of X */
+  SIS_FORM_Q12 = 4     /* fused MaxCut dSB only (no reference counterpart; the
+                          reference keeps X, Y in FP64, src/ising.cu:71-72):
+                          X and Y stored as 12-bit fixed-point codes,
+                            X: round(x 2^10) in [-1024, 1024], +-1025 for any
+                               |x| > 1 (the next step confines it anyway),
+                            Y: round(y 2^s), |code| <= 2047, s the largest
+                               integer in [0, 11] with 2047 2^-s >= 2/dt + dt
+                               (the bound of |y| while |x| <= 1; dt > 0),
+                          rounded to nearest even after every step; arithmetic
+                          in FP32 on the decoded values; gathers read 1 sign
+                          bit per agent, so J sgn(X) is exact.  A code of 0 is
+                          +0 (sign +1).  Agents are padded to a multiple of
+                          256.  Not bit-identical to SIS_FORM_F32: its own
+                          trajectory.  Per node-agent and step on a
+                          degree-4 graph it moves 6 + 1/8 + 4/8 = 6.6 bytes
+                          (state, sign write, sign gathers) against 32 for
+                          F32. */
 };
 
 enum sis_weights { SIS_W_ONE = 0, SIS_W_PM1 = 1, SIS_W_INT3 = 2 };

@@ -167,5 +167,124 @@ __device__ __forceinline__ void st_state(float *p, float v) {
   if (NT) __builtin_nontemporal_store(v, p);
   else *p = v;
 }
+template <int NT>
+__device__ __forceinline__ uint32_t ld_state(const uint32_t *p) {
+  if (NT) return __builtin_nontemporal_load(p);
+  return *p;
+}
+template <int NT>
+__device__ __forceinline__ void st_state(uint32_t *p, uint32_t v) {
+  if (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
+// ------------------------------------------------- 12-bit fixed-point state
+//
+// Narrow-state dSB form: element (x, y) is the 24-bit unit
+// u = (xq + 2048) | (yq + 2048) << 12 of biased 12-bit codes
+//   xq = round(x 2^10) in [-1024, 1024], or +-1025 for any |x| > 1,
+//   yq = round(y 2^sy) clamped to [-2047, 2047] (0 beside a +-1025 code),
+// rounded to nearest even; the element is negative iff bit 11 is clear.  A
+// lane keeps four elements (agents l, l + 64, l + 128, l + 192 of a 256-agent
+// quad) in three dwords, bytes 0..2 of units 0..3 in order:
+//   d0 = u0 | u1 << 24,  d1 = u1 >> 8 | u2 << 16,  d2 = u2 >> 16 | u3 << 8.
+//
+// Decoding and rounding go through M = 1.5 2^23 (bits 0x4b400000, low 12 bits
+// zero): M + k for an integer |k| < 2^22 is exact with bits(M) + k, and for
+// |v| < 2^22, fma(v, s, M + 2048) with v s exact rounds v s + M + 2048 to an
+// integer (ulp 1, ties to even; the constant is even), whose low 12 bits are
+// the biased code round(v s) + 2048.
+
+// Per-launch constants: step coefficients; the Y code scale 2^sy, the
+// largest |y| with a code (2047 2^-sy); the decoding offsets.  All are small
+// integers times powers of two, so every product below is exact.
+struct Q12K {
+  float c0, c1, c2, yscale, ylim, yinv, xoff, yoff;
+};
+
+__device__ __forceinline__ Q12K q12_consts(float c0, float c1, float c2, int qshift) {
+  Q12K k;
+  k.c0 = c0;
+  k.c1 = c1;
+  k.c2 = c2;
+  k.yscale = __builtin_ldexpf(1.0f, qshift);
+  k.yinv = __builtin_ldexpf(1.0f, -qshift);
+  k.ylim = 2047.0f * k.yinv;
+  k.xoff = -(12582912.0f + 2048.0f) * (1.0f / 1024.0f);  // -(M + 2048) 2^-10
+  k.yoff = -(12582912.0f + 2048.0f) * k.yinv;            // -(M + 2048) 2^-sy
+  return k;
+}
+
+// The four units of a lane's three dwords; bits 24..31 are not cleared.
+__device__ __forceinline__ void q12_units(uint32_t d0, uint32_t d1, uint32_t d2, uint32_t u[4]) {
+  u[0] = d0;                                     // bits 0..23
+  u[1] = __builtin_amdgcn_alignbit(d1, d0, 24);  // bits 24..47
+  u[2] = __builtin_amdgcn_alignbit(d2, d1, 16);  // bits 48..71
+  u[3] = d2 >> 8;                                // bits 72..95
+}
+
+// x = (field - 2048) 2^-10 as fma(M + field, 2^-10, -(M + 2048) 2^-10): both
+// terms and the result are exact.  Likewise y with 2^-sy.
+__device__ __forceinline__ void q12_decode(uint32_t u, const Q12K &k, float &x, float &y) {
+  const float fx = __uint_as_float((u & 0xfffu) | 0x4b400000u);
+  const float fy = __uint_as_float(((u >> 12) & 0xfffu) | 0x4b400000u);
+  x = __builtin_fmaf(fx, 1.0f / 1024.0f, k.xoff);
+  y = __builtin_fmaf(fy, k.yinv, k.yoff);
+}
+
+__device__ __forceinline__ void q12_unpack(uint32_t d0, uint32_t d1, uint32_t d2, float x[4],
+                                           float y[4], const Q12K &k) {
+  uint32_t u[4];
+  q12_units(d0, d1, d2, u);
+#pragma unroll
+  for (int j = 0; j < 4; j++) q12_decode(u[j], k, x[j], y[j]);
+}
+
+// Unit of an updated (x, y), bits 24..31 unspecified, and the element's
+// sign.  x 2^10 stays within [-1024, 1024] while |x| <= 1; y is clamped to
+// +-2047 codes first (clamping y or y 2^sy is the same, the scale being a
+// power of two).  Elements beyond the wall take the +-1025 X code and a 0 Y
+// code.  The sign: f = fma(x, 2^10, M + 2048) is monotonic in x and equals
+// M + 2048 exactly where the X code is 0, so the code (or, beyond the wall,
+// x) is negative iff f < M + 2048, i.e. iff bits(f) < bits(M + 2048) as
+// signed integers (a negative f has negative bits).  Branch-free.
+__device__ __forceinline__ uint32_t q12_encode(float x, float y, const Q12K &k, bool &neg) {
+  const float kMb = 12582912.0f + 2048.0f;
+  const uint32_t xb = __float_as_uint(__builtin_fmaf(x, 1024.0f, kMb));
+  const float yc = __builtin_amdgcn_fmed3f(y, -k.ylim, k.ylim);
+  const uint32_t yb = __float_as_uint(__builtin_fmaf(yc, k.yscale, kMb));
+  const uint32_t in = (xb & 0xfffu) | (yb << 12);  // bits 0..23 valid
+  neg = (int32_t)xb < (int32_t)0x4b400800u;
+  // beyond the wall: X code 2048 -+ 1025, Y code 2048
+  const uint32_t wall = neg ? (0x800000u | 1023u) : (0x800000u | 3073u);
+  return __builtin_fabsf(x) > 1.0f ? wall : in;
+}
+
+__device__ __forceinline__ uint32_t q12_encode(float x, float y, const Q12K &k) {
+  bool neg;
+  return q12_encode(x, y, k, neg);
+}
+
+__device__ __forceinline__ bool q12_negative(uint32_t unit) { return (unit & 0x800u) == 0; }
+
+// Three dwords from four units (bytes 0..2 of each; upper bytes ignored).
+__device__ __forceinline__ void q12_pack(const uint32_t u[4], uint32_t &d0, uint32_t &d1,
+                                         uint32_t &d2) {
+  d0 = __builtin_amdgcn_perm(u[1], u[0], 0x04020100u);  // u0.b0 u0.b1 u0.b2 u1.b0
+  d1 = __builtin_amdgcn_perm(u[2], u[1], 0x05040201u);  // u1.b1 u1.b2 u2.b0 u2.b1
+  d2 = __builtin_amdgcn_perm(u[3], u[2], 0x06050402u);  // u2.b2 u3.b0 u3.b1 u3.b2
+}
+
+// The fused MaxCut dSB update of sb_update<float, PROB_MAXCUT> without
+// branches: the same operations, selected.
+__device__ __forceinline__ void sb_update_maxcut_sel(float &x, float &y, float F, float c0,
+                                                     float c1, float c2) {
+  const bool conf = __builtin_fabsf(x) > 1.0f;
+  const float yk = __builtin_fmaf(c2, F, y);
+  x = conf ? __builtin_copysignf(1.0f, x) : x;
+  y = conf ? 0.0f : yk;
+  x = __builtin_fmaf(c0, y, x);
+  y = __builtin_fmaf(c1, x, y);
+}
 
 }  // namespace sis

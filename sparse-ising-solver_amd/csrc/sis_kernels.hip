@@ -313,6 +313,243 @@ __global__ __launch_bounds__(kBlock) void k_step_packed(StepArgs a) {
   }
 }
 
+// Narrow-state dSB form (MaxCut, fused): X and Y as 12-bit codes (see
+// sis_device.hpp), three dword planes of N x Q x 64 lanes (Q = lda / 256
+// quads); lane l of quad q owns agents 256 q + 64 j + l, j = 0..3, i.e. bit l
+// of sign word 4 q + j.  The gather is the packed form's (exact sum of +-J
+// from the snapshot's sign words, in CSR order); the update is the FP32 dSB
+// update on the decoded values, rounded back to codes.  A wavefront owns R
+// rows of one quad; their state loads are issued before the gathers.  Bytes
+// per node-agent: 3 read + 3 written + 1/8 sign write + degree / 8 gathers.
+//
+// Staged gathers (VG = 1): the R rows' entries (at most kQ12Stage) are
+// fetched at once -- lane e loads column, weight and the four sign words of
+// entry e with vector loads -- into the wavefront's LDS slice; each row then
+// adds its entries in CSR order from LDS broadcasts.  One dependent chain of
+// memory round trips per R rows instead of one per row: the scalar path is
+// bound by that latency, not by HBM.  Waves whose rows hold more entries take
+// the scalar path.
+constexpr int kQ12Stage = 128;
+
+// New codes and sign words of one row: decode, fused dSB update, encode,
+// one ballot per sign word.  rb: the row's plane-0 dwords of this quad
+// (wave-uniform), ps: plane stride, sw: the row's four sign words.
+template <int NT>
+__device__ __forceinline__ void q12_finish_row(const uint32_t d[3], const float F[4], const Q12K &k,
+                                               uint32_t *rb, int64_t ps, uint64_t *sw, int lane) {
+  uint32_t u[4], un[4];
+  q12_units(d[0], d[1], d[2], u);
+  uint64_t b[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    float x, y;
+    q12_decode(u[j], k, x, y);
+    sb_update_maxcut_sel(x, y, F[j], k.c0, k.c1, k.c2);
+    bool neg;
+    un[j] = q12_encode(x, y, k, neg);
+    b[j] = __ballot(neg);
+  }
+  uint32_t e0, e1, e2;
+  q12_pack(un, e0, e1, e2);
+  st_state<NT>(rb + lane, e0);
+  st_state<NT>(rb + ps + lane, e1);
+  st_state<NT>(rb + 2 * ps + lane, e2);
+  // lane j writes sign word j
+  if (lane < 4) sw[lane] = lane == 0 ? b[0] : lane == 1 ? b[1] : lane == 2 ? b[2] : b[3];
+}
+
+template <int R, int NT, int VG>
+__device__ __forceinline__ void q12_step_body(const StepArgs &a) {
+  const Ctrl *ctrl = a.ctrl;
+  if (ctrl->done) return;
+  const int64_t W = a.W, Q = W / 4;
+  const int64_t g = wave_index();
+  const int64_t row0 = (g / Q) * R;
+  if (row0 >= a.N) return;
+  const int lane = threadIdx.x & 63;
+  const int64_t q = g - (g / Q) * Q;
+  const int64_t w0 = 4 * q;
+  const int64_t t = ctrl->base + a.offset;
+  const float *sc = (const float *)a.sched + 4 * t;
+  const Q12K kc = q12_consts(sc[0], sc[1], sc[2], a.qshift);
+  const uint64_t *__restrict__ S = (const uint64_t *)a.gsrc;
+  const float *__restrict__ val = (const float *)a.val;
+  const int32_t *__restrict__ col = a.col;
+  const int64_t *__restrict__ rp = a.rp;
+  uint32_t *P = (uint32_t *)a.X;
+  const int64_t ps = a.N * Q * 64;  // plane stride (dwords)
+  uint64_t *Sn = (uint64_t *)a.gdst;
+
+  uint32_t d[R][3];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    if (row0 + r < a.N) {
+      const uint32_t *rb = P + ((row0 + r) * Q + q) * 64;  // wave-uniform
+#pragma unroll
+      for (int p = 0; p < 3; p++) d[r][p] = ld_state<NT>(rb + p * ps + lane);
+    }
+  }
+  if constexpr (VG) {
+    // entry e: the low dwords of its four sign words, then the high dwords,
+    // so lanes 0-31 and 32-63 each read the 16 bytes holding their bits
+    __shared__ uint4 lsig[kWaves][kQ12Stage][2];
+    __shared__ float lval[kWaves][kQ12Stage];
+    const int wv = threadIdx.x >> 6, half = lane >> 5, sh = 31 - (lane & 31);
+    const int64_t rl = a.N - row0 < R ? a.N - row0 : R;
+    const int64_t kb = rp[row0], ne = rp[row0 + rl] - kb;
+    if (ne <= kQ12Stage) {
+      for (int64_t e = lane; e < ne; e += 64) {
+        const int64_t c = col[kb + e];
+        const uint4 *sp = reinterpret_cast<const uint4 *>(S + c * W + w0);
+        const uint4 s01 = sp[0], s23 = sp[1];  // words 0,1 and 2,3 as dword pairs
+        lsig[wv][e][0] = make_uint4(s01.x, s01.z, s23.x, s23.z);
+        lsig[wv][e][1] = make_uint4(s01.y, s01.w, s23.y, s23.w);
+        lval[wv][e] = val[kb + e];
+      }
+      // the wavefront's LDS writes complete before its reads below
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      // F += bit ? v : -v, as (-v) with its sign flipped where the bit is set
+      auto add = [&](float F[4], const uint4 &d, float v) {
+        const uint32_t nv = __float_as_uint(-v);
+        F[0] += __uint_as_float(nv ^ ((d.x << sh) & 0x80000000u));
+        F[1] += __uint_as_float(nv ^ ((d.y << sh) & 0x80000000u));
+        F[2] += __uint_as_float(nv ^ ((d.z << sh) & 0x80000000u));
+        F[3] += __uint_as_float(nv ^ ((d.w << sh) & 0x80000000u));
+      };
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int64_t row = row0 + r;
+        if (row >= a.N) break;
+        const int k0 = (int)(rp[row] - kb), k1 = (int)(rp[row + 1] - kb);
+        float F[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        int k = k0;
+        for (; k + 2 <= k1; k += 2) {
+          const uint4 d0 = lsig[wv][k][half], d1 = lsig[wv][k + 1][half];
+          const float v0 = lval[wv][k], v1 = lval[wv][k + 1];
+          add(F, d0, v0);
+          add(F, d1, v1);
+        }
+        if (k < k1) add(F, lsig[wv][k][half], lval[wv][k]);
+        q12_finish_row<NT>(d[r], F, kc, P + (row * Q + q) * 64, ps, Sn + row * W + w0, lane);
+      }
+      return;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int64_t row = row0 + r;
+    if (row >= a.N) break;
+    const int64_t k0 = rp[row], k1 = rp[row + 1];
+    float F[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    int64_t k = k0;
+    for (; k + 4 <= k1; k += 4) {
+      const int64_t j0 = col[k], j1 = col[k + 1], j2 = col[k + 2], j3 = col[k + 3];
+      const float v0 = val[k], v1 = val[k + 1], v2 = val[k + 2], v3 = val[k + 3];
+      uint64_t s0[4], s1[4], s2[4], s3[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        s0[u] = S[j0 * W + w0 + u];
+        s1[u] = S[j1 * W + w0 + u];
+        s2[u] = S[j2 * W + w0 + u];
+        s3[u] = S[j3 * W + w0 + u];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        F[u] = ((s0[u] >> lane) & 1) ? F[u] + v0 : F[u] - v0;
+        F[u] = ((s1[u] >> lane) & 1) ? F[u] + v1 : F[u] - v1;
+        F[u] = ((s2[u] >> lane) & 1) ? F[u] + v2 : F[u] - v2;
+        F[u] = ((s3[u] >> lane) & 1) ? F[u] + v3 : F[u] - v3;
+      }
+    }
+    for (; k < k1; k++) {
+      const int64_t j = col[k];
+      const float v = val[k];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const uint64_t sw = S[j * W + w0 + u];
+        F[u] = ((sw >> lane) & 1) ? F[u] + v : F[u] - v;
+      }
+    }
+    q12_finish_row<NT>(d[r], F, kc, P + (row * Q + q) * 64, ps, Sn + row * W + w0, lane);
+  }
+}
+
+template <int R, int NT, int VG>
+__global__ __launch_bounds__(kBlock) void k_step_q12(StepArgs a) {
+  q12_step_body<R, NT, VG>(a);
+}
+
+// Initial state of the narrow form: the counter-based stream of k_init_state
+// (element e = i A + a, FP32 values), rounded to codes; padding agents 0.
+__global__ void k_init_q12(uint32_t *P, int64_t N, int64_t A, int64_t Q, uint64_t seed, int qshift) {
+  const int64_t n = N * Q * 64, ps = n;
+  const Q12K k = q12_consts(0.0f, 0.0f, 0.0f, qshift);
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = idx / (Q * 64), rem = idx - i * Q * 64, q = rem / 64, l = rem - q * 64;
+    uint32_t u[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int64_t ag = 256 * q + 64 * j + l;
+      float x = 0.0f, y = 0.0f;
+      if (ag < A) {
+        const uint64_t e = (uint64_t)(i * A + ag);
+        x = (float)((d_uniform(seed, 2 * e) * 2 - 1) / 10);
+        y = (float)((d_uniform(seed, 2 * e + 1) * 2 - 1) / 10);
+      }
+      u[j] = q12_encode(x, y, k);
+    }
+    uint32_t d0, d1, d2;
+    q12_pack(u, d0, d1, d2);
+    P[idx] = d0;
+    P[ps + idx] = d1;
+    P[2 * ps + idx] = d2;
+  }
+}
+
+// Pre-step (src/kernels.cu:15-20 at t = 0) on the decoded values, rounded.
+__global__ void k_prestep_q12(uint32_t *P, int64_t n, float cy, float cx, int qshift) {
+  const Q12K k = q12_consts(0.0f, 0.0f, 0.0f, qshift);
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    float x[4], y[4];
+    q12_unpack(P[idx], P[n + idx], P[2 * n + idx], x, y, k);
+    uint32_t u[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const float yy = fmaT(cy, x[j], y[j]);
+      const float xx = fmaT(cx, yy, x[j]);
+      u[j] = q12_encode(xx, yy, k);
+    }
+    uint32_t d0, d1, d2;
+    q12_pack(u, d0, d1, d2);
+    P[idx] = d0;
+    P[n + idx] = d1;
+    P[2 * n + idx] = d2;
+  }
+}
+
+// Sign words of the narrow state: one wavefront per (row, quad), one ballot
+// per word (bit l of word 4 q + j = X code of agent 256 q + 64 j + l < 0).
+__global__ __launch_bounds__(kBlock) void k_pack_signs_q12(const uint32_t *P, uint64_t *S,
+                                                           int64_t N, int64_t Q) {
+  const int lane = threadIdx.x & 63;
+  const int64_t g = wave_index();
+  if (g >= N * Q) return;
+  const int64_t ps = N * Q * 64, o = g * 64 + lane;
+  uint32_t u[4];
+  q12_units(P[o], P[ps + o], P[2 * ps + o], u);
+  uint64_t mine = 0;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const uint64_t b = __ballot(q12_negative(u[j]));
+    if (lane == j) mine = b;
+  }
+  if (lane < 4) S[g * 4 + lane] = mine;
+}
+
 // ------------------------------------------------------------ row split
 //
 // FORM as in launch_step: 0 plain FP32, 1 plain FP64, 2 packed signs, 3 BF16
@@ -1685,6 +1922,26 @@ void launch_prestep(hipStream_t st, bool f64, void *X, void *Y, int64_t n, doubl
                        (float)cy, (float)cx);
 }
 
+void launch_init_q12(hipStream_t st, uint32_t *P, int64_t N, int64_t A, int64_t lda,
+                     uint64_t seed, int qshift) {
+  const int64_t Q = lda / 256;
+  hipLaunchKernelGGL(k_init_q12, dim3(grid_for(N * Q * 64, 256, 16384)), dim3(256), 0, st, P, N,
+                     A, Q, seed, qshift);
+}
+
+void launch_prestep_q12(hipStream_t st, uint32_t *P, int64_t N, int64_t lda, double dt, int qshift) {
+  const int64_t n = N * (lda / 256) * 64;
+  const double cy = dt * (0.0 - 1), cx = dt * (0.0 + 1);
+  hipLaunchKernelGGL(k_prestep_q12, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, P, n,
+                     (float)cy, (float)cx, qshift);
+}
+
+void launch_pack_signs_q12(hipStream_t st, const uint32_t *P, uint64_t *S, int64_t N, int64_t lda) {
+  const int64_t Q = lda / 256;
+  hipLaunchKernelGGL(k_pack_signs_q12, dim3(grid_for(N * Q, kWaves, INT32_MAX)), dim3(kBlock), 0,
+                     st, P, S, N, Q);
+}
+
 void launch_bf16_shadow(hipStream_t st, const float *X, uint16_t *sh, int64_t n) {
   hipLaunchKernelGGL(k_bf16_shadow, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, X, sh, n);
 }
@@ -1712,6 +1969,14 @@ void launch_xy_update(hipStream_t st, bool f64, void *X, void *Y, int64_t n, con
 }
 
 bool step_supported(int form, int prob, int gather, int vec) {
+  if (form == 4) {
+    // 4 | rows << 8 | non-temporal << 16 | staged gathers << 17
+    const int rows = (vec >> 8) & 0xff, vg = (vec >> 17) & 1;
+    if (prob != PROB_MAXCUT || gather != G_DSB || (vec & 0xff) != 4 || (vec >> 18) != 0)
+      return false;
+    return vg ? (rows == 4 || rows == 8 || rows == 16 || rows == 32)
+              : (rows == 1 || rows == 2 || rows == 4);
+  }
   if (form == 2) {
     const int wt = vec & 0xff, rows = (vec >> 8) & 0xff;
     return (prob == PROB_MAXCUT || prob == PROB_TSP) && gather == G_DSB && (vec >> 17) == 0 &&
@@ -1788,6 +2053,24 @@ void launch_step(hipStream_t st, int form, int prob, int gather, int vec, const 
 #undef SIS_PACKED_V
 #undef SIS_PACKED_R
 #undef SIS_PACKED
+  } else if (form == 4) {
+    const int rows = (vec >> 8) & 0xff, vg = (vec >> 17) & 1;
+    const int64_t waves = ((a.N + rows - 1) / rows) * (a.W / 4);
+#define SIS_Q12(R, NT, VG)                                      \
+  if (rows == R && nt == NT && vg == VG) {                      \
+    launch_waves(st, k_step_q12<R, NT, VG>, waves, a);          \
+    return;                                                     \
+  }
+#define SIS_Q12_NT(R, VG) SIS_Q12(R, 0, VG) SIS_Q12(R, 1, VG)
+    SIS_Q12_NT(1, 0)
+    SIS_Q12_NT(2, 0)
+    SIS_Q12_NT(4, 0)
+    SIS_Q12_NT(4, 1)
+    SIS_Q12_NT(8, 1)
+    SIS_Q12_NT(16, 1)
+    SIS_Q12_NT(32, 1)
+#undef SIS_Q12_NT
+#undef SIS_Q12
   } else if (form == 3) {
 #define SIS_BF16(P, V, NT)                                                      \
   if (prob == P && v == V && nt == NT) {                                        \

@@ -40,6 +40,8 @@ struct sis_solver {
   DevBuf<double> Xd[2], Yd;      // FP64 state
   DevBuf<uint16_t> sh[2];        // BF16 shadow
   DevBuf<uint64_t> sg[2];        // packed sign words
+  DevBuf<uint32_t> Xq;           // narrow form: 12-bit X / Y codes, three dword planes
+  int qshift = 0;                // narrow form: Y codes are round(y 2^qshift)
   int par = 0;                   // index of the current snapshot buffer
 
   DevBuf<uint64_t> cur, fin, newmask, part;
@@ -132,11 +134,60 @@ struct sis_solver {
     if (stream) (void)hipStreamDestroy(stream);
   }
 
+  bool q12() const { return p.form == SIS_FORM_Q12; }
   void *Xbuf(int i) { return f64 ? (void *)Xd[i].p : (void *)Xf[i].p; }
   void *Ybuf() { return f64 ? (void *)Yd.p : (void *)Yf.p; }
   // X holding the current positions.
   void *Xcur() { return p.form == SIS_FORM_F32 || p.form == SIS_FORM_F64 ? Xbuf(par) : Xbuf(0); }
 };
+
+// Narrow-state form on the host: the same rounding as q12_encode (device) --
+// X code round(x 2^10), +-1025 beyond the wall with a 0 Y code, Y code
+// round(y 2^qshift) clamped to +-2047, nearest even, each stored plus 2048
+// in 12 bits -- and the plane layout (dword p of lane l, quad q, row i at
+// p N Q 64 + (i Q + q) 64 + l).
+static int q12_shift_for(double dt) {
+  if (!(dt > 0)) return -1;
+  const double ymax = 2.0 / dt + dt;
+  int sh = 11;
+  while (sh > 0 && std::ldexp(2047.0, -sh) < ymax) sh--;
+  return sh;
+}
+
+static uint32_t q12_unit_host(float x, float y, int qshift) {
+  int xq, yq;
+  if (std::fabs(x) > 1.0f) {
+    xq = std::signbit(x) ? -1025 : 1025;
+    yq = 0;
+  } else {
+    xq = (int)std::rint(std::ldexp(x, 10));
+    const float yr = std::rint(std::ldexp(y, qshift));
+    yq = yr > 2047.0f ? 2047 : yr < -2047.0f ? -2047 : (int)yr;
+  }
+  return (uint32_t)(xq + 2048) | ((uint32_t)(yq + 2048) << 12);
+}
+
+// Unit j of a lane occupies bits 24 j .. 24 j + 23 of its 96-bit (d0, d1, d2):
+// read and written through the 64-bit window of dwords p, p + 1 that holds it.
+static void q12_put(std::vector<uint32_t> &P, int64_t N, int64_t Q, int64_t i, int64_t a,
+                    uint32_t u) {
+  const int64_t q = a / 256, j = (a % 256) / 64, l = a % 64, ps = N * Q * 64;
+  const int64_t o = (i * Q + q) * 64 + l, p = (24 * j) / 32, r = (24 * j) % 32;
+  uint32_t &lo = P[(size_t)(p * ps + o)];
+  uint64_t win = lo;
+  if (p < 2) win |= (uint64_t)P[(size_t)((p + 1) * ps + o)] << 32;
+  win = (win & ~((uint64_t)0xffffff << r)) | ((uint64_t)(u & 0xffffff) << r);
+  lo = (uint32_t)win;
+  if (p < 2) P[(size_t)((p + 1) * ps + o)] = (uint32_t)(win >> 32);
+}
+
+static uint32_t q12_get(const std::vector<uint32_t> &P, int64_t N, int64_t Q, int64_t i, int64_t a) {
+  const int64_t q = a / 256, j = (a % 256) / 64, l = a % 64, ps = N * Q * 64;
+  const int64_t o = (i * Q + q) * 64 + l, p = (24 * j) / 32, r = (24 * j) % 32;
+  uint64_t win = P[(size_t)(p * ps + o)];
+  if (p < 2) win |= (uint64_t)P[(size_t)((p + 1) * ps + o)] << 32;
+  return (uint32_t)(win >> r) & 0xffffffu;
+}
 
 // Resident tiles (k_resident): bit 20, log2 of the agents per workgroup in
 // bits 21-22.  Fused MaxCut in FP32 (plain dSB / bSB, packed dSB), weights
@@ -241,11 +292,19 @@ static bool tile_ok(const sis_solver *s, int vec) {
   if (vec & kResidentBit) return resident_layout(s, vec, nullptr);
   if (!step_supported(s->p.form, s->prob, s->gather, vec)) return false;
   if (s->p.form == SIS_FORM_PACKED) return s->W % (vec & 0xff) == 0;
+  if (s->p.form == SIS_FORM_Q12) return s->lda % 256 == 0;
   return s->lda % (64 * (vec & 0xffff)) == 0;
 }
 
 static int default_tile(const sis_solver *s) {
   int v = 1;
+  if (s->p.form == SIS_FORM_Q12) {
+    // staged gathers of 16 rows per wavefront; non-temporal codes once they
+    // exceed the Infinity Cache (3 bytes per node-agent)
+    v = 4 | 16 << 8 | 1 << 17;
+    if ((double)s->N * (double)s->lda * 3.0 > 256.0 * 1024 * 1024) v |= 1 << 16;
+    return v;
+  }
   if (s->p.form == SIS_FORM_PACKED) {
     while (v < 4 && s->W % (v * 2) == 0) v *= 2;
     v |= 1 << 8;
@@ -268,6 +327,13 @@ static std::vector<int> candidate_tiles(const sis_solver *s) {
     if (s->f64) return {1};
     if (s->tsp_res_G) return {1, 2, 3};
     return {1, 2};
+  }
+  if (s->p.form == SIS_FORM_Q12) {
+    for (int nt : {0, 1}) {
+      for (int r : {1, 2, 4}) c.push_back(4 | (r << 8) | (nt << 16));
+      for (int r : {4, 8, 16, 32}) c.push_back(4 | (r << 8) | (nt << 16) | (1 << 17));
+    }
+    return c;
   }
   if (s->p.form == SIS_FORM_PACKED) {
     for (int wt : {1, 2, 4, 8})
@@ -388,8 +454,16 @@ int sis_solver_create(const sis_csr *J, const double *field, const sis_params *p
     return fail(SIS_ERR_ARG, "sis_solver_create: the packed-sign form is dSB only");
   if (p.form == SIS_FORM_BF16 && p.dynamics != SIS_BSB)
     return fail(SIS_ERR_ARG, "sis_solver_create: the BF16 form is bSB only");
-  if (p.form < SIS_FORM_F32 || p.form > SIS_FORM_BF16)
+  if (p.form < SIS_FORM_F32 || p.form > SIS_FORM_Q12)
     return fail(SIS_ERR_ARG, "sis_solver_create: unknown form");
+  if (p.form == SIS_FORM_Q12) {
+    if (p.problem != SIS_MAXCUT || p.dynamics != SIS_DSB || !p.fused)
+      return fail(SIS_ERR_ARG, "sis_solver_create: the 12-bit form is fused MaxCut dSB only");
+    if (!(p.dt > 0))
+      return fail(SIS_ERR_ARG, "sis_solver_create: the 12-bit form needs dt > 0");
+    if (p.split != 0)
+      return fail(SIS_ERR_ARG, "sis_solver_create: the 12-bit form has no row split");
+  }
   if (p.agents < 1 || p.agents > (1 << 18))
     return fail(SIS_ERR_ARG, "sis_solver_create: agents must be in [1, 262144]");
   if (!(p.dt == p.dt)) return fail(SIS_ERR_ARG, "sis_solver_create: dt is NaN");
@@ -421,7 +495,8 @@ int sis_solver_create(const sis_csr *J, const double *field, const sis_params *p
     }
   }
   s->A = p.agents;
-  s->lda = (s->A + 63) / 64 * 64;
+  s->lda = p.form == SIS_FORM_Q12 ? (s->A + 255) / 256 * 256 : (s->A + 63) / 64 * 64;
+  s->qshift = p.form == SIS_FORM_Q12 ? q12_shift_for(p.dt) : 0;
   s->W = s->lda / 64;
   s->f64 = p.form == SIS_FORM_F64;
   s->gather = p.dynamics == SIS_DSB ? G_DSB : G_BSB;
@@ -477,6 +552,11 @@ int sis_solver_create(const sis_csr *J, const double *field, const sis_params *p
   if (s->f64) {
     if ((st = s->Xd[0].alloc((size_t)NL)) || (st = s->Xd[1].alloc((size_t)NL)) ||
         (st = s->Yd.alloc((size_t)NL)))
+      return st;
+  } else if (p.form == SIS_FORM_Q12) {
+    // three dword planes of N x (lda / 256) x 64 lanes: 3 bytes per node-agent
+    if ((st = s->Xq.alloc((size_t)(3 * N * (s->lda / 4)))) ||
+        (st = s->sg[0].alloc((size_t)NW)) || (st = s->sg[1].alloc((size_t)NW)))
       return st;
   } else {
     if ((st = s->Xf[0].alloc((size_t)NL)) || (st = s->Yf.alloc((size_t)NL))) return st;
@@ -634,8 +714,17 @@ static int refresh_shadow(sis_solver *s) {
   if (s->p.form == SIS_FORM_BF16) launch_bf16_shadow(s->stream, s->Xf[0].p, s->sh[s->par].p, NL);
   if (s->p.form == SIS_FORM_PACKED)
     launch_pack_signs(s->stream, SRC_F32, s->Xf[0].p, s->sg[s->par].p, s->N, s->W, s->lda);
+  if (s->q12()) launch_pack_signs_q12(s->stream, s->Xq.p, s->sg[s->par].p, s->N, s->lda);
   HIPCHK(hipGetLastError());
   return SIS_OK;
+}
+
+// Pre-step (src/kernels.cu:8-22 at t = 0) on the current state.
+static void enqueue_prestep(sis_solver *s) {
+  if (s->q12())
+    launch_prestep_q12(s->stream, s->Xq.p, s->N, s->lda, s->p.dt, s->qshift);
+  else
+    launch_prestep(s->stream, s->f64, s->Xcur(), s->Ybuf(), s->N * s->lda, s->p.dt);
 }
 
 // Schedule table for horizon T covering steps [0, len).
@@ -717,6 +806,13 @@ static void enqueue_step(sis_solver *s, int32_t off) {
       a.gsrc = s->sg[cur].p;
       a.gdst = s->sg[nxt].p;
       a.X = s->Xf[0].p;
+      break;
+    case SIS_FORM_Q12:
+      a.gsrc = s->sg[cur].p;
+      a.gdst = s->sg[nxt].p;
+      a.X = s->Xq.p;
+      a.Y = nullptr;
+      a.qshift = s->qshift;
       break;
   }
   if (s->prob == PROB_UNFUSED)
@@ -809,7 +905,8 @@ static void enqueue_window(sis_solver *s, int64_t P, int uw) {
   int src;
   const void *x;
   switch (s->p.form) {
-    case SIS_FORM_PACKED: src = SRC_BITS; x = s->sg[s->par].p; break;
+    case SIS_FORM_PACKED:
+    case SIS_FORM_Q12: src = SRC_BITS; x = s->sg[s->par].p; break;
     case SIS_FORM_F64: src = SRC_F64; x = s->Xcur(); break;
     default: src = SRC_F32; x = s->Xcur(); break;
   }
@@ -902,8 +999,11 @@ int sis_solver_init_random(sis_solver *s, uint64_t seed) {
   int st = reset_windows(s);
   if (st) return st;
   s->par = 0;
-  launch_init_state(s->stream, s->f64, s->Xbuf(0), s->Ybuf(), s->N, s->A, s->lda, seed,
-                    s->p.problem == SIS_TSP);
+  if (s->q12())
+    launch_init_q12(s->stream, s->Xq.p, s->N, s->A, s->lda, seed, s->qshift);
+  else
+    launch_init_state(s->stream, s->f64, s->Xbuf(0), s->Ybuf(), s->N, s->A, s->lda, seed,
+                      s->p.problem == SIS_TSP);
   HIPCHK(hipGetLastError());
   if ((st = refresh_shadow(s))) return st;
   HIPCHK(hipStreamSynchronize(s->stream));
@@ -917,7 +1017,16 @@ int sis_solver_set_state(sis_solver *s, const double *x, const double *y) {
   if (st) return st;
   s->par = 0;
   const int64_t N = s->N, A = s->A, lda = s->lda;
-  if (s->f64) {
+  if (s->q12()) {
+    // FP32 values (as the other FP32 forms take them), rounded to codes;
+    // padding agents 0
+    const int64_t Q = lda / 256;
+    std::vector<uint32_t> P((size_t)(3 * N * Q * 64), 0u);
+    for (int64_t i = 0; i < N; i++)
+      for (int64_t a = 0; a < A; a++)
+        q12_put(P, N, Q, i, a, q12_unit_host((float)x[i * A + a], (float)y[i * A + a], s->qshift));
+    HIPCHK(hipMemcpy(s->Xq.p, P.data(), sizeof(uint32_t) * P.size(), hipMemcpyHostToDevice));
+  } else if (s->f64) {
     std::vector<double> bx((size_t)(N * lda), 0.0), by((size_t)(N * lda), 0.0);
     for (int64_t i = 0; i < N; i++)
       for (int64_t a = 0; a < A; a++) {
@@ -961,6 +1070,20 @@ int sis_solver_get_state(sis_solver *s, double *x, double *y) {
     }
     return SIS_OK;
   };
+  if (s->q12()) {
+    const int64_t Q = lda / 256;
+    std::vector<uint32_t> P((size_t)(3 * N * Q * 64));
+    HIPCHK(hipMemcpy(P.data(), s->Xq.p, sizeof(uint32_t) * P.size(), hipMemcpyDeviceToHost));
+    const double yinv = std::ldexp(1.0, -s->qshift);
+    for (int64_t i = 0; i < N; i++)
+      for (int64_t a = 0; a < A; a++) {
+        const uint32_t u = q12_get(P, N, Q, i, a);
+        const int32_t xq = (int32_t)(u & 0xfffu) - 2048, yq = (int32_t)((u >> 12) & 0xfffu) - 2048;
+        if (x) x[i * A + a] = std::ldexp((double)xq, -10);
+        if (y) y[i * A + a] = (double)yq * yinv;
+      }
+    return SIS_OK;
+  }
   int st = copy_out(s->Xcur(), x);
   if (st) return st;
   return copy_out(s->Ybuf(), y);
@@ -968,7 +1091,7 @@ int sis_solver_get_state(sis_solver *s, double *x, double *y) {
 
 int sis_solver_prestep(sis_solver *s) {
   if (!s || !s->state_ready) return fail(SIS_ERR_STATE, "sis_solver_prestep: state not set");
-  launch_prestep(s->stream, s->f64, s->Xcur(), s->Ybuf(), s->N * s->lda, s->p.dt);
+  enqueue_prestep(s);
   HIPCHK(hipGetLastError());
   int st = refresh_shadow(s);
   if (st) return st;
@@ -1007,27 +1130,29 @@ int sis_solver_autotune(sis_solver *s, int64_t trial_steps, int32_t *tile, doubl
   if ((st = ensure_events(s, 2))) return st;
   // Snapshot of the state every candidate starts from (and that is restored).
   const int64_t NL = s->N * s->lda, NW = s->N * s->W;
-  const size_t xbytes = (size_t)NL * (s->f64 ? 8 : 4);
+  // the narrow form keeps X and Y codes in one buffer
+  const size_t xbytes = s->q12() ? sizeof(uint32_t) * s->Xq.n : (size_t)NL * (s->f64 ? 8 : 4);
+  const size_t ybytes = s->q12() ? 0 : xbytes;
   size_t gbytes = 0;
   void *gcur = nullptr;
   if (s->p.form == SIS_FORM_BF16) {
     gbytes = (size_t)NL * 2;
     gcur = s->sh[s->par].p;
-  } else if (s->p.form == SIS_FORM_PACKED) {
+  } else if (s->p.form == SIS_FORM_PACKED || s->q12()) {
     gbytes = (size_t)NW * 8;
     gcur = s->sg[s->par].p;
   }
   DevBuf<char> sx, sy, sgb;
-  if ((st = sx.alloc(xbytes)) || (st = sy.alloc(xbytes)) || (st = sgb.alloc(gbytes))) return st;
+  if ((st = sx.alloc(xbytes)) || (st = sy.alloc(ybytes)) || (st = sgb.alloc(gbytes))) return st;
   const int par0 = s->par;
-  void *x0 = s->Xcur();
+  void *x0 = s->q12() ? (void *)s->Xq.p : s->Xcur();
   HIPCHK(hipMemcpyAsync(sx.p, x0, xbytes, hipMemcpyDeviceToDevice, s->stream));
-  HIPCHK(hipMemcpyAsync(sy.p, s->Ybuf(), xbytes, hipMemcpyDeviceToDevice, s->stream));
+  if (ybytes) HIPCHK(hipMemcpyAsync(sy.p, s->Ybuf(), ybytes, hipMemcpyDeviceToDevice, s->stream));
   if (gbytes) HIPCHK(hipMemcpyAsync(sgb.p, gcur, gbytes, hipMemcpyDeviceToDevice, s->stream));
   auto restore = [&]() -> int {
     s->par = par0;
     HIPCHK(hipMemcpyAsync(x0, sx.p, xbytes, hipMemcpyDeviceToDevice, s->stream));
-    HIPCHK(hipMemcpyAsync(s->Ybuf(), sy.p, xbytes, hipMemcpyDeviceToDevice, s->stream));
+    if (ybytes) HIPCHK(hipMemcpyAsync(s->Ybuf(), sy.p, ybytes, hipMemcpyDeviceToDevice, s->stream));
     if (gbytes) HIPCHK(hipMemcpyAsync(gcur, sgb.p, gbytes, hipMemcpyDeviceToDevice, s->stream));
     Ctrl c{};
     c.merge = s->p.problem == SIS_TSP ? 1 : 0;
@@ -1107,7 +1232,7 @@ int sis_solver_run(sis_solver *s, int64_t max_steps, int update_window, sis_run_
 
   auto t_start = std::chrono::high_resolution_clock::now();
   HIPCHK(hipEventRecord(s->events[0], s->stream));
-  launch_prestep(s->stream, s->f64, s->Xcur(), s->Ybuf(), s->N * s->lda, s->p.dt);
+  enqueue_prestep(s);
   if ((st = refresh_shadow(s))) return st;
   // Early exit is a device flag; the host checks it every `chunk` windows.
   const int64_t chunk = uw ? 4 : nwin;

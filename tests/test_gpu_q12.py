@@ -1,0 +1,210 @@
+"""GPU parity of the narrow-state dSB form (SIS_FORM_Q12: X and Y stored as
+12-bit fixed-point codes, FP32 arithmetic, exact J sgn(X) from packed sign
+words) against the oracle's restatement of the same form (orc_*_q12).
+
+Bar: bit-exact.  The codes, the FP32 update on the decoded values and the
+rounding are deterministic, so the device's X, Y codes, spins, locks, step
+counts and cuts equal the oracle's exactly, for every tile, agent count
+(padded to 256 per quad), launch mode and window setting.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle as orc
+import sis
+
+pytestmark = pytest.mark.gpu
+
+Q12 = sis.FORM_Q12
+VG = 1 << 17  # staged gathers: a wavefront's rows' entries fetched at once into LDS
+TILES = [4 | 1 << 8, 4 | 2 << 8, 4 | 4 << 8, 4 | 1 << 8 | 1 << 16, 4 | 2 << 8 | 1 << 16,
+         4 | 4 << 8 | 1 << 16, 4 | 4 << 8 | VG, 4 | 8 << 8 | VG | 1 << 16, 4 | 16 << 8 | VG,
+         4 | 32 << 8 | VG | 1 << 16]
+
+
+def problem(n=203, p=0.04, weights=sis.W_INT3, seed=12):
+    J = sis.Csr.gen_er(n, p, weights, seed)
+    rp, col, val = J.arrays()
+    return J, orc.Problem(rp, col, val, S=J.sum)
+
+
+def seeded(n, A, seed, scale=0.1):
+    rng = np.random.default_rng(seed)
+    X = rng.uniform(-scale, scale, (n, A)).astype(np.float32)
+    Y = rng.uniform(-scale, scale, (n, A)).astype(np.float32)
+    return X, Y
+
+
+def test_init_random_equals_rounded_oracle_stream():
+    J, _ = problem(70)
+    for A in (64, 300):
+        s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=A, dt=0.5, form=Q12))
+        s.init_random(9)
+        X, Y = orc.init_state(9, orc.MAXCUT, J.n, A)
+        orc.q12_round(X, Y, orc.q12_shift(0.5))
+        x, y = s.get_state()
+        assert np.array_equal(x, X.astype(np.float64)) and np.array_equal(y, Y.astype(np.float64))
+
+
+@pytest.mark.parametrize("dt", [0.5, 0.1])
+def test_set_state_rounds_like_the_oracle(dt):
+    # values off the grid, halfway cases, beyond the wall, negative zero and
+    # Y beyond the code range: the device's codes are the oracle's rounding
+    J, _ = problem(50)
+    A = 130
+    X, Y = seeded(J.n, A, 3, scale=1.5)
+    X[0, :8] = np.array([2.5, 3.5, -2.5, -0.3, 1024.0, -0.0, 1023.5, -1024.5]) / 1024
+    Y[1, :4] = [100.0, -100.0, 0.5 / 256, 1.5 / 256]
+    s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=A, dt=dt, form=Q12))
+    s.set_state(X.astype(np.float64), Y.astype(np.float64))
+    orc.q12_round(X, Y, orc.q12_shift(dt))
+    x, y = s.get_state()
+    assert np.array_equal(x, X.astype(np.float64)) and np.array_equal(y, Y.astype(np.float64))
+    assert not np.signbit(x[x == 0]).any()
+
+
+@pytest.mark.parametrize("tile", TILES)
+@pytest.mark.parametrize("agents", [96, 256, 300])
+def test_trajectory_bit_exact(tile, agents):
+    # 203 rows (not a multiple of the rows per wavefront), integer weights,
+    # a state that reaches the walls; two advances cross p = 1 at t = 390
+    J, pb = problem()
+    T, dt = 400, 0.5
+    s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=agents, dt=dt, form=Q12, tile=tile))
+    assert s.p.tile == tile
+    sy = orc.q12_shift(dt)
+    X, Y = seeded(J.n, agents, 13, scale=0.9)
+    s.set_state(X.astype(np.float64), Y.astype(np.float64))
+    orc.q12_round(X, Y, sy)
+    s.prestep()
+    orc.prestep_q12(X, Y, dt, sy)
+    x, y = s.get_state()
+    assert np.array_equal(x, X.astype(np.float64)) and np.array_equal(y, Y.astype(np.float64))
+    for t0, n in ((0, 150), (150, 250)):
+        s.advance(t0, n, T)
+        orc.advance_q12(pb, X, Y, t0, n, T, dt, s.p.xi0, sy)
+        x, y = s.get_state()
+        assert np.array_equal(x, X.astype(np.float64)), (tile, agents, t0 + n)
+        assert np.array_equal(y, Y.astype(np.float64)), (tile, agents, t0 + n)
+    assert (np.abs(X) > 1).any()  # the wall code occurred
+
+
+@pytest.mark.parametrize("tile", [4 | 8 << 8 | VG, 4 | 16 << 8 | VG | 1 << 16, 4 | 4 << 8 | VG])
+def test_staged_gathers_hub_and_empty_rows(tile):
+    # 150 rows: a hub joined to rows 1..140 (its wave exceeds the staging
+    # capacity and takes the scalar path), a sparse tail among rows 1..129,
+    # rows 141..149 without entries (waves whose rows hold no entry at all),
+    # real weights
+    rng = np.random.default_rng(31)
+    rows = [0] * 140 + list(rng.integers(1, 130, 80))
+    cols = list(range(1, 141)) + list(rng.integers(1, 130, 80))
+    keep = [(r, c) for r, c in zip(rows, cols) if r != c and r < 130 and c < 141]
+    vals = rng.normal(size=len(keep))
+    J = sis.Csr.from_coo(150, [r for r, _ in keep], [c for _, c in keep], vals)
+    rp, col, val = J.arrays()
+    deg = np.diff(rp)
+    assert deg[0] > 128 and (deg[141:] == 0).all()
+    pb = orc.Problem(rp, col, val, S=J.sum)
+    A, T, dt = 256, 300, 0.5
+    sy = orc.q12_shift(dt)
+    s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=A, dt=dt, form=Q12, tile=tile, xi0=0.2))
+    assert s.p.tile == tile
+    X, Y = seeded(J.n, A, 32, scale=0.5)
+    s.set_state(X.astype(np.float64), Y.astype(np.float64))
+    orc.q12_round(X, Y, sy)
+    s.advance(0, 251, T)
+    orc.advance_q12(pb, X, Y, 0, 251, T, dt, 0.2, sy)
+    x, y = s.get_state()
+    assert np.array_equal(x, X.astype(np.float64)) and np.array_equal(y, Y.astype(np.float64))
+
+
+@pytest.mark.parametrize("uw,T", [(1, 6000), (0, 2345)])
+@pytest.mark.parametrize("graph", [0, 1, 2])
+def test_full_run_matches_oracle(uw, T, graph):
+    J, pb = problem(120, 0.06, sis.W_PM1, 5)
+    A, dt = 100, 0.5
+    s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=A, dt=dt, form=Q12, use_graph=graph))
+    runs = 2 if graph else 1
+    for _ in range(runs):  # graph replays across runs must agree with eager
+        s.init_random(21)
+        info = s.run(T, uw)
+        X, Y = orc.init_state(21, orc.MAXCUT, J.n, A)
+        r = orc.solve_q12(pb, X, Y, T, uw, P=100, K=25, dt=dt, xi0=s.p.xi0)
+        assert info["steps_executed"] == r["steps"]
+        x, y = s.get_state()
+        assert np.array_equal(x, X.astype(np.float64)) and np.array_equal(y, Y.astype(np.float64))
+        assert np.array_equal(s.spins(sis.SPINS_CURRENT), r["cur"])
+        assert np.array_equal(s.spins(sis.SPINS_FINAL), r["fin"])
+        st, lk, _ = s.agent_state()
+        assert np.array_equal(st, r["stab"]) and np.array_equal(lk, r["bif"])
+        cuts, best = s.cuts()
+        want = orc.cut(pb, r["cur"])
+        assert np.array_equal(cuts, want) and best == int(np.argmax(want))
+        assert info["all_bifurcated"] == int(uw == 1 and r["bif"].all())
+
+
+def test_autotune_keeps_state_and_results():
+    J, pb = problem(300, 0.03, sis.W_PM1, 14)
+    A, dt = 256, 0.5
+    s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=A, dt=dt, form=Q12))
+    s.init_random(15)
+    x0, y0 = s.get_state()
+    tile, ms = s.autotune(4)
+    assert ms > 0 and s.p.tile == tile and tile & 0xff == 4 and tile >> 18 == 0
+    x, y = s.get_state()
+    assert np.array_equal(x, x0) and np.array_equal(y, y0)
+    info = s.run(700, 0)
+    X, Y = orc.init_state(15, orc.MAXCUT, J.n, A)
+    r = orc.solve_q12(pb, X, Y, 700, 0, P=100, dt=dt, xi0=s.p.xi0)
+    assert info["steps_executed"] == r["steps"]
+    assert np.array_equal(s.get_state()[0], X.astype(np.float64))
+    assert np.array_equal(s.cuts()[0], orc.cut(pb, r["cur"]))
+
+
+def test_full_size_on_agent_sample_and_exact_cut():
+    """Random 4-regular N = 2^20, 256 agents (the benchmark graph): 30 steps
+    after the pre-step, every 8th agent bit-exact with the oracle; after a
+    300-step run the device cut of the returned spins equals an exact
+    integer host evaluation."""
+    J = sis.Csr.gen_regular(1 << 20, 4, sis.W_PM1, 2024)
+    rp, col, val = J.arrays()
+    pb = orc.Problem(rp, col, val, S=J.sum)
+    A, T, dt = 256, 20000, 0.5
+    sy = orc.q12_shift(dt)
+    s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=A, dt=dt, form=Q12))
+    s.init_random(77)
+    s.prestep()
+    s.advance(0, 30, T)
+    cols = np.arange(0, A, 8)
+    X, Y = orc.init_state(77, orc.MAXCUT, J.n, A)
+    X, Y = np.ascontiguousarray(X[:, cols]), np.ascontiguousarray(Y[:, cols])
+    orc.q12_round(X, Y, sy)
+    orc.prestep_q12(X, Y, dt, sy)
+    orc.advance_q12(pb, X, Y, 0, 30, T, dt, s.p.xi0, sy)
+    x, y = s.get_state()
+    assert np.array_equal(x[:, cols], X.astype(np.float64))
+    assert np.array_equal(y[:, cols], Y.astype(np.float64))
+    s.init_random(78)
+    assert s.run(300, 0)["steps_executed"] == 300
+    cuts, _ = s.cuts()
+    M = sp.csr_matrix((val, col, rp), shape=(J.n, J.n))
+    s64 = s.spins().astype(np.int64)
+    H = 0.5 * np.einsum("ia,ia->a", s64, M @ s64)
+    assert np.array_equal(cuts, -0.5 * (H - J.sum))
+
+
+def test_rejected_where_it_does_not_apply():
+    J, _ = problem(60)
+    bad = [dict(dynamics=sis.BSB), dict(fused=0), dict(dt=0.0), dict(split=8), dict(split=-1),
+           dict(tile=1 << 20)]
+    for extra in bad:
+        params = dict(problem=sis.MAXCUT, agents=64, dt=0.5, form=Q12)
+        params.update(extra)
+        with pytest.raises(sis.SisError):
+            sis.Solver(J, params)
+    t = sis.Tsp.generate(5, 100, 0, 1)
+    Jt, h = t.qubo()
+    with pytest.raises(sis.SisError):
+        sis.Solver(Jt, dict(problem=sis.TSP, agents=64, form=Q12, dynamics=sis.DSB, ncities=5),
+                   field=h)
