@@ -4,10 +4,12 @@ Workload (default): MaxCut dSB on a random 4-regular graph, N = 2^20 nodes,
 Z = 4 N = 4,194,304 symmetric nonzeros, weights +-1 uniform from seed, 256
 agents, dt = 0.5, max_steps = K (20,000 by default), windows every 100 steps.
 A "step" is one fused SB step over all N x 256 node-agents.  The headline
-form (--form) runs beside the plain FP32 dSB kernel for the speed-up and the
-equal-steps best-cut comparison; --also adds more forms, and the line also
-carries bSB with BF16 gathers against plain FP32 bSB, a batched-serving
-summary (4096 small instances) and the CPU baseline.
+form (--form, default q12: X and Y stored as 12-bit fixed-point codes, FP32
+arithmetic, J sgn(X) exact from packed sign words) runs beside the plain
+FP32 dSB kernel for the speed-up and the equal-steps best-cut comparison;
+--also adds more forms (default: the packed-sign form, bit-identical to plain
+FP32), and the line also carries bSB with BF16 gathers against plain FP32
+bSB, a batched-serving summary (4096 small instances) and the CPU baseline.
 
 The timed region is one sis_solver_run (pre-step + steps + windows), with the
 state resident in HBM; the update window's early exit is off inside the timed
@@ -463,7 +465,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--form", default="packed", choices=list(sis.FORMS))
+    ap.add_argument("--form", default="q12", choices=list(sis.FORMS))
     ap.add_argument("--baseline-form", default="f32", choices=list(sis.FORMS) + ["none"])
     ap.add_argument("--nodes-log2", type=int, default=20)
     ap.add_argument("--agents", type=int, default=256)
@@ -471,8 +473,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-bf16", action="store_true", help="skip the BF16 bSB legs")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving sub-object")
-    ap.add_argument("--also", default="", help="comma-separated extra forms timed beside the "
-                    "headline (same dynamics)")
+    ap.add_argument("--also", default="packed", help="comma-separated extra forms timed beside "
+                    "the headline (same dynamics); the default adds the bit-exact packed form")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--workload", default="regular2p20",
                     choices=["regular2p20", "er800", "torus100", "tsp48", "powerlaw", "tsp_struct",
@@ -548,6 +550,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32" if a.form != "f64" else "f64",
+        "state_storage": {"q12": "X, Y as 12-bit fixed-point codes (3 bytes per node-agent)",
+                          "packed": "X, Y FP32", "f32": "X, Y FP32", "bf16": "X, Y FP32",
+                          "f64": "X, Y FP64"}[a.form],
         "data": "synthetic (seeded random 4-regular graph, +-1 weights, seeded initial state)",
         "config": {
             "workload": f"maxcut {'bSB' if dyn == sis.BSB else 'dSB'}, random 4-regular "
