@@ -313,6 +313,26 @@ int sis_solver_get_spins(sis_solver *s, int which, int32_t *out);
  * cut may be NULL; best_agent may be NULL. */
 int sis_solver_maxcut_cuts(sis_solver *s, double *cut, int64_t *best_agent);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the per-agent cut -0.5 (H - S) and the best agent of
+ * sis_solver_maxcut_cuts, from caller-owned DEVICE arrays -- rp (n + 1 int64),
+ * col / val (stored entries, int32 / double), spins (n x W uint64 words,
+ * W = ceil(agents / 64), bit a % 64 of word i W + a / 64 set for spin -1: the
+ * solver's layout), cut (agents doubles), best_agent (one int64, may be NULL).
+ * Same kernels and summation order as sis_solver_maxcut_cuts, so the results
+ * are identical.  The per-block partial sums live in a fixed device array of
+ * SIS_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends; its address
+ * is resolved on the first call on each device); shapes whose partial sums
+ * exceed it (ceil(n / max(4, min(256, ceil(n / 1024)))) x 64 W doubles) fail
+ * with SIS_ERR_ARG.  Enqueued on `stream` (a hipStream_t; NULL is the default
+ * stream) without synchronizing.  Calls that overlap on one GPU share the
+ * array and race.  Calls no hipMalloc / hipFree. */
+#define SIS_NOALLOC_SCRATCH_BYTES (16u << 20)
+int sis_maxcut_cuts_noalloc(const int64_t *rp, const int32_t *col, const double *val, int64_t n,
+                            const uint64_t *spins, int64_t agents, double S, double *cut,
+                            int64_t *best_agent, void *stream);
+
 /* Per-agent Ising energy s^T J s - h.s of the chosen spins (FP64, device). */
 int sis_solver_energy(sis_solver *s, int which, double *energy);
 

@@ -116,6 +116,8 @@ _SIGS = {
     "sis_solver_advance": (None, [_P, C.c_int64, C.c_int64, C.c_int64]),
     "sis_solver_get_spins": (None, [_P, C.c_int, _I32P]),
     "sis_solver_maxcut_cuts": (None, [_P, _DP, _I64P]),
+    "sis_maxcut_cuts_noalloc": (None, [_P, _P, _P, C.c_int64, _P, C.c_int64, C.c_double, _P, _P,
+                                       _P]),
     "sis_solver_energy": (None, [_P, C.c_int, _DP]),
     "sis_solver_agent_state": (None, [_P, _I32P, _I32P, _I32P]),
     "sis_solver_tsp_lengths": (None, [_P, _P, _DP, _I64P]),
