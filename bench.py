@@ -317,6 +317,61 @@ def small_configs(a):
                                   valid=int(len(ok)), best=float(ok.min()) if len(ok) else None,
                                   **tsp_energies(s, lens)))
                 s.close()
+    elif a.workload == "tsp_valid":
+        # Valid-tour rate of the one-hot TSP QUBO against the number of cities
+        # and the coordinate span, with the reference's fixed coupling 0.002
+        # (src/ising_tsp.cu:61) and its alternative 0.5 / (sigma sqrt N)
+        # (src/ising_tsp.cu:62-83): bSB FP32, the reference's 10,000 agents,
+        # 10,000 steps, dt 0.1, window every 1,000 steps; structured product
+        # on resident workgroups (tile 3).
+        T = a.steps if a.steps != 20000 else 10000
+        A = a.agents if a.agents != 256 else 10000
+        for n in (8, 10, 12, 14, 16, 20, 24, 32, 48):
+            for span in (100, 1000):
+                t = sis.Tsp.generate(n, span, 1, a.seed + n)
+                for rule in ("fixed", "scaled"):
+                    xi0 = 0.002 if rule == "fixed" else t.xi0()
+                    s = sis.Solver.from_tsp(t, dict(agents=A, form=sis.FORM_F32, dynamics=sis.BSB,
+                                                    dt=0.1, xi0=xi0, structured=1, tile=3,
+                                                    use_graph=1))
+                    s.init_random(a.seed)
+                    t0 = time.perf_counter()
+                    info = s.run(T, 0)
+                    wall = time.perf_counter() - t0
+                    lens, _ = s.tsp_lengths(t)
+                    ok = lens[lens >= 0]
+                    lines.append(dict(workload=f"tsp bSB f32 structured resident, {A} agents, "
+                                               f"{T} steps, dt 0.1", n=n, span=span,
+                                      xi0_rule=rule, xi0=xi0, valid=int(len(ok)),
+                                      valid_fraction=len(ok) / A,
+                                      best=float(ok.min()) if len(ok) else None,
+                                      mean=float(ok.mean()) if len(ok) else None,
+                                      random_tour=t.random_length(a.seed),
+                                      steps_executed=info["steps_executed"], wall_s=wall,
+                                      **tsp_energies(s, lens)))
+                    s.close()
+        # The same against c = xi0 n max(d), the scale of xi0 times the field
+        # (h ~ (n - 2)(B + C) / 2 with B = C = max d): 1,000-span instances.
+        for n in (16, 32, 48, 64):
+            t = sis.Tsp.generate(n, 1000, 1, a.seed + n)
+            for c in (10, 15, 20, 30, 45, 60, 90):
+                xi0 = c / (n * t.maxd)
+                s = sis.Solver.from_tsp(t, dict(agents=A, form=sis.FORM_F32, dynamics=sis.BSB,
+                                                dt=0.1, xi0=xi0, structured=1, tile=3,
+                                                use_graph=1))
+                s.init_random(a.seed)
+                info = s.run(T, 0)
+                lens, _ = s.tsp_lengths(t)
+                ok = lens[lens >= 0]
+                lines.append(dict(workload=f"tsp bSB f32 structured resident, {A} agents, "
+                                           f"{T} steps, dt 0.1", n=n, span=1000,
+                                  xi0_rule=f"c/(n maxd), c={c}", xi0=xi0, valid=int(len(ok)),
+                                  valid_fraction=len(ok) / A,
+                                  best=float(ok.min()) if len(ok) else None,
+                                  mean=float(ok.mean()) if len(ok) else None,
+                                  random_tour=t.random_length(a.seed),
+                                  steps_executed=info["steps_executed"], **tsp_energies(s, lens)))
+                s.close()
     elif a.workload == "powerlaw":
         # Chung-Lu N = 2^20, mean degree 8, exponent 2.1 (largest row ~1.2e5
         # entries), +-1 weights, 256 agents, dSB dt 0.5; without and with the
@@ -478,7 +533,7 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--workload", default="regular2p20",
                     choices=["regular2p20", "er800", "torus100", "tsp48", "powerlaw", "tsp_struct",
-                             "serving"],
+                             "tsp_valid", "serving"],
                     help="regular2p20: the headline (random 4-regular graph, N = 2^20); the "
                          "others print one JSON line per configuration")
     ap.add_argument("--instances", type=int, default=4096)
@@ -499,7 +554,7 @@ def main():
             a.steps = 5000
         serving(a)
         return
-    if a.workload in ("er800", "torus100", "tsp48", "powerlaw", "tsp_struct"):
+    if a.workload in ("er800", "torus100", "tsp48", "powerlaw", "tsp_struct", "tsp_valid"):
         small_configs(a)
         return
 

@@ -154,6 +154,14 @@ int sis_tsp_export(const sis_tsp *t, double *x, double *y, double *dist, double 
  * h_(i,k) = -(A D_k + (n-2)(B+C)) / 2 (src/kernels.cu:591-592). */
 int sis_tsp_qubo(const sis_tsp *t, sis_csr **J, double *field);
 
+/* The reference's alternative TSP coupling scale (the optimal_xi0 = false
+ * branch of Ising_TSP::reset_initialization, src/ising_tsp.cu:62-83; the
+ * default branch is xi0 = 0.002): xi0 = 0.5 / (sigma sqrt N) with N = n^2 and
+ * sigma = sqrt(sum of the squared stored QUBO entries / (N^2 - 1)) -- the
+ * uncentred second moment (the reference computes the mean and drops it).
+ * Evaluated in closed form from the distances, with N^2 in 64 bits. */
+int sis_tsp_xi0(const sis_tsp *t, double *xi0);
+
 /* Cyclic length of a tour (n city indices) from the packed table
  * (Ising_TSP::calculate_distance_of_tour, src/ising_tsp.cu:272-287). */
 int sis_tsp_tour_length(const sis_tsp *t, const int32_t *tour, double *length);

@@ -19,7 +19,9 @@
 //     would not reproduce the value exactly, then with 17;
 //   - the seed comes from SIS_SEED (default: the clock, as srand(time(0))).
 // Extra knobs (environment, so the argv layout is unchanged):
-//   SIS_FORM = f32 | f64 | packed | bf16   precision / gather form (f32)
+//   SIS_FORM = f32 | f64 | packed | bf16 | q12
+//                                          precision / state form (f32); q12 =
+//                                          12-bit X/Y codes, MaxCut dSB only
 //   SIS_DYNAMICS = dsb | bsb               dynamics (MaxCut dsb, TSP bsb)
 //   SIS_EUC_ROUND = 0 | 1                  EUC_2D truncation (0, reference) or nint
 //   SIS_GRAPH = 0 | 1 | 2                  hipGraph per window period / whole run
@@ -28,6 +30,9 @@
 //   SIS_SPLIT = <int>                      row-split threshold (0 off, -1 tuned)
 //   SIS_TUNE = 0 | 1                       autotune the tile before the run
 //   SIS_STRUCTURED = 0 | 1                 TSP: structured J x, no QUBO matrix
+//   SIS_TSP_XI0_RULE = fixed | scaled      TSP coupling: 0.002 (the reference's
+//                                          default) or 0.5 / (sigma sqrt N), its
+//                                          optimal_xi0 = false branch
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -92,6 +97,7 @@ void apply_env(sis_params &p) {
     if (s == "f64") p.form = SIS_FORM_F64;
     else if (s == "packed") p.form = SIS_FORM_PACKED;
     else if (s == "bf16") p.form = SIS_FORM_BF16;
+    else if (s == "q12") p.form = SIS_FORM_Q12;
     else p.form = SIS_FORM_F32;
   }
   if (const char *d = env("SIS_DYNAMICS")) p.dynamics = std::string(d) == "bsb" ? SIS_BSB : SIS_DSB;
@@ -160,6 +166,13 @@ int traveling_salesman(const Opts &o, const std::string &ifile, const std::strin
   sis_params_default(SIS_TSP, &p);
   p.agents = o.agents;
   p.dt = o.dt;
+  if (const char *r = env("SIS_TSP_XI0_RULE")) {
+    if (std::strcmp(r, "scaled") == 0) {
+      if (sis_tsp_xi0(t, &p.xi0)) return die("computing the coupling scale");
+    } else if (std::strcmp(r, "fixed") != 0) {
+      return die("SIS_TSP_XI0_RULE must be fixed or scaled");
+    }
+  }
   apply_env(p);
   if (const char *st = env("SIS_STRUCTURED")) p.structured = std::atoi(st) != 0;
   sis_solver *s = nullptr;  // builds the QUBO (or, structured, no matrix) and the field

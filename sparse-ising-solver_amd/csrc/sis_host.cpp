@@ -601,6 +601,35 @@ int sis_tsp_qubo(const sis_tsp *t, sis_csr **J, double *field) {
   return SIS_OK;
 }
 
+int sis_tsp_xi0(const sis_tsp *t, double *xi0) {
+  if (!t || !xi0) return fail(SIS_ERR_ARG, "sis_tsp_xi0: null argument");
+  const int64_t n = t->n;
+  if (n < 2) return fail(SIS_ERR_ARG, "sis_tsp_xi0: needs at least 2 cities");
+  // Sum of squares of the QUBO's stored entries (src/ising_tsp.cu:74-77) by
+  // rule (src/tsp_gpu.cu:85-106): i != j, k = l: C/4 (n^2 (n-1) entries);
+  // i = j, k != l: B/4 (n^2 (n-1)); the diagonal (B+C)/4 (n^2); adjacent
+  // positions (a(n) of them: 2 for n >= 3, 1 for n = 2), k != l:
+  // A d_kl / 8, i.e. n a(n) times both orders of every city pair.  Zero
+  // entries are not stored and add nothing.  Every square is a multiple of
+  // 1/64, so the sum is exact in any order while below 2^47.
+  const double A = 1, B = t->maxd, C = t->maxd, nn = (double)n;
+  double pairs = 0.0;
+  for (int64_t c1 = 0; c1 < n; c1++)
+    for (int64_t c2 = c1 + 1; c2 < n; c2++) {
+      const double v = A * t->dist[(size_t)tri(c1, c2, n)] / 8;
+      pairs += v * v;
+    }
+  const double adj = n >= 3 ? 2.0 : 1.0;
+  const double sq = nn * nn * (nn - 1) * ((C / 4) * (C / 4) + (B / 4) * (B / 4)) +
+                    nn * nn * ((B + C) / 4) * ((B + C) / 4) + 2.0 * nn * adj * pairs;
+  // sigma = sqrt(sum / (N^2 - 1)) with N = n^2 rows (64-bit N^2; the
+  // reference's int product overflows for n >= 216), xi0 = 0.5 / (sigma sqrt N)
+  const double N = nn * nn;
+  const double sigma = std::pow(sq / (double)(n * n * n * n - 1), 0.5);
+  *xi0 = 0.5 / (sigma * std::pow(N, 0.5));
+  return SIS_OK;
+}
+
 int sis_tsp_tour_length(const sis_tsp *t, const int32_t *tour, double *length) {
   if (!t || !tour || !length) return fail(SIS_ERR_ARG, "sis_tsp_tour_length: null argument");
   const int64_t n = t->n;

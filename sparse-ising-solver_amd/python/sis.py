@@ -93,6 +93,7 @@ _SIGS = {
     "sis_tsp_export": (None, [_P, _DP, _DP, _DP, _DP]),
     "sis_tsp_qubo": (None, [_P, C.POINTER(_P), _DP]),
     "sis_tsp_tour_length": (None, [_P, _I32P, _DP]),
+    "sis_tsp_xi0": (None, [_P, _DP]),
     "sis_tsp_random_length": (None, [_P, C.c_uint64, _DP]),
     "sis_tsp_write": (None, [_P, C.c_char_p]),
     "sis_tsp_free": ("void", [_P]),
@@ -296,6 +297,12 @@ class Tsp:
         v = C.c_double()
         _ck(lib().sis_tsp_tour_length(self.h, _ptr(tour, C.c_int32), C.byref(v)),
             "sis_tsp_tour_length")
+        return v.value
+
+    def xi0(self):
+        """The reference's alternative coupling scale 0.5 / (sigma sqrt N)."""
+        v = C.c_double()
+        _ck(lib().sis_tsp_xi0(self.h, C.byref(v)), "sis_tsp_xi0")
         return v.value
 
     def random_length(self, seed):

@@ -73,10 +73,32 @@ def test_maxcut_line_matches_oracle(tmp_path, fused, knobs):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("knobs", [{}, {"SIS_STRUCTURED": "1", "SIS_DYNAMICS": "dsb"}])
+def test_maxcut_line_q12_form_matches_oracle(tmp_path):
+    """SIS_FORM=q12 (12-bit X/Y codes): the line's cut statistics equal the
+    oracle's narrow-state solve from the same seed."""
+    J = sis.Csr.gen_er(300, 0.05, sis.W_PM1, 3)
+    mtx = tmp_path / "g.mtx"
+    J.write_mtx(mtx)
+    out = tmp_path / "out.csv"
+    A, T, dt, seed = 128, 1500, 0.5, 99
+    r = run(["0", mtx, out, "maxcut", "1", "1", "1", dt, T, A],
+            env={"SIS_SEED": str(seed), "SIS_FORM": "q12"})
+    assert r.returncode == 0, r.stderr
+    f = _fields(out.read_text())
+    rp, col, val = J.arrays()
+    pb = orc.Problem(rp, col, val, S=J.sum)
+    X, Y = orc.init_state(seed, orc.MAXCUT, J.n, A)
+    res = orc.solve_q12(pb, X, Y, T, 1, P=100, K=25, dt=dt, xi0=J.xi0())
+    assert [float(v) for v in f[1:6]] == list(orc.cut_stats(orc.cut(pb, res["cur"])))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("knobs", [{}, {"SIS_STRUCTURED": "1", "SIS_DYNAMICS": "dsb"},
+                                   {"SIS_TSP_XI0_RULE": "scaled"}])
 def test_tsp_line_matches_oracle(tmp_path, knobs):
-    """bSB through the QUBO's CSR (the reference's TSP); and dSB through the
-    structured product, whose trajectory is bit-identical to the CSR's."""
+    """bSB through the QUBO's CSR (the reference's TSP); dSB through the
+    structured product, whose trajectory is bit-identical to the CSR's; and
+    the coupling 0.5 / (sigma sqrt N) in place of 0.002."""
     src = os.path.join(INPUTS, "euc8.tsp")
     out = tmp_path / "out.csv"
     A, T, dt, seed = 256, 2000, 0.1, 5
@@ -89,7 +111,8 @@ def test_tsp_line_matches_oracle(tmp_path, knobs):
     pb = orc.Problem(rp, col, val, field=h, ncities=8)
     X, Y = orc.init_state(seed, orc.TSP, 64, A)
     dyn = orc.DSB if knobs.get("SIS_DYNAMICS") == "dsb" else orc.BSB
-    res = orc.solve(pb, orc.TSP, dyn, X, Y, T, 0, P=T // 10, K=50, dt=dt, xi0=0.002,
+    xi0 = t.xi0() if knobs.get("SIS_TSP_XI0_RULE") == "scaled" else 0.002
+    res = orc.solve(pb, orc.TSP, dyn, X, Y, T, 0, P=T // 10, K=50, dt=dt, xi0=xi0,
                     coupling=1.005)
     _, _, d, _ = t.arrays()
     lens = np.array([orc.tsp_tour_length(8, d, res["fin"], a)[0] for a in range(A) if res["valid"][a] > 0])
@@ -98,3 +121,10 @@ def test_tsp_line_matches_oracle(tmp_path, knobs):
         assert float(f[1]) == lens.min() and float(f[2]) == lens.max()
     assert r.stdout.startswith("8\nEUC_2D\n\n")
     assert "traveling salesman distance from random spin assignment" in r.stdout
+
+
+def test_tsp_xi0_rule_rejects_unknown_value(tmp_path):
+    src = os.path.join(INPUTS, "euc8.tsp")
+    r = run(["0", src, tmp_path / "o.csv", "tsp", "0", "1", "0", "0.1", "100", "64"],
+            env={"SIS_TSP_XI0_RULE": "optimal"})
+    assert r.returncode == 1 and "SIS_TSP_XI0_RULE must be fixed or scaled" in r.stderr

@@ -176,6 +176,39 @@ def test_tsp48_whole_solve_on_agent_sample(form, prec):
     assert np.array_equal(E[cols], orc.energy_field(pb, r["cur"]))
 
 
+def test_tsp16_valid_tours_bit_exact_on_agent_sample():
+    """The valid-tour path, where the n = 48 instance ends with none: n = 16 on
+    a 1,000-span instance with xi0 = 30 / (n max d), at which nearly every
+    agent ends on a valid tour.  bSB FP32 over the CSR, 1024 agents, 10,000
+    steps with the window every 1,000: valid_list, final spins and the tour
+    lengths of every 16th agent equal the oracle's, most of them valid."""
+    n = 16
+    t = sis.Tsp.generate(n, 1000, 1, 2024 + n)
+    J, h = t.qubo()
+    rp, col, val = J.arrays()
+    pb = orc.Problem(rp, col, val, field=h, ncities=n)
+    A, T, dt, xi0 = 1024, 10000, 0.1, 30.0 / (n * t.maxd)
+    s = sis.Solver(J, dict(problem=sis.TSP, agents=A, form=sis.FORM_F32, dynamics=sis.BSB,
+                           dt=dt, xi0=xi0, ncities=n, use_graph=1), field=h)
+    s.init_random(3)
+    assert s.run(T, 0)["steps_executed"] == T
+    cols = np.arange(0, A, 16)
+    X, Y = orc.init_state(3, orc.TSP, n * n, A)
+    X, Y = np.ascontiguousarray(X[:, cols]), np.ascontiguousarray(Y[:, cols])
+    r = orc.solve(pb, orc.TSP, orc.BSB, X, Y, T, 0, P=T // 10, K=50, dt=dt, xi0=xi0,
+                  coupling=1.005, precision="f32")
+    assert np.array_equal(s.get_state()[0][:, cols], X.astype(np.float64))
+    _, _, vs = s.agent_state()
+    assert np.array_equal(vs[cols], r["valid"])
+    assert np.array_equal(s.spins(sis.SPINS_FINAL)[:, cols], r["fin"])
+    lens, _ = s.tsp_lengths(t)
+    _, _, d, _ = t.arrays()
+    want = np.array([orc.tsp_tour_length(n, d, r["fin"], k)[0] if r["valid"][k] > 0 else -1.0
+                     for k in range(len(cols))])
+    assert np.array_equal(lens[cols], want)
+    assert (want >= 0).sum() >= len(cols) // 2, (want >= 0).sum()
+
+
 def test_tsp48_resident_structured_bsb_within_bars():
     """Resident structured tile (bSB FP32, 1024 agents) against the CSR oracle
     on a sample of agents: one step within 1e-6 relative; 1000 steps of the

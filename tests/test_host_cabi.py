@@ -196,3 +196,34 @@ def test_tsp_structured_identity(n, seed):
         v = np.roll(G, 1, axis=0) + np.roll(G, -1, axis=0)  # g(i-1, .) + g(i+1, .)
         S = C / 4 * G.sum(axis=0)[None, :] + B / 4 * G.sum(axis=1)[:, None] + (v @ D.T) / 8
         np.testing.assert_allclose(S.ravel(), M @ g, rtol=1e-12, atol=1e-9)
+
+
+@pytest.mark.parametrize("n", [2, 3, 4, 5, 7, 9])
+@pytest.mark.parametrize("seed,span,rnd", [(1, 100, 0), (4, 1000, 1), (9, 13, 0)])
+def test_tsp_scaled_xi0_against_dense_qubo(n, seed, span, rnd):
+    """The reference's alternative coupling (src/ising_tsp.cu:62-83):
+    0.5 / (sigma sqrt N), sigma^2 = sum of squared QUBO entries / (N^2 - 1),
+    recomputed with numpy over the oracle's dense n^2 x n^2 matrix (and over the
+    stored CSR entries), so a wrong entry count (the adjacent positions of
+    n = 2 and n >= 3, the diagonal) or a centred moment would fail."""
+    import oracle as orc
+
+    t = sis.Tsp.generate(n, span, rnd, seed)
+    _, _, d, _ = t.arrays()
+    D, maxd = orc.tsp_totals(n, d)
+    dense = orc.tsp_qubo_dense(n, d, maxd)
+    N = n * n
+    want = 0.5 / (np.sqrt(np.sum(dense ** 2) / (N * N - 1)) * np.sqrt(N))
+    np.testing.assert_allclose(t.xi0(), want, rtol=1e-15)
+    J, _ = t.qubo()
+    _, _, val = J.arrays()
+    assert np.sum(val ** 2) == np.sum(dense ** 2)  # multiples of 1/64: exact
+    # the mean the reference computes but drops would change the value
+    cen = 0.5 / (np.sqrt((np.sum((dense - dense.mean()) ** 2)) / (N * N - 1)) * np.sqrt(N))
+    assert t.xi0() != cen
+
+
+def test_tsp_scaled_xi0_rejects_one_city():
+    t = sis.Tsp.from_coords(np.array([1.0]), np.array([2.0]))
+    with pytest.raises(sis.SisError):
+        t.xi0()
