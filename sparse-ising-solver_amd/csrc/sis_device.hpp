@@ -189,17 +189,19 @@ __device__ __forceinline__ void st_state(uint32_t *p, uint32_t v) {
 // quad) in three dwords, bytes 0..2 of units 0..3 in order:
 //   d0 = u0 | u1 << 24,  d1 = u1 >> 8 | u2 << 16,  d2 = u2 >> 16 | u3 << 8.
 //
-// Decoding and rounding go through M = 1.5 2^23 (bits 0x4b400000, low 12 bits
-// zero): M + k for an integer |k| < 2^22 is exact with bits(M) + k, and for
-// |v| < 2^22, fma(v, s, M + 2048) with v s exact rounds v s + M + 2048 to an
-// integer (ulp 1, ties to even; the constant is even), whose low 12 bits are
-// the biased code round(v s) + 2048.
+// Decoding and rounding go through magic constants.  X: M = 2^23 (bits
+// 0x4b000000): M + k for an integer 0 <= k < 2^23 is exact with bits
+// bits(M) + k, so bits 12..23 are zero for k < 4096; and for v s + 2048 in
+// [0, 2^23), fma(v, s, M + 2048) rounds v s + M + 2048 to an integer (ulp 1,
+// ties to even), whose low 12 bits are the biased code round(v s) + 2048.
 
 // Per-launch constants: step coefficients; the Y code scale 2^sy, the
-// largest |y| with a code (2047 2^-sy); the decoding offsets.  All are small
+// largest |y| with a code (2047 2^-sy); the decoding offsets; the field mask,
+// held in a register so that a field's decode is one and-or.  All are small
 // integers times powers of two, so every product below is exact.
 struct Q12K {
   float c0, c1, c2, yscale, ylim, yinv, xoff, yoff;
+  uint32_t mask;
 };
 
 __device__ __forceinline__ Q12K q12_consts(float c0, float c1, float c2, int qshift) {
@@ -210,8 +212,11 @@ __device__ __forceinline__ Q12K q12_consts(float c0, float c1, float c2, int qsh
   k.yscale = __builtin_ldexpf(1.0f, qshift);
   k.yinv = __builtin_ldexpf(1.0f, -qshift);
   k.ylim = 2047.0f * k.yinv;
-  k.xoff = -(12582912.0f + 2048.0f) * (1.0f / 1024.0f);  // -(M + 2048) 2^-10
-  k.yoff = -(12582912.0f + 2048.0f) * k.yinv;            // -(M + 2048) 2^-sy
+  k.xoff = -(8388608.0f + 2048.0f) * (1.0f / 1024.0f);  // -(2^23 + 2048) 2^-10
+  k.yoff = -(8388608.0f + 2048.0f) * k.yinv;            // -(2^23 + 2048) 2^-sy
+  uint32_t mask = 0xfffu;
+  asm volatile("" : "+v"(mask));
+  k.mask = mask;
   return k;
 }
 
@@ -223,11 +228,13 @@ __device__ __forceinline__ void q12_units(uint32_t d0, uint32_t d1, uint32_t d2,
   u[3] = d2 >> 8;                                // bits 72..95
 }
 
-// x = (field - 2048) 2^-10 as fma(M + field, 2^-10, -(M + 2048) 2^-10): both
-// terms and the result are exact.  Likewise y with 2^-sy.
+// x = (field - 2048) 2^-10 as fma(2^23 + field, 2^-10, -(2^23 + 2048) 2^-10):
+// both terms and the result are exact.  Likewise y with 2^-sy.  (The Y field
+// is shifted down first: at bits 12..23 its top bit would be the float's
+// exponent bit.)
 __device__ __forceinline__ void q12_decode(uint32_t u, const Q12K &k, float &x, float &y) {
-  const float fx = __uint_as_float((u & 0xfffu) | 0x4b400000u);
-  const float fy = __uint_as_float(((u >> 12) & 0xfffu) | 0x4b400000u);
+  const float fx = __uint_as_float((u & k.mask) | 0x4b000000u);
+  const float fy = __uint_as_float(((u >> 12) & k.mask) | 0x4b000000u);
   x = __builtin_fmaf(fx, 1.0f / 1024.0f, k.xoff);
   y = __builtin_fmaf(fy, k.yinv, k.yoff);
 }
@@ -243,18 +250,20 @@ __device__ __forceinline__ void q12_unpack(uint32_t d0, uint32_t d1, uint32_t d2
 // Unit of an updated (x, y), bits 24..31 unspecified, and the element's
 // sign.  x 2^10 stays within [-1024, 1024] while |x| <= 1; y is clamped to
 // +-2047 codes first (clamping y or y 2^sy is the same, the scale being a
-// power of two).  Elements beyond the wall take the +-1025 X code and a 0 Y
-// code.  The sign: f = fma(x, 2^10, M + 2048) is monotonic in x and equals
-// M + 2048 exactly where the X code is 0, so the code (or, beyond the wall,
-// x) is negative iff f < M + 2048, i.e. iff bits(f) < bits(M + 2048) as
-// signed integers (a negative f has negative bits).  Branch-free.
+// power of two).  Both biased codes lie in [1, 4095], so the rounded floats'
+// bits 12..23 are zero and one shift-or joins them.  Elements beyond the
+// wall take the +-1025 X code and a 0 Y code.  The sign: f = fma(x, 2^10,
+// 2^23 + 2048) is monotonic in x and equals 2^23 + 2048 exactly where the X
+// code is 0, so the code (or, beyond the wall, x) is negative iff
+// f < 2^23 + 2048, i.e. iff bits(f) < bits(2^23 + 2048) as signed integers
+// (a negative f, for x below -8194, has negative bits).  Branch-free.
 __device__ __forceinline__ uint32_t q12_encode(float x, float y, const Q12K &k, bool &neg) {
-  const float kMb = 12582912.0f + 2048.0f;
+  const float kMb = 8388608.0f + 2048.0f;
   const uint32_t xb = __float_as_uint(__builtin_fmaf(x, 1024.0f, kMb));
   const float yc = __builtin_amdgcn_fmed3f(y, -k.ylim, k.ylim);
   const uint32_t yb = __float_as_uint(__builtin_fmaf(yc, k.yscale, kMb));
-  const uint32_t in = (xb & 0xfffu) | (yb << 12);  // bits 0..23 valid
-  neg = (int32_t)xb < (int32_t)0x4b400800u;
+  const uint32_t in = (yb << 12) | xb;  // bits 0..23 valid
+  neg = (int32_t)xb < (int32_t)0x4b000800u;
   // beyond the wall: X code 2048 -+ 1025, Y code 2048
   const uint32_t wall = neg ? (0x800000u | 1023u) : (0x800000u | 3073u);
   return __builtin_fabsf(x) > 1.0f ? wall : in;
@@ -281,7 +290,7 @@ __device__ __forceinline__ void sb_update_maxcut_sel(float &x, float &y, float F
                                                      float c1, float c2) {
   const bool conf = __builtin_fabsf(x) > 1.0f;
   const float yk = __builtin_fmaf(c2, F, y);
-  x = conf ? __builtin_copysignf(1.0f, x) : x;
+  x = __builtin_amdgcn_fmed3f(x, -1.0f, 1.0f);  // sgn(x) where |x| > 1
   y = conf ? 0.0f : yk;
   x = __builtin_fmaf(c0, y, x);
   y = __builtin_fmaf(c1, x, y);

@@ -327,7 +327,12 @@ __global__ __launch_bounds__(kBlock) void k_step_packed(StepArgs a) {
 // entry e with vector loads -- into the wavefront's LDS slice; each row then
 // adds its entries in CSR order from LDS broadcasts.  One dependent chain of
 // memory round trips per R rows instead of one per row: the scalar path is
-// bound by that latency, not by HBM.  Waves whose rows hold more entries take
+// bound by that latency, not by HBM.  The staged words carry the weight's
+// sign (complemented where v < 0), so each term is +-|v|: the same value as
+// +-v, and the sum never meets a signed zero (it starts at +0 and x + (-x) is
+// +0).  Where every staged weight is +-1 the sums are integers, counted from
+// the bits (two bit extracts and one three-way add per two entries and
+// agent) instead of three operations per term.  Waves whose rows hold more entries take
 // the scalar path.
 constexpr int kQ12Stage = 128;
 
@@ -390,39 +395,85 @@ __device__ __forceinline__ void q12_step_body(const StepArgs &a) {
     }
   }
   if constexpr (VG) {
-    // entry e: the low dwords of its four sign words, then the high dwords,
-    // so lanes 0-31 and 32-63 each read the 16 bytes holding their bits
-    __shared__ uint4 lsig[kWaves][kQ12Stage][2];
-    __shared__ float lval[kWaves][kQ12Stage];
-    const int wv = threadIdx.x >> 6, half = lane >> 5, sh = 31 - (lane & 31);
-    const int64_t rl = a.N - row0 < R ? a.N - row0 : R;
-    const int64_t kb = rp[row0], ne = rp[row0 + rl] - kb;
+    // Row pointers rp[row0 .. row0 + rl] in lanes 0..rl, loaded before any
+    // store so they stay in registers; their offsets from the first come
+    // back one row at a time by readlane.
+    const int rl = (int)(a.N - row0 < R ? a.N - row0 : R);
+    const int64_t rpl = rp[row0 + (lane < rl ? lane : rl)];
+    const int64_t kb = (int64_t)readlane64((uint64_t)rpl, 0);
+    const int64_t ne = (int64_t)readlane64((uint64_t)rpl, rl) - kb;
     if (ne <= kQ12Stage) {
+      const int off = (int)(rpl - kb);
+      // entry e: the low dwords of its four sign words, then the high
+      // dwords, so lanes 0-31 and 32-63 each read the 16 bytes holding their
+      // bits; all complemented where the weight is negative, so that a set
+      // bit means +|v|.  lval: -|v|.
+      __shared__ uint4 lsig[kWaves][kQ12Stage][2];
+      __shared__ float lval[kWaves][kQ12Stage];
+      const int wv = threadIdx.x >> 6, half = lane >> 5, bit = lane & 31, sh = 31 - bit;
+      bool unit = true;
       for (int64_t e = lane; e < ne; e += 64) {
         const int64_t c = col[kb + e];
+        const float v = val[kb + e];
+        const uint32_t f = v < 0.0f ? 0xffffffffu : 0u;
         const uint4 *sp = reinterpret_cast<const uint4 *>(S + c * W + w0);
         const uint4 s01 = sp[0], s23 = sp[1];  // words 0,1 and 2,3 as dword pairs
-        lsig[wv][e][0] = make_uint4(s01.x, s01.z, s23.x, s23.z);
-        lsig[wv][e][1] = make_uint4(s01.y, s01.w, s23.y, s23.w);
-        lval[wv][e] = val[kb + e];
+        lsig[wv][e][0] = make_uint4(s01.x ^ f, s01.z ^ f, s23.x ^ f, s23.z ^ f);
+        lsig[wv][e][1] = make_uint4(s01.y ^ f, s01.w ^ f, s23.y ^ f, s23.w ^ f);
+        lval[wv][e] = -__builtin_fabsf(v);
+        unit = unit && __builtin_fabsf(v) == 1.0f;
       }
+      const bool all_unit = __all(unit);
       // the wavefront's LDS writes complete before its reads below
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      // F += bit ? v : -v, as (-v) with its sign flipped where the bit is set
-      auto add = [&](float F[4], const uint4 &d, float v) {
-        const uint32_t nv = __float_as_uint(-v);
-        F[0] += __uint_as_float(nv ^ ((d.x << sh) & 0x80000000u));
-        F[1] += __uint_as_float(nv ^ ((d.y << sh) & 0x80000000u));
-        F[2] += __uint_as_float(nv ^ ((d.z << sh) & 0x80000000u));
-        F[3] += __uint_as_float(nv ^ ((d.w << sh) & 0x80000000u));
+      if (all_unit) {
+        // weights +-1: F = (entries with the bit set) - (the others), an
+        // integer, as the FP32 sum of +-1 terms is in any order
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          if (r >= rl) break;
+          const int k0 = __builtin_amdgcn_readlane(off, r);
+          const int k1 = __builtin_amdgcn_readlane(off, r + 1);
+          uint32_t cnt[4] = {0u, 0u, 0u, 0u};
+          int k = k0;
+          for (; k + 2 <= k1; k += 2) {
+            const uint4 d0 = lsig[wv][k][half], d1 = lsig[wv][k + 1][half];
+            cnt[0] += ((d0.x >> bit) & 1u) + ((d1.x >> bit) & 1u);
+            cnt[1] += ((d0.y >> bit) & 1u) + ((d1.y >> bit) & 1u);
+            cnt[2] += ((d0.z >> bit) & 1u) + ((d1.z >> bit) & 1u);
+            cnt[3] += ((d0.w >> bit) & 1u) + ((d1.w >> bit) & 1u);
+          }
+          if (k < k1) {
+            const uint4 d0 = lsig[wv][k][half];
+            cnt[0] += (d0.x >> bit) & 1u;
+            cnt[1] += (d0.y >> bit) & 1u;
+            cnt[2] += (d0.z >> bit) & 1u;
+            cnt[3] += (d0.w >> bit) & 1u;
+          }
+          const int deg = k1 - k0;
+          float F[4];
+#pragma unroll
+          for (int j = 0; j < 4; j++) F[j] = (float)(2 * (int)cnt[j] - deg);
+          const int64_t row = row0 + r;
+          q12_finish_row<NT>(d[r], F, kc, P + (row * Q + q) * 64, ps, Sn + row * W + w0, lane);
+        }
+        return;
+      }
+      // F += bit ? |v| : -|v|, as -|v| with its sign flipped where the bit is set
+      auto add = [&](float F[4], const uint4 &d, float nv) {
+        const uint32_t b = __float_as_uint(nv);
+        F[0] += __uint_as_float(b ^ ((d.x << sh) & 0x80000000u));
+        F[1] += __uint_as_float(b ^ ((d.y << sh) & 0x80000000u));
+        F[2] += __uint_as_float(b ^ ((d.z << sh) & 0x80000000u));
+        F[3] += __uint_as_float(b ^ ((d.w << sh) & 0x80000000u));
       };
 #pragma unroll
       for (int r = 0; r < R; r++) {
-        const int64_t row = row0 + r;
-        if (row >= a.N) break;
-        const int k0 = (int)(rp[row] - kb), k1 = (int)(rp[row + 1] - kb);
+        if (r >= rl) break;
+        const int k0 = __builtin_amdgcn_readlane(off, r);
+        const int k1 = __builtin_amdgcn_readlane(off, r + 1);
         float F[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         int k = k0;
         for (; k + 2 <= k1; k += 2) {
@@ -432,6 +483,7 @@ __device__ __forceinline__ void q12_step_body(const StepArgs &a) {
           add(F, d1, v1);
         }
         if (k < k1) add(F, lsig[wv][k][half], lval[wv][k]);
+        const int64_t row = row0 + r;
         q12_finish_row<NT>(d[r], F, kc, P + (row * Q + q) * 64, ps, Sn + row * W + w0, lane);
       }
       return;

@@ -59,3 +59,6 @@ def test_headline_line_has_the_contract_fields():
     assert abs(bf["best_cut_rel_diff_vs_plain_bsb"]) < 0.01
     sv = d["serving"]
     assert sv["alone_vs_mix_identical"] == "4/4" and sv["closed_loop_solves_per_s"] > 0
+    gr = d["graph"]
+    assert gr["use_graph"] == 2 and gr["steps_executed"] == 200 and gr["value"] > 0
+    assert gr["identical_to_direct"] is True

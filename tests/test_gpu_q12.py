@@ -90,17 +90,52 @@ def test_trajectory_bit_exact(tile, agents):
     assert (np.abs(X) > 1).any()  # the wall code occurred
 
 
+@pytest.mark.parametrize("tile", [4 | 4 << 8 | VG, 4 | 8 << 8 | VG | 1 << 16, 4 | 32 << 8 | VG])
+@pytest.mark.parametrize("agents", [96, 300])
+def test_unit_weight_waves_beside_real_weight_waves(tile, agents):
+    # +-1 weights except in rows 40..47 (2 and -0.75) and row 150 (an exact
+    # zero): waves whose staged weights are all +-1 count sign bits, the
+    # others add +-|v|; both equal the oracle's CSR-order FP32 sums
+    J0 = sis.Csr.gen_er(203, 0.04, sis.W_PM1, 12)
+    rp, col, val = J0.arrays()
+    rows = np.repeat(np.arange(203), np.diff(rp))
+    val = val.astype(np.float64).copy()
+    sel = (rows >= 40) & (rows < 48)
+    val[sel] = np.where(val[sel] > 0, 2.0, -0.75)
+    val[np.flatnonzero(rows == 150)[:1]] = 0.0
+    J = sis.Csr.from_coo(203, rows, col, val, mirror=False)
+    rp, col, val = J.arrays()
+    pb = orc.Problem(rp, col, val, S=J.sum)
+    T, dt = 400, 0.5
+    s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=agents, dt=dt, form=Q12, tile=tile))
+    assert s.p.tile == tile
+    sy = orc.q12_shift(dt)
+    X, Y = seeded(J.n, agents, 17, scale=0.9)
+    s.set_state(X.astype(np.float64), Y.astype(np.float64))
+    orc.q12_round(X, Y, sy)
+    for t0, n in ((0, 150), (150, 250)):
+        s.advance(t0, n, T)
+        orc.advance_q12(pb, X, Y, t0, n, T, dt, s.p.xi0, sy)
+        x, y = s.get_state()
+        assert np.array_equal(x, X.astype(np.float64)), (tile, agents, t0 + n)
+        assert np.array_equal(y, Y.astype(np.float64)), (tile, agents, t0 + n)
+    assert (np.abs(X) > 1).any()
+
+
 @pytest.mark.parametrize("tile", [4 | 8 << 8 | VG, 4 | 16 << 8 | VG | 1 << 16, 4 | 4 << 8 | VG])
-def test_staged_gathers_hub_and_empty_rows(tile):
+@pytest.mark.parametrize("unit", [False, True])
+def test_staged_gathers_hub_and_empty_rows(tile, unit):
     # 150 rows: a hub joined to rows 1..140 (its wave exceeds the staging
     # capacity and takes the scalar path), a sparse tail among rows 1..129,
     # rows 141..149 without entries (waves whose rows hold no entry at all),
-    # real weights
+    # real weights or +-1
     rng = np.random.default_rng(31)
     rows = [0] * 140 + list(rng.integers(1, 130, 80))
     cols = list(range(1, 141)) + list(rng.integers(1, 130, 80))
     keep = [(r, c) for r, c in zip(rows, cols) if r != c and r < 130 and c < 141]
     vals = rng.normal(size=len(keep))
+    if unit:
+        vals = np.sign(vals)
     J = sis.Csr.from_coo(150, [r for r, _ in keep], [c for _, c in keep], vals)
     rp, col, val = J.arrays()
     deg = np.diff(rp)
