@@ -294,7 +294,7 @@ def small_configs(a):
                               ms_per_step=r["wall"] * 1e3 / r["info"]["steps_executed"]))
     elif a.workload == "torus100":
         J = sis.Csr.gen_torus(100, 100, sis.W_PM1, a.seed)
-        for form in ("f32", "packed"):
+        for form in ("f32", "packed", "q12"):
             r = run_solver(J, dict(problem=sis.MAXCUT, agents=1024, dt=0.5, form=sis.FORMS[form],
                                    use_graph=1), a.steps if a.steps != 20000 else 10000, 1, a.seed)
             ms = r["info"]["step_kernel_ms"] / max(r["info"]["steps_executed"], 1)

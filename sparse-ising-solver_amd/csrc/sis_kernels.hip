@@ -1402,9 +1402,10 @@ void launch_tsp_struct_energy(hipStream_t st, const uint64_t *spins, const doubl
 // Pair q = row * G + agent is owned by thread q mod 1024 (register slot
 // q / 1024), so the G agents of a row sit in consecutive lanes of one
 // wavefront.
-// The gather source holds each term ready to add: slot q the term of a +1
-// weight on pair q, slot q + N G that of a -1 weight, and G zero slots at
-// 2 N G for the padding.  dSB: int8 -sgn(x) (+1 where x has its sign bit set),
+// The gather source holds each term ready to add.  dSB: byte slot 2q the term
+// of a +1 weight on pair q, 2q + 1 that of a -1 weight (one 2-byte write per
+// pair); bSB: float slot q and q + N G (split, so that graphs with only +1
+// weights still read every LDS bank).  2G zero slots at 2 N G pad the rows.  dSB: int8 -sgn(x) (+1 where x has its sign bit set),
 // summed as integers; bSB: float -(x * coupling), so F + slot equals the row
 // kernels' fma(-v, x * coupling, F).  The matrix is in LDS in ELL groups of the
 // R = 64 / G rows of one wavefront register slot: entry k of row g R + l is
@@ -1429,7 +1430,7 @@ __global__ __launch_bounds__(kResBlock) void k_resident(StepArgs a, ResArgs r) {
   unsigned char *snap = s_raw + r.snap_off;         // two buffers of r.snap_bytes
   const uint2 *__restrict__ gell = reinterpret_cast<const uint2 *>(r.ell);
   for (int k = t; k < (r.ell_words >> 2); k += kResBlock) sent[k] = gell[k];
-  if (t < G) {  // padding slots of both buffers
+  if (t < 2 * G) {  // padding slots of both buffers
     if (GATHER == G_DSB) {
       reinterpret_cast<int8_t *>(snap)[2 * NG + t] = 0;
       reinterpret_cast<int8_t *>(snap + r.snap_bytes)[2 * NG + t] = 0;
@@ -1474,10 +1475,8 @@ __global__ __launch_bounds__(kResBlock) void k_resident(StepArgs a, ResArgs r) {
       const int q = t + kResBlock * m;
       if (q < NG) {
         const int qs = SORTED ? qp[m] : q;
-        if (GATHER == G_DSB) {
-          const int8_t sg = signbit(x[m]) ? (int8_t)1 : (int8_t)-1;
-          reinterpret_cast<int8_t *>(buf)[qs] = sg;
-          reinterpret_cast<int8_t *>(buf)[qs + NG] = (int8_t)-sg;
+        if (GATHER == G_DSB) {  // bytes (sg, -sg): 0x01 0xff or 0xff 0x01
+          reinterpret_cast<uint16_t *>(buf)[qs] = signbit(x[m]) ? 0xff01u : 0x01ffu;
         } else {
           const float p = x[m] * coupling;
           reinterpret_cast<float *>(buf)[qs] = -p;
@@ -1503,7 +1502,7 @@ __global__ __launch_bounds__(kResBlock) void k_resident(StepArgs a, ResArgs r) {
     for (int m = 0; m < M; m++) {
       const int q = t + kResBlock * m;
       if (q < NG) {
-        const int ag = q & (G - 1);
+        const int ag = (GATHER == G_DSB ? 2 : 1) * (q & (G - 1));
         const uint2 *ent = sent + gbase[m] + rloc;
         const uint32_t w = gwid[m];
         float F;
@@ -1549,6 +1548,135 @@ __global__ __launch_bounds__(kResBlock) void k_resident(StepArgs a, ResArgs r) {
   }
 }
 
+// The dSB resident tile with the ELL in registers (tile bit 23): for
+// matrices whose ELL groups are at most CW chunks of four entries wide (the
+// torus: one), each thread loads the chunks of its own register slots once,
+// so LDS holds only the two int8 gather-slot buffers.  With no ELL in LDS two
+// 1024-thread workgroups fit on one CU (8 waves per SIMD: 64 VGPRs), and a term
+// takes one LDS round trip (the slot) instead of two (ELL entry, then slot).
+// Same sums in the same order as k_resident<G_DSB>: bit-identical.
+template <int M, int SORTED, int CW>
+__global__ __launch_bounds__(kResBlock, 8) void k_resident_reg(StepArgs a, ResArgs r) {
+  extern __shared__ __align__(16) unsigned char s_raw[];
+  if (a.ctrl->done) return;
+  const int N = (int)a.N, G = r.G, lg = r.lgG;
+  const int NG = N * G;
+  const int64_t lda = a.lda;
+  const int t = threadIdx.x, lane = t & 63;
+  const int64_t g0 = (int64_t)blockIdx.x * G;
+  unsigned char *snap = s_raw;  // two buffers of r.snap_bytes
+  if (t < 2 * G) {              // padding slots of both buffers
+    reinterpret_cast<int8_t *>(snap)[2 * NG + t] = 0;
+    reinterpret_cast<int8_t *>(snap + r.snap_bytes)[2 * NG + t] = 0;
+  }
+  const int R = 64 >> lg, rloc = lane >> lg;
+  const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+  const uint2 *__restrict__ gell = reinterpret_cast<const uint2 *>(r.ell);
+  // chunks past a group's width, and slots past the last group, read the
+  // zero slots: the gather below has no branch, so the reads of all slots
+  // can be in flight together
+  const uint32_t z = (uint32_t)(2 * NG), zz = z | z << 16;
+  uint2 ent[M][CW];
+#pragma unroll
+  for (int m = 0; m < M; m++) {
+    const int grp = wv + 16 * m;
+    const bool has = grp < r.ngroups;
+    const uint32_t gb = has ? r.gofs[grp] >> 2 : 0;
+    const uint32_t gw = has ? ((r.gofs[grp + 1] >> 2) - gb) / (uint32_t)R : 0;
+#pragma unroll
+    for (int c = 0; c < CW; c++)
+      ent[m][c] = (uint32_t)c < gw ? gell[gb + c * R + rloc] : make_uint2(zz, zz);
+  }
+  float x[M], y[M];
+  int qp[M];
+  const float *__restrict__ Xin = r.Xin;
+  float *Y = (float *)a.Y;
+#pragma unroll
+  for (int m = 0; m < M; m++) {
+    const int q = t + kResBlock * m;
+    x[m] = y[m] = 0.0f;
+    qp[m] = q;
+    if (q < NG) {
+      if (SORTED) qp[m] = ((int)r.perm[q >> lg] << lg) + (q & (G - 1));
+      const int64_t e = (int64_t)(qp[m] >> lg) * lda + g0 + (q & (G - 1));
+      x[m] = Xin[e];
+      y[m] = Y[e];
+    }
+  }
+  auto put_snapshot = [&](unsigned char *buf) {
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+      const int q = t + kResBlock * m;
+      if (q < NG) {
+        const int qs = SORTED ? qp[m] : q;
+        reinterpret_cast<uint16_t *>(buf)[qs] = signbit(x[m]) ? 0xff01u : 0x01ffu;
+      }
+    }
+  };
+  put_snapshot(snap);
+  __syncthreads();
+  const float *sc = (const float *)a.sched + 4 * (a.ctrl->base + a.offset);
+  float n0 = sc[0], n1 = sc[1], n2 = sc[2];
+  for (int st = 0; st < r.steps; st++) {
+    const float c0 = n0, c1 = n1, c2 = n2;
+    if (st + 1 < r.steps) {
+      n0 = sc[4 * (st + 1)];
+      n1 = sc[4 * (st + 1) + 1];
+      n2 = sc[4 * (st + 1) + 2];
+    }
+    const int8_t *src = reinterpret_cast<const int8_t *>(snap + (st & 1) * r.snap_bytes) +
+                        2 * (t & (G - 1));
+    // opaque each step: the entry extracts are redone per step (one SDWA add
+    // each) instead of being hoisted into 4 M registers of addresses
+#pragma unroll
+    for (int m = 0; m < M; m++)
+#pragma unroll
+      for (int c = 0; c < CW; c++) asm volatile("" : "+v"(ent[m][c].x), "+v"(ent[m][c].y));
+    // two slots at a time: their reads in flight together, then their update
+    // and the next step's gather slots (the other buffer); the scheduling
+    // barrier keeps batches apart so live ranges stay within 64 VGPRs
+    unsigned char *nxt = snap + ((st + 1) & 1) * r.snap_bytes;
+#pragma unroll
+    for (int m0 = 0; m0 < M; m0 += 2) {
+      int Fi[2] = {0, 0};
+#pragma unroll
+      for (int b = 0; b < 2; b++) {
+        if (m0 + b < M) {
+#pragma unroll
+          for (int c = 0; c < CW; c++) {
+            const uint2 e = ent[m0 + b][c];
+            Fi[b] += src[e.x & 0xffff] + src[e.x >> 16] + src[e.y & 0xffff] + src[e.y >> 16];
+          }
+        }
+      }
+      // pairs past N G update zeros and are neither stored nor gathered
+#pragma unroll
+      for (int b = 0; b < 2; b++)
+        if (m0 + b < M) sb_update_maxcut_sel(x[m0 + b], y[m0 + b], (float)Fi[b], c0, c1, c2);
+#pragma unroll
+      for (int b = 0; b < 2; b++) {
+        const int m = m0 + b, q = t + kResBlock * m;
+        if (m < M && q < NG) {
+          const int qs = SORTED ? qp[m] : q;
+          reinterpret_cast<uint16_t *>(nxt)[qs] = signbit(x[m]) ? 0xff01u : 0x01ffu;
+        }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    __syncthreads();
+  }
+  float *Xout = r.Xout;
+#pragma unroll
+  for (int m = 0; m < M; m++) {
+    const int q = t + kResBlock * m;
+    if (q < NG) {
+      const int64_t e = (int64_t)((SORTED ? qp[m] : q) >> lg) * lda + g0 + (q & (G - 1));
+      Xout[e] = x[m];
+      Y[e] = y[m];
+    }
+  }
+}
+
 int resident_pairs_max() { return kResBlock * 12; }  // the largest instance below: M = 12
 int resident_sorted_pairs_max() { return kResBlock * 8; }  // sorted instances up to M = 8
 
@@ -1557,6 +1685,25 @@ bool launch_resident(hipStream_t st, int gather, const StepArgs &a, const ResArg
   const dim3 grid((unsigned)(a.lda / r.G));
   const size_t shm = (size_t)r.snap_off + 2 * (size_t)r.snap_bytes;
   const int sorted = r.perm != nullptr;
+  if (r.ell_cw > 0) {
+    if (gather != G_DSB || r.snap_off != 0) return false;
+#define SIS_RESR(MM, SO, CW)                                                                 \
+  if (sorted == SO && r.ell_cw <= CW && NG <= (int64_t)kResBlock * MM) {                    \
+    allow_big_lds<k_resident_reg<MM, SO, CW>>();                                            \
+    hipLaunchKernelGGL((k_resident_reg<MM, SO, CW>), grid, dim3(kResBlock), shm, st, a, r); \
+    return true;                                                                            \
+  }
+    SIS_RESR(4, 0, 1)
+    SIS_RESR(8, 0, 1)
+    SIS_RESR(10, 0, 1)
+    SIS_RESR(12, 0, 1)
+    SIS_RESR(4, 0, 2)
+    SIS_RESR(4, 1, 1)
+    SIS_RESR(8, 1, 1)
+    SIS_RESR(4, 1, 2)
+#undef SIS_RESR
+    return false;
+  }
 #define SIS_RES(GA, MM, SO)                                                                  \
   if (gather == GA && sorted == SO && NG <= (int64_t)kResBlock * MM) {                      \
     allow_big_lds<k_resident<GA, MM, SO>>();                                                \

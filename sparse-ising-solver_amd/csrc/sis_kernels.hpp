@@ -117,16 +117,17 @@ void launch_tsp_struct_energy(hipStream_t st, const uint64_t *spins, const doubl
 // Resident small problems (k_resident): one 1024-thread workgroup per G
 // agents runs `steps` steps with X, Y in registers and the matrix and gather
 // source in LDS.  MaxCut fused step, FP32, weights +-1, N x G <= 12288.
-// The gather source has 2 N G + G slots: slot q (pair q = row G + agent) the
-// term of a +1 weight on pair q, q + N G that of a -1 weight, zeros from 2 N G
-// (dSB int8 -sgn x; bSB float -(x * coupling)).  The matrix is in ELL groups
+// The gather source has 2 N G + 2 G slots: the terms of a +1 and a -1 weight
+// on pair q = row G + agent at 2q and 2q + 1 (dSB, int8 -sgn x) or at q and
+// q + N G (bSB, float -(x * coupling)), zeros from 2 N G.  The matrix is in ELL groups
 // of R = 64 / G rows (the rows of one wavefront register slot), padded to
 // whole chunks of four entries: entry k of (sorted) row g R + l is uint16
 // number gofs[g] + 4 ((k / 4) R + l) + k % 4, the gather slot of its column
 // and weight sign for agent 0 (2 N G past the row's end).  With perm the rows
 // are in degree order (sorted row s is row perm[s]), so a group pads to
 // nearly equal degrees.  LDS: the ell_words entries, then two gather-slot
-// buffers of snap_bytes at snap_off.
+// buffers of snap_bytes at snap_off; with ell_cw > 0 (dSB, groups at most
+// ell_cw chunks wide) the ELL stays in registers and snap_off is 0.
 struct ResArgs {
   const float *Xin;  // X at the first step (plain: the snapshot buffer)
   float *Xout;       // X after the last step
@@ -137,6 +138,7 @@ struct ResArgs {
   int32_t G, lgG;    // agents per workgroup, log2 G
   int32_t steps;
   int32_t snap_off, snap_bytes;
+  int32_t ell_cw;  // > 0: dSB with the ELL in registers, groups at most ell_cw chunks wide
 };
 int resident_pairs_max();         // N x G limit of one workgroup
 int resident_sorted_pairs_max();  // the same with rows in degree order (perm)

@@ -67,6 +67,7 @@ struct sis_solver {
   // ELL copies of the matrix for the resident tiles, per log2 G
   bool ell_ok[4] = {false, false, false, false};
   int32_t ell_words[4] = {0, 0, 0, 0}, ell_groups[4] = {0, 0, 0, 0};
+  int32_t ell_maxw[4] = {0, 0, 0, 0};  // widest group, in chunks of four entries
   DevBuf<uint16_t> ell[4];
   DevBuf<uint32_t> gofs[4];
   DevBuf<uint16_t> ell_perm[4];  // rows in degree order, when the ELL uses it
@@ -193,23 +194,33 @@ static uint32_t q12_get(const std::vector<uint32_t> &P, int64_t N, int64_t Q, in
 // bits 21-22.  Fused MaxCut in FP32 (plain dSB / bSB, packed dSB), weights
 // +-1, N x G pairs per 1024-thread workgroup within the kernel's register
 // slots (12,288), the ELL matrix and two buffers of gather slots within 160 KB
-// of LDS.
+// of LDS.  Bit 23 (dSB only): the ELL in registers (k_resident_reg), for
+// groups one chunk wide (N G up to 12,288) or two (N G up to 4,096), with the
+// two gather-slot buffers within 80 KB so that two workgroups share a CU.
 constexpr int kResidentBit = 1 << 20;
+constexpr int kResRegBit = 1 << 23;
 
 static bool resident_layout(const sis_solver *s, int vec, ResArgs *r) {
-  if (!(vec & kResidentBit) || (vec & ~(kResidentBit | (3 << 21))) != 0) return false;
+  if (!(vec & kResidentBit) || (vec & ~(kResidentBit | (3 << 21) | kResRegBit)) != 0)
+    return false;
   const int lg = (vec >> 21) & 3, G = 1 << lg;
+  const bool reg = (vec & kResRegBit) != 0;
   if (s->prob != PROB_MAXCUT || s->structured || !s->pm1) return false;
   if (!(s->p.form == SIS_FORM_F32 || (s->p.form == SIS_FORM_PACKED && s->gather == G_DSB)))
     return false;
   if (s->N * G > resident_pairs_max() || s->lda % G != 0 || !s->ell_ok[lg])
     return false;
+  // register budget (64 VGPRs at two workgroups per CU): one chunk per row
+  // up to twelve register slots, two chunks up to four
+  if (reg && (s->gather != G_DSB || s->ell_maxw[lg] > 2 ||
+              (s->ell_maxw[lg] == 2 && s->N * G > 4 * 1024)))
+    return false;
   auto al16 = [](int64_t b) { return (b + 15) / 16 * 16; };
-  const int64_t off = al16(2 * (int64_t)s->ell_words[lg]);
+  const int64_t off = reg ? 0 : al16(2 * (int64_t)s->ell_words[lg]);
   // gather slots: +1 and -1 weight terms of every pair, then G zeros
-  const int64_t slots = 2 * s->N * G + G;
+  const int64_t slots = 2 * s->N * G + 2 * G;
   const int64_t snap = al16(s->gather == G_DSB ? slots : 4 * slots);
-  if (off + 2 * snap > 160 * 1024) return false;
+  if (off + 2 * snap > (reg ? 80 : 160) * 1024) return false;
   if (r) {
     r->G = G;
     r->lgG = lg;
@@ -220,6 +231,7 @@ static bool resident_layout(const sis_solver *s, int vec, ResArgs *r) {
     r->perm = s->ell_perm[lg].p;  // null: rows in natural order
     r->ngroups = s->ell_groups[lg];
     r->ell_words = s->ell_words[lg];
+    r->ell_cw = reg ? s->ell_maxw[lg] : 0;
   }
   return true;
 }
@@ -227,8 +239,10 @@ static bool resident_layout(const sis_solver *s, int vec, ResArgs *r) {
 // ELL groups of R = 64 / G rows for the resident tiles (only for matrices
 // whose entries fit the kernel's LDS at all), padded to whole chunks of four
 // entries: entry k of row g R + l at gofs[g] + 4 ((k / 4) R + l) + k % 4.  An
-// entry is the gather slot of agent 0: column G, plus N G for a -1 weight;
-// 2 N G (a zero slot) past a row's end.
+// entry is the gather slot of agent 0 -- dSB (byte slots, interleaved):
+// 2 column G, plus 1 for a -1 weight; bSB (float slots, split so that +1
+// weights alone still spread over every LDS bank): column G, plus N G for a
+// -1 weight -- and 2 N G (a zero slot) past a row's end.
 static int build_ell(sis_solver *s, const sis_csr *J) {
   if (!s->pm1 || s->N > resident_pairs_max() || s->prob != PROB_MAXCUT || 2 * s->Z > 160 * 1024) return SIS_OK;
   std::vector<int64_t> byd((size_t)s->N);  // rows by descending degree (stable)
@@ -237,7 +251,7 @@ static int build_ell(sis_solver *s, const sis_csr *J) {
   std::stable_sort(byd.begin(), byd.end(), [&](int64_t x, int64_t y) { return deg(x) > deg(y); });
   for (int lg = 0; lg < 4; lg++) {
     const int64_t G = int64_t(1) << lg, R = 64 >> lg, ng = (s->N + R - 1) / R;
-    if (2 * s->N * G + G > 65535) continue;  // slots must fit the 16-bit entries
+    if (2 * s->N * G + 2 * G > 65535) continue;  // slots must fit the 16-bit entries
     // group widths for rows in natural order and in degree order; the sorted
     // layout (kernel instances up to 8 register slots) when 10% smaller
     auto layout = [&](bool sorted, std::vector<uint32_t> &go) {
@@ -265,7 +279,9 @@ static int build_ell(sis_solver *s, const sis_csr *J) {
       for (int64_t k = J->rp[row]; k < J->rp[row + 1]; k++) {
         const int64_t kk = k - J->rp[row];
         e[(size_t)(go[(size_t)g] + 4 * ((kk / 4) * R + l) + kk % 4)] =
-            (uint16_t)(J->col[(size_t)k] * G + (J->val[(size_t)k] < 0 ? s->N * G : 0));
+            (uint16_t)(s->gather == G_DSB
+                           ? 2 * J->col[(size_t)k] * G + (J->val[(size_t)k] < 0 ? 1 : 0)
+                           : J->col[(size_t)k] * G + (J->val[(size_t)k] < 0 ? s->N * G : 0));
       }
     }
     int st;
@@ -280,6 +296,10 @@ static int build_ell(sis_solver *s, const sis_csr *J) {
     }
     s->ell_words[lg] = (int32_t)words;
     s->ell_groups[lg] = (int32_t)ng;
+    uint32_t maxw = 0;
+    for (int64_t g = 0; g < ng; g++)
+      maxw = std::max<uint32_t>(maxw, (go[(size_t)g + 1] - go[(size_t)g]) / (uint32_t)(4 * R));
+    s->ell_maxw[lg] = (int32_t)maxw;
     s->ell_ok[lg] = true;
   }
   return SIS_OK;
@@ -343,7 +363,10 @@ static std::vector<int> candidate_tiles(const sis_solver *s) {
     for (int v : {1, 2, 4})
       for (int nt : {0, 1}) c.push_back(v | (nt << 16));
   }
-  for (int lg = 0; lg < 4; lg++) c.push_back(kResidentBit | (lg << 21));
+  for (int lg = 0; lg < 4; lg++) {
+    c.push_back(kResidentBit | (lg << 21));
+    c.push_back(kResidentBit | (lg << 21) | kResRegBit);
+  }
   std::vector<int> ok;
   for (int v : c)
     if (tile_ok(s, v)) ok.push_back(v);

@@ -330,6 +330,96 @@ def test_resident_full_runs_with_windows(use_graph, form, lg):
     assert np.array_equal(s.cuts()[0], orc.cut(pb, r["cur"]))
 
 
+RREG = 1 << 23  # resident dSB tile with the ELL in registers (groups at most two chunks wide)
+
+
+def low_degree_graph(n, seed):
+    # +-1 weights, degrees 0..8 (ELL groups of zero, one and two chunks), the
+    # last rows empty; mixed enough that the degree-sorted layout applies
+    rng = np.random.default_rng(seed)
+    deg = np.zeros(n, int)
+    rows, cols = [], []
+    for _ in range(3 * n):
+        a, b = rng.integers(0, n - 9, 2)
+        if a != b and deg[a] < 8 and deg[b] < 8 and rng.random() < (0.9 if a % 3 else 0.2):
+            rows.append(a)
+            cols.append(b)
+            deg[a] += 1
+            deg[b] += 1
+    vals = rng.choice([-1.0, 1.0], len(rows))
+    J = sis.Csr.from_coo(n, rows, cols, vals)
+    rp, col, val = J.arrays()
+    return J, orc.Problem(rp, col, val, S=J.sum)
+
+
+@pytest.mark.parametrize("graph,form,lg,A", [("torus", "f32", 0, 64), ("torus", "packed", 2, 128),
+                                             ("regular", "f32", 1, 128), ("low", "packed", 0, 64),
+                                             ("low", "f32", 2, 64)])
+def test_resident_register_ell_bit_exact(graph, form, lg, A):
+    # ELL entries held in registers, gather slots alone in LDS: the same sums
+    # as the LDS tile; two launches of odd length flip the snapshot parity
+    if graph == "torus":
+        J = sis.Csr.gen_torus(40, 40, sis.W_PM1, 3)
+        rp, col, val = J.arrays()
+        pb = orc.Problem(rp, col, val, S=J.sum)
+    elif graph == "regular":
+        J = sis.Csr.gen_regular(1000, 4, sis.W_PM1, 4)
+        rp, col, val = J.arrays()
+        pb = orc.Problem(rp, col, val, S=J.sum)
+    else:
+        J, pb = low_degree_graph(700, 9)
+        assert np.diff(J.arrays()[0]).max() <= 8 and np.diff(J.arrays()[0])[-9:].max() == 0
+    T, dt = 300, 0.5
+    tile = RES | lg << 21 | RREG
+    s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=A, dt=dt, form=sis.FORMS[form], tile=tile))
+    assert s.p.tile == tile
+    X0, Y0 = seeded_state(J.n, A, 23)
+    s.set_state(X0, Y0)
+    s.advance(0, 137, T)
+    s.advance(137, 63, T)
+    X, Y = X0.astype(np.float32), Y0.astype(np.float32)
+    orc.advance(pb, orc.MAXCUT, orc.DSB, X, Y, 0, 200, T, dt, s.p.xi0, 1.0, "f32")
+    x, y = s.get_state()
+    assert np.array_equal(x, X.astype(np.float64)) and np.array_equal(y, Y.astype(np.float64))
+
+
+@pytest.mark.parametrize("use_graph", [0, 1, 2])
+@pytest.mark.parametrize("form", ["packed", "f32"])
+def test_resident_register_ell_torus_full_runs(use_graph, form):
+    # torus 100 x 100 at G = 1: ten register slots per thread, two workgroups
+    # per CU; windows with early exit and graph replay against the oracle
+    J = sis.Csr.gen_torus(100, 100, sis.W_PM1, 5)
+    rp, col, val = J.arrays()
+    pb = orc.Problem(rp, col, val, S=J.sum)
+    A, T, dt = 128, 1500, 0.5
+    tile = RES | RREG
+    s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=A, dt=dt, form=sis.FORMS[form], tile=tile,
+                           use_graph=use_graph, window_threshold=5))
+    assert s.p.tile == tile
+    s.init_random(7)
+    info = s.run(T, 1)
+    X, Y = orc.init_state(7, orc.MAXCUT, J.n, A)
+    r = orc.solve(pb, orc.MAXCUT, orc.DSB, X, Y, T, 1, P=100, K=5, dt=dt, xi0=s.p.xi0)
+    assert info["steps_executed"] == r["steps"]
+    assert np.array_equal(s.get_state()[0], X.astype(np.float64))
+    assert np.array_equal(s.spins(sis.SPINS_FINAL), r["fin"])
+    assert np.array_equal(s.cuts()[0], orc.cut(pb, r["cur"]))
+
+
+def test_resident_register_ell_rejected_where_it_does_not_apply():
+    J, _ = maxcut_problem(203, 0.08, sis.W_PM1, 21)  # rows wider than two chunks
+    assert np.diff(J.arrays()[0]).max() > 8
+    with pytest.raises(sis.SisError):
+        sis.Solver(J, dict(problem=sis.MAXCUT, agents=64, form=sis.FORM_PACKED, tile=RES | RREG))
+    L, _ = low_degree_graph(700, 9)  # two-chunk groups beyond 4,096 pairs (G = 8: 5,600)
+    with pytest.raises(sis.SisError):
+        sis.Solver(L, dict(problem=sis.MAXCUT, agents=64, form=sis.FORM_F32, tile=RES | 3 << 21 | RREG))
+    T = sis.Csr.gen_torus(40, 40, sis.W_PM1, 3)
+    with pytest.raises(sis.SisError):  # bSB keeps float slots: no register-ELL tile
+        sis.Solver(T, dict(problem=sis.MAXCUT, agents=64, form=sis.FORM_F32, dynamics=sis.BSB,
+                           tile=RES | RREG))
+
+
 def test_resident_tile_rejected_where_it_does_not_apply():
     J, _ = maxcut_problem(203, 0.04, sis.W_INT3, 21)  # weights outside +-1
     with pytest.raises(sis.SisError):
