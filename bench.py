@@ -425,6 +425,35 @@ def small_configs(a):
         print(json.dumps(ln), flush=True)
 
 
+def batch_stream(a):
+    """The batch engine's streaming step alone (the fallback of resident
+    chunks): the serving mix (N log-uniform in [64, 4096], ER mean degree 8,
+    weights {-3..3}\\{0}, 64 agents, dt 0.5) admitted whole, then `steps`
+    steps of each tile in --batch-tiles, direct launches, one JSON line each."""
+    rng = np.random.default_rng(a.seed)
+    A, count = 64, a.instances
+    sizes = np.floor(np.exp(rng.uniform(np.log(64), np.log(4096), count))).astype(int)
+    Js = [sis.Csr.gen_er(int(n), 8.0 / (n - 1), sis.W_INT3, 7000 + k) for k, n in enumerate(sizes)]
+    rows, nnz = int(sizes.sum()), int(sum(J.nnz for J in Js))
+    K = a.steps if a.steps != 20000 else 300
+    for tile in [int(t, 0) for t in a.batch_tiles.split(",") if t]:
+        b = sis.Batch(agents_max=A, max_instances=count, max_rows=rows, max_nnz=nnz,
+                      chunk_steps=100, tile=tile, eager=True)
+        for k, J in enumerate(Js):
+            b.submit(J, A, 0.5, 1 << 30, 0, seed=k)
+        b.advance(0)
+        b.advance(100)  # warm-up chunk
+        t0 = time.perf_counter()
+        b.advance(K)
+        wall = time.perf_counter() - t0
+        b.close()
+        state = rows * A * 16  # X and Y read and written, FP32
+        print(json.dumps(dict(workload=f"batch streaming step, {count} MaxCut instances "
+                                       f"({rows} rows, {nnz} stored entries), 64 agents",
+                              tile=hex(tile), steps=K, ms_per_step=wall * 1e3 / K,
+                              state_bytes=state, state_GBs=state / (wall / K) / 1e9)), flush=True)
+
+
 def serving_run(seed, count, T, chunk, eager, rate, seq_sample):
     """Batched serving of many small MaxCut dSB instances: N log-uniform in
     [64, 4096], Erdos-Renyi with mean degree 8, weights uniform in
@@ -563,7 +592,7 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--workload", default="regular2p20",
                     choices=["regular2p20", "er800", "torus100", "tsp48", "powerlaw", "tsp_struct",
-                             "tsp_valid", "serving"],
+                             "tsp_valid", "serving", "batch_stream"],
                     help="regular2p20: the headline (random 4-regular graph, N = 2^20); the "
                          "others print one JSON line per configuration")
     ap.add_argument("--instances", type=int, default=4096)
@@ -572,6 +601,8 @@ def main():
     ap.add_argument("--seq-sample", type=int, default=32)
     ap.add_argument("--eager", action="store_true",
                     help="serving: launch chunk kernels directly instead of replaying a hipGraph")
+    ap.add_argument("--batch-tiles", default="0x90004,0x2090004,0x2090008",
+                    help="batch_stream: comma-separated streaming tiles of the batch engine")
     a = ap.parse_args()
     a.also = [f for f in a.also.split(",") if f]
     if a.gpus != 1:
@@ -583,6 +614,9 @@ def main():
         if a.steps == 20000:
             a.steps = 5000
         serving(a)
+        return
+    if a.workload == "batch_stream":
+        batch_stream(a)
         return
     if a.workload in ("er800", "torus100", "tsp48", "powerlaw", "tsp_struct", "tsp_valid"):
         small_configs(a)

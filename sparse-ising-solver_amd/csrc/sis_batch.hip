@@ -49,9 +49,17 @@ struct BInst {
   int32_t intw;      // every weight an integer in [-3, 3] (bit-transpose path)
 };
 
+// A task: nrows consecutive rows of one instance from global row row0.  For
+// tasks of at most 8 rows, kb is the first row's entry in the global CSR and
+// off[r] the entry of row r relative to kb (off[nrows] = the task's entries),
+// so the pipelined step finds its column range from the record alone.  64
+// bytes: one scalar load.
 struct BTask {
   int32_t slot, nrows;
   int64_t row0;
+  int64_t kb;
+  int32_t off[9];
+  int32_t pad;
 };
 
 constexpr int kXcds = 8;  // MI355X: workgroups go to the 8 XCDs round-robin
@@ -66,6 +74,7 @@ struct BArgs {
   const float *val;
   const double *vald;
   float *X, *Y;
+  float *sink;  // 16 lda floats + 16 W words that nothing reads (pipelined step)
   uint64_t *S0, *S1, *cur, *fin;
   int32_t *stab, *bif;
   double *cuts;
@@ -488,6 +497,284 @@ __device__ __forceinline__ void batch_step_bt(const BArgs &b) {
 template <int R, int WT, int NT, int PF>
 __global__ __launch_bounds__(kBlock) void k_batch_step_bt(BArgs b) {
   batch_step_bt<R, WT, NT, PF>(b);
+}
+
+// Scalar loads of the task and instance records: the host writes the task
+// table and k_batch_post the instance records, never a step kernel, so they
+// may go through the scalar cache.  Scalar loads count on lgkmcnt, apart from
+// the vector memory counter, so they never wait for earlier vector stores.
+template <typename T>
+__device__ __forceinline__ T ld_const(const T *p) {
+  return *(const __attribute__((address_space(4))) T *)(p);
+}
+
+__device__ __forceinline__ BTask ld_task(const BTask *p) {
+  BTask t;
+  t.slot = ld_const(&p->slot);
+  t.nrows = ld_const(&p->nrows);
+  t.row0 = ld_const(&p->row0);
+  t.kb = ld_const(&p->kb);
+#pragma unroll
+  for (int i = 0; i < 9; i++) t.off[i] = ld_const(&p->off[i]);
+  t.pad = 0;
+  return t;
+}
+
+// Pipelined bit-transpose step (tile bit 25).  gfx9 retires vector loads and
+// stores in order on one counter, so in k_batch_step_bt every load issued
+// after a task's stores -- the next task's row ranges, columns and sign words
+// -- waits for those stores too: each task exposes the store latency plus a
+// chain of four dependent round trips.  Here, for task k, in this order:
+//   (1) task k's columns and weights (issued one task earlier) have arrived;
+//       issue its sign-word loads;
+//   (2) issue task k+1's state rows and pass-0 columns and weights, their
+//       addresses from its record alone (entry offsets in BTask);
+//   (3) issue task k-1's stores, held in registers since;
+//   (4) wait for task k's sign words -- older than (2) and (3), so no wait
+//       covers a store -- and compute task k; its results wait for (3) of
+//       task k+1.
+// One round trip (the sign words) per task is exposed.  Rows longer than one
+// pass (64 / R slots) take their further passes synchronously.  Same
+// arithmetic, same order: bit-identical to the other step kernels.
+template <int R, int WT, int NT>
+__device__ __forceinline__ void batch_step_pl(const BArgs &b) {
+  constexpr int SL = 64 / R;  // neighbour slots per row and pass
+  constexpr int kPlanes = 2;  // |v| <= 3
+  const int lane = threadIdx.x & 63;
+  const TaskRange tr = task_range(b);
+  const int64_t lda = b.lda, W = WT;
+  const int my_r = lane / SL, my_i = lane % SL;
+
+  // The wait counts the compiler places are exact only if every path issues
+  // the same vector memory operations, so staging and storing never branch:
+  // rows past a task's end load and store (again) the task's first row,
+  // slots past a row's end load entry 0 and are masked when used, a stage
+  // past the last task reloads row 0, and the first task's "previous"
+  // stores go to a sink.
+  int ntask = tr.first;
+  BTask ntk{};
+  int32_t nt_ = 0;
+  float nx[R][WT], ny[R][WT];
+  int nkb_lo = 0, ndeg = 0;  // this lane's row: entries past the task's kb, degree
+  int32_t ncol = 0;
+  float nval = 0.0f;
+  bool nok = false;
+  auto stage = [&](int task) {
+    for (; task < tr.end; task += tr.stride) {
+      ntk = ld_task(b.tasks + task);
+      if (ld_const(&b.inst[ntk.slot].status) == SLOT_ACTIVE) break;
+    }
+    ntask = task;
+    const bool have = task < tr.end;
+    if (have) nt_ = ld_const(&b.inst[ntk.slot].t);
+    const int nr = have ? ntk.nrows : 1;
+    const int64_t r0 = have ? ntk.row0 : 0;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int64_t row = r0 + (r < nr ? r : 0);
+#pragma unroll
+      for (int u = 0; u < WT; u++) {
+        nx[r][u] = ld_state<NT>(b.X + row * lda + 64 * u + lane);
+        ny[r][u] = ld_state<NT>(b.Y + row * lda + 64 * u + lane);
+      }
+    }
+    int o0 = 0, o1 = 0;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      o0 = my_r == r ? ntk.off[r] : o0;
+      o1 = my_r == r ? ntk.off[r + 1] : o1;
+    }
+    nkb_lo = o0;
+    ndeg = have && my_r < ntk.nrows ? o1 - o0 : 0;
+    nok = my_i < ndeg;
+    const int64_t k = nok ? ntk.kb + o0 + my_i : 0;
+    ncol = b.col[k];
+    nval = b.val[k];
+  };
+  // task k-1's results, stored after task k+1's loads are issued; before the
+  // first task, R distinct sink rows (no store may fold into another)
+  float *pX = b.sink, *pY = b.sink + R * lda;
+  uint64_t *pS = reinterpret_cast<uint64_t *>(b.sink + 2 * R * lda);
+  int64_t pSW = 2 * W;  // sign-word row stride: 2 W in the sink, so no two stores merge
+  int pnrows = R;
+  float px[R][WT], py[R][WT];
+  uint64_t pw[R];  // this lane's sign word (word lane % WT) of each row
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    pw[r] = 0;
+#pragma unroll
+    for (int u = 0; u < WT; u++) px[r][u] = py[r][u] = 0.0f;
+  }
+  auto flush = [&]() {
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int64_t rr = r < pnrows ? r : 0;
+#pragma unroll
+      for (int u = 0; u < WT; u++) {
+        st_state<NT>(pX + rr * lda + 64 * u + lane, r < pnrows ? px[r][u] : px[0][u]);
+        st_state<NT>(pY + rr * lda + 64 * u + lane, r < pnrows ? py[r][u] : py[0][u]);
+      }
+      pS[rr * pSW + lane % WT] = r < pnrows ? pw[r] : pw[0];
+    }
+  };
+
+  // compiler memory barriers keep the phases in program order (the loads and
+  // stores touch different types, which alias analysis would let it swap)
+  stage(tr.first);
+  asm volatile("" ::: "memory");
+  flush();  // to the sink: the loop is entered with as many stores in flight as from its end
+  while (ntask < tr.end) {
+    // task k: take over the staged record, state and pass-0 slots
+    const BTask tk = ntk;
+    const int task = ntask;
+    const int32_t t = nt_;
+    float x[R][WT], y[R][WT];
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+      for (int u = 0; u < WT; u++) {
+        x[r][u] = nx[r][u];
+        y[r][u] = ny[r][u];
+      }
+    const int mykb_lo = nkb_lo, mydeg = ndeg;
+    const int64_t j0 = nok ? (int64_t)ncol : tk.row0;
+    const float v0 = nok ? nval : 0.0f;
+    const uint64_t *S = (t & 1) ? b.S1 : b.S0;
+    uint64_t *Sn = (t & 1) ? b.S0 : b.S1;
+    // (1) pass-0 sign words
+    uint64_t w0[WT];
+#pragma unroll
+    for (int u = 0; u < WT; u++) w0[u] = S[j0 * W + u];
+    // (2) the next task's loads, (3) the previous task's stores
+    asm volatile("" ::: "memory");
+    stage(task + tr.stride);
+    asm volatile("" ::: "memory");
+    flush();
+    // (4) task k
+    const int32_t T = ld_const(&b.inst[tk.slot].T);
+    const double dt = ld_const(&b.inst[tk.slot].dt), xi0 = ld_const(&b.inst[tk.slot].xi0);
+    const int32_t intw = ld_const(&b.inst[tk.slot].intw);
+    double p = 1.025 * (double)t / (double)T;
+    p = p < 1.0 ? p : 1.0;
+    const float c0 = (float)(dt * (p + 1)), c1 = (float)(dt * (p - 1));
+    const float c2 = (float)(dt * xi0);
+    int maxdeg = 0;
+#pragma unroll
+    for (int r = 0; r < R; r++)
+      if (r < tk.nrows) maxdeg = max(maxdeg, tk.off[r + 1] - tk.off[r]);
+    float F[R][WT];
+    if (intw) {
+      int Sa[R][WT], Vr[R];
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        Vr[r] = 0;
+#pragma unroll
+        for (int u = 0; u < WT; u++) Sa[r][u] = 0;
+      }
+      for (int d0 = 0; d0 < maxdeg; d0 += SL) {
+        float vf = v0;
+        uint64_t w[WT];
+#pragma unroll
+        for (int u = 0; u < WT; u++) w[u] = w0[u];
+        if (d0 > 0) {  // further passes: load this pass's slots now
+          const bool ok = d0 + my_i < mydeg;
+          const int64_t k = ok ? tk.kb + mykb_lo + d0 + my_i : 0;
+          const int64_t j = ok ? (int64_t)b.col[k] : tk.row0;
+          vf = ok ? b.val[k] : 0.0f;
+#pragma unroll
+          for (int u = 0; u < WT; u++) w[u] = S[j * W + u];
+        }
+        const int vi = (int)vf;
+        const int av = vi < 0 ? -vi : vi;
+        uint64_t Pm[kPlanes], Nm[kPlanes];
+#pragma unroll
+        for (int q = 0; q < kPlanes; q++) {
+          Pm[q] = __ballot(vi > 0 && ((av >> q) & 1));
+          Nm[q] = __ballot(vi < 0 && ((av >> q) & 1));
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const uint64_t rm = (SL == 64 ? ~0ull : ((1ull << SL) - 1)) << (r * SL);
+#pragma unroll
+          for (int q = 0; q < kPlanes; q++)
+            Vr[r] += (__popcll(Pm[q] & rm) - __popcll(Nm[q] & rm)) * (1 << q);
+        }
+#pragma unroll
+        for (int u = 0; u < WT; u++) {
+          const uint64_t m = bit_transpose64(w[u], lane);
+#pragma unroll
+          for (int r = 0; r < R; r++) {
+            const uint64_t rm = (SL == 64 ? ~0ull : ((1ull << SL) - 1)) << (r * SL);
+            int acc = 0;
+#pragma unroll
+            for (int q = 0; q < kPlanes; q++)
+              acc += (__popcll(m & Pm[q] & rm) - __popcll(m & Nm[q] & rm)) * (1 << q);
+            Sa[r][u] += acc;
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int u = 0; u < WT; u++) F[r][u] = (float)(2 * Sa[r][u] - Vr[r]);
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int u = 0; u < WT; u++) F[r][u] = 0.0f;
+      for (int d0 = 0; d0 < maxdeg; d0 += SL) {
+        float vf = v0;
+        uint64_t w[WT];
+#pragma unroll
+        for (int u = 0; u < WT; u++) w[u] = w0[u];
+        if (d0 > 0) {
+          const bool ok = d0 + my_i < mydeg;
+          const int64_t k = ok ? tk.kb + mykb_lo + d0 + my_i : 0;
+          const int64_t j = ok ? (int64_t)b.col[k] : tk.row0;
+          vf = ok ? b.val[k] : 0.0f;
+#pragma unroll
+          for (int u = 0; u < WT; u++) w[u] = S[j * W + u];
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+#pragma unroll
+          for (int i = 0; i < SL; i++) {
+            const int l = r * SL + i;
+            const float vv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(vf), l));
+#pragma unroll
+            for (int u = 0; u < WT; u++) {
+              const uint64_t ww = readlane64(w[u], l);
+              F[r][u] = ((ww >> lane) & 1) ? F[r][u] + vv : F[r][u] - vv;
+            }
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      uint64_t mine = 0;
+#pragma unroll
+      for (int u = 0; u < WT; u++) {
+        sb_update<float, PROB_MAXCUT>(x[r][u], y[r][u], F[r][u], c0, c1, c2, 0.0f, 0.0f);
+        px[r][u] = x[r][u];
+        py[r][u] = y[r][u];
+        const uint64_t bits = __ballot(signbit(x[r][u]));
+        mine = lane % WT == u ? bits : mine;
+      }
+      pw[r] = mine;
+    }
+    pX = b.X + tk.row0 * lda;
+    pY = b.Y + tk.row0 * lda;
+    pS = Sn + tk.row0 * W;
+    pSW = W;
+    pnrows = tk.nrows;
+  }
+  flush();
+}
+
+template <int R, int WT, int NT>
+__global__ __launch_bounds__(kBlock) void k_batch_step_pl(BArgs b) {
+  batch_step_pl<R, WT, NT>(b);
 }
 
 // Same step as k_batch_step_bt with each instance's sign words staged in LDS.
@@ -1230,6 +1517,7 @@ struct sis_batch {
   int64_t W = 1, lda = 64;
   hipStream_t stream = nullptr;
   DevBuf<float> X, Y;
+  DevBuf<uint64_t> sink;  // 8 lda + 16 W words: the pipelined step's first stores
   DevBuf<uint64_t> S0, S1, cur, fin;
   DevBuf<int64_t> rbeg, rend;
   DevBuf<int32_t> col;
@@ -1284,6 +1572,7 @@ struct sis_batch {
     a.vald = vald.p;
     a.X = X.p;
     a.Y = Y.p;
+    a.sink = reinterpret_cast<float *>(sink.p);
     a.S0 = S0.p;
     a.S1 = S1.p;
     a.cur = cur.p;
@@ -1306,16 +1595,22 @@ namespace {
 // sign words staged in LDS << 21 (2, 4 or 8 rows per pass, no other bit but
 // non-temporal and XCD-affine) | segment code s << 22 (LDS staging: segments
 // of 512 << s rows, s = 0..3) | resident chunks << 24 (with any streaming tile,
-// which runs the chunks whenever a live instance does not qualify).
+// which runs the chunks whenever a live instance does not qualify) |
+// pipelined bit-transpose step << 25 (with the bit-transpose bit, 2, 4 or 8
+// rows, no vector-gather, prefetch or LDS bit).
 constexpr int kBatchResBit = 1 << 24;
+constexpr int kBatchPipeBit = 1 << 25;
 
 bool batch_tile_ok(int W, int tile) {
   tile &= ~kBatchResBit;
+  const int pl = (tile & kBatchPipeBit) != 0;
+  tile &= ~kBatchPipeBit;
   const int r = tile & 0xffff, nt = (tile >> 16) & 1, vg = (tile >> 17) & 1;
   const int bt = (tile >> 19) & 1, pf = (tile >> 20) & 1, ls = (tile >> 21) & 1;
   if ((tile >> 24) != 0 || (nt != 0 && nt != 1)) return false;
   if (!ls && (tile >> 22) != 0) return false;  // segment code only with LDS staging
   if (!(W == 1 || W == 2 || W == 4)) return false;
+  if (pl) return bt && !vg && !pf && !ls && (r == 2 || r == 4 || r == 8);
   if (ls) return !vg && !bt && !pf && (r == 2 || r == 4 || r == 8);
   if (pf && !(bt && nt)) return false;  // prefetch: bit-transpose, non-temporal only
   if (bt) return !vg && (r == 2 || r == 4 || r == 8);
@@ -1360,6 +1655,14 @@ void launch_bstep_bt(const sis_batch *b, const BArgs &a, bool launch) {
 }
 
 template <int R, int WT, int NT>
+void launch_bstep_pl(const sis_batch *b, const BArgs &a, bool launch) {
+  static unsigned grid = 0;  // per kernel; the engine runs on one kind of device
+  if (!grid) grid = persistent_grid(b, k_batch_step_pl<R, WT, NT>);
+  if (launch)
+    hipLaunchKernelGGL((k_batch_step_pl<R, WT, NT>), dim3(grid), dim3(kBlock), 0, b->stream, a);
+}
+
+template <int R, int WT, int NT>
 void launch_bstep_lds(const sis_batch *b, const BArgs &a, bool launch) {
   static unsigned grid = 0;  // per kernel; the engine runs on one kind of device
   if (!grid) grid = persistent_grid(b, k_batch_step_lds<R, WT, NT>);
@@ -1373,7 +1676,19 @@ void launch_batch_step(sis_batch *b, bool launch = true) {
   const BArgs a = b->args();
   const int r = b->tile & 0xffff, nt = (b->tile >> 16) & 1, vg = (b->tile >> 17) & 1;
   const int bt = (b->tile >> 19) & 1, pf = (b->tile >> 20) & 1, ls = (b->tile >> 21) & 1;
+  const int pl = (b->tile >> 25) & 1;
   const int w = (int)b->W;
+#define SIS_BSTEPP(R, WT, NT) \
+  if (pl && r == R && w == WT && nt == NT) return launch_bstep_pl<R, WT, NT>(b, a, launch);
+#define SIS_BSTEPP_W(R, NT) SIS_BSTEPP(R, 1, NT) SIS_BSTEPP(R, 2, NT) SIS_BSTEPP(R, 4, NT)
+  SIS_BSTEPP_W(2, 0)
+  SIS_BSTEPP_W(4, 0)
+  SIS_BSTEPP_W(8, 0)
+  SIS_BSTEPP_W(2, 1)
+  SIS_BSTEPP_W(4, 1)
+  SIS_BSTEPP_W(8, 1)
+#undef SIS_BSTEPP_W
+#undef SIS_BSTEPP
 #define SIS_BSTEPL(R, WT, NT) \
   if (ls && r == R && w == WT && nt == NT) return launch_bstep_lds<R, WT, NT>(b, a, launch);
 #define SIS_BSTEPL_W(R, NT) SIS_BSTEPL(R, 1, NT) SIS_BSTEPL(R, 2, NT) SIS_BSTEPL(R, 4, NT)
@@ -1534,7 +1849,16 @@ int rebuild_tasks(sis_batch *b) {
     int64_t k = 0;
     for (int64_t r = 0; r < l->req.n; r += R, k++) {
       const size_t x = spread ? (size_t)(k % kXcds) : dst;
-      per[x].push_back(BTask{l->slot, (int32_t)std::min<int64_t>(R, l->req.n - r), l->row_off + r});
+      BTask tk{};
+      tk.slot = l->slot;
+      tk.nrows = (int32_t)std::min<int64_t>(R, l->req.n - r);
+      tk.row0 = l->row_off + r;
+      const std::vector<int64_t> &rp = l->req.rp;
+      tk.kb = l->nnz_off + rp[(size_t)r];
+      if (tk.nrows <= 8)
+        for (int i = 0; i <= tk.nrows; i++)
+          tk.off[i] = (int32_t)(rp[(size_t)(r + i)] - rp[(size_t)r]);
+      per[x].push_back(tk);
     }
     if (spread)
       for (auto &v : xl) v += load(l) / kXcds;
@@ -1829,6 +2153,7 @@ int sis_batch_create(const sis_batch_params *pin, sis_batch **out) {
   const int64_t R = p.max_rows, RW = R * b->W, L = b->W * 64 * (int64_t)p.max_instances;
   int st;
   if ((st = b->X.alloc((size_t)(R * b->lda))) || (st = b->Y.alloc((size_t)(R * b->lda))) ||
+      (st = b->sink.alloc((size_t)(8 * b->lda + 16 * b->W))) ||
       (st = b->S0.alloc((size_t)RW)) || (st = b->S1.alloc((size_t)RW)) ||
       (st = b->cur.alloc((size_t)RW)) || (st = b->fin.alloc((size_t)RW)) ||
       (st = b->rbeg.alloc((size_t)R)) || (st = b->rend.alloc((size_t)R)) ||
@@ -2009,11 +2334,12 @@ int sis_batch_autotune(sis_batch *b, int64_t trial_steps, int32_t *tile, double 
           for (int bt : {0, 1})
             for (int pf : {0, 1})
               for (int ls = 0; ls < (try_lds ? 2 : 1); ls++)
-                for (int sg = 0; sg < (ls ? 4 : 1); sg++) {
-                  const int32_t tl = r | (nt << 16) | (vg << 17) | (xc << 18) | (bt << 19) |
-                                     (pf << 20) | (ls << 21) | (sg << 22);
-                  if (batch_tile_ok((int)b->W, tl)) cand.push_back(tl);
-                }
+                for (int sg = 0; sg < (ls ? 4 : 1); sg++)
+                  for (int pl : {0, 1}) {
+                    const int32_t tl = r | (nt << 16) | (vg << 17) | (xc << 18) | (bt << 19) |
+                                       (pf << 20) | (ls << 21) | (sg << 22) | (pl << 25);
+                    if (batch_tile_ok((int)b->W, tl)) cand.push_back(tl);
+                  }
   for (int32_t tl : cand) {
     {
       b->tile = tl;

@@ -67,6 +67,7 @@ XCD = 1 << 18  # XCD-affine task order: with 9 uneven instances both placements
 BT = 1 << 19  # bit-transpose integer sums (instances with small integer weights)
 PF = 1 << 20  # next-task prefetch (bit-transpose, non-temporal)
 LS = 1 << 21  # sign words staged in LDS, segments of 512 rows
+PL = 1 << 25  # pipelined bit-transpose step (stores deferred behind the next task's loads)
 
 
 @pytest.mark.parametrize("agents_max,tile", [(64, 1), (64, 8 | 1 << 16), (64, 2 | 1 << 17),
@@ -79,7 +80,9 @@ LS = 1 << 21  # sign words staged in LDS, segments of 512 rows
                                              (64, 8 | 1 << 16 | BT | PF), (64, 2 | 1 << 16 | BT | PF | XCD),
                                              (256, 4 | 1 << 16 | BT | PF), (64, 4 | LS),
                                              (64, 8 | 1 << 16 | LS | XCD), (128, 2 | LS),
-                                             (256, 4 | 1 << 16 | LS)])
+                                             (256, 4 | 1 << 16 | LS), (64, 4 | 1 << 16 | BT | PL),
+                                             (64, 8 | BT | PL | XCD), (64, 2 | 1 << 16 | BT | PL),
+                                             (128, 4 | BT | PL), (256, 8 | 1 << 16 | BT | PL)])
 def test_batch_tiles_bit_exact(agents_max, tile):
     Js = instances(9, 21, 64, 500)
     A = min(agents_max, 100) if agents_max > 64 else 64
@@ -161,7 +164,8 @@ def _weighted(n, p, seed, kind):
     return sis.Csr.from_coo(n, rows, cols, vals.astype(np.float64), mirror=True)
 
 
-@pytest.mark.parametrize("tile", [4 | BT, 8 | 1 << 16 | BT | XCD, 2 | BT, 8 | 1 << 16 | BT | PF])
+@pytest.mark.parametrize("tile", [4 | BT, 8 | 1 << 16 | BT | XCD, 2 | BT, 8 | 1 << 16 | BT | PF,
+                                  4 | 1 << 16 | BT | PL, 8 | BT | PL | XCD, 2 | 1 << 16 | BT | PL])
 def test_batch_bit_transpose_mixed_weights(tile):
     """Small-integer instances take the bit-plane sums, others the readlane
     path, in the same launch; every result equals the solo solve."""
@@ -244,7 +248,8 @@ RES = 1 << 24  # resident chunks (with the streaming tile as fallback)
                                                          (64, 4 | RES, 50, True),
                                                          (128, 2 | 1 << 16 | RES, 100, False),
                                                          (256, 4 | BT | RES, 50, False),
-                                                         (64, 8 | BT | RES, 30, False)])
+                                                         (64, 8 | BT | RES, 30, False),
+                                                         (64, 4 | 1 << 16 | BT | PL | RES, 100, False)])
 def test_batch_resident_chunks(agents_max, tile, chunk, eager):
     """Resident chunks against solo solves: instances of 1 to 6000 rows (one
     to 64 agents per workgroup), early exits, admission in waves, and a
