@@ -8,7 +8,7 @@ form (--form, default q12: X and Y stored as 12-bit fixed-point codes, FP32
 arithmetic, J sgn(X) exact from packed sign words) runs beside the plain
 FP32 dSB kernel for the speed-up and the equal-steps best-cut comparison;
 --also adds more forms (default: the packed-sign form, bit-identical to plain
-FP32), and the line also carries bSB with BF16 gathers against plain FP32
+FP32, and the 8-bit narrow-state form q8), and the line also carries bSB with BF16 gathers against plain FP32
 bSB, a batched-serving summary (4096 small instances) and the CPU baseline.
 
 The timed region is one sis_solver_run (pre-step + steps + windows), with the
@@ -48,6 +48,8 @@ def step_bytes(form, N, Z, A):
         return state + N * A // 8 + Z * A // 8 + csr
     if form == "q12":  # 12-bit X and Y codes: 3 bytes read + 3 written
         return 6 * N * A + N * A // 8 + Z * A // 8 + csr
+    if form == "q8":  # 8-bit X and Y codes: 2 bytes read + 2 written
+        return 4 * N * A + N * A // 8 + Z * A // 8 + csr
     if form == "bf16":
         return state + 2 * N * A + 2 * Z * A + csr
     if form == "f64":
@@ -294,7 +296,7 @@ def small_configs(a):
                               ms_per_step=r["wall"] * 1e3 / r["info"]["steps_executed"]))
     elif a.workload == "torus100":
         J = sis.Csr.gen_torus(100, 100, sis.W_PM1, a.seed)
-        for form in ("f32", "packed", "q12"):
+        for form in ("f32", "packed", "q12", "q8"):
             r = run_solver(J, dict(problem=sis.MAXCUT, agents=1024, dt=0.5, form=sis.FORMS[form],
                                    use_graph=1), a.steps if a.steps != 20000 else 10000, 1, a.seed)
             ms = r["info"]["step_kernel_ms"] / max(r["info"]["steps_executed"], 1)
@@ -587,8 +589,9 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving sub-object")
     ap.add_argument("--no-graph", action="store_true",
                     help="skip the whole-run hipGraph replay of the headline solve")
-    ap.add_argument("--also", default="packed", help="comma-separated extra forms timed beside "
-                    "the headline (same dynamics); the default adds the bit-exact packed form")
+    ap.add_argument("--also", default="packed,q8", help="comma-separated extra forms timed "
+                    "beside the headline (same dynamics); the default adds the packed form "
+                    "(bit-identical to plain FP32) and the 8-bit narrow-state form")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--workload", default="regular2p20",
                     choices=["regular2p20", "er800", "torus100", "tsp48", "powerlaw", "tsp_struct",
@@ -670,6 +673,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32" if a.form != "f64" else "f64",
         "state_storage": {"q12": "X, Y as 12-bit fixed-point codes (3 bytes per node-agent)",
+                          "q8": "X, Y as 8-bit fixed-point codes (2 bytes per node-agent)",
                           "packed": "X, Y FP32", "f32": "X, Y FP32", "bf16": "X, Y FP32",
                           "f64": "X, Y FP64"}[a.form],
         "data": "synthetic (seeded random 4-regular graph, +-1 weights, seeded initial state)",

@@ -50,14 +50,15 @@ enum sis_form {
   SIS_FORM_F64 = 1,    /* plain FP64, the reference's own precision */
   SIS_FORM_PACKED = 2, /* dSB only: gathers read 1 sign bit per agent */
   SIS_FORM_BF16 = 3,   /* bSB only: gathers read a BF16 shadow of X */
-  SIS_FORM_Q12 = 4     /* fused MaxCut dSB only (no reference counterpart; the
+  SIS_FORM_Q12 = 4,    /* fused MaxCut dSB only (no reference counterpart; the
                           reference keeps X, Y in FP64, src/ising.cu:71-72):
                           X and Y stored as 12-bit fixed-point codes,
                             X: round(x 2^10) in [-1024, 1024], +-1025 for any
                                |x| > 1 (the next step confines it anyway),
                             Y: round(y 2^s), |code| <= 2047, s the largest
                                integer in [0, 11] with 2047 2^-s >= 2/dt + dt
-                               (the bound of |y| while |x| <= 1; dt > 0),
+                               (the bound of |y| while |x| <= 1; dt > 0, and
+                               dt with 2047 < 2/dt + dt is refused),
                           rounded to nearest even after every step; arithmetic
                           in FP32 on the decoded values; gathers read 1 sign
                           bit per agent, so J sgn(X) is exact.  A code of 0 is
@@ -67,6 +68,16 @@ enum sis_form {
                           degree-4 graph it moves 6 + 1/8 + 4/8 = 6.6 bytes
                           (state, sign write, sign gathers) against 32 for
                           F32. */
+  SIS_FORM_Q8 = 5      /* as SIS_FORM_Q12 with 8-bit codes (no reference
+                          counterpart):
+                            X: round(x 2^6) in [-64, 64], +-65 for any |x| > 1,
+                            Y: round(y 2^s), |code| <= 127, s the largest
+                               integer in [0, 7] with 127 2^-s >= 2/dt + dt
+                               (dt with 127 < 2/dt + dt, i.e. dt < 0.0158, is
+                               refused);
+                          2 bytes per node-agent, so on a degree-4 graph
+                          4 + 1/8 + 4/8 = 4.6 bytes per node-agent and step.
+                          Its own trajectory; J sgn(X) exact. */
 };
 
 enum sis_weights { SIS_W_ONE = 0, SIS_W_PM1 = 1, SIS_W_INT3 = 2 };

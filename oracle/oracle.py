@@ -229,44 +229,76 @@ def tri(c1, c2, n):
     return c2 + c1 * n - ((c1 + 2) * (c1 + 1)) // 2
 
 
-# ---------------------------------------------------------------- narrow-state dSB form
+# ---------------------------------------------------------------- narrow-state dSB forms
+# Code width w: 12 (X codes +-1024 and the wall code +-1025, Y codes +-2047)
+# or 8 (X codes +-64, wall +-65, Y codes +-127).
 
-def q12_shift(dt):
-    """Y code shift sy of the 12-bit form for time step dt (-1 if dt <= 0)."""
-    lib().orc_q12_shift.restype = C.c_int
-    return int(lib().orc_q12_shift(C.c_double(dt)))
+def qn_shift(dt, w):
+    """Y code shift sy of the w-bit form for time step dt (-1 if dt <= 0)."""
+    lib().orc_qn_shift.restype = C.c_int
+    return int(lib().orc_qn_shift(C.c_int(w), C.c_double(dt)))
 
 
-def q12_round(X, Y, sy):
-    """Rounds float32 X, Y in place to the 12-bit codes (decoded values)."""
+def qn_round(X, Y, sy, w):
+    """Rounds float32 X, Y in place to the w-bit codes (decoded values)."""
     assert X.dtype == np.float32 and Y.dtype == np.float32
-    lib().orc_q12_round_state(C.c_int64(X.size), _p(X), _p(Y), C.c_int(sy))
+    lib().orc_qn_round_state(C.c_int(w), C.c_int64(X.size), _p(X), _p(Y), C.c_int(sy))
 
 
-def advance_q12(pb, X, Y, t0, nsteps, T, dt, xi0, sy):
+def step_qn(pb, Xs, Y, c, sy, w):
+    """One narrow-state step from the snapshot Xs: returns the new X, updates Y."""
+    n, A = Xs.shape
+    Xn = np.zeros_like(Xs)
+    cc = np.ascontiguousarray(c, np.float64)
+    lib().orc_step_qn(C.c_int(w), _p(pb.rp), _p(pb.col), _p(pb.val32), C.c_int64(n), C.c_int64(A),
+                      _p(Xs), _p(Xn), _p(Y), _p(cc), C.c_int(sy))
+    return Xn
+
+
+def advance_qn(pb, X, Y, t0, nsteps, T, dt, xi0, sy, w):
     """Narrow-state dSB MaxCut steps t0..t0+nsteps-1 in place (decoded codes)."""
     n, A = X.shape
-    lib().orc_advance_q12(_p(pb.rp), _p(pb.col), _p(pb.val32), C.c_int64(n), C.c_int64(A),
-                          C.c_int64(t0), C.c_int64(nsteps), C.c_int64(T), C.c_double(dt),
-                          C.c_double(xi0), C.c_int(sy), _p(X), _p(Y))
+    lib().orc_advance_qn(C.c_int(w), _p(pb.rp), _p(pb.col), _p(pb.val32), C.c_int64(n),
+                         C.c_int64(A), C.c_int64(t0), C.c_int64(nsteps), C.c_int64(T),
+                         C.c_double(dt), C.c_double(xi0), C.c_int(sy), _p(X), _p(Y))
 
 
-def prestep_q12(X, Y, dt, sy):
-    lib().orc_prestep_q12(C.c_int64(X.size), C.c_double(dt), C.c_int(sy), _p(X), _p(Y))
+def prestep_qn(X, Y, dt, sy, w):
+    lib().orc_prestep_qn(C.c_int(w), C.c_int64(X.size), C.c_double(dt), C.c_int(sy), _p(X), _p(Y))
 
 
-def solve_q12(pb, X, Y, T, uw, P, K=25, dt=0.5, xi0=0.0, sy=None):
+def solve_qn(pb, X, Y, T, uw, P, K=25, dt=0.5, xi0=0.0, sy=None, w=12):
     """Whole narrow-state dSB MaxCut solve in place on float32 X, Y (rounded to
     the codes first).  Returns dict(steps, cur, fin, stab, bif)."""
     n, A = X.shape
-    sy = q12_shift(dt) if sy is None else sy
+    sy = qn_shift(dt, w) if sy is None else sy
     cur = np.full((n, A), -1, np.int32)
     fin = np.full((n, A), -1, np.int32)
     stab = np.zeros(A, np.int32)
     bif = np.zeros(A, np.uint8)
-    lib().orc_solve_q12.restype = C.c_int64
-    steps = lib().orc_solve_q12(_p(pb.rp), _p(pb.col), _p(pb.val32), C.c_int64(n), C.c_int64(A),
-                                C.c_int64(T), C.c_int(uw), C.c_int64(P), C.c_int(K),
-                                C.c_double(dt), C.c_double(xi0), C.c_int(sy), _p(X), _p(Y),
-                                _p(cur), _p(fin), _p(stab), _p(bif))
+    lib().orc_solve_qn.restype = C.c_int64
+    steps = lib().orc_solve_qn(C.c_int(w), _p(pb.rp), _p(pb.col), _p(pb.val32), C.c_int64(n),
+                               C.c_int64(A), C.c_int64(T), C.c_int(uw), C.c_int64(P), C.c_int(K),
+                               C.c_double(dt), C.c_double(xi0), C.c_int(sy), _p(X), _p(Y),
+                               _p(cur), _p(fin), _p(stab), _p(bif))
     return dict(steps=steps, cur=cur, fin=fin, stab=stab, bif=bif.astype(np.int32))
+
+
+def q12_shift(dt):
+    return qn_shift(dt, 12)
+
+
+def q12_round(X, Y, sy):
+    qn_round(X, Y, sy, 12)
+
+
+def advance_q12(pb, X, Y, t0, nsteps, T, dt, xi0, sy):
+    advance_qn(pb, X, Y, t0, nsteps, T, dt, xi0, sy, 12)
+
+
+def prestep_q12(X, Y, dt, sy):
+    prestep_qn(X, Y, dt, sy, 12)
+
+
+def solve_q12(pb, X, Y, T, uw, P, K=25, dt=0.5, xi0=0.0, sy=None):
+    return solve_qn(pb, X, Y, T, uw, P, K, dt, xi0, sy, 12)

@@ -217,40 +217,57 @@ static void orc_unfused_f32(const int64_t *rp, const int32_t *col, const float *
   free(Xu);
 }
 
-/* 12-bit fixed-point state (the narrow-state dSB form).  X is stored as the
- * integer code round(x * 2^10) in [-1024, 1024], or +-1025 for any |x| > 1
- * (the value of an element beyond the wall does not matter: the next step
- * confines it to sgn x with y = 0, src/kernels.cu:236-242); Y as
- * round(y * 2^sy) clamped to [-2047, 2047] (0 beside a +-1025 X code).  Codes
- * are rounded to nearest even and stored as integers, so a code of 0 is +0.0
- * (sgn +1).  sy is the largest s in [0, 11] with 2047 * 2^-s >= 2/dt + dt,
- * the bound of |y| while |x| <= 1: |x_new - x| = c0 |y'| <= 2 with
- * c0 = dt (p + 1) >= dt gives |y'| <= 2 / dt, and |c1 x| <= dt. */
-int orc_q12_shift(double dt) {
-  if (!(dt > 0)) return -1;
+/* Narrow fixed-point state (the narrow-state dSB forms), code width w = 12
+ * or w = 8 bits per variable.  X is stored as the integer code
+ * round(x * 2^xs), xs = w - 2, in [-2^xs, 2^xs], or +-(2^xs + 1) for any
+ * |x| > 1 (the value of an element beyond the wall does not matter: the next
+ * step confines it to sgn x with y = 0, src/kernels.cu:236-242); Y as
+ * round(y * 2^sy) clamped to [-ymax, ymax], ymax = 2^(w-1) - 1 (0 beside a
+ * wall X code).  Codes are rounded to nearest even and stored as integers,
+ * so a code of 0 is +0.0 (sgn +1).  sy is the largest s in [0, w - 1] with
+ * ymax * 2^-s >= 2/dt + dt, the bound of |y| while |x| <= 1:
+ * |x_new - x| = c0 |y'| <= 2 with c0 = dt (p + 1) >= dt gives |y'| <= 2 / dt,
+ * and |c1 x| <= dt.  w = 12: X codes +-1024 (+-1025), Y codes +-2047;
+ * w = 8: X codes +-64 (+-65), Y codes +-127. */
+static inline int orc_qn_ymax(int w) { return (1 << (w - 1)) - 1; }
+
+int orc_qn_shift(int w, double dt) {
+  if (!(dt > 0) || (w != 12 && w != 8)) return -1;
   double ymax = 2.0 / dt + dt;
-  int s = 11;
-  while (s > 0 && ldexp(2047.0, -s) < ymax) s--;
+  int s = w - 1;
+  while (s > 0 && ldexp((double)orc_qn_ymax(w), -s) < ymax) s--;
   return s;
 }
 
-static inline void orc_q12_round(float *x, float *y, int sy) {
+int orc_q12_shift(double dt) { return orc_qn_shift(12, dt); }
+
+static inline void orc_qn_round(int w, float *x, float *y, int sy) {
+  const int xs = w - 2;
+  const float ym = (float)orc_qn_ymax(w);
   if (fabsf(*x) > 1.0f) {
-    *x = signbit(*x) ? -1025.0f / 1024.0f : 1025.0f / 1024.0f;
+    const float wall = ldexpf((float)((1 << xs) + 1), -xs);
+    *x = signbit(*x) ? -wall : wall;
     *y = 0.0f;
     return;
   }
-  int xq = (int)rintf(ldexpf(*x, 10));
+  int xq = (int)rintf(ldexpf(*x, xs));
   float yr = rintf(ldexpf(*y, sy));
-  int yq = yr > 2047.0f ? 2047 : yr < -2047.0f ? -2047 : (int)yr;
-  *x = ldexpf((float)xq, -10);
+  int yq = yr > ym ? (int)ym : yr < -ym ? -(int)ym : (int)yr;
+  *x = ldexpf((float)xq, -xs);
   *y = ldexpf((float)yq, -sy);
 }
 
 /* Rounds FP32 (x, y) pairs to the codes above in place. */
-void orc_q12_round_state(int64_t n, float *X, float *Y, int sy) {
-  for (int64_t e = 0; e < n; e++) orc_q12_round(&X[e], &Y[e], sy);
+void orc_qn_round_state(int w, int64_t n, float *X, float *Y, int sy) {
+  for (int64_t e = 0; e < n; e++) orc_qn_round(w, &X[e], &Y[e], sy);
 }
+
+void orc_q12_round_state(int64_t n, float *X, float *Y, int sy) {
+  orc_qn_round_state(12, n, X, Y, sy);
+}
+
+/* precision codes of the FP32 core below: 2 = 12-bit state, 3 = 8-bit state */
+static inline int orc_qn_width(int precision) { return precision == 2 ? 12 : 8; }
 
 /* One fused step over all (i, a), FP64.
  * MaxCut (src/kernels.cu:236-262): confine |x| > 1 to sgn x and y = 0, else
@@ -316,8 +333,8 @@ void orc_step_f64(int problem, int gather, const int64_t *rp, const int32_t *col
 
 /* FP32 form of the same step.  precision 1 rounds every gathered x to
  * bfloat16 (round to nearest even) before use: the BF16-shadow bSB form.
- * precision 2 (MaxCut dSB) rounds every updated (x, y) to the 12-bit codes
- * above with Y shift sy: the narrow-state form. */
+ * precision 2 / 3 (MaxCut dSB) round every updated (x, y) to the 12-bit /
+ * 8-bit codes above with Y shift sy: the narrow-state forms. */
 static void orc_step_f32_core(int problem, int gather, int precision, int sy, const int64_t *rp,
                               const int32_t *col, const float *val, const float *h, int64_t N,
                               int64_t A, const float *Xs, float *Xn, float *Y, const double *cd,
@@ -375,7 +392,7 @@ static void orc_step_f32_core(int problem, int gather, int precision, int sy, co
             y = fmaf(c1, x, y);
           }
         }
-        if (precision == 2) orc_q12_round(&x, &y, sy);
+        if (precision >= 2) orc_qn_round(orc_qn_width(precision), &x, &y, sy);
         Xn[e] = x;
         Y[e] = y;
       }
@@ -392,9 +409,15 @@ void orc_step_f32(int problem, int gather, int bf16_gather, const int64_t *rp, c
 }
 
 /* Narrow-state dSB step (precision 2): Xs / Y hold decoded codes. */
+void orc_step_qn(int w, const int64_t *rp, const int32_t *col, const float *val, int64_t N,
+                 int64_t A, const float *Xs, float *Xn, float *Y, const double *cd, int sy) {
+  orc_step_f32_core(ORC_MAXCUT, ORC_DSB, w == 12 ? 2 : 3, sy, rp, col, val, NULL, N, A, Xs, Xn, Y,
+                    cd, 1.0);
+}
+
 void orc_step_q12(const int64_t *rp, const int32_t *col, const float *val, int64_t N, int64_t A,
                   const float *Xs, float *Xn, float *Y, const double *cd, int sy) {
-  orc_step_f32_core(ORC_MAXCUT, ORC_DSB, 2, sy, rp, col, val, NULL, N, A, Xs, Xn, Y, cd, 1.0);
+  orc_step_qn(12, rp, col, val, N, A, Xs, Xn, Y, cd, sy);
 }
 
 /* ------------------------------------------------------------- windows */
@@ -476,7 +499,7 @@ static int64_t orc_solve_f32_core(int problem, int gather, int precision, int sy
                                   uint8_t *bif, int32_t *valid_list) {
   float *Xb = (float *)malloc(sizeof(float) * (size_t)(N * A));
   orc_prestep_f32(N * A, dt, X, Y);
-  if (precision == 2) orc_q12_round_state(N * A, X, Y, sy);
+  if (precision >= 2) orc_qn_round_state(orc_qn_width(precision), N * A, X, Y, sy);
   int64_t steps = 0;
   double c[4];
   while (steps < T) {
@@ -515,16 +538,22 @@ int64_t orc_solve_f32(int problem, int gather, int precision, const int64_t *rp,
                             valid_list);
 }
 
-/* The same for the narrow-state dSB MaxCut form: X, Y arrive as FP32 values
- * and are rounded to the 12-bit codes after the pre-step (the device rounds
- * at set_state too, which leaves values already on the code grid unchanged)
- * and after every step. */
+/* The same for the narrow-state dSB MaxCut forms (code width w): X, Y arrive
+ * as FP32 values and are rounded to the codes before and after the
+ * pre-step (the device rounds at set_state too, which leaves values already
+ * on the code grid unchanged) and after every step. */
+int64_t orc_solve_qn(int w, const int64_t *rp, const int32_t *col, const float *val, int64_t N,
+                     int64_t A, int64_t T, int uw, int64_t P, int K, double dt, double xi0, int sy,
+                     float *X, float *Y, int32_t *cur, int32_t *fin, int32_t *stab, uint8_t *bif) {
+  orc_qn_round_state(w, N * A, X, Y, sy);
+  return orc_solve_f32_core(ORC_MAXCUT, ORC_DSB, w == 12 ? 2 : 3, sy, rp, col, val, NULL, N, A, T,
+                            uw, P, K, 0, dt, xi0, 1.0, X, Y, cur, fin, stab, bif, NULL);
+}
+
 int64_t orc_solve_q12(const int64_t *rp, const int32_t *col, const float *val, int64_t N,
                       int64_t A, int64_t T, int uw, int64_t P, int K, double dt, double xi0, int sy,
                       float *X, float *Y, int32_t *cur, int32_t *fin, int32_t *stab, uint8_t *bif) {
-  orc_q12_round_state(N * A, X, Y, sy);
-  return orc_solve_f32_core(ORC_MAXCUT, ORC_DSB, 2, sy, rp, col, val, NULL, N, A, T, uw, P, K, 0,
-                            dt, xi0, 1.0, X, Y, cur, fin, stab, bif, NULL);
+  return orc_solve_qn(12, rp, col, val, N, A, T, uw, P, K, dt, xi0, sy, X, Y, cur, fin, stab, bif);
 }
 
 int64_t orc_solve_f64(int problem, int gather, const int64_t *rp, const int32_t *col,
@@ -571,23 +600,33 @@ void orc_advance_f32(int problem, int gather, int precision, const int64_t *rp, 
 }
 
 /* Narrow-state dSB MaxCut steps t0 .. t0+nsteps-1 on decoded codes. */
-void orc_advance_q12(const int64_t *rp, const int32_t *col, const float *val, int64_t N, int64_t A,
-                     int64_t t0, int64_t nsteps, int64_t T, double dt, double xi0, int sy, float *X,
-                     float *Y) {
+void orc_advance_qn(int w, const int64_t *rp, const int32_t *col, const float *val, int64_t N,
+                    int64_t A, int64_t t0, int64_t nsteps, int64_t T, double dt, double xi0, int sy,
+                    float *X, float *Y) {
   float *Xb = (float *)malloc(sizeof(float) * (size_t)(N * A));
   double c[4];
   for (int64_t s = 0; s < nsteps; s++) {
     orc_sched(ORC_MAXCUT, t0 + s, T, dt, xi0, c);
-    orc_step_q12(rp, col, val, N, A, X, Xb, Y, c, sy);
+    orc_step_qn(w, rp, col, val, N, A, X, Xb, Y, c, sy);
     memcpy(X, Xb, sizeof(float) * (size_t)(N * A));
   }
   free(Xb);
 }
 
+void orc_advance_q12(const int64_t *rp, const int32_t *col, const float *val, int64_t N, int64_t A,
+                     int64_t t0, int64_t nsteps, int64_t T, double dt, double xi0, int sy, float *X,
+                     float *Y) {
+  orc_advance_qn(12, rp, col, val, N, A, t0, nsteps, T, dt, xi0, sy, X, Y);
+}
+
 /* Pre-step (src/kernels.cu:15-20 at t = 0) on decoded codes, then rounding. */
-void orc_prestep_q12(int64_t n, double dt, int sy, float *X, float *Y) {
+void orc_prestep_qn(int w, int64_t n, double dt, int sy, float *X, float *Y) {
   orc_prestep_f32(n, dt, X, Y);
-  orc_q12_round_state(n, X, Y, sy);
+  orc_qn_round_state(w, n, X, Y, sy);
+}
+
+void orc_prestep_q12(int64_t n, double dt, int sy, float *X, float *Y) {
+  orc_prestep_qn(12, n, dt, sy, X, Y);
 }
 
 void orc_advance_f64(int problem, int gather, const int64_t *rp, const int32_t *col,

@@ -178,110 +178,189 @@ __device__ __forceinline__ void st_state(uint32_t *p, uint32_t v) {
   else *p = v;
 }
 
-// ------------------------------------------------- 12-bit fixed-point state
+// ------------------------------------------------- narrow fixed-point state
 //
-// Narrow-state dSB form: element (x, y) is the 24-bit unit
-// u = (xq + 2048) | (yq + 2048) << 12 of biased 12-bit codes
-//   xq = round(x 2^10) in [-1024, 1024], or +-1025 for any |x| > 1,
-//   yq = round(y 2^sy) clamped to [-2047, 2047] (0 beside a +-1025 code),
-// rounded to nearest even; the element is negative iff bit 11 is clear.  A
-// lane keeps four elements (agents l, l + 64, l + 128, l + 192 of a 256-agent
-// quad) in three dwords, bytes 0..2 of units 0..3 in order:
-//   d0 = u0 | u1 << 24,  d1 = u1 >> 8 | u2 << 16,  d2 = u2 >> 16 | u3 << 8.
+// Narrow-state dSB forms, code width QW = 12 or 8 bits per variable.
+// Element (x, y) is the 2 QW-bit unit u = (xq + B) | (yq + B) << QW of codes
+// biased by B = 2^(QW-1):
+//   xq = round(x 2^XS) in [-2^XS, 2^XS], XS = QW - 2, or +-(2^XS + 1) for
+//        any |x| > 1 (the wall code),
+//   yq = round(y 2^sy) clamped to [-YM, YM], YM = B - 1 (0 beside a wall
+//        code),
+// rounded to nearest even; the element is negative iff bit QW - 1 is clear.
+// QW = 12: X codes +-1024 (wall +-1025), Y codes +-2047, 24-bit units;
+// QW = 8: X codes +-64 (wall +-65), Y codes +-127, 16-bit units.
+// A lane keeps four elements (agents l, l + 64, l + 128, l + 192 of a
+// 256-agent quad) in NP dwords, the units' low 2 QW bits in order:
+//   QW = 12, NP = 3: d0 = u0 | u1 << 24, d1 = u1 >> 8 | u2 << 16,
+//                    d2 = u2 >> 16 | u3 << 8;
+//   QW = 8,  NP = 2: d0 = u0 | u1 << 16, d1 = u2 | u3 << 16.
 //
-// Decoding and rounding go through magic constants.  X: M = 2^23 (bits
+// Decoding and rounding go through magic constants.  M = 2^23 (bits
 // 0x4b000000): M + k for an integer 0 <= k < 2^23 is exact with bits
-// bits(M) + k, so bits 12..23 are zero for k < 4096; and for v s + 2048 in
-// [0, 2^23), fma(v, s, M + 2048) rounds v s + M + 2048 to an integer (ulp 1,
-// ties to even), whose low 12 bits are the biased code round(v s) + 2048.
-
-// Per-launch constants: step coefficients; the Y code scale 2^sy, the
-// largest |y| with a code (2047 2^-sy); the decoding offsets; the field mask,
-// held in a register so that a field's decode is one and-or.  All are small
-// integers times powers of two, so every product below is exact.
-struct Q12K {
-  float c0, c1, c2, yscale, ylim, yinv, xoff, yoff;
-  uint32_t mask;
+// bits(M) + k, so bits QW..22 are zero for k < 2^QW; and for v s + B in
+// [0, 2^23), fma(v, s, M + B) rounds v s + M + B to an integer (ulp 1, ties
+// to even), whose low QW bits are the biased code round(v s) + B.
+template <int QW> struct QFmt;
+template <> struct QFmt<12> {
+  static constexpr int NP = 3, XS = 10;
+  static constexpr uint32_t MASK = 0xfffu, SIGN = 0x800u;
+  static constexpr uint32_t WALL_NEG = 0x800000u | 1023u, WALL_POS = 0x800000u | 3073u;
+};
+template <> struct QFmt<8> {
+  static constexpr int NP = 2, XS = 6;
+  static constexpr uint32_t MASK = 0xffu, SIGN = 0x80u;
+  static constexpr uint32_t WALL_NEG = 0x8000u | 63u, WALL_POS = 0x8000u | 193u;
 };
 
-__device__ __forceinline__ Q12K q12_consts(float c0, float c1, float c2, int qshift) {
+// Per-launch constants: step coefficients; the Y code scale 2^sy, the
+// largest |y| with a code (YM 2^-sy); the decoding offsets (of 2^23 + field,
+// and of the field itself for byte fields); the field mask and the two wall
+// units.  The mask, the wall units and the offsets are held in registers
+// (opaque to the compiler), so that a field's decode is one and-or and the
+// constants are not rebuilt for every row.  All are small integers times
+// powers of two, so every product below is exact.
+struct Q12K {
+  float c0, c1, c2, yscale, ylim, yinv, xoff, yoff, xbo, ybo, mb;
+  uint32_t mask, wpos, wneg;
+};
+
+template <int QW>
+__device__ __forceinline__ Q12K qn_consts(float c0, float c1, float c2, int qshift) {
+  constexpr float B = (float)(1 << (QW - 1));
   Q12K k;
   k.c0 = c0;
   k.c1 = c1;
   k.c2 = c2;
   k.yscale = __builtin_ldexpf(1.0f, qshift);
   k.yinv = __builtin_ldexpf(1.0f, -qshift);
-  k.ylim = 2047.0f * k.yinv;
-  k.xoff = -(8388608.0f + 2048.0f) * (1.0f / 1024.0f);  // -(2^23 + 2048) 2^-10
-  k.yoff = -(8388608.0f + 2048.0f) * k.yinv;            // -(2^23 + 2048) 2^-sy
-  uint32_t mask = 0xfffu;
-  asm volatile("" : "+v"(mask));
+  k.ylim = (B - 1.0f) * k.yinv;
+  float xoff = -(8388608.0f + B) * (1.0f / (float)(1 << QFmt<QW>::XS));  // -(2^23 + B) 2^-XS
+  float xbo = -B * (1.0f / (float)(1 << QFmt<QW>::XS));                  // -B 2^-XS
+  k.yoff = -(8388608.0f + B) * k.yinv;                                   // -(2^23 + B) 2^-sy
+  k.ybo = -B * k.yinv;                                                   // -B 2^-sy
+  uint32_t mask = QFmt<QW>::MASK, wpos = QFmt<QW>::WALL_POS, wneg = QFmt<QW>::WALL_NEG;
+  float mb = 8388608.0f + B;  // 2^23 + B: the rounding addend of the encoder
+  asm volatile("" : "+v"(mask), "+v"(wpos), "+v"(wneg), "+v"(xoff), "+v"(xbo), "+v"(mb));
+  k.mb = mb;
+  k.xoff = xoff;
+  k.xbo = xbo;
   k.mask = mask;
+  k.wpos = wpos;
+  k.wneg = wneg;
   return k;
 }
 
-// The four units of a lane's three dwords; bits 24..31 are not cleared.
-__device__ __forceinline__ void q12_units(uint32_t d0, uint32_t d1, uint32_t d2, uint32_t u[4]) {
-  u[0] = d0;                                     // bits 0..23
-  u[1] = __builtin_amdgcn_alignbit(d1, d0, 24);  // bits 24..47
-  u[2] = __builtin_amdgcn_alignbit(d2, d1, 16);  // bits 48..71
-  u[3] = d2 >> 8;                                // bits 72..95
+// The four units of a lane's NP dwords; bits above 2 QW are not cleared.
+template <int QW>
+__device__ __forceinline__ void qn_units(const uint32_t d[QFmt<QW>::NP], uint32_t u[4]) {
+  if constexpr (QW == 12) {
+    u[0] = d[0];                                       // bits 0..23
+    u[1] = __builtin_amdgcn_alignbit(d[1], d[0], 24);  // bits 24..47
+    u[2] = __builtin_amdgcn_alignbit(d[2], d[1], 16);  // bits 48..71
+    u[3] = d[2] >> 8;                                  // bits 72..95
+  } else {
+    u[0] = d[0];
+    u[1] = d[0] >> 16;
+    u[2] = d[1];
+    u[3] = d[1] >> 16;
+  }
 }
 
-// x = (field - 2048) 2^-10 as fma(2^23 + field, 2^-10, -(2^23 + 2048) 2^-10):
-// both terms and the result are exact.  Likewise y with 2^-sy.  (The Y field
-// is shifted down first: at bits 12..23 its top bit would be the float's
+// x = (field - B) 2^-XS as fma(2^23 + field, 2^-XS, -(2^23 + B) 2^-XS): both
+// terms and the result are exact.  Likewise y with 2^-sy.  (The Y field is
+// shifted down first: at bits QW..2 QW-1 its top bit could be the float's
 // exponent bit.)
-__device__ __forceinline__ void q12_decode(uint32_t u, const Q12K &k, float &x, float &y) {
+template <int QW>
+__device__ __forceinline__ void qn_decode(uint32_t u, const Q12K &k, float &x, float &y) {
   const float fx = __uint_as_float((u & k.mask) | 0x4b000000u);
-  const float fy = __uint_as_float(((u >> 12) & k.mask) | 0x4b000000u);
-  x = __builtin_fmaf(fx, 1.0f / 1024.0f, k.xoff);
+  const float fy = __uint_as_float(((u >> QW) & k.mask) | 0x4b000000u);
+  x = __builtin_fmaf(fx, 1.0f / (float)(1 << QFmt<QW>::XS), k.xoff);
   y = __builtin_fmaf(fy, k.yinv, k.yoff);
 }
 
-__device__ __forceinline__ void q12_unpack(uint32_t d0, uint32_t d1, uint32_t d2, float x[4],
-                                           float y[4], const Q12K &k) {
-  uint32_t u[4];
-  q12_units(d0, d1, d2, u);
-#pragma unroll
-  for (int j = 0; j < 4; j++) q12_decode(u[j], k, x[j], y[j]);
+// Element j of a lane's NP dwords, decoded.  8-bit codes are whole bytes:
+// each field converts to float in one operation (byte b of a dword:
+// v_cvt_f32_ubyte<b>), then x = (field - B) 2^-XS = fma(field, 2^-XS, -B 2^-XS)
+// exactly; 12-bit codes go through qn_units / qn_decode.
+template <int QW>
+__device__ __forceinline__ void qn_decode_at(const uint32_t d[QFmt<QW>::NP], int j,
+                                             const Q12K &k, float &x, float &y) {
+  if constexpr (QW == 8) {
+    const uint32_t w = d[j >> 1];
+    const int sh = (j & 1) * 16;
+    const float fx = (float)((w >> sh) & 0xffu), fy = (float)((w >> (sh + 8)) & 0xffu);
+    x = __builtin_fmaf(fx, 1.0f / (float)(1 << QFmt<QW>::XS), k.xbo);
+    y = __builtin_fmaf(fy, k.yinv, k.ybo);
+  } else {
+    uint32_t u[4];
+    qn_units<QW>(d, u);
+    qn_decode<QW>(u[j], k, x, y);
+  }
 }
 
-// Unit of an updated (x, y), bits 24..31 unspecified, and the element's
-// sign.  x 2^10 stays within [-1024, 1024] while |x| <= 1; y is clamped to
-// +-2047 codes first (clamping y or y 2^sy is the same, the scale being a
-// power of two).  Both biased codes lie in [1, 4095], so the rounded floats'
-// bits 12..23 are zero and one shift-or joins them.  Elements beyond the
-// wall take the +-1025 X code and a 0 Y code.  The sign: f = fma(x, 2^10,
-// 2^23 + 2048) is monotonic in x and equals 2^23 + 2048 exactly where the X
-// code is 0, so the code (or, beyond the wall, x) is negative iff
-// f < 2^23 + 2048, i.e. iff bits(f) < bits(2^23 + 2048) as signed integers
-// (a negative f, for x below -8194, has negative bits).  Branch-free.
-__device__ __forceinline__ uint32_t q12_encode(float x, float y, const Q12K &k, bool &neg) {
-  const float kMb = 8388608.0f + 2048.0f;
-  const uint32_t xb = __float_as_uint(__builtin_fmaf(x, 1024.0f, kMb));
-  const float yc = __builtin_amdgcn_fmed3f(y, -k.ylim, k.ylim);
-  const uint32_t yb = __float_as_uint(__builtin_fmaf(yc, k.yscale, kMb));
-  const uint32_t in = (yb << 12) | xb;  // bits 0..23 valid
-  neg = (int32_t)xb < (int32_t)0x4b000800u;
-  // beyond the wall: X code 2048 -+ 1025, Y code 2048
-  const uint32_t wall = neg ? (0x800000u | 1023u) : (0x800000u | 3073u);
-  return __builtin_fabsf(x) > 1.0f ? wall : in;
+// Unit of an updated (x, y), bits above 2 QW unspecified, and the element's
+// sign.  x 2^XS stays within [-2^XS, 2^XS] while |x| <= 1; y is clamped to
+// +-YM codes first (clamping y or y 2^sy is the same, the scale being a power
+// of two).  Both biased codes lie in [1, 2^QW - 1], so the rounded floats'
+// bits QW..22 are zero and one shift-or joins them.  Elements beyond the
+// wall take the wall X code and a 0 Y code.  The sign: f = fma(x, 2^XS,
+// 2^23 + B) is monotonic in x and equals 2^23 + B exactly where the X code
+// is 0, so the code (or, beyond the wall, x) is negative iff f < 2^23 + B,
+// i.e. iff bits(f) < bits(2^23 + B) as signed integers (a negative f, for x
+// far below -1, has negative bits).  Branch-free.
+//
+// CLAMP = false leaves the Y clamp out.  That is exact after a fused MaxCut
+// step whenever YM 2^-sy >= 2/dt + dt (the rule that picks sy; the solver
+// refuses a dt for which no sy >= 0 meets it): an element that stays in the
+// box has |x_new - x| <= 2 with both in [-1, 1], so c0 |y'| <= 2 + 2^-24 with
+// c0 = fl(dt (p + 1)) >= fl(dt), and |y''| = |y' + c1 x_new| <= |y'| + fl(dt);
+// then |y''| 2^sy <= YM + 1e-5, which rounds to at most YM.  An element that
+// leaves the box takes the wall unit, whatever its y.
+template <int QW, bool CLAMP = true>
+__device__ __forceinline__ uint32_t qn_encode(float x, float y, const Q12K &k, bool &neg) {
+  const uint32_t xb = __float_as_uint(__builtin_fmaf(x, (float)(1 << QFmt<QW>::XS), k.mb));
+  const float yc = CLAMP ? __builtin_amdgcn_fmed3f(y, -k.ylim, k.ylim) : y;
+  const uint32_t yb = __float_as_uint(__builtin_fmaf(yc, k.yscale, k.mb));
+  uint32_t in = (yb << QW) | xb;  // bits 0 .. 2 QW - 1 valid
+  neg = (int32_t)xb < (int32_t)(0x4b000000u | (1u << (QW - 1)));
+  // computed for every element, so that the wall choice below is two selects
+  // rather than a branch around the encoding
+  asm volatile("" : "+v"(in));
+  return __builtin_fabsf(x) > 1.0f ? (neg ? k.wneg : k.wpos) : in;
 }
 
-__device__ __forceinline__ uint32_t q12_encode(float x, float y, const Q12K &k) {
+template <int QW>
+__device__ __forceinline__ uint32_t qn_encode(float x, float y, const Q12K &k) {
   bool neg;
-  return q12_encode(x, y, k, neg);
+  return qn_encode<QW, true>(x, y, k, neg);
 }
 
-__device__ __forceinline__ bool q12_negative(uint32_t unit) { return (unit & 0x800u) == 0; }
+template <int QW>
+__device__ __forceinline__ bool qn_negative(uint32_t unit) {
+  return (unit & QFmt<QW>::SIGN) == 0;
+}
 
-// Three dwords from four units (bytes 0..2 of each; upper bytes ignored).
-__device__ __forceinline__ void q12_pack(const uint32_t u[4], uint32_t &d0, uint32_t &d1,
-                                         uint32_t &d2) {
-  d0 = __builtin_amdgcn_perm(u[1], u[0], 0x04020100u);  // u0.b0 u0.b1 u0.b2 u1.b0
-  d1 = __builtin_amdgcn_perm(u[2], u[1], 0x05040201u);  // u1.b1 u1.b2 u2.b0 u2.b1
-  d2 = __builtin_amdgcn_perm(u[3], u[2], 0x06050402u);  // u2.b2 u3.b0 u3.b1 u3.b2
+// NP dwords from four units (their low 2 QW bits; upper bits ignored).
+template <int QW>
+__device__ __forceinline__ void qn_pack(const uint32_t u[4], uint32_t d[QFmt<QW>::NP]) {
+  if constexpr (QW == 12) {
+    d[0] = __builtin_amdgcn_perm(u[1], u[0], 0x04020100u);  // u0.b0 u0.b1 u0.b2 u1.b0
+    d[1] = __builtin_amdgcn_perm(u[2], u[1], 0x05040201u);  // u1.b1 u1.b2 u2.b0 u2.b1
+    d[2] = __builtin_amdgcn_perm(u[3], u[2], 0x06050402u);  // u2.b2 u3.b0 u3.b1 u3.b2
+  } else {
+    d[0] = __builtin_amdgcn_perm(u[1], u[0], 0x05040100u);  // u0.b0 u0.b1 u1.b0 u1.b1
+    d[1] = __builtin_amdgcn_perm(u[3], u[2], 0x05040100u);  // u2.b0 u2.b1 u3.b0 u3.b1
+  }
+}
+
+template <int QW>
+__device__ __forceinline__ void qn_unpack(const uint32_t d[QFmt<QW>::NP], float x[4], float y[4],
+                                          const Q12K &k) {
+  uint32_t u[4];
+  qn_units<QW>(d, u);
+#pragma unroll
+  for (int j = 0; j < 4; j++) qn_decode<QW>(u[j], k, x[j], y[j]);
 }
 
 // The fused MaxCut dSB update of sb_update<float, PROB_MAXCUT> without

@@ -313,14 +313,14 @@ __global__ __launch_bounds__(kBlock) void k_step_packed(StepArgs a) {
   }
 }
 
-// Narrow-state dSB form (MaxCut, fused): X and Y as 12-bit codes (see
-// sis_device.hpp), three dword planes of N x Q x 64 lanes (Q = lda / 256
-// quads); lane l of quad q owns agents 256 q + 64 j + l, j = 0..3, i.e. bit l
-// of sign word 4 q + j.  The gather is the packed form's (exact sum of +-J
+// Narrow-state dSB forms (MaxCut, fused): X and Y as QW-bit codes, QW = 12
+// or 8 (see sis_device.hpp), NP = 3 or 2 dword planes of N x Q x 64 lanes
+// (Q = lda / 256 quads); lane l of quad q owns agents 256 q + 64 j + l,
+// j = 0..3, i.e. bit l of sign word 4 q + j.  The gather is the packed form's (exact sum of +-J
 // from the snapshot's sign words, in CSR order); the update is the FP32 dSB
 // update on the decoded values, rounded back to codes.  A wavefront owns R
 // rows of one quad; their state loads are issued before the gathers.  Bytes
-// per node-agent: 3 read + 3 written + 1/8 sign write + degree / 8 gathers.
+// per node-agent: NP read + NP written + 1/8 sign write + degree / 8 gathers.
 //
 // Staged gathers (VG = 1): the R rows' entries (at most kQ12Stage) are
 // fetched at once -- lane e loads column, weight and the four sign words of
@@ -339,32 +339,37 @@ constexpr int kQ12Stage = 128;
 // New codes and sign words of one row: decode, fused dSB update, encode,
 // one ballot per sign word.  rb: the row's plane-0 dwords of this quad
 // (wave-uniform), ps: plane stride, sw: the row's four sign words.
-template <int NT>
-__device__ __forceinline__ void q12_finish_row(const uint32_t d[3], const float F[4], const Q12K &k,
-                                               uint32_t *rb, int64_t ps, uint64_t *sw, int lane) {
-  uint32_t u[4], un[4];
-  q12_units(d[0], d[1], d[2], u);
+template <int QW, int NT>
+__device__ __forceinline__ void qn_finish_row(const uint32_t d[QFmt<QW>::NP], const float F[4],
+                                              const Q12K &k, uint32_t *rb, int64_t ps,
+                                              uint64_t *sw, int lane) {
+  constexpr int NP = QFmt<QW>::NP;
+  uint32_t un[4];
   uint64_t b[4];
 #pragma unroll
   for (int j = 0; j < 4; j++) {
     float x, y;
-    q12_decode(u[j], k, x, y);
+    qn_decode_at<QW>(d, j, k, x, y);
     sb_update_maxcut_sel(x, y, F[j], k.c0, k.c1, k.c2);
     bool neg;
-    un[j] = q12_encode(x, y, k, neg);
+    un[j] = qn_encode<QW, false>(x, y, k, neg);  // no Y clamp after a fused step
     b[j] = __ballot(neg);
   }
-  uint32_t e0, e1, e2;
-  q12_pack(un, e0, e1, e2);
-  st_state<NT>(rb + lane, e0);
-  st_state<NT>(rb + ps + lane, e1);
-  st_state<NT>(rb + 2 * ps + lane, e2);
-  // lane j writes sign word j
-  if (lane < 4) sw[lane] = lane == 0 ? b[0] : lane == 1 ? b[1] : lane == 2 ? b[2] : b[3];
+  uint32_t e[NP];
+  qn_pack<QW>(un, e);
+#pragma unroll
+  for (int p = 0; p < NP; p++) st_state<NT>(rb + p * ps + lane, e[p]);
+  // lane 0 writes the four (wave-uniform) sign words, 32 aligned bytes
+  if (lane == 0) {
+    uint4 *o = reinterpret_cast<uint4 *>(sw);
+    o[0] = make_uint4((uint32_t)b[0], (uint32_t)(b[0] >> 32), (uint32_t)b[1], (uint32_t)(b[1] >> 32));
+    o[1] = make_uint4((uint32_t)b[2], (uint32_t)(b[2] >> 32), (uint32_t)b[3], (uint32_t)(b[3] >> 32));
+  }
 }
 
-template <int R, int NT, int VG>
-__device__ __forceinline__ void q12_step_body(const StepArgs &a) {
+template <int QW, int R, int NT, int VG>
+__device__ __forceinline__ void qn_step_body(const StepArgs &a) {
+  constexpr int NP = QFmt<QW>::NP;
   const Ctrl *ctrl = a.ctrl;
   if (ctrl->done) return;
   const int64_t W = a.W, Q = W / 4;
@@ -376,7 +381,7 @@ __device__ __forceinline__ void q12_step_body(const StepArgs &a) {
   const int64_t w0 = 4 * q;
   const int64_t t = ctrl->base + a.offset;
   const float *sc = (const float *)a.sched + 4 * t;
-  const Q12K kc = q12_consts(sc[0], sc[1], sc[2], a.qshift);
+  const Q12K kc = qn_consts<QW>(sc[0], sc[1], sc[2], a.qshift);
   const uint64_t *__restrict__ S = (const uint64_t *)a.gsrc;
   const float *__restrict__ val = (const float *)a.val;
   const int32_t *__restrict__ col = a.col;
@@ -385,13 +390,13 @@ __device__ __forceinline__ void q12_step_body(const StepArgs &a) {
   const int64_t ps = a.N * Q * 64;  // plane stride (dwords)
   uint64_t *Sn = (uint64_t *)a.gdst;
 
-  uint32_t d[R][3];
+  uint32_t d[R][NP];
 #pragma unroll
   for (int r = 0; r < R; r++) {
     if (row0 + r < a.N) {
       const uint32_t *rb = P + ((row0 + r) * Q + q) * 64;  // wave-uniform
 #pragma unroll
-      for (int p = 0; p < 3; p++) d[r][p] = ld_state<NT>(rb + p * ps + lane);
+      for (int p = 0; p < NP; p++) d[r][p] = ld_state<NT>(rb + p * ps + lane);
     }
   }
   if constexpr (VG) {
@@ -457,7 +462,7 @@ __device__ __forceinline__ void q12_step_body(const StepArgs &a) {
 #pragma unroll
           for (int j = 0; j < 4; j++) F[j] = (float)(2 * (int)cnt[j] - deg);
           const int64_t row = row0 + r;
-          q12_finish_row<NT>(d[r], F, kc, P + (row * Q + q) * 64, ps, Sn + row * W + w0, lane);
+          qn_finish_row<QW, NT>(d[r], F, kc, P + (row * Q + q) * 64, ps, Sn + row * W + w0, lane);
         }
         return;
       }
@@ -484,7 +489,7 @@ __device__ __forceinline__ void q12_step_body(const StepArgs &a) {
         }
         if (k < k1) add(F, lsig[wv][k][half], lval[wv][k]);
         const int64_t row = row0 + r;
-        q12_finish_row<NT>(d[r], F, kc, P + (row * Q + q) * 64, ps, Sn + row * W + w0, lane);
+        qn_finish_row<QW, NT>(d[r], F, kc, P + (row * Q + q) * 64, ps, Sn + row * W + w0, lane);
       }
       return;
     }
@@ -524,20 +529,23 @@ __device__ __forceinline__ void q12_step_body(const StepArgs &a) {
         F[u] = ((sw >> lane) & 1) ? F[u] + v : F[u] - v;
       }
     }
-    q12_finish_row<NT>(d[r], F, kc, P + (row * Q + q) * 64, ps, Sn + row * W + w0, lane);
+    qn_finish_row<QW, NT>(d[r], F, kc, P + (row * Q + q) * 64, ps, Sn + row * W + w0, lane);
   }
 }
 
-template <int R, int NT, int VG>
-__global__ __launch_bounds__(kBlock) void k_step_q12(StepArgs a) {
-  q12_step_body<R, NT, VG>(a);
+template <int QW, int R, int NT, int VG>
+__global__ __launch_bounds__(kBlock) void k_step_qn(StepArgs a) {
+  qn_step_body<QW, R, NT, VG>(a);
 }
 
-// Initial state of the narrow form: the counter-based stream of k_init_state
-// (element e = i A + a, FP32 values), rounded to codes; padding agents 0.
-__global__ void k_init_q12(uint32_t *P, int64_t N, int64_t A, int64_t Q, uint64_t seed, int qshift) {
+// Initial state of the narrow forms: the counter-based stream of
+// k_init_state (element e = i A + a, FP32 values), rounded to codes; padding
+// agents 0.
+template <int QW>
+__global__ void k_init_qn(uint32_t *P, int64_t N, int64_t A, int64_t Q, uint64_t seed, int qshift) {
+  constexpr int NP = QFmt<QW>::NP;
   const int64_t n = N * Q * 64, ps = n;
-  const Q12K k = q12_consts(0.0f, 0.0f, 0.0f, qshift);
+  const Q12K k = qn_consts<QW>(0.0f, 0.0f, 0.0f, qshift);
   for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
        idx += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = idx / (Q * 64), rem = idx - i * Q * 64, q = rem / 64, l = rem - q * 64;
@@ -551,52 +559,58 @@ __global__ void k_init_q12(uint32_t *P, int64_t N, int64_t A, int64_t Q, uint64_
         x = (float)((d_uniform(seed, 2 * e) * 2 - 1) / 10);
         y = (float)((d_uniform(seed, 2 * e + 1) * 2 - 1) / 10);
       }
-      u[j] = q12_encode(x, y, k);
+      u[j] = qn_encode<QW>(x, y, k);
     }
-    uint32_t d0, d1, d2;
-    q12_pack(u, d0, d1, d2);
-    P[idx] = d0;
-    P[ps + idx] = d1;
-    P[2 * ps + idx] = d2;
+    uint32_t d[NP];
+    qn_pack<QW>(u, d);
+#pragma unroll
+    for (int p = 0; p < NP; p++) P[p * ps + idx] = d[p];
   }
 }
 
 // Pre-step (src/kernels.cu:15-20 at t = 0) on the decoded values, rounded.
-__global__ void k_prestep_q12(uint32_t *P, int64_t n, float cy, float cx, int qshift) {
-  const Q12K k = q12_consts(0.0f, 0.0f, 0.0f, qshift);
+template <int QW>
+__global__ void k_prestep_qn(uint32_t *P, int64_t n, float cy, float cx, int qshift) {
+  constexpr int NP = QFmt<QW>::NP;
+  const Q12K k = qn_consts<QW>(0.0f, 0.0f, 0.0f, qshift);
   for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
        idx += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t d[NP];
+#pragma unroll
+    for (int p = 0; p < NP; p++) d[p] = P[p * n + idx];
     float x[4], y[4];
-    q12_unpack(P[idx], P[n + idx], P[2 * n + idx], x, y, k);
+    qn_unpack<QW>(d, x, y, k);
     uint32_t u[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       const float yy = fmaT(cy, x[j], y[j]);
       const float xx = fmaT(cx, yy, x[j]);
-      u[j] = q12_encode(xx, yy, k);
+      u[j] = qn_encode<QW>(xx, yy, k);
     }
-    uint32_t d0, d1, d2;
-    q12_pack(u, d0, d1, d2);
-    P[idx] = d0;
-    P[n + idx] = d1;
-    P[2 * n + idx] = d2;
+    qn_pack<QW>(u, d);
+#pragma unroll
+    for (int p = 0; p < NP; p++) P[p * n + idx] = d[p];
   }
 }
 
 // Sign words of the narrow state: one wavefront per (row, quad), one ballot
 // per word (bit l of word 4 q + j = X code of agent 256 q + 64 j + l < 0).
-__global__ __launch_bounds__(kBlock) void k_pack_signs_q12(const uint32_t *P, uint64_t *S,
-                                                           int64_t N, int64_t Q) {
+template <int QW>
+__global__ __launch_bounds__(kBlock) void k_pack_signs_qn(const uint32_t *P, uint64_t *S,
+                                                          int64_t N, int64_t Q) {
+  constexpr int NP = QFmt<QW>::NP;
   const int lane = threadIdx.x & 63;
   const int64_t g = wave_index();
   if (g >= N * Q) return;
   const int64_t ps = N * Q * 64, o = g * 64 + lane;
-  uint32_t u[4];
-  q12_units(P[o], P[ps + o], P[2 * ps + o], u);
+  uint32_t d[NP], u[4];
+#pragma unroll
+  for (int p = 0; p < NP; p++) d[p] = P[p * ps + o];
+  qn_units<QW>(d, u);
   uint64_t mine = 0;
 #pragma unroll
   for (int j = 0; j < 4; j++) {
-    const uint64_t b = __ballot(q12_negative(u[j]));
+    const uint64_t b = __ballot(qn_negative<QW>(u[j]));
     if (lane == j) mine = b;
   }
   if (lane < 4) S[g * 4 + lane] = mine;
@@ -2121,24 +2135,35 @@ void launch_prestep(hipStream_t st, bool f64, void *X, void *Y, int64_t n, doubl
                        (float)cy, (float)cx);
 }
 
-void launch_init_q12(hipStream_t st, uint32_t *P, int64_t N, int64_t A, int64_t lda,
-                     uint64_t seed, int qshift) {
+void launch_init_qn(hipStream_t st, int qw, uint32_t *P, int64_t N, int64_t A, int64_t lda,
+                    uint64_t seed, int qshift) {
   const int64_t Q = lda / 256;
-  hipLaunchKernelGGL(k_init_q12, dim3(grid_for(N * Q * 64, 256, 16384)), dim3(256), 0, st, P, N,
-                     A, Q, seed, qshift);
+  const dim3 grid(grid_for(N * Q * 64, 256, 16384));
+  if (qw == 12)
+    hipLaunchKernelGGL(k_init_qn<12>, grid, dim3(256), 0, st, P, N, A, Q, seed, qshift);
+  else
+    hipLaunchKernelGGL(k_init_qn<8>, grid, dim3(256), 0, st, P, N, A, Q, seed, qshift);
 }
 
-void launch_prestep_q12(hipStream_t st, uint32_t *P, int64_t N, int64_t lda, double dt, int qshift) {
+void launch_prestep_qn(hipStream_t st, int qw, uint32_t *P, int64_t N, int64_t lda, double dt,
+                       int qshift) {
   const int64_t n = N * (lda / 256) * 64;
   const double cy = dt * (0.0 - 1), cx = dt * (0.0 + 1);
-  hipLaunchKernelGGL(k_prestep_q12, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, P, n,
-                     (float)cy, (float)cx, qshift);
+  const dim3 grid(grid_for(n, 256, 16384));
+  if (qw == 12)
+    hipLaunchKernelGGL(k_prestep_qn<12>, grid, dim3(256), 0, st, P, n, (float)cy, (float)cx, qshift);
+  else
+    hipLaunchKernelGGL(k_prestep_qn<8>, grid, dim3(256), 0, st, P, n, (float)cy, (float)cx, qshift);
 }
 
-void launch_pack_signs_q12(hipStream_t st, const uint32_t *P, uint64_t *S, int64_t N, int64_t lda) {
+void launch_pack_signs_qn(hipStream_t st, int qw, const uint32_t *P, uint64_t *S, int64_t N,
+                          int64_t lda) {
   const int64_t Q = lda / 256;
-  hipLaunchKernelGGL(k_pack_signs_q12, dim3(grid_for(N * Q, kWaves, INT32_MAX)), dim3(kBlock), 0,
-                     st, P, S, N, Q);
+  const dim3 grid(grid_for(N * Q, kWaves, INT32_MAX));
+  if (qw == 12)
+    hipLaunchKernelGGL(k_pack_signs_qn<12>, grid, dim3(kBlock), 0, st, P, S, N, Q);
+  else
+    hipLaunchKernelGGL(k_pack_signs_qn<8>, grid, dim3(kBlock), 0, st, P, S, N, Q);
 }
 
 void launch_bf16_shadow(hipStream_t st, const float *X, uint16_t *sh, int64_t n) {
@@ -2168,7 +2193,7 @@ void launch_xy_update(hipStream_t st, bool f64, void *X, void *Y, int64_t n, con
 }
 
 bool step_supported(int form, int prob, int gather, int vec) {
-  if (form == 4) {
+  if (form == 4 || form == 5) {
     // 4 | rows << 8 | non-temporal << 16 | staged gathers << 17
     const int rows = (vec >> 8) & 0xff, vg = (vec >> 17) & 1;
     if (prob != PROB_MAXCUT || gather != G_DSB || (vec & 0xff) != 4 || (vec >> 18) != 0)
@@ -2252,24 +2277,26 @@ void launch_step(hipStream_t st, int form, int prob, int gather, int vec, const 
 #undef SIS_PACKED_V
 #undef SIS_PACKED_R
 #undef SIS_PACKED
-  } else if (form == 4) {
+  } else if (form == 4 || form == 5) {
     const int rows = (vec >> 8) & 0xff, vg = (vec >> 17) & 1;
     const int64_t waves = ((a.N + rows - 1) / rows) * (a.W / 4);
-#define SIS_Q12(R, NT, VG)                                      \
-  if (rows == R && nt == NT && vg == VG) {                      \
-    launch_waves(st, k_step_q12<R, NT, VG>, waves, a);          \
+#define SIS_QN(QW, R, NT, VG)                                   \
+  if (qw == QW && rows == R && nt == NT && vg == VG) {          \
+    launch_waves(st, k_step_qn<QW, R, NT, VG>, waves, a);       \
     return;                                                     \
   }
-#define SIS_Q12_NT(R, VG) SIS_Q12(R, 0, VG) SIS_Q12(R, 1, VG)
-    SIS_Q12_NT(1, 0)
-    SIS_Q12_NT(2, 0)
-    SIS_Q12_NT(4, 0)
-    SIS_Q12_NT(4, 1)
-    SIS_Q12_NT(8, 1)
-    SIS_Q12_NT(16, 1)
-    SIS_Q12_NT(32, 1)
-#undef SIS_Q12_NT
-#undef SIS_Q12
+#define SIS_QN_NT(R, VG) \
+  SIS_QN(12, R, 0, VG) SIS_QN(12, R, 1, VG) SIS_QN(8, R, 0, VG) SIS_QN(8, R, 1, VG)
+    const int qw = form == 4 ? 12 : 8;
+    SIS_QN_NT(1, 0)
+    SIS_QN_NT(2, 0)
+    SIS_QN_NT(4, 0)
+    SIS_QN_NT(4, 1)
+    SIS_QN_NT(8, 1)
+    SIS_QN_NT(16, 1)
+    SIS_QN_NT(32, 1)
+#undef SIS_QN_NT
+#undef SIS_QN
   } else if (form == 3) {
 #define SIS_BF16(P, V, NT)                                                      \
   if (prob == P && v == V && nt == NT) {                                        \

@@ -40,7 +40,7 @@ struct sis_solver {
   DevBuf<double> Xd[2], Yd;      // FP64 state
   DevBuf<uint16_t> sh[2];        // BF16 shadow
   DevBuf<uint64_t> sg[2];        // packed sign words
-  DevBuf<uint32_t> Xq;           // narrow form: 12-bit X / Y codes, three dword planes
+  DevBuf<uint32_t> Xq;           // narrow forms: X / Y codes, three (12-bit) or two (8-bit) dword planes
   int qshift = 0;                // narrow form: Y codes are round(y 2^qshift)
   int par = 0;                   // index of the current snapshot buffer
 
@@ -135,59 +135,68 @@ struct sis_solver {
     if (stream) (void)hipStreamDestroy(stream);
   }
 
-  bool q12() const { return p.form == SIS_FORM_Q12; }
+  // narrow-state forms: X, Y as qw()-bit codes
+  bool narrow() const { return p.form == SIS_FORM_Q12 || p.form == SIS_FORM_Q8; }
+  int qw() const { return p.form == SIS_FORM_Q12 ? 12 : 8; }
   void *Xbuf(int i) { return f64 ? (void *)Xd[i].p : (void *)Xf[i].p; }
   void *Ybuf() { return f64 ? (void *)Yd.p : (void *)Yf.p; }
   // X holding the current positions.
   void *Xcur() { return p.form == SIS_FORM_F32 || p.form == SIS_FORM_F64 ? Xbuf(par) : Xbuf(0); }
 };
 
-// Narrow-state form on the host: the same rounding as q12_encode (device) --
-// X code round(x 2^10), +-1025 beyond the wall with a 0 Y code, Y code
-// round(y 2^qshift) clamped to +-2047, nearest even, each stored plus 2048
-// in 12 bits -- and the plane layout (dword p of lane l, quad q, row i at
-// p N Q 64 + (i Q + q) 64 + l).
-static int q12_shift_for(double dt) {
+// Narrow-state forms on the host (code width qw = 12 or 8): the same
+// rounding as qn_encode (device) -- X code round(x 2^(qw-2)), +-(2^(qw-2) + 1)
+// beyond the wall with a 0 Y code, Y code round(y 2^qshift) clamped to
+// +-(2^(qw-1) - 1), nearest even, each stored plus 2^(qw-1) in qw bits -- and
+// the plane layout (dword p of lane l, quad q, row i at
+// p N Q 64 + (i Q + q) 64 + l; 3 planes for qw = 12, 2 for qw = 8).
+static int qn_shift_for(int qw, double dt) {
   if (!(dt > 0)) return -1;
-  const double ymax = 2.0 / dt + dt;
-  int sh = 11;
-  while (sh > 0 && std::ldexp(2047.0, -sh) < ymax) sh--;
+  const double ymax = 2.0 / dt + dt, ym = (double)((1 << (qw - 1)) - 1);
+  int sh = qw - 1;
+  while (sh > 0 && std::ldexp(ym, -sh) < ymax) sh--;
   return sh;
 }
 
-static uint32_t q12_unit_host(float x, float y, int qshift) {
+static uint32_t qn_unit_host(int qw, float x, float y, int qshift) {
+  const int xs = qw - 2, bias = 1 << (qw - 1), ym = bias - 1;
   int xq, yq;
   if (std::fabs(x) > 1.0f) {
-    xq = std::signbit(x) ? -1025 : 1025;
+    xq = std::signbit(x) ? -((1 << xs) + 1) : (1 << xs) + 1;
     yq = 0;
   } else {
-    xq = (int)std::rint(std::ldexp(x, 10));
+    xq = (int)std::rint(std::ldexp(x, xs));
     const float yr = std::rint(std::ldexp(y, qshift));
-    yq = yr > 2047.0f ? 2047 : yr < -2047.0f ? -2047 : (int)yr;
+    yq = yr > (float)ym ? ym : yr < -(float)ym ? -ym : (int)yr;
   }
-  return (uint32_t)(xq + 2048) | ((uint32_t)(yq + 2048) << 12);
+  return (uint32_t)(xq + bias) | ((uint32_t)(yq + bias) << qw);
 }
 
-// Unit j of a lane occupies bits 24 j .. 24 j + 23 of its 96-bit (d0, d1, d2):
-// read and written through the 64-bit window of dwords p, p + 1 that holds it.
-static void q12_put(std::vector<uint32_t> &P, int64_t N, int64_t Q, int64_t i, int64_t a,
-                    uint32_t u) {
+// Unit j of a lane occupies bits ub j .. ub j + ub - 1 (ub = 2 qw) of its
+// NP dwords: read and written through the 64-bit window of dwords p, p + 1
+// that holds it.
+static void qn_put(int qw, std::vector<uint32_t> &P, int64_t N, int64_t Q, int64_t i, int64_t a,
+                   uint32_t u) {
+  const int64_t ub = 2 * qw, np = qw == 12 ? 3 : 2;
+  const uint64_t m = ((uint64_t)1 << ub) - 1;
   const int64_t q = a / 256, j = (a % 256) / 64, l = a % 64, ps = N * Q * 64;
-  const int64_t o = (i * Q + q) * 64 + l, p = (24 * j) / 32, r = (24 * j) % 32;
+  const int64_t o = (i * Q + q) * 64 + l, p = (ub * j) / 32, r = (ub * j) % 32;
   uint32_t &lo = P[(size_t)(p * ps + o)];
   uint64_t win = lo;
-  if (p < 2) win |= (uint64_t)P[(size_t)((p + 1) * ps + o)] << 32;
-  win = (win & ~((uint64_t)0xffffff << r)) | ((uint64_t)(u & 0xffffff) << r);
+  if (p < np - 1) win |= (uint64_t)P[(size_t)((p + 1) * ps + o)] << 32;
+  win = (win & ~(m << r)) | (((uint64_t)u & m) << r);
   lo = (uint32_t)win;
-  if (p < 2) P[(size_t)((p + 1) * ps + o)] = (uint32_t)(win >> 32);
+  if (p < np - 1) P[(size_t)((p + 1) * ps + o)] = (uint32_t)(win >> 32);
 }
 
-static uint32_t q12_get(const std::vector<uint32_t> &P, int64_t N, int64_t Q, int64_t i, int64_t a) {
+static uint32_t qn_get(int qw, const std::vector<uint32_t> &P, int64_t N, int64_t Q, int64_t i,
+                       int64_t a) {
+  const int64_t ub = 2 * qw, np = qw == 12 ? 3 : 2;
   const int64_t q = a / 256, j = (a % 256) / 64, l = a % 64, ps = N * Q * 64;
-  const int64_t o = (i * Q + q) * 64 + l, p = (24 * j) / 32, r = (24 * j) % 32;
+  const int64_t o = (i * Q + q) * 64 + l, p = (ub * j) / 32, r = (ub * j) % 32;
   uint64_t win = P[(size_t)(p * ps + o)];
-  if (p < 2) win |= (uint64_t)P[(size_t)((p + 1) * ps + o)] << 32;
-  return (uint32_t)(win >> r) & 0xffffffu;
+  if (p < np - 1) win |= (uint64_t)P[(size_t)((p + 1) * ps + o)] << 32;
+  return (uint32_t)(win >> r) & (uint32_t)(((uint64_t)1 << ub) - 1);
 }
 
 // Resident tiles (k_resident): bit 20, log2 of the agents per workgroup in
@@ -312,17 +321,18 @@ static bool tile_ok(const sis_solver *s, int vec) {
   if (vec & kResidentBit) return resident_layout(s, vec, nullptr);
   if (!step_supported(s->p.form, s->prob, s->gather, vec)) return false;
   if (s->p.form == SIS_FORM_PACKED) return s->W % (vec & 0xff) == 0;
-  if (s->p.form == SIS_FORM_Q12) return s->lda % 256 == 0;
+  if (s->narrow()) return s->lda % 256 == 0;
   return s->lda % (64 * (vec & 0xffff)) == 0;
 }
 
 static int default_tile(const sis_solver *s) {
   int v = 1;
-  if (s->p.form == SIS_FORM_Q12) {
+  if (s->narrow()) {
     // staged gathers of 16 rows per wavefront; non-temporal codes once they
-    // exceed the Infinity Cache (3 bytes per node-agent)
+    // exceed the Infinity Cache (3 or 2 bytes per node-agent)
     v = 4 | 16 << 8 | 1 << 17;
-    if ((double)s->N * (double)s->lda * 3.0 > 256.0 * 1024 * 1024) v |= 1 << 16;
+    if ((double)s->N * (double)s->lda * (s->qw() == 12 ? 3.0 : 2.0) > 256.0 * 1024 * 1024)
+      v |= 1 << 16;
     return v;
   }
   if (s->p.form == SIS_FORM_PACKED) {
@@ -348,7 +358,7 @@ static std::vector<int> candidate_tiles(const sis_solver *s) {
     if (s->tsp_res_G) return {1, 2, 3};
     return {1, 2};
   }
-  if (s->p.form == SIS_FORM_Q12) {
+  if (s->narrow()) {
     for (int nt : {0, 1}) {
       for (int r : {1, 2, 4}) c.push_back(4 | (r << 8) | (nt << 16));
       for (int r : {4, 8, 16, 32}) c.push_back(4 | (r << 8) | (nt << 16) | (1 << 17));
@@ -477,15 +487,15 @@ int sis_solver_create(const sis_csr *J, const double *field, const sis_params *p
     return fail(SIS_ERR_ARG, "sis_solver_create: the packed-sign form is dSB only");
   if (p.form == SIS_FORM_BF16 && p.dynamics != SIS_BSB)
     return fail(SIS_ERR_ARG, "sis_solver_create: the BF16 form is bSB only");
-  if (p.form < SIS_FORM_F32 || p.form > SIS_FORM_Q12)
+  if (p.form < SIS_FORM_F32 || p.form > SIS_FORM_Q8)
     return fail(SIS_ERR_ARG, "sis_solver_create: unknown form");
-  if (p.form == SIS_FORM_Q12) {
+  if (p.form == SIS_FORM_Q12 || p.form == SIS_FORM_Q8) {
     if (p.problem != SIS_MAXCUT || p.dynamics != SIS_DSB || !p.fused)
-      return fail(SIS_ERR_ARG, "sis_solver_create: the 12-bit form is fused MaxCut dSB only");
+      return fail(SIS_ERR_ARG, "sis_solver_create: the narrow-state forms are fused MaxCut dSB only");
     if (!(p.dt > 0))
-      return fail(SIS_ERR_ARG, "sis_solver_create: the 12-bit form needs dt > 0");
+      return fail(SIS_ERR_ARG, "sis_solver_create: the narrow-state forms need dt > 0");
     if (p.split != 0)
-      return fail(SIS_ERR_ARG, "sis_solver_create: the 12-bit form has no row split");
+      return fail(SIS_ERR_ARG, "sis_solver_create: the narrow-state forms have no row split");
   }
   if (p.agents < 1 || p.agents > (1 << 18))
     return fail(SIS_ERR_ARG, "sis_solver_create: agents must be in [1, 262144]");
@@ -518,8 +528,17 @@ int sis_solver_create(const sis_csr *J, const double *field, const sis_params *p
     }
   }
   s->A = p.agents;
-  s->lda = p.form == SIS_FORM_Q12 ? (s->A + 255) / 256 * 256 : (s->A + 63) / 64 * 64;
-  s->qshift = p.form == SIS_FORM_Q12 ? q12_shift_for(p.dt) : 0;
+  const bool narrow = p.form == SIS_FORM_Q12 || p.form == SIS_FORM_Q8;
+  s->lda = narrow ? (s->A + 255) / 256 * 256 : (s->A + 63) / 64 * 64;
+  s->qshift = narrow ? qn_shift_for(p.form == SIS_FORM_Q12 ? 12 : 8, p.dt) : 0;
+  if (narrow) {
+    // The step leaves the Y clamp out, which is exact only while the Y code
+    // range covers the bound 2/dt + dt of a live element's y.
+    const int qw = p.form == SIS_FORM_Q12 ? 12 : 8;
+    if (std::ldexp((double)((1 << (qw - 1)) - 1), -s->qshift) < 2.0 / p.dt + p.dt)
+      return fail(SIS_ERR_ARG, "sis_solver_create: dt too small for the narrow-state form's "
+                               "Y code range (needs (2^(w-1) - 1) >= 2/dt + dt)");
+  }
   s->W = s->lda / 64;
   s->f64 = p.form == SIS_FORM_F64;
   s->gather = p.dynamics == SIS_DSB ? G_DSB : G_BSB;
@@ -576,9 +595,11 @@ int sis_solver_create(const sis_csr *J, const double *field, const sis_params *p
     if ((st = s->Xd[0].alloc((size_t)NL)) || (st = s->Xd[1].alloc((size_t)NL)) ||
         (st = s->Yd.alloc((size_t)NL)))
       return st;
-  } else if (p.form == SIS_FORM_Q12) {
-    // three dword planes of N x (lda / 256) x 64 lanes: 3 bytes per node-agent
-    if ((st = s->Xq.alloc((size_t)(3 * N * (s->lda / 4)))) ||
+  } else if (p.form == SIS_FORM_Q12 || p.form == SIS_FORM_Q8) {
+    // three (12-bit) or two (8-bit) dword planes of N x (lda / 256) x 64
+    // lanes: 3 or 2 bytes per node-agent
+    const int64_t np = p.form == SIS_FORM_Q12 ? 3 : 2;
+    if ((st = s->Xq.alloc((size_t)(np * N * (s->lda / 4)))) ||
         (st = s->sg[0].alloc((size_t)NW)) || (st = s->sg[1].alloc((size_t)NW)))
       return st;
   } else {
@@ -737,15 +758,16 @@ static int refresh_shadow(sis_solver *s) {
   if (s->p.form == SIS_FORM_BF16) launch_bf16_shadow(s->stream, s->Xf[0].p, s->sh[s->par].p, NL);
   if (s->p.form == SIS_FORM_PACKED)
     launch_pack_signs(s->stream, SRC_F32, s->Xf[0].p, s->sg[s->par].p, s->N, s->W, s->lda);
-  if (s->q12()) launch_pack_signs_q12(s->stream, s->Xq.p, s->sg[s->par].p, s->N, s->lda);
+  if (s->narrow())
+    launch_pack_signs_qn(s->stream, s->qw(), s->Xq.p, s->sg[s->par].p, s->N, s->lda);
   HIPCHK(hipGetLastError());
   return SIS_OK;
 }
 
 // Pre-step (src/kernels.cu:8-22 at t = 0) on the current state.
 static void enqueue_prestep(sis_solver *s) {
-  if (s->q12())
-    launch_prestep_q12(s->stream, s->Xq.p, s->N, s->lda, s->p.dt, s->qshift);
+  if (s->narrow())
+    launch_prestep_qn(s->stream, s->qw(), s->Xq.p, s->N, s->lda, s->p.dt, s->qshift);
   else
     launch_prestep(s->stream, s->f64, s->Xcur(), s->Ybuf(), s->N * s->lda, s->p.dt);
 }
@@ -831,6 +853,7 @@ static void enqueue_step(sis_solver *s, int32_t off) {
       a.X = s->Xf[0].p;
       break;
     case SIS_FORM_Q12:
+    case SIS_FORM_Q8:
       a.gsrc = s->sg[cur].p;
       a.gdst = s->sg[nxt].p;
       a.X = s->Xq.p;
@@ -929,7 +952,8 @@ static void enqueue_window(sis_solver *s, int64_t P, int uw) {
   const void *x;
   switch (s->p.form) {
     case SIS_FORM_PACKED:
-    case SIS_FORM_Q12: src = SRC_BITS; x = s->sg[s->par].p; break;
+    case SIS_FORM_Q12:
+    case SIS_FORM_Q8: src = SRC_BITS; x = s->sg[s->par].p; break;
     case SIS_FORM_F64: src = SRC_F64; x = s->Xcur(); break;
     default: src = SRC_F32; x = s->Xcur(); break;
   }
@@ -1022,8 +1046,8 @@ int sis_solver_init_random(sis_solver *s, uint64_t seed) {
   int st = reset_windows(s);
   if (st) return st;
   s->par = 0;
-  if (s->q12())
-    launch_init_q12(s->stream, s->Xq.p, s->N, s->A, s->lda, seed, s->qshift);
+  if (s->narrow())
+    launch_init_qn(s->stream, s->qw(), s->Xq.p, s->N, s->A, s->lda, seed, s->qshift);
   else
     launch_init_state(s->stream, s->f64, s->Xbuf(0), s->Ybuf(), s->N, s->A, s->lda, seed,
                       s->p.problem == SIS_TSP);
@@ -1040,14 +1064,15 @@ int sis_solver_set_state(sis_solver *s, const double *x, const double *y) {
   if (st) return st;
   s->par = 0;
   const int64_t N = s->N, A = s->A, lda = s->lda;
-  if (s->q12()) {
+  if (s->narrow()) {
     // FP32 values (as the other FP32 forms take them), rounded to codes;
     // padding agents 0
-    const int64_t Q = lda / 256;
-    std::vector<uint32_t> P((size_t)(3 * N * Q * 64), 0u);
+    const int64_t Q = lda / 256, qw = s->qw();
+    std::vector<uint32_t> P((size_t)(s->Xq.n), 0u);
     for (int64_t i = 0; i < N; i++)
       for (int64_t a = 0; a < A; a++)
-        q12_put(P, N, Q, i, a, q12_unit_host((float)x[i * A + a], (float)y[i * A + a], s->qshift));
+        qn_put(qw, P, N, Q, i, a,
+               qn_unit_host(qw, (float)x[i * A + a], (float)y[i * A + a], s->qshift));
     HIPCHK(hipMemcpy(s->Xq.p, P.data(), sizeof(uint32_t) * P.size(), hipMemcpyHostToDevice));
   } else if (s->f64) {
     std::vector<double> bx((size_t)(N * lda), 0.0), by((size_t)(N * lda), 0.0);
@@ -1093,16 +1118,19 @@ int sis_solver_get_state(sis_solver *s, double *x, double *y) {
     }
     return SIS_OK;
   };
-  if (s->q12()) {
+  if (s->narrow()) {
     const int64_t Q = lda / 256;
-    std::vector<uint32_t> P((size_t)(3 * N * Q * 64));
+    const int qw = s->qw();
+    const uint32_t m = (1u << qw) - 1;
+    const int32_t bias = 1 << (qw - 1);
+    std::vector<uint32_t> P((size_t)(s->Xq.n));
     HIPCHK(hipMemcpy(P.data(), s->Xq.p, sizeof(uint32_t) * P.size(), hipMemcpyDeviceToHost));
     const double yinv = std::ldexp(1.0, -s->qshift);
     for (int64_t i = 0; i < N; i++)
       for (int64_t a = 0; a < A; a++) {
-        const uint32_t u = q12_get(P, N, Q, i, a);
-        const int32_t xq = (int32_t)(u & 0xfffu) - 2048, yq = (int32_t)((u >> 12) & 0xfffu) - 2048;
-        if (x) x[i * A + a] = std::ldexp((double)xq, -10);
+        const uint32_t u = qn_get(qw, P, N, Q, i, a);
+        const int32_t xq = (int32_t)(u & m) - bias, yq = (int32_t)((u >> qw) & m) - bias;
+        if (x) x[i * A + a] = std::ldexp((double)xq, -(qw - 2));
         if (y) y[i * A + a] = (double)yq * yinv;
       }
     return SIS_OK;
@@ -1154,21 +1182,21 @@ int sis_solver_autotune(sis_solver *s, int64_t trial_steps, int32_t *tile, doubl
   // Snapshot of the state every candidate starts from (and that is restored).
   const int64_t NL = s->N * s->lda, NW = s->N * s->W;
   // the narrow form keeps X and Y codes in one buffer
-  const size_t xbytes = s->q12() ? sizeof(uint32_t) * s->Xq.n : (size_t)NL * (s->f64 ? 8 : 4);
-  const size_t ybytes = s->q12() ? 0 : xbytes;
+  const size_t xbytes = s->narrow() ? sizeof(uint32_t) * s->Xq.n : (size_t)NL * (s->f64 ? 8 : 4);
+  const size_t ybytes = s->narrow() ? 0 : xbytes;
   size_t gbytes = 0;
   void *gcur = nullptr;
   if (s->p.form == SIS_FORM_BF16) {
     gbytes = (size_t)NL * 2;
     gcur = s->sh[s->par].p;
-  } else if (s->p.form == SIS_FORM_PACKED || s->q12()) {
+  } else if (s->p.form == SIS_FORM_PACKED || s->narrow()) {
     gbytes = (size_t)NW * 8;
     gcur = s->sg[s->par].p;
   }
   DevBuf<char> sx, sy, sgb;
   if ((st = sx.alloc(xbytes)) || (st = sy.alloc(ybytes)) || (st = sgb.alloc(gbytes))) return st;
   const int par0 = s->par;
-  void *x0 = s->q12() ? (void *)s->Xq.p : s->Xcur();
+  void *x0 = s->narrow() ? (void *)s->Xq.p : s->Xcur();
   HIPCHK(hipMemcpyAsync(sx.p, x0, xbytes, hipMemcpyDeviceToDevice, s->stream));
   if (ybytes) HIPCHK(hipMemcpyAsync(sy.p, s->Ybuf(), ybytes, hipMemcpyDeviceToDevice, s->stream));
   if (gbytes) HIPCHK(hipMemcpyAsync(sgb.p, gcur, gbytes, hipMemcpyDeviceToDevice, s->stream));

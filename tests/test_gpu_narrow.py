@@ -1,10 +1,11 @@
-"""GPU parity of the narrow-state dSB form (SIS_FORM_Q12: X and Y stored as
-12-bit fixed-point codes, FP32 arithmetic, exact J sgn(X) from packed sign
-words) against the oracle's restatement of the same form (orc_*_q12).
+"""GPU parity of the narrow-state dSB forms (SIS_FORM_Q12 / SIS_FORM_Q8: X
+and Y stored as 12-bit / 8-bit fixed-point codes, FP32 arithmetic, exact
+J sgn(X) from packed sign words) against the oracle's restatement of the same
+forms (orc_*_qn with width w = 12 / 8).
 
 Bar: bit-exact.  The codes, the FP32 update on the decoded values and the
 rounding are deterministic, so the device's X, Y codes, spins, locks, step
-counts and cuts equal the oracle's exactly, for every tile, agent count
+counts and cuts equal the oracle's exactly, for every width, tile, agent count
 (padded to 256 per quad), launch mode and window setting.
 """
 import numpy as np
@@ -16,7 +17,8 @@ import sis
 
 pytestmark = pytest.mark.gpu
 
-Q12 = sis.FORM_Q12
+FORMW = {12: sis.FORM_Q12, 8: sis.FORM_Q8}
+XS = {12: 10, 8: 6}  # X code scale 2^xs; the wall code is +-(2^xs + 1)
 VG = 1 << 17  # staged gathers: a wavefront's rows' entries fetched at once into LDS
 TILES = [4 | 1 << 8, 4 | 2 << 8, 4 | 4 << 8, 4 | 1 << 8 | 1 << 16, 4 | 2 << 8 | 1 << 16,
          4 | 4 << 8 | 1 << 16, 4 | 4 << 8 | VG, 4 | 8 << 8 | VG | 1 << 16, 4 | 16 << 8 | VG,
@@ -36,63 +38,69 @@ def seeded(n, A, seed, scale=0.1):
     return X, Y
 
 
-def test_init_random_equals_rounded_oracle_stream():
+@pytest.mark.parametrize("w", [12, 8])
+def test_init_random_equals_rounded_oracle_stream(w):
     J, _ = problem(70)
     for A in (64, 300):
-        s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=A, dt=0.5, form=Q12))
+        s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=A, dt=0.5, form=FORMW[w]))
         s.init_random(9)
         X, Y = orc.init_state(9, orc.MAXCUT, J.n, A)
-        orc.q12_round(X, Y, orc.q12_shift(0.5))
+        orc.qn_round(X, Y, orc.qn_shift(0.5, w), w)
         x, y = s.get_state()
         assert np.array_equal(x, X.astype(np.float64)) and np.array_equal(y, Y.astype(np.float64))
 
 
+@pytest.mark.parametrize("w", [12, 8])
 @pytest.mark.parametrize("dt", [0.5, 0.1])
-def test_set_state_rounds_like_the_oracle(dt):
+def test_set_state_rounds_like_the_oracle(w, dt):
     # values off the grid, halfway cases, beyond the wall, negative zero and
     # Y beyond the code range: the device's codes are the oracle's rounding
     J, _ = problem(50)
     A = 130
+    top = 2.0 ** XS[w]
+    sy = orc.qn_shift(dt, w)
     X, Y = seeded(J.n, A, 3, scale=1.5)
-    X[0, :8] = np.array([2.5, 3.5, -2.5, -0.3, 1024.0, -0.0, 1023.5, -1024.5]) / 1024
-    Y[1, :4] = [100.0, -100.0, 0.5 / 256, 1.5 / 256]
-    s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=A, dt=dt, form=Q12))
+    X[0, :8] = np.array([2.5, 3.5, -2.5, -0.3, top, -0.0, top - 0.5, -top - 0.5]) / top
+    Y[1, :4] = [100.0, -100.0, 0.5 / 2.0 ** sy, 1.5 / 2.0 ** sy]
+    s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=A, dt=dt, form=FORMW[w]))
     s.set_state(X.astype(np.float64), Y.astype(np.float64))
-    orc.q12_round(X, Y, orc.q12_shift(dt))
+    orc.qn_round(X, Y, sy, w)
     x, y = s.get_state()
     assert np.array_equal(x, X.astype(np.float64)) and np.array_equal(y, Y.astype(np.float64))
     assert not np.signbit(x[x == 0]).any()
 
 
+@pytest.mark.parametrize("w", [12, 8])
 @pytest.mark.parametrize("tile", TILES)
 @pytest.mark.parametrize("agents", [96, 256, 300])
-def test_trajectory_bit_exact(tile, agents):
+def test_trajectory_bit_exact(w, tile, agents):
     # 203 rows (not a multiple of the rows per wavefront), integer weights,
     # a state that reaches the walls; two advances cross p = 1 at t = 390
     J, pb = problem()
     T, dt = 400, 0.5
-    s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=agents, dt=dt, form=Q12, tile=tile))
+    s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=agents, dt=dt, form=FORMW[w], tile=tile))
     assert s.p.tile == tile
-    sy = orc.q12_shift(dt)
+    sy = orc.qn_shift(dt, w)
     X, Y = seeded(J.n, agents, 13, scale=0.9)
     s.set_state(X.astype(np.float64), Y.astype(np.float64))
-    orc.q12_round(X, Y, sy)
+    orc.qn_round(X, Y, sy, w)
     s.prestep()
-    orc.prestep_q12(X, Y, dt, sy)
+    orc.prestep_qn(X, Y, dt, sy, w)
     x, y = s.get_state()
     assert np.array_equal(x, X.astype(np.float64)) and np.array_equal(y, Y.astype(np.float64))
     for t0, n in ((0, 150), (150, 250)):
         s.advance(t0, n, T)
-        orc.advance_q12(pb, X, Y, t0, n, T, dt, s.p.xi0, sy)
+        orc.advance_qn(pb, X, Y, t0, n, T, dt, s.p.xi0, sy, w)
         x, y = s.get_state()
-        assert np.array_equal(x, X.astype(np.float64)), (tile, agents, t0 + n)
-        assert np.array_equal(y, Y.astype(np.float64)), (tile, agents, t0 + n)
+        assert np.array_equal(x, X.astype(np.float64)), (w, tile, agents, t0 + n)
+        assert np.array_equal(y, Y.astype(np.float64)), (w, tile, agents, t0 + n)
     assert (np.abs(X) > 1).any()  # the wall code occurred
 
 
+@pytest.mark.parametrize("w", [12, 8])
 @pytest.mark.parametrize("tile", [4 | 4 << 8 | VG, 4 | 8 << 8 | VG | 1 << 16, 4 | 32 << 8 | VG])
 @pytest.mark.parametrize("agents", [96, 300])
-def test_unit_weight_waves_beside_real_weight_waves(tile, agents):
+def test_unit_weight_waves_beside_real_weight_waves(w, tile, agents):
     # +-1 weights except in rows 40..47 (2 and -0.75) and row 150 (an exact
     # zero): waves whose staged weights are all +-1 count sign bits, the
     # others add +-|v|; both equal the oracle's CSR-order FP32 sums
@@ -107,24 +115,25 @@ def test_unit_weight_waves_beside_real_weight_waves(tile, agents):
     rp, col, val = J.arrays()
     pb = orc.Problem(rp, col, val, S=J.sum)
     T, dt = 400, 0.5
-    s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=agents, dt=dt, form=Q12, tile=tile))
+    s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=agents, dt=dt, form=FORMW[w], tile=tile))
     assert s.p.tile == tile
-    sy = orc.q12_shift(dt)
+    sy = orc.qn_shift(dt, w)
     X, Y = seeded(J.n, agents, 17, scale=0.9)
     s.set_state(X.astype(np.float64), Y.astype(np.float64))
-    orc.q12_round(X, Y, sy)
+    orc.qn_round(X, Y, sy, w)
     for t0, n in ((0, 150), (150, 250)):
         s.advance(t0, n, T)
-        orc.advance_q12(pb, X, Y, t0, n, T, dt, s.p.xi0, sy)
+        orc.advance_qn(pb, X, Y, t0, n, T, dt, s.p.xi0, sy, w)
         x, y = s.get_state()
-        assert np.array_equal(x, X.astype(np.float64)), (tile, agents, t0 + n)
-        assert np.array_equal(y, Y.astype(np.float64)), (tile, agents, t0 + n)
+        assert np.array_equal(x, X.astype(np.float64)), (w, tile, agents, t0 + n)
+        assert np.array_equal(y, Y.astype(np.float64)), (w, tile, agents, t0 + n)
     assert (np.abs(X) > 1).any()
 
 
+@pytest.mark.parametrize("w", [12, 8])
 @pytest.mark.parametrize("tile", [4 | 8 << 8 | VG, 4 | 16 << 8 | VG | 1 << 16, 4 | 4 << 8 | VG])
 @pytest.mark.parametrize("unit", [False, True])
-def test_staged_gathers_hub_and_empty_rows(tile, unit):
+def test_staged_gathers_hub_and_empty_rows(w, tile, unit):
     # 150 rows: a hub joined to rows 1..140 (its wave exceeds the staging
     # capacity and takes the scalar path), a sparse tail among rows 1..129,
     # rows 141..149 without entries (waves whose rows hold no entry at all),
@@ -142,30 +151,31 @@ def test_staged_gathers_hub_and_empty_rows(tile, unit):
     assert deg[0] > 128 and (deg[141:] == 0).all()
     pb = orc.Problem(rp, col, val, S=J.sum)
     A, T, dt = 256, 300, 0.5
-    sy = orc.q12_shift(dt)
-    s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=A, dt=dt, form=Q12, tile=tile, xi0=0.2))
+    sy = orc.qn_shift(dt, w)
+    s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=A, dt=dt, form=FORMW[w], tile=tile, xi0=0.2))
     assert s.p.tile == tile
     X, Y = seeded(J.n, A, 32, scale=0.5)
     s.set_state(X.astype(np.float64), Y.astype(np.float64))
-    orc.q12_round(X, Y, sy)
+    orc.qn_round(X, Y, sy, w)
     s.advance(0, 251, T)
-    orc.advance_q12(pb, X, Y, 0, 251, T, dt, 0.2, sy)
+    orc.advance_qn(pb, X, Y, 0, 251, T, dt, 0.2, sy, w)
     x, y = s.get_state()
     assert np.array_equal(x, X.astype(np.float64)) and np.array_equal(y, Y.astype(np.float64))
 
 
+@pytest.mark.parametrize("w", [12, 8])
 @pytest.mark.parametrize("uw,T", [(1, 6000), (0, 2345)])
 @pytest.mark.parametrize("graph", [0, 1, 2])
-def test_full_run_matches_oracle(uw, T, graph):
+def test_full_run_matches_oracle(w, uw, T, graph):
     J, pb = problem(120, 0.06, sis.W_PM1, 5)
     A, dt = 100, 0.5
-    s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=A, dt=dt, form=Q12, use_graph=graph))
+    s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=A, dt=dt, form=FORMW[w], use_graph=graph))
     runs = 2 if graph else 1
     for _ in range(runs):  # graph replays across runs must agree with eager
         s.init_random(21)
         info = s.run(T, uw)
         X, Y = orc.init_state(21, orc.MAXCUT, J.n, A)
-        r = orc.solve_q12(pb, X, Y, T, uw, P=100, K=25, dt=dt, xi0=s.p.xi0)
+        r = orc.solve_qn(pb, X, Y, T, uw, P=100, K=25, dt=dt, xi0=s.p.xi0, w=w)
         assert info["steps_executed"] == r["steps"]
         x, y = s.get_state()
         assert np.array_equal(x, X.astype(np.float64)) and np.array_equal(y, Y.astype(np.float64))
@@ -179,10 +189,11 @@ def test_full_run_matches_oracle(uw, T, graph):
         assert info["all_bifurcated"] == int(uw == 1 and r["bif"].all())
 
 
-def test_autotune_keeps_state_and_results():
+@pytest.mark.parametrize("w", [12, 8])
+def test_autotune_keeps_state_and_results(w):
     J, pb = problem(300, 0.03, sis.W_PM1, 14)
     A, dt = 256, 0.5
-    s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=A, dt=dt, form=Q12))
+    s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=A, dt=dt, form=FORMW[w]))
     s.init_random(15)
     x0, y0 = s.get_state()
     tile, ms = s.autotune(4)
@@ -191,13 +202,14 @@ def test_autotune_keeps_state_and_results():
     assert np.array_equal(x, x0) and np.array_equal(y, y0)
     info = s.run(700, 0)
     X, Y = orc.init_state(15, orc.MAXCUT, J.n, A)
-    r = orc.solve_q12(pb, X, Y, 700, 0, P=100, dt=dt, xi0=s.p.xi0)
+    r = orc.solve_qn(pb, X, Y, 700, 0, P=100, dt=dt, xi0=s.p.xi0, w=w)
     assert info["steps_executed"] == r["steps"]
     assert np.array_equal(s.get_state()[0], X.astype(np.float64))
     assert np.array_equal(s.cuts()[0], orc.cut(pb, r["cur"]))
 
 
-def test_full_size_on_agent_sample_and_exact_cut():
+@pytest.mark.parametrize("w", [12, 8])
+def test_full_size_on_agent_sample_and_exact_cut(w):
     """Random 4-regular N = 2^20, 256 agents (the benchmark graph): 30 steps
     after the pre-step, every 8th agent bit-exact with the oracle; after a
     300-step run the device cut of the returned spins equals an exact
@@ -206,17 +218,17 @@ def test_full_size_on_agent_sample_and_exact_cut():
     rp, col, val = J.arrays()
     pb = orc.Problem(rp, col, val, S=J.sum)
     A, T, dt = 256, 20000, 0.5
-    sy = orc.q12_shift(dt)
-    s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=A, dt=dt, form=Q12))
+    sy = orc.qn_shift(dt, w)
+    s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=A, dt=dt, form=FORMW[w]))
     s.init_random(77)
     s.prestep()
     s.advance(0, 30, T)
     cols = np.arange(0, A, 8)
     X, Y = orc.init_state(77, orc.MAXCUT, J.n, A)
     X, Y = np.ascontiguousarray(X[:, cols]), np.ascontiguousarray(Y[:, cols])
-    orc.q12_round(X, Y, sy)
-    orc.prestep_q12(X, Y, dt, sy)
-    orc.advance_q12(pb, X, Y, 0, 30, T, dt, s.p.xi0, sy)
+    orc.qn_round(X, Y, sy, w)
+    orc.prestep_qn(X, Y, dt, sy, w)
+    orc.advance_qn(pb, X, Y, 0, 30, T, dt, s.p.xi0, sy, w)
     x, y = s.get_state()
     assert np.array_equal(x[:, cols], X.astype(np.float64))
     assert np.array_equal(y[:, cols], Y.astype(np.float64))
@@ -229,17 +241,20 @@ def test_full_size_on_agent_sample_and_exact_cut():
     assert np.array_equal(cuts, -0.5 * (H - J.sum))
 
 
-def test_rejected_where_it_does_not_apply():
+@pytest.mark.parametrize("w", [12, 8])
+def test_rejected_where_it_does_not_apply(w):
     J, _ = problem(60)
+    # dt below the Y code range's bound (127 < 2/dt + dt for 8 bits, 2047 for 12)
+    tiny = 0.015 if w == 8 else 0.0009
     bad = [dict(dynamics=sis.BSB), dict(fused=0), dict(dt=0.0), dict(split=8), dict(split=-1),
-           dict(tile=1 << 20)]
+           dict(tile=1 << 20), dict(dt=tiny)]
     for extra in bad:
-        params = dict(problem=sis.MAXCUT, agents=64, dt=0.5, form=Q12)
+        params = dict(problem=sis.MAXCUT, agents=64, dt=0.5, form=FORMW[w])
         params.update(extra)
         with pytest.raises(sis.SisError):
             sis.Solver(J, params)
     t = sis.Tsp.generate(5, 100, 0, 1)
     Jt, h = t.qubo()
     with pytest.raises(sis.SisError):
-        sis.Solver(Jt, dict(problem=sis.TSP, agents=64, form=Q12, dynamics=sis.DSB, ncities=5),
+        sis.Solver(Jt, dict(problem=sis.TSP, agents=64, form=FORMW[w], dynamics=sis.DSB, ncities=5),
                    field=h)
