@@ -409,58 +409,89 @@ __device__ __forceinline__ void qn_step_body(const StepArgs &a) {
     const int64_t ne = (int64_t)readlane64((uint64_t)rpl, rl) - kb;
     if (ne <= kQ12Stage) {
       const int off = (int)(rpl - kb);
-      // entry e: the low dwords of its four sign words, then the high
-      // dwords, so lanes 0-31 and 32-63 each read the 16 bytes holding their
-      // bits; all complemented where the weight is negative, so that a set
-      // bit means +|v|.  lval: -|v|.
       __shared__ uint4 lsig[kWaves][kQ12Stage][2];
       __shared__ float lval[kWaves][kQ12Stage];
       const int wv = threadIdx.x >> 6, half = lane >> 5, bit = lane & 31, sh = 31 - bit;
+      // Pass 1: the columns and weights of this lane's entries (lane, lane + 64).
+      constexpr int kSlots = kQ12Stage / 64;
+      int64_t ce[kSlots];
+      float ve[kSlots];
       bool unit = true;
-      for (int64_t e = lane; e < ne; e += 64) {
-        const int64_t c = col[kb + e];
-        const float v = val[kb + e];
-        const uint32_t f = v < 0.0f ? 0xffffffffu : 0u;
-        const uint4 *sp = reinterpret_cast<const uint4 *>(S + c * W + w0);
-        const uint4 s01 = sp[0], s23 = sp[1];  // words 0,1 and 2,3 as dword pairs
-        lsig[wv][e][0] = make_uint4(s01.x ^ f, s01.z ^ f, s23.x ^ f, s23.z ^ f);
-        lsig[wv][e][1] = make_uint4(s01.y ^ f, s01.w ^ f, s23.y ^ f, s23.w ^ f);
-        lval[wv][e] = -__builtin_fabsf(v);
-        unit = unit && __builtin_fabsf(v) == 1.0f;
+#pragma unroll
+      for (int u = 0; u < kSlots; u++) {
+        const int64_t e = lane + 64 * u;
+        ce[u] = 0;
+        ve[u] = 1.0f;
+        if (e < ne) {
+          ce[u] = col[kb + e];
+          ve[u] = val[kb + e];
+          unit = unit && __builtin_fabsf(ve[u]) == 1.0f;
+        }
       }
       const bool all_unit = __all(unit);
+      // Pass 2: the four sign words of each entry, complemented where the
+      // weight is negative so that a set bit means +|v|.  Lanes 0-31 use the
+      // words' low dwords L0..L3, lanes 32-63 the high dwords.  With real
+      // weights, entry e holds L0..L3 of each half (a lane reads the 16 bytes
+      // holding its bits) and lval[e] = -|v|.  With +-1 weights, entry e
+      // holds the byte transpose T0..T3 of each half's L0..L3:
+      // byte j of Tb = byte b of Lj, so lane l's four bits (bit l mod 32 of
+      // L0..L3) sit at bits (l mod 8) + 8 j of the one dword T(l mod 32 / 8).
+#pragma unroll
+      for (int u = 0; u < kSlots; u++) {
+        const int64_t e = lane + 64 * u;
+        if (e < ne) {
+          const uint32_t f = ve[u] < 0.0f ? 0xffffffffu : 0u;
+          const uint4 *sp = reinterpret_cast<const uint4 *>(S + ce[u] * W + w0);
+          const uint4 s01 = sp[0], s23 = sp[1];  // words 0,1 and 2,3 as dword pairs
+          const uint4 lo = make_uint4(s01.x ^ f, s01.z ^ f, s23.x ^ f, s23.z ^ f);
+          const uint4 hi = make_uint4(s01.y ^ f, s01.w ^ f, s23.y ^ f, s23.w ^ f);
+          if (all_unit) {
+            auto transpose = [](const uint4 &L) {
+              // 2 x 2 byte blocks, then pairs of half-words
+              const uint32_t A = __builtin_amdgcn_perm(L.y, L.x, 0x06020400u);  // L0b0 L1b0 L0b2 L1b2
+              const uint32_t B = __builtin_amdgcn_perm(L.y, L.x, 0x07030501u);  // L0b1 L1b1 L0b3 L1b3
+              const uint32_t C = __builtin_amdgcn_perm(L.w, L.z, 0x06020400u);  // L2b0 L3b0 L2b2 L3b2
+              const uint32_t D = __builtin_amdgcn_perm(L.w, L.z, 0x07030501u);  // L2b1 L3b1 L2b3 L3b3
+              return make_uint4(__builtin_amdgcn_perm(C, A, 0x05040100u),   // b0 of L0..L3
+                                __builtin_amdgcn_perm(D, B, 0x05040100u),   // b1
+                                __builtin_amdgcn_perm(C, A, 0x07060302u),   // b2
+                                __builtin_amdgcn_perm(D, B, 0x07060302u));  // b3
+            };
+            lsig[wv][e][0] = transpose(lo);
+            lsig[wv][e][1] = transpose(hi);
+          } else {
+            lsig[wv][e][0] = lo;
+            lsig[wv][e][1] = hi;
+            lval[wv][e] = -__builtin_fabsf(ve[u]);
+          }
+        }
+      }
       // the wavefront's LDS writes complete before its reads below
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       if (all_unit) {
         // weights +-1: F = (entries with the bit set) - (the others), an
-        // integer, as the FP32 sum of +-1 terms is in any order
+        // integer, as the FP32 sum of +-1 terms is in any order.  The four
+        // counts accumulate as bytes of one dword (a row has at most
+        // kQ12Stage < 256 entries here): one shift, mask and add per entry.
+        const uint32_t *lt = reinterpret_cast<const uint32_t *>(&lsig[wv][0][half]) + (bit >> 3);
+        const int bs = bit & 7;
 #pragma unroll
         for (int r = 0; r < R; r++) {
           if (r >= rl) break;
           const int k0 = __builtin_amdgcn_readlane(off, r);
           const int k1 = __builtin_amdgcn_readlane(off, r + 1);
-          uint32_t cnt[4] = {0u, 0u, 0u, 0u};
+          uint32_t acc = 0u;
           int k = k0;
-          for (; k + 2 <= k1; k += 2) {
-            const uint4 d0 = lsig[wv][k][half], d1 = lsig[wv][k + 1][half];
-            cnt[0] += ((d0.x >> bit) & 1u) + ((d1.x >> bit) & 1u);
-            cnt[1] += ((d0.y >> bit) & 1u) + ((d1.y >> bit) & 1u);
-            cnt[2] += ((d0.z >> bit) & 1u) + ((d1.z >> bit) & 1u);
-            cnt[3] += ((d0.w >> bit) & 1u) + ((d1.w >> bit) & 1u);
-          }
-          if (k < k1) {
-            const uint4 d0 = lsig[wv][k][half];
-            cnt[0] += (d0.x >> bit) & 1u;
-            cnt[1] += (d0.y >> bit) & 1u;
-            cnt[2] += (d0.z >> bit) & 1u;
-            cnt[3] += (d0.w >> bit) & 1u;
-          }
-          const int deg = k1 - k0;
+          for (; k + 2 <= k1; k += 2)
+            acc += ((lt[8 * k] >> bs) & 0x01010101u) + ((lt[8 * k + 8] >> bs) & 0x01010101u);
+          if (k < k1) acc += (lt[8 * k] >> bs) & 0x01010101u;
+          const float md = -(float)(k1 - k0);
           float F[4];
 #pragma unroll
-          for (int j = 0; j < 4; j++) F[j] = (float)(2 * (int)cnt[j] - deg);
+          for (int j = 0; j < 4; j++) F[j] = __builtin_fmaf((float)((acc >> (8 * j)) & 0xffu), 2.0f, md);
           const int64_t row = row0 + r;
           qn_finish_row<QW, NT>(d[r], F, kc, P + (row * Q + q) * 64, ps, Sn + row * W + w0, lane);
         }
