@@ -73,22 +73,23 @@ def test_maxcut_line_matches_oracle(tmp_path, fused, knobs):
 
 
 @pytest.mark.gpu
-def test_maxcut_line_q12_form_matches_oracle(tmp_path):
-    """SIS_FORM=q12 (12-bit X/Y codes): the line's cut statistics equal the
-    oracle's narrow-state solve from the same seed."""
+@pytest.mark.parametrize("form,w", [("q12", 12), ("q8", 8)])
+def test_maxcut_line_narrow_form_matches_oracle(tmp_path, form, w):
+    """SIS_FORM=q12 / q8 (12-bit / 8-bit X/Y codes): the line's cut statistics
+    equal the oracle's narrow-state solve from the same seed."""
     J = sis.Csr.gen_er(300, 0.05, sis.W_PM1, 3)
     mtx = tmp_path / "g.mtx"
     J.write_mtx(mtx)
     out = tmp_path / "out.csv"
     A, T, dt, seed = 128, 1500, 0.5, 99
     r = run(["0", mtx, out, "maxcut", "1", "1", "1", dt, T, A],
-            env={"SIS_SEED": str(seed), "SIS_FORM": "q12"})
+            env={"SIS_SEED": str(seed), "SIS_FORM": form})
     assert r.returncode == 0, r.stderr
     f = _fields(out.read_text())
     rp, col, val = J.arrays()
     pb = orc.Problem(rp, col, val, S=J.sum)
     X, Y = orc.init_state(seed, orc.MAXCUT, J.n, A)
-    res = orc.solve_q12(pb, X, Y, T, 1, P=100, K=25, dt=dt, xi0=J.xi0())
+    res = orc.solve_qn(pb, X, Y, T, 1, P=100, K=25, dt=dt, xi0=J.xi0(), w=w)
     assert [float(v) for v in f[1:6]] == list(orc.cut_stats(orc.cut(pb, res["cur"])))
 
 
