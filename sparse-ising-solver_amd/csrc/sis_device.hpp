@@ -375,4 +375,18 @@ __device__ __forceinline__ void sb_update_maxcut_sel(float &x, float &y, float F
   y = __builtin_fmaf(c1, x, y);
 }
 
+// The same update for loops that carry x in a fixed register across
+// steps: the x FMA is the three-operand v_fma_f32, so it writes x's register
+// from the clamped copy directly (the two-operand form the compiler picks
+// needs a move there).  Same operations, same rounding, same result.
+__device__ __forceinline__ void sb_update_maxcut_carried(float &x, float &y, float F, float c0,
+                                                         float c1, float c2) {
+  const bool conf = __builtin_fabsf(x) > 1.0f;
+  const float yk = __builtin_fmaf(c2, F, y);
+  const float xm = __builtin_amdgcn_fmed3f(x, -1.0f, 1.0f);  // sgn(x) where |x| > 1
+  y = conf ? 0.0f : yk;
+  asm("v_fma_f32 %0, %1, %2, %3" : "=v"(x) : "v"(c0), "v"(y), "v"(xm));
+  y = __builtin_fmaf(c1, x, y);
+}
+
 }  // namespace sis

@@ -1697,7 +1697,7 @@ __global__ __launch_bounds__(kResBlock, 8) void k_resident_reg(StepArgs a, ResAr
       // pairs past N G update zeros and are neither stored nor gathered
 #pragma unroll
       for (int b = 0; b < 2; b++)
-        if (m0 + b < M) sb_update_maxcut_sel(x[m0 + b], y[m0 + b], (float)Fi[b], c0, c1, c2);
+        if (m0 + b < M) sb_update_maxcut_carried(x[m0 + b], y[m0 + b], (float)Fi[b], c0, c1, c2);
 #pragma unroll
       for (int b = 0; b < 2; b++) {
         const int m = m0 + b, q = t + kResBlock * m;
