@@ -76,8 +76,10 @@ def time_form(J, form, A, dt, K, W, seed, dyn, autotune=True):
     s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=A, dt=dt, form=sis.FORMS[form], dynamics=dyn))
     tune = None
     if autotune:  # tile choice, outside the timed region; every tile is bit-identical
+        # 50-step trials: the narrow forms' best two tiles are within a few
+        # percent, which 10-step trials do not resolve reliably
         s.init_random(seed + 1)
-        tile, ms = s.autotune(10)
+        tile, ms = s.autotune(50)
         tune = dict(tile=tile, trial_ms_per_step=ms)
     if W > 0:  # untimed warm-up steps (same kernels, discarded state)
         s.init_random(seed + 1)

@@ -242,6 +242,22 @@ typedef struct sis_params {
   int32_t structured;       /* TSP from sis_solver_create_tsp: 1 evaluates J x from row
                                sums, column sums and the distance table instead of the
                                QUBO's CSR (n >= 3; f32 n <= 512, f64 n <= 256) */
+  int32_t row_order;        /* storage order of the rows (no reference counterpart;
+                               every result is the same in either order):
+                               0 = chosen by the solver: a locality order for the
+                               fused MaxCut dSB forms that gather packed sign words
+                               (packed, 12-bit, 8-bit) on graphs whose sign words
+                               exceed an L2 slice (N x agents / 8 > 4 MiB) and whose
+                               weights are integers; 1 = the locality order whenever
+                               those forms and integer weights allow it; -1 = node
+                               order.  The locality order groups rows joined by
+                               edges (multilevel greedy matching, pairs of pairs up
+                               to 64 rows), so a wavefront's sign gathers share
+                               cache lines; states, spins and results still come
+                               and go in node order, each row keeps its entries in
+                               their CSR order, and the seeded initial state is
+                               keyed by node.  sis_solver_params reports 1 if the
+                               solver stores rows in the locality order, else -1. */
 } sis_params;
 
 typedef struct sis_run_info {

@@ -15,14 +15,17 @@ namespace sis {
 
 template <typename T>
 __global__ void k_init_state(T *X, T *Y, int64_t N, int64_t A, int64_t lda, uint64_t seed,
-                             int tsp) {
+                             int tsp, const int32_t *rowid) {
   const int64_t n = N * lda;
   for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
        idx += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = idx / lda, a = idx - i * lda;
     T x = 0, y = 0;
     if (a < A) {
-      const uint64_t e = (uint64_t)(i * A + a);
+      // the element's index in the caller's node order (rows may be stored
+      // in another order: rowid[stored row] = node)
+      const int64_t node = rowid ? (int64_t)rowid[i] : i;
+      const uint64_t e = (uint64_t)(node * A + a);
       const double ux = d_uniform(seed, 2 * e), uy = d_uniform(seed, 2 * e + 1);
       x = tsp ? (T)0 : (T)((ux * 2 - 1) / 10);
       y = (T)((uy * 2 - 1) / 10);
@@ -573,20 +576,22 @@ __global__ __launch_bounds__(kBlock) void k_step_qn(StepArgs a) {
 // k_init_state (element e = i A + a, FP32 values), rounded to codes; padding
 // agents 0.
 template <int QW>
-__global__ void k_init_qn(uint32_t *P, int64_t N, int64_t A, int64_t Q, uint64_t seed, int qshift) {
+__global__ void k_init_qn(uint32_t *P, int64_t N, int64_t A, int64_t Q, uint64_t seed, int qshift,
+                          const int32_t *rowid) {
   constexpr int NP = QFmt<QW>::NP;
   const int64_t n = N * Q * 64, ps = n;
   const Q12K k = qn_consts<QW>(0.0f, 0.0f, 0.0f, qshift);
   for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
        idx += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = idx / (Q * 64), rem = idx - i * Q * 64, q = rem / 64, l = rem - q * 64;
+    const int64_t node = rowid ? (int64_t)rowid[i] : i;
     uint32_t u[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       const int64_t ag = 256 * q + 64 * j + l;
       float x = 0.0f, y = 0.0f;
       if (ag < A) {
-        const uint64_t e = (uint64_t)(i * A + ag);
+        const uint64_t e = (uint64_t)(node * A + ag);
         x = (float)((d_uniform(seed, 2 * e) * 2 - 1) / 10);
         y = (float)((d_uniform(seed, 2 * e + 1) * 2 - 1) / 10);
       }
@@ -2145,14 +2150,14 @@ __global__ __launch_bounds__(1024) void k_best(const double *v, int64_t A, int m
 // ------------------------------------------------------------ launchers
 
 void launch_init_state(hipStream_t st, bool f64, void *X, void *Y, int64_t N, int64_t A,
-                       int64_t lda, uint64_t seed, bool tsp) {
+                       int64_t lda, uint64_t seed, bool tsp, const int32_t *rowid) {
   const unsigned gb = grid_for(N * lda, 256, 16384);
   if (f64)
     hipLaunchKernelGGL(k_init_state<double>, dim3(gb), dim3(256), 0, st, (double *)X, (double *)Y,
-                       N, A, lda, seed, (int)tsp);
+                       N, A, lda, seed, (int)tsp, rowid);
   else
     hipLaunchKernelGGL(k_init_state<float>, dim3(gb), dim3(256), 0, st, (float *)X, (float *)Y, N,
-                       A, lda, seed, (int)tsp);
+                       A, lda, seed, (int)tsp, rowid);
 }
 
 void launch_prestep(hipStream_t st, bool f64, void *X, void *Y, int64_t n, double dt) {
@@ -2167,13 +2172,13 @@ void launch_prestep(hipStream_t st, bool f64, void *X, void *Y, int64_t n, doubl
 }
 
 void launch_init_qn(hipStream_t st, int qw, uint32_t *P, int64_t N, int64_t A, int64_t lda,
-                    uint64_t seed, int qshift) {
+                    uint64_t seed, int qshift, const int32_t *rowid) {
   const int64_t Q = lda / 256;
   const dim3 grid(grid_for(N * Q * 64, 256, 16384));
   if (qw == 12)
-    hipLaunchKernelGGL(k_init_qn<12>, grid, dim3(256), 0, st, P, N, A, Q, seed, qshift);
+    hipLaunchKernelGGL(k_init_qn<12>, grid, dim3(256), 0, st, P, N, A, Q, seed, qshift, rowid);
   else
-    hipLaunchKernelGGL(k_init_qn<8>, grid, dim3(256), 0, st, P, N, A, Q, seed, qshift);
+    hipLaunchKernelGGL(k_init_qn<8>, grid, dim3(256), 0, st, P, N, A, Q, seed, qshift, rowid);
 }
 
 void launch_prestep_qn(hipStream_t st, int qw, uint32_t *P, int64_t N, int64_t lda, double dt,

@@ -43,8 +43,10 @@ constexpr int kBlock = 256;  // 4 wavefronts of 64 lanes
 constexpr int kWaves = kBlock / 64;
 
 // Launchers (defined in sis_kernels.hip).  All take the solver's stream.
+// rowid: the node of each stored row (null: rows in node order); the seeded
+// stream is keyed by node.
 void launch_init_state(hipStream_t st, bool f64, void *X, void *Y, int64_t N, int64_t A,
-                       int64_t lda, uint64_t seed, bool tsp);
+                       int64_t lda, uint64_t seed, bool tsp, const int32_t *rowid);
 void launch_prestep(hipStream_t st, bool f64, void *X, void *Y, int64_t n, double dt);
 void launch_bf16_shadow(hipStream_t st, const float *X, uint16_t *sh, int64_t n);
 void launch_pack_signs(hipStream_t st, int src, const void *X, uint64_t *S, int64_t N, int64_t W,
@@ -161,7 +163,7 @@ void launch_step_split(hipStream_t st, int form, int prob, int gather, int64_t n
 // sis_device.hpp): 3 or 2 dword planes of N x (lda / 256) x 64 lanes; lda is
 // a multiple of 256.
 void launch_init_qn(hipStream_t st, int qw, uint32_t *P, int64_t N, int64_t A, int64_t lda,
-                    uint64_t seed, int qshift);
+                    uint64_t seed, int qshift, const int32_t *rowid);
 void launch_prestep_qn(hipStream_t st, int qw, uint32_t *P, int64_t N, int64_t lda, double dt,
                        int qshift);
 void launch_pack_signs_qn(hipStream_t st, int qw, const uint32_t *P, uint64_t *S, int64_t N,

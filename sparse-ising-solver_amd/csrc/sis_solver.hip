@@ -41,6 +41,12 @@ struct sis_solver {
   DevBuf<uint16_t> sh[2];        // BF16 shadow
   DevBuf<uint64_t> sg[2];        // packed sign words
   DevBuf<uint32_t> Xq;           // narrow forms: X / Y codes, three (12-bit) or two (8-bit) dword planes
+  // Row storage order: node_of[r] is the node stored in row r (empty: node
+  // order); a copy on the device keys the seeded initial state by node.
+  std::vector<int32_t> node_of;
+  DevBuf<int32_t> node_of_d;
+  int64_t row_of(int64_t node) const { return node_of.empty() ? node : row_inv[(size_t)node]; }
+  std::vector<int32_t> row_inv;  // row_inv[node] = its row
   int qshift = 0;                // narrow form: Y codes are round(y 2^qshift)
   int par = 0;                   // index of the current snapshot buffer
 
@@ -437,6 +443,117 @@ static int build_split(sis_solver *s, int64_t tau) {
   return SIS_OK;
 }
 
+// Locality order of the rows: multilevel greedy matching.  Level 1 pairs
+// each node with an unpaired neighbour (nodes visited in a fixed
+// pseudo-random order, the first unpaired neighbour in CSR order); level 2
+// pairs the pairs over the graph of pairs, and so on for `levels` levels, so
+// clusters of up to 2^levels nodes are connected subgraphs.  The order lists
+// the top-level clusters in turn, each expanded recursively.  For the
+// packed-sign gathers this puts a row's neighbours, where the graph allows,
+// in the row's own 128-byte sign line (4 rows of 256 agents) or among the
+// rows of its wavefront, so their gathers share cache lines.  O(Z) per level.
+static void locality_order(const sis_csr &J, int levels, std::vector<int32_t> &node_of) {
+  const int64_t n = J.n;
+  std::vector<int64_t> rp(J.rp.begin(), J.rp.begin() + n + 1);
+  std::vector<int32_t> col(J.col.begin(), J.col.end());
+  std::vector<std::vector<int32_t>> up;  // up[l][v]: cluster of level-l node v
+  std::vector<int64_t> sizes;            // nodes per level, finest first
+  int64_t cn = n;
+  uint64_t z = 0x243F6A8885A308D3ull;
+  auto rnd = [&]() {  // splitmix64
+    z += 0x9E3779B97F4A7C15ull;
+    uint64_t r = z;
+    r = (r ^ (r >> 30)) * 0xBF58476D1CE4E5B9ull;
+    r = (r ^ (r >> 27)) * 0x94D049BB133111EBull;
+    return r ^ (r >> 31);
+  };
+  for (int l = 0; l < levels && cn > 1; l++) {
+    std::vector<int32_t> visit((size_t)cn);
+    for (int64_t v = 0; v < cn; v++) visit[(size_t)v] = (int32_t)v;
+    for (int64_t v = cn - 1; v > 0; v--) std::swap(visit[(size_t)v], visit[(size_t)(rnd() % (uint64_t)(v + 1))]);
+    std::vector<int32_t> cid((size_t)cn, -1);
+    int32_t nc = 0;
+    for (int32_t u : visit) {
+      if (cid[(size_t)u] >= 0) continue;
+      cid[(size_t)u] = nc;
+      for (int64_t k = rp[(size_t)u]; k < rp[(size_t)u + 1]; k++) {
+        const int32_t v = col[(size_t)k];
+        if (v != u && cid[(size_t)v] < 0) {
+          cid[(size_t)v] = nc;
+          break;
+        }
+      }
+      nc++;
+    }
+    // the graph of clusters: distinct neighbouring clusters, no self loops
+    std::vector<int64_t> crp((size_t)nc + 1, 0);
+    for (int64_t v = 0; v < cn; v++) crp[(size_t)cid[(size_t)v] + 1] += rp[(size_t)v + 1] - rp[(size_t)v];
+    for (int64_t c = 0; c < nc; c++) crp[(size_t)c + 1] += crp[(size_t)c];
+    std::vector<int32_t> ccol((size_t)crp[(size_t)nc]);
+    std::vector<int64_t> fill(crp.begin(), crp.end() - 1);
+    for (int64_t v = 0; v < cn; v++)
+      for (int64_t k = rp[(size_t)v]; k < rp[(size_t)v + 1]; k++)
+        ccol[(size_t)fill[(size_t)cid[(size_t)v]]++] = cid[(size_t)col[(size_t)k]];
+    std::vector<int64_t> nrp((size_t)nc + 1, 0);
+    int64_t w = 0;
+    for (int64_t c = 0; c < nc; c++) {
+      auto b = ccol.begin() + crp[(size_t)c], e = ccol.begin() + crp[(size_t)c + 1];
+      std::sort(b, e);
+      int32_t last = -1;
+      for (auto it = b; it != e; ++it)
+        if (*it != c && *it != last) ccol[(size_t)w++] = last = *it;
+      nrp[(size_t)c + 1] = w;
+    }
+    ccol.resize((size_t)w);
+    up.push_back(std::move(cid));
+    sizes.push_back(cn);
+    rp.swap(nrp);
+    col.swap(ccol);
+    cn = nc;
+  }
+  // expand top-down: the members of each cluster in increasing id order
+  std::vector<int32_t> order((size_t)cn);
+  for (int64_t c = 0; c < cn; c++) order[(size_t)c] = (int32_t)c;
+  for (int64_t l = (int64_t)up.size() - 1; l >= 0; l--) {
+    const std::vector<int32_t> &cid = up[(size_t)l];
+    const int64_t fine = sizes[(size_t)l];
+    std::vector<int64_t> start((size_t)cn + 1, 0);
+    for (int64_t v = 0; v < fine; v++) start[(size_t)cid[(size_t)v] + 1]++;
+    for (int64_t c = 0; c < cn; c++) start[(size_t)c + 1] += start[(size_t)c];
+    std::vector<int32_t> members((size_t)fine);
+    std::vector<int64_t> pos(start.begin(), start.end() - 1);
+    for (int64_t v = 0; v < fine; v++) members[(size_t)pos[(size_t)cid[(size_t)v]]++] = (int32_t)v;
+    std::vector<int32_t> next;
+    next.reserve((size_t)fine);
+    for (int32_t c : order)
+      for (int64_t k = start[(size_t)c]; k < start[(size_t)c + 1]; k++) next.push_back(members[(size_t)k]);
+    order.swap(next);
+    cn = fine;
+  }
+  node_of.swap(order);
+}
+
+// The rows of J in the order node_of (row r holds node node_of[r]): each row
+// keeps its entries in their CSR order, with columns renamed to rows.
+static void reorder_csr(const sis_csr &J, const std::vector<int32_t> &node_of,
+                        const std::vector<int32_t> &row_inv, sis_csr &out) {
+  const int64_t n = J.n;
+  out.n = n;
+  out.sum = J.sum;
+  out.rp.assign((size_t)n + 1, 0);
+  out.col.resize(J.col.size());
+  out.val.resize(J.val.size());
+  for (int64_t r = 0; r < n; r++) {
+    const int64_t v = node_of[(size_t)r], b = J.rp[(size_t)v], e = J.rp[(size_t)v + 1];
+    out.rp[(size_t)r + 1] = out.rp[(size_t)r] + (e - b);
+    for (int64_t k = b; k < e; k++) {
+      const int64_t o = out.rp[(size_t)r] + (k - b);
+      out.col[(size_t)o] = row_inv[(size_t)J.col[(size_t)k]];
+      out.val[(size_t)o] = J.val[(size_t)k];
+    }
+  }
+}
+
 static int resolve_period(const sis_solver *s, int64_t T) {
   if (s->p.window_period > 0) return s->p.window_period;
   if (s->p.problem == SIS_TSP) return (int)std::max<int64_t>(1, T / 10);
@@ -548,6 +665,32 @@ int sis_solver_create(const sis_csr *J, const double *field, const sis_params *p
       (s->gather != G_DSB || (p.form != SIS_FORM_F32 && p.form != SIS_FORM_F64)))
     return fail(SIS_ERR_ARG, "sis_solver_create: the unfused step is plain-form dSB only");
   s->prob = prob;
+  // Row storage order (sis_params.row_order): the locality order for the
+  // sign-gather forms when the weights are integers (the FP64 cut sums are
+  // then exact in any row order) and, by default, when the sign words
+  // outgrow an L2 slice.
+  sis_csr Jr;
+  {
+    bool integer_w = true;
+    double wsum = 0.0;
+    for (double v : J->val) {
+      if (!(v == std::floor(v))) integer_w = false;
+      wsum += std::fabs(v);
+    }
+    integer_w = integer_w && wsum < 9007199254740992.0;  // 2^53
+    const bool sign_gather = prob == PROB_MAXCUT && s->gather == G_DSB &&
+                             (p.form == SIS_FORM_PACKED || narrow);
+    const bool want = p.row_order == 1 ||
+                      (p.row_order == 0 && (double)s->N * (double)s->lda / 8.0 > 4.0 * 1024 * 1024);
+    if (sign_gather && integer_w && want && s->N > 1) {
+      locality_order(*J, 6, s->node_of);
+      s->row_inv.assign((size_t)s->N, 0);
+      for (int64_t r = 0; r < s->N; r++) s->row_inv[(size_t)s->node_of[(size_t)r]] = (int32_t)r;
+      reorder_csr(*J, s->node_of, s->row_inv, Jr);
+      J = &Jr;
+    }
+    p.row_order = s->node_of.empty() ? -1 : 1;
+  }
   s->p = p;
   if (int st0 = build_ell(s.get(), J)) return st0;  // resident tiles' matrix copies
   const int vec = p.tile > 0 ? p.tile : default_tile(s.get());
@@ -569,6 +712,10 @@ int sis_solver_create(const sis_csr *J, const double *field, const sis_params *p
     return st;
   HIPCHK(hipMemcpy(s->rp.p, J->rp.data(), sizeof(int64_t) * (N + 1), hipMemcpyHostToDevice));
   s->rp_host.assign(J->rp.begin(), J->rp.begin() + N + 1);
+  if (!s->node_of.empty()) {
+    if ((st = s->node_of_d.alloc((size_t)N))) return st;
+    HIPCHK(hipMemcpy(s->node_of_d.p, s->node_of.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice));
+  }
   for (int64_t i = 0; i < N; i++) s->max_deg = std::max(s->max_deg, J->rp[i + 1] - J->rp[i]);
 
   if (Z) {
@@ -1047,10 +1194,11 @@ int sis_solver_init_random(sis_solver *s, uint64_t seed) {
   if (st) return st;
   s->par = 0;
   if (s->narrow())
-    launch_init_qn(s->stream, s->qw(), s->Xq.p, s->N, s->A, s->lda, seed, s->qshift);
+    launch_init_qn(s->stream, s->qw(), s->Xq.p, s->N, s->A, s->lda, seed, s->qshift,
+                   s->node_of_d.p);
   else
     launch_init_state(s->stream, s->f64, s->Xbuf(0), s->Ybuf(), s->N, s->A, s->lda, seed,
-                      s->p.problem == SIS_TSP);
+                      s->p.problem == SIS_TSP, s->node_of_d.p);
   HIPCHK(hipGetLastError());
   if ((st = refresh_shadow(s))) return st;
   HIPCHK(hipStreamSynchronize(s->stream));
@@ -1071,15 +1219,15 @@ int sis_solver_set_state(sis_solver *s, const double *x, const double *y) {
     std::vector<uint32_t> P((size_t)(s->Xq.n), 0u);
     for (int64_t i = 0; i < N; i++)
       for (int64_t a = 0; a < A; a++)
-        qn_put(qw, P, N, Q, i, a,
+        qn_put(qw, P, N, Q, s->row_of(i), a,
                qn_unit_host(qw, (float)x[i * A + a], (float)y[i * A + a], s->qshift));
     HIPCHK(hipMemcpy(s->Xq.p, P.data(), sizeof(uint32_t) * P.size(), hipMemcpyHostToDevice));
   } else if (s->f64) {
     std::vector<double> bx((size_t)(N * lda), 0.0), by((size_t)(N * lda), 0.0);
     for (int64_t i = 0; i < N; i++)
       for (int64_t a = 0; a < A; a++) {
-        bx[(size_t)(i * lda + a)] = x[i * A + a];
-        by[(size_t)(i * lda + a)] = y[i * A + a];
+        bx[(size_t)(s->row_of(i) * lda + a)] = x[i * A + a];
+        by[(size_t)(s->row_of(i) * lda + a)] = y[i * A + a];
       }
     HIPCHK(hipMemcpy(s->Xd[0].p, bx.data(), sizeof(double) * bx.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(s->Yd.p, by.data(), sizeof(double) * by.size(), hipMemcpyHostToDevice));
@@ -1087,8 +1235,8 @@ int sis_solver_set_state(sis_solver *s, const double *x, const double *y) {
     std::vector<float> bx((size_t)(N * lda), 0.0f), by((size_t)(N * lda), 0.0f);
     for (int64_t i = 0; i < N; i++)
       for (int64_t a = 0; a < A; a++) {
-        bx[(size_t)(i * lda + a)] = (float)x[i * A + a];
-        by[(size_t)(i * lda + a)] = (float)y[i * A + a];
+        bx[(size_t)(s->row_of(i) * lda + a)] = (float)x[i * A + a];
+        by[(size_t)(s->row_of(i) * lda + a)] = (float)y[i * A + a];
       }
     HIPCHK(hipMemcpy(s->Xf[0].p, bx.data(), sizeof(float) * bx.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(s->Yf.p, by.data(), sizeof(float) * by.size(), hipMemcpyHostToDevice));
@@ -1109,12 +1257,12 @@ int sis_solver_get_state(sis_solver *s, double *x, double *y) {
       std::vector<double> b((size_t)(N * lda));
       HIPCHK(hipMemcpy(b.data(), src, sizeof(double) * b.size(), hipMemcpyDeviceToHost));
       for (int64_t i = 0; i < N; i++)
-        for (int64_t a = 0; a < A; a++) dst[i * A + a] = b[(size_t)(i * lda + a)];
+        for (int64_t a = 0; a < A; a++) dst[i * A + a] = b[(size_t)(s->row_of(i) * lda + a)];
     } else {
       std::vector<float> b((size_t)(N * lda));
       HIPCHK(hipMemcpy(b.data(), src, sizeof(float) * b.size(), hipMemcpyDeviceToHost));
       for (int64_t i = 0; i < N; i++)
-        for (int64_t a = 0; a < A; a++) dst[i * A + a] = b[(size_t)(i * lda + a)];
+        for (int64_t a = 0; a < A; a++) dst[i * A + a] = b[(size_t)(s->row_of(i) * lda + a)];
     }
     return SIS_OK;
   };
@@ -1128,7 +1276,7 @@ int sis_solver_get_state(sis_solver *s, double *x, double *y) {
     const double yinv = std::ldexp(1.0, -s->qshift);
     for (int64_t i = 0; i < N; i++)
       for (int64_t a = 0; a < A; a++) {
-        const uint32_t u = qn_get(qw, P, N, Q, i, a);
+        const uint32_t u = qn_get(qw, P, N, Q, s->row_of(i), a);
         const int32_t xq = (int32_t)(u & m) - bias, yq = (int32_t)((u >> qw) & m) - bias;
         if (x) x[i * A + a] = std::ldexp((double)xq, -(qw - 2));
         if (y) y[i * A + a] = (double)yq * yinv;
@@ -1353,7 +1501,7 @@ int sis_solver_get_spins(sis_solver *s, int which, int32_t *out) {
                    sizeof(uint64_t) * b.size(), hipMemcpyDeviceToHost));
   for (int64_t i = 0; i < N; i++)
     for (int64_t a = 0; a < A; a++)
-      out[i * A + a] = ((b[(size_t)(i * W + a / 64)] >> (a % 64)) & 1) ? -1 : 1;
+      out[i * A + a] = ((b[(size_t)(s->row_of(i) * W + a / 64)] >> (a % 64)) & 1) ? -1 : 1;
   return SIS_OK;
 }
 

@@ -33,7 +33,7 @@ class Params(C.Structure):
         ("coupling", C.c_double), ("window_period", C.c_int32),
         ("window_threshold", C.c_int32), ("ncities", C.c_int32),
         ("use_graph", C.c_int32), ("tile", C.c_int32), ("fused", C.c_int32),
-        ("split", C.c_int32), ("structured", C.c_int32),
+        ("split", C.c_int32), ("structured", C.c_int32), ("row_order", C.c_int32),
     ]
 
 

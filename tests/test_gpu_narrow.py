@@ -97,6 +97,33 @@ def test_trajectory_bit_exact(w, tile, agents):
     assert (np.abs(X) > 1).any()  # the wall code occurred
 
 
+@pytest.mark.parametrize("w,dt", [(8, 0.016), (8, 1.0), (8, 2.0), (12, 0.001), (12, 2.0)])
+def test_trajectory_bit_exact_at_the_y_range_edges(w, dt):
+    # The step encodes Y without the clamp the oracle applies: exact while
+    # the code range holds 2/dt + dt.  dt 0.016 (8-bit) and 0.001 (12-bit)
+    # sit just above the refused range (shift 0: the bound 125.0 / 2000.0 is
+    # within one code range of 127 / 2047); dt 1 and 2 give large steps.  A
+    # strong coupling and a state near the walls push |y| up.
+    J, pb = problem(203, 0.04, sis.W_INT3, 19)
+    A, T = 128, 300
+    sy = orc.qn_shift(dt, w)
+    assert sy >= 0 and (2 ** (w - 1) - 1) * 2.0 ** -sy >= 2 / dt + dt
+    s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=A, dt=dt, form=FORMW[w], xi0=0.9))
+    X, Y = seeded(J.n, A, 41, scale=0.99)
+    s.set_state(X.astype(np.float64), Y.astype(np.float64))
+    orc.qn_round(X, Y, sy, w)
+    ymax = 0.0
+    for t0, n in ((0, 120), (120, 180)):
+        s.advance(t0, n, T)
+        orc.advance_qn(pb, X, Y, t0, n, T, dt, 0.9, sy, w)
+        x, y = s.get_state()
+        assert np.array_equal(x, X.astype(np.float64)), (w, dt, t0 + n)
+        assert np.array_equal(y, Y.astype(np.float64)), (w, dt, t0 + n)
+        ymax = max(ymax, float(np.abs(Y[np.abs(X) <= 1]).max()))
+    # live |y| within the bound (plus half a code of rounding), inside the range
+    assert ymax <= 2 / dt + dt + 2.0 ** -(sy + 1) and ymax < (2 ** (w - 1) - 1) * 2.0 ** -sy
+
+
 @pytest.mark.parametrize("w", [12, 8])
 @pytest.mark.parametrize("tile", [4 | 4 << 8 | VG, 4 | 8 << 8 | VG | 1 << 16, 4 | 32 << 8 | VG])
 @pytest.mark.parametrize("agents", [96, 300])
