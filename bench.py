@@ -404,6 +404,45 @@ def small_configs(a):
                                   random_tour=t.random_length(a.seed),
                                   steps_executed=info["steps_executed"], **tsp_energies(s, lens)))
                 s.close()
+    elif a.workload == "quality":
+        # Best and mean cut of the narrow-state forms against plain FP32 dSB
+        # at equal steps on several graph families (N = 2^18, 256 agents, dt
+        # 0.5, 20,000 steps, no window exit), three seeds each.  The bar is a
+        # best cut within 0.1% of plain FP32; the line states which pass.
+        T = a.steps
+        fams = [("4-regular +-1", lambda g: sis.Csr.gen_regular(1 << 18, 4, sis.W_PM1, g)),
+                ("3-regular +-1", lambda g: sis.Csr.gen_regular(1 << 18, 3, sis.W_PM1, g)),
+                ("3-regular +1", lambda g: sis.Csr.gen_regular(1 << 18, 3, sis.W_ONE, g)),
+                ("4-regular +1", lambda g: sis.Csr.gen_regular(1 << 18, 4, sis.W_ONE, g)),
+                ("ER mean degree 8 +-1", lambda g: sis.Csr.gen_er(1 << 18, 8.0 / (1 << 18),
+                                                                  sis.W_PM1, g)),
+                ("ER mean degree 8, weights {-3..3}\\{0}",
+                 lambda g: sis.Csr.gen_er(1 << 18, 8.0 / (1 << 18), sis.W_INT3, g)),
+                ("torus 512x512 +-1", lambda g: sis.Csr.gen_torus(512, 512, sis.W_PM1, g))]
+        for fam, mk in fams:
+            for g in (a.seed, a.seed + 1, a.seed + 2):
+                J = mk(g)
+                res = {}
+                for form in ("f32", "q12", "q8"):
+                    s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=256, dt=0.5,
+                                           form=sis.FORMS[form]))
+                    s.init_random(g + 100)
+                    info = s.run(T, 0)
+                    c, _ = s.cuts()
+                    res[form] = dict(best=float(c.max()), mean=float(c.mean()),
+                                     ms_per_step=info["step_kernel_ms"] / info["steps_executed"])
+                    s.close()
+                b = res["f32"]
+                for f in ("q12", "q8"):
+                    r = res[f]
+                    r["best_rel_diff"] = (r["best"] - b["best"]) / abs(b["best"])
+                    r["mean_rel_diff"] = (r["mean"] - b["mean"]) / abs(b["mean"])
+                    r["within_0.1pct"] = bool(r["best_rel_diff"] >= -1e-3)
+                # printed as each comparison ends (the whole sweep takes minutes)
+                print(json.dumps(dict(workload=f"quality: {fam}, N={J.n}, 256 agents, dt 0.5, "
+                                               f"{T} steps, dSB, no window exit", seed=g,
+                                      nnz=J.nnz, plain=b, q12=res["q12"], q8=res["q8"])),
+                      flush=True)
     elif a.workload == "powerlaw":
         # Chung-Lu N = 2^20, mean degree 8, exponent 2.1 (largest row ~1.2e5
         # entries), +-1 weights, 256 agents, dSB dt 0.5; without and with the
@@ -597,7 +636,7 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--workload", default="regular2p20",
                     choices=["regular2p20", "er800", "torus100", "tsp48", "powerlaw", "tsp_struct",
-                             "tsp_valid", "serving", "batch_stream"],
+                             "tsp_valid", "serving", "batch_stream", "quality"],
                     help="regular2p20: the headline (random 4-regular graph, N = 2^20); the "
                          "others print one JSON line per configuration")
     ap.add_argument("--instances", type=int, default=4096)
@@ -623,7 +662,8 @@ def main():
     if a.workload == "batch_stream":
         batch_stream(a)
         return
-    if a.workload in ("er800", "torus100", "tsp48", "powerlaw", "tsp_struct", "tsp_valid"):
+    if a.workload in ("er800", "torus100", "tsp48", "powerlaw", "tsp_struct", "tsp_valid",
+                      "quality"):
         small_configs(a)
         return
 

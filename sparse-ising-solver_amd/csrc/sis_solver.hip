@@ -618,6 +618,8 @@ int sis_solver_create(const sis_csr *J, const double *field, const sis_params *p
     return fail(SIS_ERR_ARG, "sis_solver_create: agents must be in [1, 262144]");
   if (!(p.dt == p.dt)) return fail(SIS_ERR_ARG, "sis_solver_create: dt is NaN");
   if (p.split < -1) return fail(SIS_ERR_ARG, "sis_solver_create: split must be >= -1");
+  if (p.row_order < -1 || p.row_order > 1)
+    return fail(SIS_ERR_ARG, "sis_solver_create: row_order must be -1, 0 or 1");
   if (p.structured)
     return fail(SIS_ERR_ARG, "sis_solver_create: structured TSP solvers come from sis_solver_create_tsp");
   if (p.window_threshold < 1) p.window_threshold = p.problem == SIS_MAXCUT ? 25 : 50;

@@ -117,6 +117,8 @@ def test_locality_order_declined_where_it_does_not_apply():
     for J, extra in cases:
         s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=64, dt=0.5, row_order=1, **extra))
         assert s.p.row_order == -1, extra
+    with pytest.raises(sis.SisError):
+        sis.Solver(I, dict(problem=sis.MAXCUT, agents=64, form=sis.FORM_PACKED, row_order=2))
     t = sis.Tsp.generate(6, 100, 0, 1)
     Jt, h = t.qubo()
     s = sis.Solver(Jt, dict(problem=sis.TSP, agents=64, ncities=6, dynamics=sis.BSB, row_order=1),
