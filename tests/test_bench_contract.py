@@ -49,11 +49,11 @@ def test_headline_line_has_the_contract_fields():
     cb = d["cpu_baseline"]
     assert cb["value"] > 0 and cb["cores"] >= 1 and cb["kind"] in ("port", "reference")
     assert cb["sample"]
-    # headline: the 8-bit form; packed and plain dSB follow the same
+    # headline: the 12-bit form; packed and plain dSB follow the same
     # trajectory (identical best cut), the narrow forms their own
-    assert d["config"]["form"] == "q8" and d["packed"]["best_cut_rel_diff_vs_plain"] == 0.0
+    assert d["config"]["form"] == "q12" and d["packed"]["best_cut_rel_diff_vs_plain"] == 0.0
     assert abs(d["best_cut_rel_diff_vs_plain"]) < 0.01
-    assert d["q12"]["form"] == "q12" and abs(d["q12"]["best_cut_rel_diff_vs_plain"]) < 0.01
+    assert d["q8"]["form"] == "q8" and abs(d["q8"]["best_cut_rel_diff_vs_plain"]) < 0.01
     assert d["energy_check"]["max_abs_err_cut"] == 0.0
     bf = d["bf16_bsb"]
     assert bf["form"] == "bf16" and bf["plain_bsb"]["form"] == "f32" and bf["value"] > 0
