@@ -91,6 +91,40 @@ def test_locality_order_set_state_and_advance(form):
     assert np.array_equal(x1, x0) and np.array_equal(y1, y0)
 
 
+def test_locality_order_self_loops_duplicates_and_row_split():
+    # a COO with self loops and duplicate entries (kept, as the reference's
+    # CSR keeps them), integer weights; then a power-law graph whose heavy rows
+    # go through the row split: the same sums in either row order
+    rng = np.random.default_rng(12)
+    n = 1500
+    rows = list(rng.integers(0, n, 6000)) + list(range(0, n, 7)) + [3] * 5
+    cols = list(rng.integers(0, n, 6000)) + list(range(0, n, 7)) + [9] * 5
+    vals = list(rng.choice([-2.0, -1.0, 1.0, 3.0], len(rows)))
+    J = sis.Csr.from_coo(n, rows, cols, vals, mirror=False)
+    rp, col, val = J.arrays()
+    r_of = np.repeat(np.arange(n), np.diff(rp))
+    assert (r_of == col).any()  # self loops present
+    pb = orc.Problem(rp, col, val, S=J.sum)
+    for form in ("packed", "q12"):
+        s1, s0 = solvers(J, form, xi0=0.2)
+        for s in (s1, s0):
+            s.init_random(3)
+        s1.run(700, 1), s0.run(700, 1)
+        same(s1, s0)
+        X, Y = orc.init_state(3, orc.MAXCUT, n, 300)
+        r = oracle_solve(pb, form, X, Y, 700, 1, 0.5, 0.2)
+        assert np.array_equal(s1.spins(sis.SPINS_CURRENT), r["cur"])
+    P = sis.Csr.gen_chung_lu(6000, 8.0, 2.1, sis.W_PM1, 4)
+    assert np.diff(P.arrays()[0]).max() > 256
+    s1, s0 = solvers(P, "packed", split=64)
+    for s in (s1, s0):
+        s.init_random(8)
+        s.advance(0, 250, 1000)
+    x1, y1 = s1.get_state()
+    x0, y0 = s0.get_state()
+    assert np.array_equal(x1, x0) and np.array_equal(y1, y0)
+
+
 def test_locality_order_default_at_the_benchmark_size():
     # the default (0) chooses the locality order once the sign words exceed
     # 4 MiB (N x agents / 8), and node order below
