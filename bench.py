@@ -50,6 +50,8 @@ def step_bytes(form, N, Z, A):
         return state + N * A // 8 + Z * A // 8 + csr
     if form == "q12":  # 12-bit X and Y codes: 3 bytes read + 3 written
         return 6 * N * A + N * A // 8 + Z * A // 8 + csr
+    if form == "q10":  # 10-bit X and Y codes: 2.5 bytes read + 2.5 written
+        return 5 * N * A + N * A // 8 + Z * A // 8 + csr
     if form == "q8":  # 8-bit X and Y codes: 2 bytes read + 2 written
         return 4 * N * A + N * A // 8 + Z * A // 8 + csr
     if form == "bf16":
@@ -300,7 +302,7 @@ def small_configs(a):
                               ms_per_step=r["wall"] * 1e3 / r["info"]["steps_executed"]))
     elif a.workload == "torus100":
         J = sis.Csr.gen_torus(100, 100, sis.W_PM1, a.seed)
-        for form in ("f32", "packed", "q12", "q8"):
+        for form in ("f32", "packed", "q12", "q10", "q8"):
             r = run_solver(J, dict(problem=sis.MAXCUT, agents=1024, dt=0.5, form=sis.FORMS[form],
                                    use_graph=1), a.steps if a.steps != 20000 else 10000, 1, a.seed)
             ms = r["info"]["step_kernel_ms"] / max(r["info"]["steps_executed"], 1)
@@ -425,7 +427,7 @@ def small_configs(a):
             for g in (a.seed, a.seed + 1, a.seed + 2):
                 J = mk(g)
                 res = {}
-                for form in ("f32", "q12", "q8"):
+                for form in ("f32", "q12", "q10", "q8"):
                     s = sis.Solver(J, dict(problem=sis.MAXCUT, agents=256, dt=0.5,
                                            form=sis.FORMS[form]))
                     s.init_random(g + 100)
@@ -435,7 +437,7 @@ def small_configs(a):
                                      ms_per_step=info["step_kernel_ms"] / info["steps_executed"])
                     s.close()
                 b = res["f32"]
-                for f in ("q12", "q8"):
+                for f in ("q12", "q10", "q8"):
                     r = res[f]
                     r["best_rel_diff"] = (r["best"] - b["best"]) / abs(b["best"])
                     r["mean_rel_diff"] = (r["mean"] - b["mean"]) / abs(b["mean"])
@@ -443,7 +445,8 @@ def small_configs(a):
                 # printed as each comparison ends (the whole sweep takes minutes)
                 print(json.dumps(dict(workload=f"quality: {fam}, N={J.n}, 256 agents, dt 0.5, "
                                                f"{T} steps, dSB, no window exit", seed=g,
-                                      nnz=J.nnz, plain=b, q12=res["q12"], q8=res["q8"])),
+                                      nnz=J.nnz, plain=b, q12=res["q12"], q10=res["q10"],
+                                      q8=res["q8"])),
                       flush=True)
     elif a.workload == "powerlaw":
         # Chung-Lu N = 2^20, mean degree 8, exponent 2.1 (largest row ~1.2e5
@@ -717,6 +720,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32" if a.form != "f64" else "f64",
         "state_storage": {"q12": "X, Y as 12-bit fixed-point codes (3 bytes per node-agent)",
+                          "q10": "X, Y as 10-bit fixed-point codes (2.5 bytes per node-agent)",
                           "q8": "X, Y as 8-bit fixed-point codes (2 bytes per node-agent)",
                           "packed": "X, Y FP32", "f32": "X, Y FP32", "bf16": "X, Y FP32",
                           "f64": "X, Y FP64"}[a.form],

@@ -68,7 +68,7 @@ enum sis_form {
                           degree-4 graph it moves 6 + 1/8 + 4/8 = 6.6 bytes
                           (state, sign write, sign gathers) against 32 for
                           F32. */
-  SIS_FORM_Q8 = 5      /* as SIS_FORM_Q12 with 8-bit codes (no reference
+  SIS_FORM_Q8 = 5,     /* as SIS_FORM_Q12 with 8-bit codes (no reference
                           counterpart):
                             X: round(x 2^6) in [-64, 64], +-65 for any |x| > 1,
                             Y: round(y 2^s), |code| <= 127, s the largest
@@ -78,6 +78,19 @@ enum sis_form {
                           2 bytes per node-agent, so on a degree-4 graph
                           4 + 1/8 + 4/8 = 4.6 bytes per node-agent and step.
                           Its own trajectory; J sgn(X) exact. */
+  SIS_FORM_Q10 = 6     /* as SIS_FORM_Q12 with 10-bit codes (no reference
+                          counterpart):
+                            X: round(x 2^8) in [-256, 256], +-257 for any
+                               |x| > 1,
+                            Y: round(y 2^s), |code| <= 511, s the largest
+                               integer in [0, 9] with 511 2^-s >= 2/dt + dt
+                               (dt with 511 < 2/dt + dt, i.e. dt < 0.0039,
+                               is refused);
+                          2.5 bytes per node-agent (two planes of 32-bit and
+                          one of 16-bit values per four agents), so on a
+                          degree-4 graph 5 + 1/8 + 4/8 = 5.6 bytes per
+                          node-agent and step.  Its own trajectory; J sgn(X)
+                          exact. */
 };
 
 enum sis_weights { SIS_W_ONE = 0, SIS_W_PM1 = 1, SIS_W_INT3 = 2 };

@@ -230,8 +230,9 @@ def tri(c1, c2, n):
 
 
 # ---------------------------------------------------------------- narrow-state dSB forms
-# Code width w: 12 (X codes +-1024 and the wall code +-1025, Y codes +-2047)
-# or 8 (X codes +-64, wall +-65, Y codes +-127).
+# Code width w: 12 (X codes +-1024 and the wall code +-1025, Y codes +-2047),
+# 10 (X codes +-256, wall +-257, Y codes +-511) or 8 (X codes +-64, wall +-65,
+# Y codes +-127).
 
 def qn_shift(dt, w):
     """Y code shift sy of the w-bit form for time step dt (-1 if dt <= 0)."""

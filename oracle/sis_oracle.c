@@ -228,11 +228,12 @@ static void orc_unfused_f32(const int64_t *rp, const int32_t *col, const float *
  * ymax * 2^-s >= 2/dt + dt, the bound of |y| while |x| <= 1:
  * |x_new - x| = c0 |y'| <= 2 with c0 = dt (p + 1) >= dt gives |y'| <= 2 / dt,
  * and |c1 x| <= dt.  w = 12: X codes +-1024 (+-1025), Y codes +-2047;
- * w = 8: X codes +-64 (+-65), Y codes +-127. */
+ * w = 10: X codes +-256 (+-257), Y codes +-511; w = 8: X codes +-64 (+-65),
+ * Y codes +-127. */
 static inline int orc_qn_ymax(int w) { return (1 << (w - 1)) - 1; }
 
 int orc_qn_shift(int w, double dt) {
-  if (!(dt > 0) || (w != 12 && w != 8)) return -1;
+  if (!(dt > 0) || (w != 12 && w != 10 && w != 8)) return -1;
   double ymax = 2.0 / dt + dt;
   int s = w - 1;
   while (s > 0 && ldexp((double)orc_qn_ymax(w), -s) < ymax) s--;
@@ -266,8 +267,12 @@ void orc_q12_round_state(int64_t n, float *X, float *Y, int sy) {
   orc_qn_round_state(12, n, X, Y, sy);
 }
 
-/* precision codes of the FP32 core below: 2 = 12-bit state, 3 = 8-bit state */
-static inline int orc_qn_width(int precision) { return precision == 2 ? 12 : 8; }
+/* precision codes of the FP32 core below: 2 = 12-bit state, 3 = 8-bit state,
+ * 4 = 10-bit state */
+static inline int orc_qn_width(int precision) {
+  return precision == 2 ? 12 : precision == 3 ? 8 : 10;
+}
+static inline int orc_qn_precision(int w) { return w == 12 ? 2 : w == 8 ? 3 : 4; }
 
 /* One fused step over all (i, a), FP64.
  * MaxCut (src/kernels.cu:236-262): confine |x| > 1 to sgn x and y = 0, else
@@ -408,11 +413,11 @@ void orc_step_f32(int problem, int gather, int bf16_gather, const int64_t *rp, c
                     coupling_d);
 }
 
-/* Narrow-state dSB step (precision 2): Xs / Y hold decoded codes. */
+/* Narrow-state dSB step (code width w): Xs / Y hold decoded codes. */
 void orc_step_qn(int w, const int64_t *rp, const int32_t *col, const float *val, int64_t N,
                  int64_t A, const float *Xs, float *Xn, float *Y, const double *cd, int sy) {
-  orc_step_f32_core(ORC_MAXCUT, ORC_DSB, w == 12 ? 2 : 3, sy, rp, col, val, NULL, N, A, Xs, Xn, Y,
-                    cd, 1.0);
+  orc_step_f32_core(ORC_MAXCUT, ORC_DSB, orc_qn_precision(w), sy, rp, col, val, NULL, N, A, Xs,
+                    Xn, Y, cd, 1.0);
 }
 
 void orc_step_q12(const int64_t *rp, const int32_t *col, const float *val, int64_t N, int64_t A,
@@ -546,8 +551,8 @@ int64_t orc_solve_qn(int w, const int64_t *rp, const int32_t *col, const float *
                      int64_t A, int64_t T, int uw, int64_t P, int K, double dt, double xi0, int sy,
                      float *X, float *Y, int32_t *cur, int32_t *fin, int32_t *stab, uint8_t *bif) {
   orc_qn_round_state(w, N * A, X, Y, sy);
-  return orc_solve_f32_core(ORC_MAXCUT, ORC_DSB, w == 12 ? 2 : 3, sy, rp, col, val, NULL, N, A, T,
-                            uw, P, K, 0, dt, xi0, 1.0, X, Y, cur, fin, stab, bif, NULL);
+  return orc_solve_f32_core(ORC_MAXCUT, ORC_DSB, orc_qn_precision(w), sy, rp, col, val, NULL, N,
+                            A, T, uw, P, K, 0, dt, xi0, 1.0, X, Y, cur, fin, stab, bif, NULL);
 }
 
 int64_t orc_solve_q12(const int64_t *rp, const int32_t *col, const float *val, int64_t N,

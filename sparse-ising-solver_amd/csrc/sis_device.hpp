@@ -177,10 +177,20 @@ __device__ __forceinline__ void st_state(uint32_t *p, uint32_t v) {
   if (NT) __builtin_nontemporal_store(v, p);
   else *p = v;
 }
+template <int NT>
+__device__ __forceinline__ uint16_t ld_state(const uint16_t *p) {
+  if (NT) return __builtin_nontemporal_load(p);
+  return *p;
+}
+template <int NT>
+__device__ __forceinline__ void st_state(uint16_t *p, uint16_t v) {
+  if (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
 
 // ------------------------------------------------- narrow fixed-point state
 //
-// Narrow-state dSB forms, code width QW = 12 or 8 bits per variable.
+// Narrow-state dSB forms, code width QW = 12, 10 or 8 bits per variable.
 // Element (x, y) is the 2 QW-bit unit u = (xq + B) | (yq + B) << QW of codes
 // biased by B = 2^(QW-1):
 //   xq = round(x 2^XS) in [-2^XS, 2^XS], XS = QW - 2, or +-(2^XS + 1) for
@@ -189,11 +199,15 @@ __device__ __forceinline__ void st_state(uint32_t *p, uint32_t v) {
 //        code),
 // rounded to nearest even; the element is negative iff bit QW - 1 is clear.
 // QW = 12: X codes +-1024 (wall +-1025), Y codes +-2047, 24-bit units;
+// QW = 10: X codes +-256 (wall +-257), Y codes +-511, 20-bit units;
 // QW = 8: X codes +-64 (wall +-65), Y codes +-127, 16-bit units.
 // A lane keeps four elements (agents l, l + 64, l + 128, l + 192 of a
-// 256-agent quad) in NP dwords, the units' low 2 QW bits in order:
+// 256-agent quad) in NP planes, the units' low 2 QW bits in order:
 //   QW = 12, NP = 3: d0 = u0 | u1 << 24, d1 = u1 >> 8 | u2 << 16,
 //                    d2 = u2 >> 16 | u3 << 8;
+//   QW = 10, NP = 3: d0 = u0 | u1 << 20, d1 = u1 >> 12 | u2 << 8 | u3 << 28,
+//                    d2 = u3 >> 4 (16 bits: the last plane holds 16-bit
+//                    values, H16);
 //   QW = 8,  NP = 2: d0 = u0 | u1 << 16, d1 = u2 | u3 << 16.
 //
 // Decoding and rounding go through magic constants.  M = 2^23 (bits
@@ -204,14 +218,39 @@ __device__ __forceinline__ void st_state(uint32_t *p, uint32_t v) {
 template <int QW> struct QFmt;
 template <> struct QFmt<12> {
   static constexpr int NP = 3, XS = 10;
+  static constexpr bool H16 = false;
   static constexpr uint32_t MASK = 0xfffu, SIGN = 0x800u;
   static constexpr uint32_t WALL_NEG = 0x800000u | 1023u, WALL_POS = 0x800000u | 3073u;
 };
+template <> struct QFmt<10> {
+  static constexpr int NP = 3, XS = 8;
+  static constexpr bool H16 = true;
+  static constexpr uint32_t MASK = 0x3ffu, SIGN = 0x200u;
+  static constexpr uint32_t WALL_NEG = 0x80000u | 255u, WALL_POS = 0x80000u | 769u;
+};
 template <> struct QFmt<8> {
   static constexpr int NP = 2, XS = 6;
+  static constexpr bool H16 = false;
   static constexpr uint32_t MASK = 0xffu, SIGN = 0x80u;
   static constexpr uint32_t WALL_NEG = 0x8000u | 63u, WALL_POS = 0x8000u | 193u;
 };
+
+// Planes of the state array P (NP planes of ps lanes each, plane p at dword
+// p ps; a 16-bit last plane holds ps 16-bit values from there): value o of
+// plane p, read and written as its width.
+template <int QW>
+__device__ __forceinline__ uint32_t qn_plane_ld(const uint32_t *P, int64_t ps, int p, int64_t o) {
+  if (QFmt<QW>::H16 && p == QFmt<QW>::NP - 1)
+    return reinterpret_cast<const uint16_t *>(P + p * ps)[o];
+  return P[p * ps + o];
+}
+template <int QW>
+__device__ __forceinline__ void qn_plane_st(uint32_t *P, int64_t ps, int p, int64_t o, uint32_t v) {
+  if (QFmt<QW>::H16 && p == QFmt<QW>::NP - 1)
+    reinterpret_cast<uint16_t *>(P + p * ps)[o] = (uint16_t)v;
+  else
+    P[p * ps + o] = v;
+}
 
 // Per-launch constants: step coefficients; the Y code scale 2^sy, the
 // largest |y| with a code (YM 2^-sy); the decoding offsets (of 2^23 + field,
@@ -259,6 +298,11 @@ __device__ __forceinline__ void qn_units(const uint32_t d[QFmt<QW>::NP], uint32_
     u[1] = __builtin_amdgcn_alignbit(d[1], d[0], 24);  // bits 24..47
     u[2] = __builtin_amdgcn_alignbit(d[2], d[1], 16);  // bits 48..71
     u[3] = d[2] >> 8;                                  // bits 72..95
+  } else if constexpr (QW == 10) {
+    u[0] = d[0];                                       // bits 0..19
+    u[1] = __builtin_amdgcn_alignbit(d[1], d[0], 20);  // bits 20..39
+    u[2] = d[1] >> 8;                                  // bits 40..59
+    u[3] = __builtin_amdgcn_alignbit(d[2], d[1], 28);  // bits 60..79
   } else {
     u[0] = d[0];
     u[1] = d[0] >> 16;
@@ -282,7 +326,7 @@ __device__ __forceinline__ void qn_decode(uint32_t u, const Q12K &k, float &x, f
 // Element j of a lane's NP dwords, decoded.  8-bit codes are whole bytes:
 // each field converts to float in one operation (byte b of a dword:
 // v_cvt_f32_ubyte<b>), then x = (field - B) 2^-XS = fma(field, 2^-XS, -B 2^-XS)
-// exactly; 12-bit codes go through qn_units / qn_decode.
+// exactly; 12- and 10-bit codes go through qn_units / qn_decode.
 template <int QW>
 __device__ __forceinline__ void qn_decode_at(const uint32_t d[QFmt<QW>::NP], int j,
                                              const Q12K &k, float &x, float &y) {
@@ -322,7 +366,10 @@ __device__ __forceinline__ uint32_t qn_encode(float x, float y, const Q12K &k, b
   const uint32_t xb = __float_as_uint(__builtin_fmaf(x, (float)(1 << QFmt<QW>::XS), k.mb));
   const float yc = CLAMP ? __builtin_amdgcn_fmed3f(y, -k.ylim, k.ylim) : y;
   const uint32_t yb = __float_as_uint(__builtin_fmaf(yc, k.yscale, k.mb));
-  uint32_t in = (yb << QW) | xb;  // bits 0 .. 2 QW - 1 valid
+  // bits 0 .. 2 QW - 1 valid; bits 2 QW .. 23 clear (xb's bits QW .. 23 are
+  // clear, bit 23 being the low exponent bit of 2^23, and yb << QW has no bit
+  // between 2 QW and 31)
+  uint32_t in = (yb << QW) | xb;
   neg = (int32_t)xb < (int32_t)(0x4b000000u | (1u << (QW - 1)));
   // computed for every element, so that the wall choice below is two selects
   // rather than a branch around the encoding
@@ -341,13 +388,19 @@ __device__ __forceinline__ bool qn_negative(uint32_t unit) {
   return (unit & QFmt<QW>::SIGN) == 0;
 }
 
-// NP dwords from four units (their low 2 QW bits; upper bits ignored).
+// NP planes from four units (their low 2 QW bits; upper bits ignored, except
+// that 10-bit units must have bits 20..23 clear, as qn_encode's units and the
+// wall units do: bits 24 and up of u2 leave d1 by the shift).
 template <int QW>
 __device__ __forceinline__ void qn_pack(const uint32_t u[4], uint32_t d[QFmt<QW>::NP]) {
   if constexpr (QW == 12) {
     d[0] = __builtin_amdgcn_perm(u[1], u[0], 0x04020100u);  // u0.b0 u0.b1 u0.b2 u1.b0
     d[1] = __builtin_amdgcn_perm(u[2], u[1], 0x05040201u);  // u1.b1 u1.b2 u2.b0 u2.b1
     d[2] = __builtin_amdgcn_perm(u[3], u[2], 0x06050402u);  // u2.b2 u3.b0 u3.b1 u3.b2
+  } else if constexpr (QW == 10) {
+    d[0] = (u[0] & 0xfffffu) | (u[1] << 20);
+    d[1] = __builtin_amdgcn_ubfe(u[1], 12, 8) | (u[2] << 8) | (u[3] << 28);
+    d[2] = u[3] >> 4;  // stored as its low 16 bits
   } else {
     d[0] = __builtin_amdgcn_perm(u[1], u[0], 0x05040100u);  // u0.b0 u0.b1 u1.b0 u1.b1
     d[1] = __builtin_amdgcn_perm(u[3], u[2], 0x05040100u);  // u2.b0 u2.b1 u3.b0 u3.b1

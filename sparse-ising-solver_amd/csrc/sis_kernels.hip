@@ -316,14 +316,16 @@ __global__ __launch_bounds__(kBlock) void k_step_packed(StepArgs a) {
   }
 }
 
-// Narrow-state dSB forms (MaxCut, fused): X and Y as QW-bit codes, QW = 12
-// or 8 (see sis_device.hpp), NP = 3 or 2 dword planes of N x Q x 64 lanes
-// (Q = lda / 256 quads); lane l of quad q owns agents 256 q + 64 j + l,
+// Narrow-state dSB forms (MaxCut, fused): X and Y as QW-bit codes, QW = 12,
+// 10 or 8 (see sis_device.hpp), NP = 3, 3 or 2 planes of N x Q x 64 lanes
+// (Q = lda / 256 quads; the 10-bit form's third plane holds 16-bit values);
+// lane l of quad q owns agents 256 q + 64 j + l,
 // j = 0..3, i.e. bit l of sign word 4 q + j.  The gather is the packed form's (exact sum of +-J
 // from the snapshot's sign words, in CSR order); the update is the FP32 dSB
 // update on the decoded values, rounded back to codes.  A wavefront owns R
 // rows of one quad; their state loads are issued before the gathers.  Bytes
-// per node-agent: NP read + NP written + 1/8 sign write + degree / 8 gathers.
+// per node-agent: QW / 4 read + QW / 4 written + 1/8 sign write + degree / 8
+// gathers.
 //
 // Staged gathers (VG = 1): the R rows' entries (at most kQ12Stage) are
 // fetched at once -- lane e loads column, weight and the four sign words of
@@ -339,12 +341,31 @@ __global__ __launch_bounds__(kBlock) void k_step_packed(StepArgs a) {
 // the scalar path.
 constexpr int kQ12Stage = 128;
 
+// Plane p of this lane at row offset ro (wave-uniform, in lanes), rb = P + ro:
+// dword planes at rb + p ps, a 16-bit last plane at value ro of its base.
+template <int QW, int NT>
+__device__ __forceinline__ uint32_t qn_ld_row(const uint32_t *rb, int64_t ro, int64_t ps, int p,
+                                              int lane) {
+  if (QFmt<QW>::H16 && p == QFmt<QW>::NP - 1)
+    return ld_state<NT>(reinterpret_cast<const uint16_t *>(rb - ro + p * ps) + ro + lane);
+  return ld_state<NT>(rb + p * ps + lane);
+}
+template <int QW, int NT>
+__device__ __forceinline__ void qn_st_row(uint32_t *rb, int64_t ro, int64_t ps, int p, int lane,
+                                          uint32_t v) {
+  if (QFmt<QW>::H16 && p == QFmt<QW>::NP - 1)
+    st_state<NT>(reinterpret_cast<uint16_t *>(rb - ro + p * ps) + ro + lane, (uint16_t)v);
+  else
+    st_state<NT>(rb + p * ps + lane, v);
+}
+
 // New codes and sign words of one row: decode, fused dSB update, encode,
 // one ballot per sign word.  rb: the row's plane-0 dwords of this quad
-// (wave-uniform), ps: plane stride, sw: the row's four sign words.
+// (wave-uniform), ro: their offset from the planes' base, ps: plane stride,
+// sw: the row's four sign words.
 template <int QW, int NT>
 __device__ __forceinline__ void qn_finish_row(const uint32_t d[QFmt<QW>::NP], const float F[4],
-                                              const Q12K &k, uint32_t *rb, int64_t ps,
+                                              const Q12K &k, uint32_t *rb, int64_t ro, int64_t ps,
                                               uint64_t *sw, int lane) {
   constexpr int NP = QFmt<QW>::NP;
   uint32_t un[4];
@@ -361,12 +382,29 @@ __device__ __forceinline__ void qn_finish_row(const uint32_t d[QFmt<QW>::NP], co
   uint32_t e[NP];
   qn_pack<QW>(un, e);
 #pragma unroll
-  for (int p = 0; p < NP; p++) st_state<NT>(rb + p * ps + lane, e[p]);
+  for (int p = 0; p < NP; p++) qn_st_row<QW, NT>(rb, ro, ps, p, lane, e[p]);
   // lane 0 writes the four (wave-uniform) sign words, 32 aligned bytes
   if (lane == 0) {
     uint4 *o = reinterpret_cast<uint4 *>(sw);
     o[0] = make_uint4((uint32_t)b[0], (uint32_t)(b[0] >> 32), (uint32_t)b[1], (uint32_t)(b[1] >> 32));
     o[1] = make_uint4((uint32_t)b[2], (uint32_t)(b[2] >> 32), (uint32_t)b[3], (uint32_t)(b[3] >> 32));
+  }
+}
+
+// Rows whose wavefront holds no entries at all: F = 0, the sum of no terms
+// (the bit-counting path gives fma(0, 2, -0) = +0 too).
+template <int QW, int NT, int R>
+__device__ __forceinline__ void qn_finish_empty(const uint32_t (&d)[R][QFmt<QW>::NP],
+                                                const Q12K &k, uint32_t *P, int64_t Q, int64_t q,
+                                                int64_t ps, uint64_t *Sn, int64_t W, int64_t w0,
+                                                int64_t row0, int rl, int lane) {
+  const float F[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    if (r >= rl) continue;
+    const int64_t row = row0 + r;
+    qn_finish_row<QW, NT>(d[r], F, k, P + (row * Q + q) * 64, (row * Q + q) * 64, ps,
+                          Sn + row * W + w0, lane);
   }
 }
 
@@ -393,21 +431,28 @@ __device__ __forceinline__ void qn_step_body(const StepArgs &a) {
   const int64_t ps = a.N * Q * 64;  // plane stride (dwords)
   uint64_t *Sn = (uint64_t *)a.gdst;
 
+  // Memory waits count loads in issue order, so a wait for one load also
+  // waits for every load issued before it.  The staged path's chain of
+  // dependent loads (row pointers, then columns and weights, then sign words)
+  // therefore starts with the row pointers, before the state loads, and each
+  // step of it issues all its loads at once, without branches.
+  // Row pointers rp[row0 .. row0 + rl] in lanes 0..rl, loaded before any
+  // store so they stay in registers; their offsets from the first come back
+  // one row at a time by readlane.
+  const int rl = (int)(a.N - row0 < R ? a.N - row0 : R);
+  int64_t rpl = 0;
+  if constexpr (VG) rpl = rp[row0 + (lane < rl ? lane : rl)];
   uint32_t d[R][NP];
 #pragma unroll
   for (int r = 0; r < R; r++) {
-    if (row0 + r < a.N) {
-      const uint32_t *rb = P + ((row0 + r) * Q + q) * 64;  // wave-uniform
+    // every row loads (one past the last reloads the last and is never
+    // stored), so that no branch hides the loads from the waits after them
+    const int64_t rr = row0 + r < a.N ? row0 + r : a.N - 1;
+    const int64_t ro = (rr * Q + q) * 64;  // wave-uniform
 #pragma unroll
-      for (int p = 0; p < NP; p++) d[r][p] = ld_state<NT>(rb + p * ps + lane);
-    }
+    for (int p = 0; p < NP; p++) d[r][p] = qn_ld_row<QW, NT>(P + ro, ro, ps, p, lane);
   }
   if constexpr (VG) {
-    // Row pointers rp[row0 .. row0 + rl] in lanes 0..rl, loaded before any
-    // store so they stay in registers; their offsets from the first come
-    // back one row at a time by readlane.
-    const int rl = (int)(a.N - row0 < R ? a.N - row0 : R);
-    const int64_t rpl = rp[row0 + (lane < rl ? lane : rl)];
     const int64_t kb = (int64_t)readlane64((uint64_t)rpl, 0);
     const int64_t ne = (int64_t)readlane64((uint64_t)rpl, rl) - kb;
     if (ne <= kQ12Stage) {
@@ -416,21 +461,26 @@ __device__ __forceinline__ void qn_step_body(const StepArgs &a) {
       __shared__ float lval[kWaves][kQ12Stage];
       const int wv = threadIdx.x >> 6, half = lane >> 5, bit = lane & 31, sh = 31 - bit;
       // Pass 1: the columns and weights of this lane's entries (lane, lane + 64).
+      // A lane past the last entry reloads entry 0 of the range (its values
+      // are never read back); a wavefront without entries loads nothing.
       constexpr int kSlots = kQ12Stage / 64;
+      if (ne == 0) {
+        qn_finish_empty<QW, NT>(d, kc, P, Q, q, ps, Sn, W, w0, row0, rl, lane);
+        return;
+      }
       int64_t ce[kSlots];
       float ve[kSlots];
-      bool unit = true;
 #pragma unroll
       for (int u = 0; u < kSlots; u++) {
         const int64_t e = lane + 64 * u;
-        ce[u] = 0;
-        ve[u] = 1.0f;
-        if (e < ne) {
-          ce[u] = col[kb + e];
-          ve[u] = val[kb + e];
-          unit = unit && __builtin_fabsf(ve[u]) == 1.0f;
-        }
+        const int64_t ei = kb + (e < ne ? e : 0);
+        ce[u] = col[ei];
+        ve[u] = val[ei];
       }
+      bool unit = true;
+#pragma unroll
+      for (int u = 0; u < kSlots; u++)
+        unit = unit && (lane + 64 * u >= ne || __builtin_fabsf(ve[u]) == 1.0f);
       const bool all_unit = __all(unit);
       // Pass 2: the four sign words of each entry, complemented where the
       // weight is negative so that a set bit means +|v|.  Lanes 0-31 use the
@@ -440,34 +490,40 @@ __device__ __forceinline__ void qn_step_body(const StepArgs &a) {
       // holds the byte transpose T0..T3 of each half's L0..L3:
       // byte j of Tb = byte b of Lj, so lane l's four bits (bit l mod 32 of
       // L0..L3) sit at bits (l mod 8) + 8 j of the one dword T(l mod 32 / 8).
+      uint4 s01v[kSlots], s23v[kSlots];  // words 0,1 and 2,3 as dword pairs
+#pragma unroll
+      for (int u = 0; u < kSlots; u++) {
+        const uint4 *sp = reinterpret_cast<const uint4 *>(S + ce[u] * W + w0);
+        s01v[u] = sp[0];
+        s23v[u] = sp[1];
+      }
+      // every lane writes its slots (e < kQ12Stage); those at or past ne are
+      // never read
 #pragma unroll
       for (int u = 0; u < kSlots; u++) {
         const int64_t e = lane + 64 * u;
-        if (e < ne) {
-          const uint32_t f = ve[u] < 0.0f ? 0xffffffffu : 0u;
-          const uint4 *sp = reinterpret_cast<const uint4 *>(S + ce[u] * W + w0);
-          const uint4 s01 = sp[0], s23 = sp[1];  // words 0,1 and 2,3 as dword pairs
-          const uint4 lo = make_uint4(s01.x ^ f, s01.z ^ f, s23.x ^ f, s23.z ^ f);
-          const uint4 hi = make_uint4(s01.y ^ f, s01.w ^ f, s23.y ^ f, s23.w ^ f);
-          if (all_unit) {
-            auto transpose = [](const uint4 &L) {
-              // 2 x 2 byte blocks, then pairs of half-words
-              const uint32_t A = __builtin_amdgcn_perm(L.y, L.x, 0x06020400u);  // L0b0 L1b0 L0b2 L1b2
-              const uint32_t B = __builtin_amdgcn_perm(L.y, L.x, 0x07030501u);  // L0b1 L1b1 L0b3 L1b3
-              const uint32_t C = __builtin_amdgcn_perm(L.w, L.z, 0x06020400u);  // L2b0 L3b0 L2b2 L3b2
-              const uint32_t D = __builtin_amdgcn_perm(L.w, L.z, 0x07030501u);  // L2b1 L3b1 L2b3 L3b3
-              return make_uint4(__builtin_amdgcn_perm(C, A, 0x05040100u),   // b0 of L0..L3
-                                __builtin_amdgcn_perm(D, B, 0x05040100u),   // b1
-                                __builtin_amdgcn_perm(C, A, 0x07060302u),   // b2
-                                __builtin_amdgcn_perm(D, B, 0x07060302u));  // b3
-            };
-            lsig[wv][e][0] = transpose(lo);
-            lsig[wv][e][1] = transpose(hi);
-          } else {
-            lsig[wv][e][0] = lo;
-            lsig[wv][e][1] = hi;
-            lval[wv][e] = -__builtin_fabsf(ve[u]);
-          }
+        const uint32_t f = ve[u] < 0.0f ? 0xffffffffu : 0u;
+        const uint4 s01 = s01v[u], s23 = s23v[u];
+        const uint4 lo = make_uint4(s01.x ^ f, s01.z ^ f, s23.x ^ f, s23.z ^ f);
+        const uint4 hi = make_uint4(s01.y ^ f, s01.w ^ f, s23.y ^ f, s23.w ^ f);
+        if (all_unit) {
+          auto transpose = [](const uint4 &L) {
+            // 2 x 2 byte blocks, then pairs of half-words
+            const uint32_t A = __builtin_amdgcn_perm(L.y, L.x, 0x06020400u);  // L0b0 L1b0 L0b2 L1b2
+            const uint32_t B = __builtin_amdgcn_perm(L.y, L.x, 0x07030501u);  // L0b1 L1b1 L0b3 L1b3
+            const uint32_t C = __builtin_amdgcn_perm(L.w, L.z, 0x06020400u);  // L2b0 L3b0 L2b2 L3b2
+            const uint32_t D = __builtin_amdgcn_perm(L.w, L.z, 0x07030501u);  // L2b1 L3b1 L2b3 L3b3
+            return make_uint4(__builtin_amdgcn_perm(C, A, 0x05040100u),   // b0 of L0..L3
+                              __builtin_amdgcn_perm(D, B, 0x05040100u),   // b1
+                              __builtin_amdgcn_perm(C, A, 0x07060302u),   // b2
+                              __builtin_amdgcn_perm(D, B, 0x07060302u));  // b3
+          };
+          lsig[wv][e][0] = transpose(lo);
+          lsig[wv][e][1] = transpose(hi);
+        } else {
+          lsig[wv][e][0] = lo;
+          lsig[wv][e][1] = hi;
+          lval[wv][e] = -__builtin_fabsf(ve[u]);
         }
       }
       // the wavefront's LDS writes complete before its reads below
@@ -483,7 +539,7 @@ __device__ __forceinline__ void qn_step_body(const StepArgs &a) {
         const int bs = bit & 7;
 #pragma unroll
         for (int r = 0; r < R; r++) {
-          if (r >= rl) break;
+          if (r >= rl) continue;  // (not break: a constant trip count unrolls)
           const int k0 = __builtin_amdgcn_readlane(off, r);
           const int k1 = __builtin_amdgcn_readlane(off, r + 1);
           uint32_t acc = 0u;
@@ -496,7 +552,8 @@ __device__ __forceinline__ void qn_step_body(const StepArgs &a) {
 #pragma unroll
           for (int j = 0; j < 4; j++) F[j] = __builtin_fmaf((float)((acc >> (8 * j)) & 0xffu), 2.0f, md);
           const int64_t row = row0 + r;
-          qn_finish_row<QW, NT>(d[r], F, kc, P + (row * Q + q) * 64, ps, Sn + row * W + w0, lane);
+          qn_finish_row<QW, NT>(d[r], F, kc, P + (row * Q + q) * 64, (row * Q + q) * 64, ps,
+                                Sn + row * W + w0, lane);
         }
         return;
       }
@@ -510,7 +567,7 @@ __device__ __forceinline__ void qn_step_body(const StepArgs &a) {
       };
 #pragma unroll
       for (int r = 0; r < R; r++) {
-        if (r >= rl) break;
+        if (r >= rl) continue;  // (not break: a constant trip count unrolls)
         const int k0 = __builtin_amdgcn_readlane(off, r);
         const int k1 = __builtin_amdgcn_readlane(off, r + 1);
         float F[4] = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -523,7 +580,8 @@ __device__ __forceinline__ void qn_step_body(const StepArgs &a) {
         }
         if (k < k1) add(F, lsig[wv][k][half], lval[wv][k]);
         const int64_t row = row0 + r;
-        qn_finish_row<QW, NT>(d[r], F, kc, P + (row * Q + q) * 64, ps, Sn + row * W + w0, lane);
+        qn_finish_row<QW, NT>(d[r], F, kc, P + (row * Q + q) * 64, (row * Q + q) * 64, ps,
+                              Sn + row * W + w0, lane);
       }
       return;
     }
@@ -531,7 +589,7 @@ __device__ __forceinline__ void qn_step_body(const StepArgs &a) {
 #pragma unroll
   for (int r = 0; r < R; r++) {
     const int64_t row = row0 + r;
-    if (row >= a.N) break;
+    if (row >= a.N) continue;
     const int64_t k0 = rp[row], k1 = rp[row + 1];
     float F[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     int64_t k = k0;
@@ -563,7 +621,8 @@ __device__ __forceinline__ void qn_step_body(const StepArgs &a) {
         F[u] = ((sw >> lane) & 1) ? F[u] + v : F[u] - v;
       }
     }
-    qn_finish_row<QW, NT>(d[r], F, kc, P + (row * Q + q) * 64, ps, Sn + row * W + w0, lane);
+    qn_finish_row<QW, NT>(d[r], F, kc, P + (row * Q + q) * 64, (row * Q + q) * 64, ps,
+                          Sn + row * W + w0, lane);
   }
 }
 
@@ -600,7 +659,7 @@ __global__ void k_init_qn(uint32_t *P, int64_t N, int64_t A, int64_t Q, uint64_t
     uint32_t d[NP];
     qn_pack<QW>(u, d);
 #pragma unroll
-    for (int p = 0; p < NP; p++) P[p * ps + idx] = d[p];
+    for (int p = 0; p < NP; p++) qn_plane_st<QW>(P, ps, p, idx, d[p]);
   }
 }
 
@@ -613,7 +672,7 @@ __global__ void k_prestep_qn(uint32_t *P, int64_t n, float cy, float cx, int qsh
        idx += (int64_t)gridDim.x * blockDim.x) {
     uint32_t d[NP];
 #pragma unroll
-    for (int p = 0; p < NP; p++) d[p] = P[p * n + idx];
+    for (int p = 0; p < NP; p++) d[p] = qn_plane_ld<QW>(P, n, p, idx);
     float x[4], y[4];
     qn_unpack<QW>(d, x, y, k);
     uint32_t u[4];
@@ -625,7 +684,7 @@ __global__ void k_prestep_qn(uint32_t *P, int64_t n, float cy, float cx, int qsh
     }
     qn_pack<QW>(u, d);
 #pragma unroll
-    for (int p = 0; p < NP; p++) P[p * n + idx] = d[p];
+    for (int p = 0; p < NP; p++) qn_plane_st<QW>(P, n, p, idx, d[p]);
   }
 }
 
@@ -641,7 +700,7 @@ __global__ __launch_bounds__(kBlock) void k_pack_signs_qn(const uint32_t *P, uin
   const int64_t ps = N * Q * 64, o = g * 64 + lane;
   uint32_t d[NP], u[4];
 #pragma unroll
-  for (int p = 0; p < NP; p++) d[p] = P[p * ps + o];
+  for (int p = 0; p < NP; p++) d[p] = qn_plane_ld<QW>(P, ps, p, o);
   qn_units<QW>(d, u);
   uint64_t mine = 0;
 #pragma unroll
@@ -2177,6 +2236,8 @@ void launch_init_qn(hipStream_t st, int qw, uint32_t *P, int64_t N, int64_t A, i
   const dim3 grid(grid_for(N * Q * 64, 256, 16384));
   if (qw == 12)
     hipLaunchKernelGGL(k_init_qn<12>, grid, dim3(256), 0, st, P, N, A, Q, seed, qshift, rowid);
+  else if (qw == 10)
+    hipLaunchKernelGGL(k_init_qn<10>, grid, dim3(256), 0, st, P, N, A, Q, seed, qshift, rowid);
   else
     hipLaunchKernelGGL(k_init_qn<8>, grid, dim3(256), 0, st, P, N, A, Q, seed, qshift, rowid);
 }
@@ -2188,6 +2249,8 @@ void launch_prestep_qn(hipStream_t st, int qw, uint32_t *P, int64_t N, int64_t l
   const dim3 grid(grid_for(n, 256, 16384));
   if (qw == 12)
     hipLaunchKernelGGL(k_prestep_qn<12>, grid, dim3(256), 0, st, P, n, (float)cy, (float)cx, qshift);
+  else if (qw == 10)
+    hipLaunchKernelGGL(k_prestep_qn<10>, grid, dim3(256), 0, st, P, n, (float)cy, (float)cx, qshift);
   else
     hipLaunchKernelGGL(k_prestep_qn<8>, grid, dim3(256), 0, st, P, n, (float)cy, (float)cx, qshift);
 }
@@ -2198,6 +2261,8 @@ void launch_pack_signs_qn(hipStream_t st, int qw, const uint32_t *P, uint64_t *S
   const dim3 grid(grid_for(N * Q, kWaves, INT32_MAX));
   if (qw == 12)
     hipLaunchKernelGGL(k_pack_signs_qn<12>, grid, dim3(kBlock), 0, st, P, S, N, Q);
+  else if (qw == 10)
+    hipLaunchKernelGGL(k_pack_signs_qn<10>, grid, dim3(kBlock), 0, st, P, S, N, Q);
   else
     hipLaunchKernelGGL(k_pack_signs_qn<8>, grid, dim3(kBlock), 0, st, P, S, N, Q);
 }
@@ -2229,7 +2294,7 @@ void launch_xy_update(hipStream_t st, bool f64, void *X, void *Y, int64_t n, con
 }
 
 bool step_supported(int form, int prob, int gather, int vec) {
-  if (form == 4 || form == 5) {
+  if (form == 4 || form == 5 || form == 6) {
     // 4 | rows << 8 | non-temporal << 16 | staged gathers << 17
     const int rows = (vec >> 8) & 0xff, vg = (vec >> 17) & 1;
     if (prob != PROB_MAXCUT || gather != G_DSB || (vec & 0xff) != 4 || (vec >> 18) != 0)
@@ -2313,7 +2378,7 @@ void launch_step(hipStream_t st, int form, int prob, int gather, int vec, const 
 #undef SIS_PACKED_V
 #undef SIS_PACKED_R
 #undef SIS_PACKED
-  } else if (form == 4 || form == 5) {
+  } else if (form == 4 || form == 5 || form == 6) {
     const int rows = (vec >> 8) & 0xff, vg = (vec >> 17) & 1;
     const int64_t waves = ((a.N + rows - 1) / rows) * (a.W / 4);
 #define SIS_QN(QW, R, NT, VG)                                   \
@@ -2321,9 +2386,10 @@ void launch_step(hipStream_t st, int form, int prob, int gather, int vec, const 
     launch_waves(st, k_step_qn<QW, R, NT, VG>, waves, a);       \
     return;                                                     \
   }
-#define SIS_QN_NT(R, VG) \
-  SIS_QN(12, R, 0, VG) SIS_QN(12, R, 1, VG) SIS_QN(8, R, 0, VG) SIS_QN(8, R, 1, VG)
-    const int qw = form == 4 ? 12 : 8;
+#define SIS_QN_NT(R, VG)                                                                    \
+  SIS_QN(12, R, 0, VG) SIS_QN(12, R, 1, VG) SIS_QN(10, R, 0, VG) SIS_QN(10, R, 1, VG) \
+  SIS_QN(8, R, 0, VG) SIS_QN(8, R, 1, VG)
+    const int qw = form == 4 ? 12 : form == 6 ? 10 : 8;
     SIS_QN_NT(1, 0)
     SIS_QN_NT(2, 0)
     SIS_QN_NT(4, 0)

@@ -159,9 +159,10 @@ bool split_supported(int form, int prob, int gather);
 void launch_step_split(hipStream_t st, int form, int prob, int gather, int64_t nent,
                        const StepArgs &a, const SplitArgs &sp);
 
-// Narrow-state dSB forms (X, Y as qw-bit codes, qw = 12 or 8,
-// sis_device.hpp): 3 or 2 dword planes of N x (lda / 256) x 64 lanes; lda is
-// a multiple of 256.
+// Narrow-state dSB forms (X, Y as qw-bit codes, qw = 12, 10 or 8,
+// sis_device.hpp): 3, 2.5 or 2 dword planes of N x (lda / 256) x 64 lanes
+// (the 10-bit form's third plane holds 16-bit values); lda is a multiple of
+// 256.
 void launch_init_qn(hipStream_t st, int qw, uint32_t *P, int64_t N, int64_t A, int64_t lda,
                     uint64_t seed, int qshift, const int32_t *rowid);
 void launch_prestep_qn(hipStream_t st, int qw, uint32_t *P, int64_t N, int64_t lda, double dt,
@@ -170,7 +171,7 @@ void launch_pack_signs_qn(hipStream_t st, int qw, const uint32_t *P, uint64_t *S
                           int64_t lda);
 
 // form: 0 f32, 1 f64, 2 packed, 3 bf16, 4 narrow (12-bit codes), 5 narrow
-// (8-bit codes).  vec = agents per lane (f32/f64/bf16); for the packed form
+// (8-bit codes), 6 narrow (10-bit codes).  vec = agents per lane (f32/f64/bf16); for the packed form
 // vec = words per wavefront | rows per wavefront << 8 | non-temporal state
 // access << 16; for the narrow forms 4 | rows per wavefront << 8 |
 // non-temporal << 16 | staged gathers << 17.

@@ -17,6 +17,15 @@
 
 using namespace sis;
 
+// Code width of a narrow-state form (12, 10 or 8 bits), 0 for the others.
+static int qn_width_of(int form) {
+  return form == SIS_FORM_Q12 ? 12 : form == SIS_FORM_Q10 ? 10 : form == SIS_FORM_Q8 ? 8 : 0;
+}
+
+// Dwords of the narrow state per 256-agent quad and lane: 3 (12-bit), 2.5
+// (10-bit: two dword planes and one of 16-bit values) or 2 (8-bit), times 2.
+static int64_t qn_half_dwords(int qw) { return qw == 12 ? 6 : qw == 10 ? 5 : 4; }
+
 struct sis_solver {
   sis_params p{};
   int dev = 0;
@@ -40,7 +49,7 @@ struct sis_solver {
   DevBuf<double> Xd[2], Yd;      // FP64 state
   DevBuf<uint16_t> sh[2];        // BF16 shadow
   DevBuf<uint64_t> sg[2];        // packed sign words
-  DevBuf<uint32_t> Xq;           // narrow forms: X / Y codes, three (12-bit) or two (8-bit) dword planes
+  DevBuf<uint32_t> Xq;           // narrow forms: X / Y codes, 3, 2.5 or 2 dword planes (12, 10, 8 bits)
   // Row storage order: node_of[r] is the node stored in row r (empty: node
   // order); a copy on the device keys the seeded initial state by node.
   std::vector<int32_t> node_of;
@@ -142,20 +151,22 @@ struct sis_solver {
   }
 
   // narrow-state forms: X, Y as qw()-bit codes
-  bool narrow() const { return p.form == SIS_FORM_Q12 || p.form == SIS_FORM_Q8; }
-  int qw() const { return p.form == SIS_FORM_Q12 ? 12 : 8; }
+  bool narrow() const { return qn_width_of(p.form) != 0; }
+  int qw() const { return qn_width_of(p.form); }
   void *Xbuf(int i) { return f64 ? (void *)Xd[i].p : (void *)Xf[i].p; }
   void *Ybuf() { return f64 ? (void *)Yd.p : (void *)Yf.p; }
   // X holding the current positions.
   void *Xcur() { return p.form == SIS_FORM_F32 || p.form == SIS_FORM_F64 ? Xbuf(par) : Xbuf(0); }
 };
 
-// Narrow-state forms on the host (code width qw = 12 or 8): the same
+// Narrow-state forms on the host (code width qw = 12, 10 or 8): the same
 // rounding as qn_encode (device) -- X code round(x 2^(qw-2)), +-(2^(qw-2) + 1)
 // beyond the wall with a 0 Y code, Y code round(y 2^qshift) clamped to
 // +-(2^(qw-1) - 1), nearest even, each stored plus 2^(qw-1) in qw bits -- and
-// the plane layout (dword p of lane l, quad q, row i at
-// p N Q 64 + (i Q + q) 64 + l; 3 planes for qw = 12, 2 for qw = 8).
+// the plane layout (value p of lane l, quad q, row i at
+// p N Q 64 + (i Q + q) 64 + l in dwords; 3 dword planes for qw = 12, 2 for
+// qw = 8; for qw = 10 two dword planes and a third of 16-bit values, value o
+// at 16-bit offset 2 (2 N Q 64) + o).
 static int qn_shift_for(int qw, double dt) {
   if (!(dt > 0)) return -1;
   const double ymax = 2.0 / dt + dt, ym = (double)((1 << (qw - 1)) - 1);
@@ -178,30 +189,51 @@ static uint32_t qn_unit_host(int qw, float x, float y, int qshift) {
   return (uint32_t)(xq + bias) | ((uint32_t)(yq + bias) << qw);
 }
 
+// Value o of plane p (a 16-bit value in the 10-bit form's third plane).
+static uint32_t qn_plane_get(int qw, const std::vector<uint32_t> &P, int64_t ps, int64_t p,
+                             int64_t o) {
+  if (qw == 10 && p == 2) {
+    uint16_t h;
+    std::memcpy(&h, reinterpret_cast<const char *>(P.data()) + 4 * (size_t)(2 * ps) + 2 * (size_t)o,
+                2);
+    return h;
+  }
+  return P[(size_t)(p * ps + o)];
+}
+
+static void qn_plane_set(int qw, std::vector<uint32_t> &P, int64_t ps, int64_t p, int64_t o,
+                         uint32_t v) {
+  if (qw == 10 && p == 2) {
+    const uint16_t h = (uint16_t)v;
+    std::memcpy(reinterpret_cast<char *>(P.data()) + 4 * (size_t)(2 * ps) + 2 * (size_t)o, &h, 2);
+    return;
+  }
+  P[(size_t)(p * ps + o)] = v;
+}
+
 // Unit j of a lane occupies bits ub j .. ub j + ub - 1 (ub = 2 qw) of its
-// NP dwords: read and written through the 64-bit window of dwords p, p + 1
-// that holds it.
+// planes taken as one bit string (plane p at bit 32 p): read and written
+// through the 64-bit window of planes p, p + 1 that holds it.
 static void qn_put(int qw, std::vector<uint32_t> &P, int64_t N, int64_t Q, int64_t i, int64_t a,
                    uint32_t u) {
-  const int64_t ub = 2 * qw, np = qw == 12 ? 3 : 2;
+  const int64_t ub = 2 * qw, np = qw == 8 ? 2 : 3;
   const uint64_t m = ((uint64_t)1 << ub) - 1;
   const int64_t q = a / 256, j = (a % 256) / 64, l = a % 64, ps = N * Q * 64;
   const int64_t o = (i * Q + q) * 64 + l, p = (ub * j) / 32, r = (ub * j) % 32;
-  uint32_t &lo = P[(size_t)(p * ps + o)];
-  uint64_t win = lo;
-  if (p < np - 1) win |= (uint64_t)P[(size_t)((p + 1) * ps + o)] << 32;
+  uint64_t win = qn_plane_get(qw, P, ps, p, o);
+  if (p < np - 1) win |= (uint64_t)qn_plane_get(qw, P, ps, p + 1, o) << 32;
   win = (win & ~(m << r)) | (((uint64_t)u & m) << r);
-  lo = (uint32_t)win;
-  if (p < np - 1) P[(size_t)((p + 1) * ps + o)] = (uint32_t)(win >> 32);
+  qn_plane_set(qw, P, ps, p, o, (uint32_t)win);
+  if (p < np - 1) qn_plane_set(qw, P, ps, p + 1, o, (uint32_t)(win >> 32));
 }
 
 static uint32_t qn_get(int qw, const std::vector<uint32_t> &P, int64_t N, int64_t Q, int64_t i,
                        int64_t a) {
-  const int64_t ub = 2 * qw, np = qw == 12 ? 3 : 2;
+  const int64_t ub = 2 * qw, np = qw == 8 ? 2 : 3;
   const int64_t q = a / 256, j = (a % 256) / 64, l = a % 64, ps = N * Q * 64;
   const int64_t o = (i * Q + q) * 64 + l, p = (ub * j) / 32, r = (ub * j) % 32;
-  uint64_t win = P[(size_t)(p * ps + o)];
-  if (p < np - 1) win |= (uint64_t)P[(size_t)((p + 1) * ps + o)] << 32;
+  uint64_t win = qn_plane_get(qw, P, ps, p, o);
+  if (p < np - 1) win |= (uint64_t)qn_plane_get(qw, P, ps, p + 1, o) << 32;
   return (uint32_t)(win >> r) & (uint32_t)(((uint64_t)1 << ub) - 1);
 }
 
@@ -335,9 +367,10 @@ static int default_tile(const sis_solver *s) {
   int v = 1;
   if (s->narrow()) {
     // staged gathers of 16 rows per wavefront; non-temporal codes once they
-    // exceed the Infinity Cache (3 or 2 bytes per node-agent)
+    // exceed the Infinity Cache (3, 2.5 or 2 bytes per node-agent)
     v = 4 | 16 << 8 | 1 << 17;
-    if ((double)s->N * (double)s->lda * (s->qw() == 12 ? 3.0 : 2.0) > 256.0 * 1024 * 1024)
+    if ((double)s->N * (double)s->lda * (double)qn_half_dwords(s->qw()) / 2.0 >
+        256.0 * 1024 * 1024)
       v |= 1 << 16;
     return v;
   }
@@ -604,9 +637,9 @@ int sis_solver_create(const sis_csr *J, const double *field, const sis_params *p
     return fail(SIS_ERR_ARG, "sis_solver_create: the packed-sign form is dSB only");
   if (p.form == SIS_FORM_BF16 && p.dynamics != SIS_BSB)
     return fail(SIS_ERR_ARG, "sis_solver_create: the BF16 form is bSB only");
-  if (p.form < SIS_FORM_F32 || p.form > SIS_FORM_Q8)
+  if (p.form < SIS_FORM_F32 || p.form > SIS_FORM_Q10)
     return fail(SIS_ERR_ARG, "sis_solver_create: unknown form");
-  if (p.form == SIS_FORM_Q12 || p.form == SIS_FORM_Q8) {
+  if (qn_width_of(p.form)) {
     if (p.problem != SIS_MAXCUT || p.dynamics != SIS_DSB || !p.fused)
       return fail(SIS_ERR_ARG, "sis_solver_create: the narrow-state forms are fused MaxCut dSB only");
     if (!(p.dt > 0))
@@ -647,13 +680,13 @@ int sis_solver_create(const sis_csr *J, const double *field, const sis_params *p
     }
   }
   s->A = p.agents;
-  const bool narrow = p.form == SIS_FORM_Q12 || p.form == SIS_FORM_Q8;
+  const int qw = qn_width_of(p.form);
+  const bool narrow = qw != 0;
   s->lda = narrow ? (s->A + 255) / 256 * 256 : (s->A + 63) / 64 * 64;
-  s->qshift = narrow ? qn_shift_for(p.form == SIS_FORM_Q12 ? 12 : 8, p.dt) : 0;
+  s->qshift = narrow ? qn_shift_for(qw, p.dt) : 0;
   if (narrow) {
     // The step leaves the Y clamp out, which is exact only while the Y code
     // range covers the bound 2/dt + dt of a live element's y.
-    const int qw = p.form == SIS_FORM_Q12 ? 12 : 8;
     if (std::ldexp((double)((1 << (qw - 1)) - 1), -s->qshift) < 2.0 / p.dt + p.dt)
       return fail(SIS_ERR_ARG, "sis_solver_create: dt too small for the narrow-state form's "
                                "Y code range (needs (2^(w-1) - 1) >= 2/dt + dt)");
@@ -744,11 +777,10 @@ int sis_solver_create(const sis_csr *J, const double *field, const sis_params *p
     if ((st = s->Xd[0].alloc((size_t)NL)) || (st = s->Xd[1].alloc((size_t)NL)) ||
         (st = s->Yd.alloc((size_t)NL)))
       return st;
-  } else if (p.form == SIS_FORM_Q12 || p.form == SIS_FORM_Q8) {
-    // three (12-bit) or two (8-bit) dword planes of N x (lda / 256) x 64
-    // lanes: 3 or 2 bytes per node-agent
-    const int64_t np = p.form == SIS_FORM_Q12 ? 3 : 2;
-    if ((st = s->Xq.alloc((size_t)(np * N * (s->lda / 4)))) ||
+  } else if (s->narrow()) {
+    // 3, 2.5 or 2 dword planes of N x (lda / 256) x 64 lanes (12, 10 or 8
+    // bits): 3, 2.5 or 2 bytes per node-agent
+    if ((st = s->Xq.alloc((size_t)(qn_half_dwords(s->qw()) * N * (s->lda / 8)))) ||
         (st = s->sg[0].alloc((size_t)NW)) || (st = s->sg[1].alloc((size_t)NW)))
       return st;
   } else {
@@ -1002,6 +1034,7 @@ static void enqueue_step(sis_solver *s, int32_t off) {
       a.X = s->Xf[0].p;
       break;
     case SIS_FORM_Q12:
+    case SIS_FORM_Q10:
     case SIS_FORM_Q8:
       a.gsrc = s->sg[cur].p;
       a.gdst = s->sg[nxt].p;
@@ -1102,6 +1135,7 @@ static void enqueue_window(sis_solver *s, int64_t P, int uw) {
   switch (s->p.form) {
     case SIS_FORM_PACKED:
     case SIS_FORM_Q12:
+    case SIS_FORM_Q10:
     case SIS_FORM_Q8: src = SRC_BITS; x = s->sg[s->par].p; break;
     case SIS_FORM_F64: src = SRC_F64; x = s->Xcur(); break;
     default: src = SRC_F32; x = s->Xcur(); break;

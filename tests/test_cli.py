@@ -73,9 +73,9 @@ def test_maxcut_line_matches_oracle(tmp_path, fused, knobs):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("form,w", [("q12", 12), ("q8", 8)])
+@pytest.mark.parametrize("form,w", [("q12", 12), ("q10", 10), ("q8", 8)])
 def test_maxcut_line_narrow_form_matches_oracle(tmp_path, form, w):
-    """SIS_FORM=q12 / q8 (12-bit / 8-bit X/Y codes): the line's cut statistics
+    """SIS_FORM=q12 / q10 / q8 (12-, 10- or 8-bit X/Y codes): the line's cut statistics
     equal the oracle's narrow-state solve from the same seed."""
     J = sis.Csr.gen_er(300, 0.05, sis.W_PM1, 3)
     mtx = tmp_path / "g.mtx"

@@ -1,7 +1,7 @@
-"""GPU parity of the narrow-state dSB forms (SIS_FORM_Q12 / SIS_FORM_Q8: X
-and Y stored as 12-bit / 8-bit fixed-point codes, FP32 arithmetic, exact
-J sgn(X) from packed sign words) against the oracle's restatement of the same
-forms (orc_*_qn with width w = 12 / 8).
+"""GPU parity of the narrow-state dSB forms (SIS_FORM_Q12 / SIS_FORM_Q10 /
+SIS_FORM_Q8: X and Y stored as 12-, 10- or 8-bit fixed-point codes, FP32
+arithmetic, exact J sgn(X) from packed sign words) against the oracle's
+restatement of the same forms (orc_*_qn with width w = 12 / 10 / 8).
 
 Bar: bit-exact.  The codes, the FP32 update on the decoded values and the
 rounding are deterministic, so the device's X, Y codes, spins, locks, step
@@ -17,8 +17,8 @@ import sis
 
 pytestmark = pytest.mark.gpu
 
-FORMW = {12: sis.FORM_Q12, 8: sis.FORM_Q8}
-XS = {12: 10, 8: 6}  # X code scale 2^xs; the wall code is +-(2^xs + 1)
+FORMW = {12: sis.FORM_Q12, 10: sis.FORM_Q10, 8: sis.FORM_Q8}
+XS = {12: 10, 10: 8, 8: 6}  # X code scale 2^xs; the wall code is +-(2^xs + 1)
 VG = 1 << 17  # staged gathers: a wavefront's rows' entries fetched at once into LDS
 TILES = [4 | 1 << 8, 4 | 2 << 8, 4 | 4 << 8, 4 | 1 << 8 | 1 << 16, 4 | 2 << 8 | 1 << 16,
          4 | 4 << 8 | 1 << 16, 4 | 4 << 8 | VG, 4 | 8 << 8 | VG | 1 << 16, 4 | 16 << 8 | VG,
@@ -38,7 +38,7 @@ def seeded(n, A, seed, scale=0.1):
     return X, Y
 
 
-@pytest.mark.parametrize("w", [12, 8])
+@pytest.mark.parametrize("w", [12, 10, 8])
 def test_init_random_equals_rounded_oracle_stream(w):
     J, _ = problem(70)
     for A in (64, 300):
@@ -50,7 +50,7 @@ def test_init_random_equals_rounded_oracle_stream(w):
         assert np.array_equal(x, X.astype(np.float64)) and np.array_equal(y, Y.astype(np.float64))
 
 
-@pytest.mark.parametrize("w", [12, 8])
+@pytest.mark.parametrize("w", [12, 10, 8])
 @pytest.mark.parametrize("dt", [0.5, 0.1])
 def test_set_state_rounds_like_the_oracle(w, dt):
     # values off the grid, halfway cases, beyond the wall, negative zero and
@@ -70,7 +70,7 @@ def test_set_state_rounds_like_the_oracle(w, dt):
     assert not np.signbit(x[x == 0]).any()
 
 
-@pytest.mark.parametrize("w", [12, 8])
+@pytest.mark.parametrize("w", [12, 10, 8])
 @pytest.mark.parametrize("tile", TILES)
 @pytest.mark.parametrize("agents", [96, 256, 300])
 def test_trajectory_bit_exact(w, tile, agents):
@@ -97,12 +97,14 @@ def test_trajectory_bit_exact(w, tile, agents):
     assert (np.abs(X) > 1).any()  # the wall code occurred
 
 
-@pytest.mark.parametrize("w,dt", [(8, 0.016), (8, 1.0), (8, 2.0), (12, 0.001), (12, 2.0)])
+@pytest.mark.parametrize("w,dt", [(8, 0.016), (8, 1.0), (8, 2.0), (10, 0.004), (10, 2.0),
+                                  (12, 0.001), (12, 2.0)])
 def test_trajectory_bit_exact_at_the_y_range_edges(w, dt):
     # The step encodes Y without the clamp the oracle applies: exact while
-    # the code range holds 2/dt + dt.  dt 0.016 (8-bit) and 0.001 (12-bit)
-    # sit just above the refused range (shift 0: the bound 125.0 / 2000.0 is
-    # within one code range of 127 / 2047); dt 1 and 2 give large steps.  A
+    # the code range holds 2/dt + dt.  dt 0.016 (8-bit), 0.004 (10-bit) and
+    # 0.001 (12-bit) sit just above the refused range (shift 0: the bound
+    # 125.0 / 500.0 / 2000.0 is within one code range of 127 / 511 / 2047);
+    # dt 1 and 2 give large steps.  A
     # strong coupling and a state near the walls push |y| up.
     J, pb = problem(203, 0.04, sis.W_INT3, 19)
     A, T = 128, 300
@@ -124,7 +126,7 @@ def test_trajectory_bit_exact_at_the_y_range_edges(w, dt):
     assert ymax <= 2 / dt + dt + 2.0 ** -(sy + 1) and ymax < (2 ** (w - 1) - 1) * 2.0 ** -sy
 
 
-@pytest.mark.parametrize("w", [12, 8])
+@pytest.mark.parametrize("w", [12, 10, 8])
 @pytest.mark.parametrize("tile", [4 | 4 << 8 | VG, 4 | 8 << 8 | VG | 1 << 16, 4 | 32 << 8 | VG])
 @pytest.mark.parametrize("agents", [96, 300])
 def test_unit_weight_waves_beside_real_weight_waves(w, tile, agents):
@@ -157,7 +159,7 @@ def test_unit_weight_waves_beside_real_weight_waves(w, tile, agents):
     assert (np.abs(X) > 1).any()
 
 
-@pytest.mark.parametrize("w", [12, 8])
+@pytest.mark.parametrize("w", [12, 10, 8])
 @pytest.mark.parametrize("tile", [4 | 8 << 8 | VG, 4 | 16 << 8 | VG | 1 << 16, 4 | 4 << 8 | VG])
 @pytest.mark.parametrize("unit", [False, True])
 def test_staged_gathers_hub_and_empty_rows(w, tile, unit):
@@ -190,7 +192,7 @@ def test_staged_gathers_hub_and_empty_rows(w, tile, unit):
     assert np.array_equal(x, X.astype(np.float64)) and np.array_equal(y, Y.astype(np.float64))
 
 
-@pytest.mark.parametrize("w", [12, 8])
+@pytest.mark.parametrize("w", [12, 10, 8])
 @pytest.mark.parametrize("uw,T", [(1, 6000), (0, 2345)])
 @pytest.mark.parametrize("graph", [0, 1, 2])
 def test_full_run_matches_oracle(w, uw, T, graph):
@@ -216,7 +218,7 @@ def test_full_run_matches_oracle(w, uw, T, graph):
         assert info["all_bifurcated"] == int(uw == 1 and r["bif"].all())
 
 
-@pytest.mark.parametrize("w", [12, 8])
+@pytest.mark.parametrize("w", [12, 10, 8])
 def test_autotune_keeps_state_and_results(w):
     J, pb = problem(300, 0.03, sis.W_PM1, 14)
     A, dt = 256, 0.5
@@ -235,7 +237,7 @@ def test_autotune_keeps_state_and_results(w):
     assert np.array_equal(s.cuts()[0], orc.cut(pb, r["cur"]))
 
 
-@pytest.mark.parametrize("w", [12, 8])
+@pytest.mark.parametrize("w", [12, 10, 8])
 def test_full_size_on_agent_sample_and_exact_cut(w):
     """Random 4-regular N = 2^20, 256 agents (the benchmark graph): 30 steps
     after the pre-step, every 8th agent bit-exact with the oracle; after a
@@ -268,11 +270,12 @@ def test_full_size_on_agent_sample_and_exact_cut(w):
     assert np.array_equal(cuts, -0.5 * (H - J.sum))
 
 
-@pytest.mark.parametrize("w", [12, 8])
+@pytest.mark.parametrize("w", [12, 10, 8])
 def test_rejected_where_it_does_not_apply(w):
     J, _ = problem(60)
-    # dt below the Y code range's bound (127 < 2/dt + dt for 8 bits, 2047 for 12)
-    tiny = 0.015 if w == 8 else 0.0009
+    # dt below the Y code range's bound (127 < 2/dt + dt for 8 bits, 511 for
+    # 10, 2047 for 12)
+    tiny = {8: 0.015, 10: 0.0039, 12: 0.0009}[w]
     bad = [dict(dynamics=sis.BSB), dict(fused=0), dict(dt=0.0), dict(split=8), dict(split=-1),
            dict(tile=1 << 20), dict(dt=tiny)]
     for extra in bad:

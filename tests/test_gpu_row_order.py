@@ -1,9 +1,9 @@
 """GPU parity of the row storage order (sis_params.row_order).
 
-The locality order stores the rows of the sign-gather forms (packed, 12-bit,
-8-bit) grouped by graph neighbourhood; states, spins and results still come and
-go in node order, each row keeps its entries in CSR order and the seeded
-initial state is keyed by node.  Bar: bit-exact -- a solver in the locality
+The locality order stores the rows of the sign-gather forms (packed, 12-, 10-
+and 8-bit codes) grouped by graph neighbourhood; states, spins and results
+still come and go in node order, each row keeps its entries in CSR order and
+the seeded initial state is keyed by node.  Bar: bit-exact -- a solver in the locality
 order equals one in node order and the oracle in every state, spin, lock, step
 count and cut.
 """
@@ -15,7 +15,7 @@ import sis
 
 pytestmark = pytest.mark.gpu
 
-WIDTH = {"q12": 12, "q8": 8}
+WIDTH = {"q12": 12, "q10": 10, "q8": 8}
 
 
 def oracle_solve(pb, form, X, Y, T, uw, dt, xi0):
@@ -45,7 +45,7 @@ def same(s1, s0):
     assert np.array_equal(c1, c0) and b1 == b0
 
 
-@pytest.mark.parametrize("form", ["packed", "q12", "q8"])
+@pytest.mark.parametrize("form", ["packed", "q12", "q10", "q8"])
 @pytest.mark.parametrize("graph", [0, 2])
 def test_locality_order_full_run_equals_node_order_and_oracle(form, graph):
     # integer weights in [-3, 3] on a 4-regular graph, 300 agents (ragged),
@@ -69,7 +69,7 @@ def test_locality_order_full_run_equals_node_order_and_oracle(form, graph):
     assert np.array_equal(s1.cuts()[0], orc.cut(pb, r["cur"]))
 
 
-@pytest.mark.parametrize("form", ["packed", "q8"])
+@pytest.mark.parametrize("form", ["packed", "q10", "q8"])
 def test_locality_order_set_state_and_advance(form):
     # explicit states in node order go in and come out in node order; an
     # isolated node and an ER graph with uneven degrees

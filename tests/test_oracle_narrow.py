@@ -1,5 +1,5 @@
 """Pins of the oracle's narrow-state dSB forms (X and Y as fixed-point codes
-of width w = 12 or w = 8 bits): known answers of the rounding rule, the
+of width w = 12, 10 or 8 bits): known answers of the rounding rule, the
 code-grid invariants, the Y range bound of the dynamics, the distance to the
 FP32 step, the exact kick against scipy, and a solve that must find a
 bipartite graph's maximum cut.
@@ -20,16 +20,20 @@ import oracle as orc
 from test_oracle_pins import random_problem
 
 # w: X scale shift xs = w - 2 (codes +-2^xs, wall +-(2^xs + 1)), Y code limit
-WIDTHS = {12: dict(xs=10, ym=2047), 8: dict(xs=6, ym=127)}
+WIDTHS = {12: dict(xs=10, ym=2047), 10: dict(xs=8, ym=511), 8: dict(xs=6, ym=127)}
 
 
 @pytest.mark.parametrize("w,dt,want", [
     (12, 0.5, 8), (12, 0.1, 6), (12, 1.0, 9), (12, 2.0, 9), (12, 0.05, 5), (12, 0.3, 8), (12, 0.2, 7),
+    (10, 0.5, 6), (10, 0.1, 4), (10, 1.0, 7), (10, 2.0, 7), (10, 0.05, 3), (10, 0.3, 6), (10, 0.2, 5),
     (8, 0.5, 4), (8, 0.1, 2), (8, 1.0, 5), (8, 2.0, 5), (8, 0.05, 1), (8, 0.3, 4), (8, 0.2, 3)])
 def test_shift_known_answers(w, dt, want):
     # largest s in [0, w - 1] with ym 2^-s >= 2/dt + dt, worked by hand:
     # w 12, dt 0.5: ymax 4.5, 2047/256 = 8.0 >= 4.5 > 2047/512; dt 0.1: 20.1 -> 64;
     # dt 0.3: 6.97 <= 2047/256; dt 0.2: 10.2 -> 128.
+    # w 10, dt 0.5: 511/64 = 7.98 >= 4.5 > 511/128; dt 0.1: 511/16 = 31.9 >= 20.1 > 511/32;
+    # dt 1 and 2: 511/128 = 3.99 >= 3.0 > 511/256; dt 0.05: 511/8 = 63.9 >= 40.05 > 511/16;
+    # dt 0.3: 6.97 <= 511/64; dt 0.2: 511/32 = 15.97 >= 10.2 > 511/64.
     # w 8, dt 0.5: 127/16 = 7.9 >= 4.5 > 127/32; dt 0.1: 127/4 = 31.75 >= 20.1 > 127/8;
     # dt 1 and 2: 3.0 <= 127/32 = 3.97; dt 0.05: 40.05 <= 63.5; dt 0.2: 10.2 <= 15.9
     s = orc.qn_shift(dt, w)
@@ -41,20 +45,21 @@ def test_shift_known_answers(w, dt, want):
         assert orc.q12_shift(dt) == s
 
 
-@pytest.mark.parametrize("w", [12, 8])
+@pytest.mark.parametrize("w", [12, 10, 8])
 def test_shift_rejects_nonpositive_dt(w):
     assert orc.qn_shift(0.0, w) == -1 and orc.qn_shift(-0.5, w) == -1
 
 
 def test_shift_rejects_unknown_width():
-    assert orc.qn_shift(0.5, 10) == -1
+    for w in (11, 9, 16, 0):
+        assert orc.qn_shift(0.5, w) == -1
 
 
-@pytest.mark.parametrize("w", [12, 8])
+@pytest.mark.parametrize("w", [12, 10, 8])
 def test_rounding_known_answers(w):
     xs, ym = WIDTHS[w]["xs"], WIDTHS[w]["ym"]
     top = 2 ** xs
-    sy = 8 if w == 12 else 4
+    sy = {12: 8, 10: 6, 8: 4}[w]
     x = np.array([2.5, 3.5, -2.5, -0.3, 0.4, top, top + 0.0001 * top / 1024, -1.5 * top, top - 0.4, -0.0],
                  np.float64) / top
     y = np.array([0.5, 1.5, -0.5, 0.0, 100.0, -100.0, 3.0, 3.0, 2.0, 0.7], np.float64) / 2.0 ** sy
@@ -82,7 +87,7 @@ def _trajectory(w, n=120, A=64, dt=0.5, seed=1):
     return pb, M, X, Y, sy
 
 
-@pytest.mark.parametrize("w", [12, 8])
+@pytest.mark.parametrize("w", [12, 10, 8])
 def test_code_grid_and_wall_invariants(w):
     xs, ym = WIDTHS[w]["xs"], WIDTHS[w]["ym"]
     pb, M, X, Y, sy = _trajectory(w)
@@ -102,7 +107,7 @@ def test_code_grid_and_wall_invariants(w):
     assert walls > 0  # the run reaches the walls, so the wall path is exercised
 
 
-@pytest.mark.parametrize("w", [12, 8])
+@pytest.mark.parametrize("w", [12, 10, 8])
 def test_y_range_bound_holds_on_live_elements(w):
     # |y| <= 2/dt + dt while |x| <= 1, so the code clamp never binds a live
     # element: the largest |y| seen stays below the code range.
@@ -119,7 +124,7 @@ def test_y_range_bound_holds_on_live_elements(w):
         assert ymax <= 2 / dt + dt < ym * 2.0 ** -sy
 
 
-@pytest.mark.parametrize("w", [12, 8])
+@pytest.mark.parametrize("w", [12, 10, 8])
 def test_one_step_within_half_a_code_of_fp32(w):
     # From the same on-grid state, the narrow step equals the FP32 step
     # rounded once at the end: within half a code of X and of Y (and the wall
@@ -141,7 +146,7 @@ def test_one_step_within_half_a_code_of_fp32(w):
         assert np.abs(Yq[live] - Y32[live]).max() <= 2.0 ** -(sy + 1)
 
 
-@pytest.mark.parametrize("w,sy", [(12, 6), (8, 2)])
+@pytest.mark.parametrize("w,sy", [(12, 6), (10, 4), (8, 2)])
 def test_kick_is_exact_minus_J_sgn_x(w, sy):
     # c = (0, 0, 1, 0), y = 0: the new Y is the force F = -J sgn(x) exactly
     # (integers within the code range), with sgn(+0) = +1 for zero codes.
@@ -176,7 +181,7 @@ def test_q12_wrappers_are_width_12():
     assert np.array_equal(Xa, Xb) and np.array_equal(Y3, Y4)
 
 
-@pytest.mark.parametrize("w", [12, 8])
+@pytest.mark.parametrize("w", [12, 10, 8])
 def test_narrow_dsb_finds_bipartite_max_cut(w):
     # An even cycle with +1 weights is bipartite, so its maximum cut takes
     # every edge; the narrow form must find it as the FP32 form does.
