@@ -4,14 +4,15 @@ Workload (default): MaxCut dSB on a random 4-regular graph, N = 2^20 nodes,
 Z = 4 N = 4,194,304 symmetric nonzeros, weights +-1 uniform from seed, 256
 agents, dt = 0.5, max_steps = K (20,000 by default), windows every 100 steps.
 A "step" is one fused SB step over all N x 256 node-agents.  The headline
-form (--form, default q12: X and Y stored as 12-bit fixed-point codes, FP32
-arithmetic, J sgn(X) exact from packed sign words) runs beside the plain
-FP32 dSB kernel for the speed-up and the equal-steps best-cut comparison;
---also adds more forms (default: the 8-bit narrow-state form q8, faster but
-outside the 0.1% best-cut bar on some graph families -- see --workload
-quality -- and the packed-sign form, bit-identical to plain FP32), and the
-line also carries bSB with BF16 gathers against plain FP32
-bSB, a batched-serving summary (4096 small instances) and the CPU baseline.
+form (--form, default q10: X and Y stored as 10-bit fixed-point codes, 2.5
+bytes per node-agent, FP32 arithmetic, J sgn(X) exact from packed sign
+words) runs beside the plain FP32 dSB kernel for the speed-up and the
+equal-steps best-cut comparison; --also adds more forms (default: the
+12-bit form q12; the 8-bit form q8, faster but outside the 0.1% best-cut bar
+on some graph families -- see --workload quality; and the packed-sign form,
+bit-identical to plain FP32), and the line also carries bSB with BF16
+gathers against plain FP32 bSB, a batched-serving summary (4096 small
+instances) and the CPU baseline.
 
 The timed region is one sis_solver_run (pre-step + steps + windows), with the
 state resident in HBM; the update window's early exit is off inside the timed
@@ -625,7 +626,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--form", default="q12", choices=list(sis.FORMS))
+    ap.add_argument("--form", default="q10", choices=list(sis.FORMS))
     ap.add_argument("--baseline-form", default="f32", choices=list(sis.FORMS) + ["none"])
     ap.add_argument("--nodes-log2", type=int, default=20)
     ap.add_argument("--agents", type=int, default=256)
@@ -635,9 +636,10 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving sub-object")
     ap.add_argument("--no-graph", action="store_true",
                     help="skip the whole-run hipGraph replay of the headline solve")
-    ap.add_argument("--also", default="q8,packed", help="comma-separated extra forms timed "
-                    "beside the headline (same dynamics); the default adds the 8-bit "
-                    "narrow-state form and the packed form (bit-identical to plain FP32)")
+    ap.add_argument("--also", default="q12,q8,packed", help="comma-separated extra forms "
+                    "timed beside the headline (same dynamics); the default adds the 12- and "
+                    "8-bit narrow-state forms and the packed form (bit-identical to plain "
+                    "FP32)")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--workload", default="regular2p20",
                     choices=["regular2p20", "er800", "torus100", "tsp48", "powerlaw", "tsp_struct",
