@@ -259,7 +259,7 @@ typedef struct sis_params {
                                every result is the same in either order):
                                0 = chosen by the solver: a locality order for the
                                fused MaxCut dSB forms that gather packed sign words
-                               (packed, 12-bit, 8-bit) on graphs whose sign words
+                               (packed, 12-, 10- and 8-bit) on graphs whose sign words
                                exceed an L2 slice (N x agents / 8 > 4 MiB) and whose
                                weights are integers; 1 = the locality order whenever
                                those forms and integer weights allow it; -1 = node
