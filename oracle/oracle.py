@@ -232,7 +232,9 @@ def tri(c1, c2, n):
 # ---------------------------------------------------------------- narrow-state dSB forms
 # Code width w: 12 (X codes +-1024 and the wall code +-1025, Y codes +-2047),
 # 10 (X codes +-256, wall +-257, Y codes +-511) or 8 (X codes +-64, wall +-65,
-# Y codes +-127).
+# Y codes +-127).  The 8-bit form also comes with stochastic Y rounding
+# (sr=True, SIS_FORM_Q8S): Y rounds to floor(y 2^sy + u), u a hashed dither
+# of (step, node, agent).
 
 def qn_shift(dt, w):
     """Y code shift sy of the w-bit form for time step dt (-1 if dt <= 0)."""
@@ -246,42 +248,60 @@ def qn_round(X, Y, sy, w):
     lib().orc_qn_round_state(C.c_int(w), C.c_int64(X.size), _p(X), _p(Y), C.c_int(sy))
 
 
-def step_qn(pb, Xs, Y, c, sy, w):
-    """One narrow-state step from the snapshot Xs: returns the new X, updates Y."""
+def step_qn(pb, Xs, Y, c, sy, w, sr=False, t=0):
+    """One narrow-state step from the snapshot Xs: returns the new X, updates Y.
+    sr (8-bit only): Y rounded stochastically with the dither of step index t."""
     n, A = Xs.shape
     Xn = np.zeros_like(Xs)
     cc = np.ascontiguousarray(c, np.float64)
-    lib().orc_step_qn(C.c_int(w), _p(pb.rp), _p(pb.col), _p(pb.val32), C.c_int64(n), C.c_int64(A),
-                      _p(Xs), _p(Xn), _p(Y), _p(cc), C.c_int(sy))
+    if sr:
+        assert w == 8
+        lib().orc_step_q8s(_p(pb.rp), _p(pb.col), _p(pb.val32), C.c_int64(n), C.c_int64(A),
+                           _p(Xs), _p(Xn), _p(Y), _p(cc), C.c_int(sy), C.c_int64(t))
+    else:
+        lib().orc_step_qn(C.c_int(w), _p(pb.rp), _p(pb.col), _p(pb.val32), C.c_int64(n),
+                          C.c_int64(A), _p(Xs), _p(Xn), _p(Y), _p(cc), C.c_int(sy))
     return Xn
 
 
-def advance_qn(pb, X, Y, t0, nsteps, T, dt, xi0, sy, w):
-    """Narrow-state dSB MaxCut steps t0..t0+nsteps-1 in place (decoded codes)."""
+def advance_qn(pb, X, Y, t0, nsteps, T, dt, xi0, sy, w, sr=False):
+    """Narrow-state dSB MaxCut steps t0..t0+nsteps-1 in place (decoded codes);
+    sr: the 8-bit form with stochastic Y rounding."""
     n, A = X.shape
-    lib().orc_advance_qn(C.c_int(w), _p(pb.rp), _p(pb.col), _p(pb.val32), C.c_int64(n),
-                         C.c_int64(A), C.c_int64(t0), C.c_int64(nsteps), C.c_int64(T),
-                         C.c_double(dt), C.c_double(xi0), C.c_int(sy), _p(X), _p(Y))
+    args = (_p(pb.rp), _p(pb.col), _p(pb.val32), C.c_int64(n), C.c_int64(A), C.c_int64(t0),
+            C.c_int64(nsteps), C.c_int64(T), C.c_double(dt), C.c_double(xi0), C.c_int(sy),
+            _p(X), _p(Y))
+    if sr:
+        assert w == 8
+        lib().orc_advance_q8s(*args)
+    else:
+        lib().orc_advance_qn(C.c_int(w), *args)
 
 
 def prestep_qn(X, Y, dt, sy, w):
     lib().orc_prestep_qn(C.c_int(w), C.c_int64(X.size), C.c_double(dt), C.c_int(sy), _p(X), _p(Y))
 
 
-def solve_qn(pb, X, Y, T, uw, P, K=25, dt=0.5, xi0=0.0, sy=None, w=12):
+def solve_qn(pb, X, Y, T, uw, P, K=25, dt=0.5, xi0=0.0, sy=None, w=12, sr=False):
     """Whole narrow-state dSB MaxCut solve in place on float32 X, Y (rounded to
-    the codes first).  Returns dict(steps, cur, fin, stab, bif)."""
+    the codes first).  sr: the 8-bit form with stochastic Y rounding.
+    Returns dict(steps, cur, fin, stab, bif)."""
     n, A = X.shape
     sy = qn_shift(dt, w) if sy is None else sy
     cur = np.full((n, A), -1, np.int32)
     fin = np.full((n, A), -1, np.int32)
     stab = np.zeros(A, np.int32)
     bif = np.zeros(A, np.uint8)
-    lib().orc_solve_qn.restype = C.c_int64
-    steps = lib().orc_solve_qn(C.c_int(w), _p(pb.rp), _p(pb.col), _p(pb.val32), C.c_int64(n),
-                               C.c_int64(A), C.c_int64(T), C.c_int(uw), C.c_int64(P), C.c_int(K),
-                               C.c_double(dt), C.c_double(xi0), C.c_int(sy), _p(X), _p(Y),
-                               _p(cur), _p(fin), _p(stab), _p(bif))
+    args = (_p(pb.rp), _p(pb.col), _p(pb.val32), C.c_int64(n), C.c_int64(A), C.c_int64(T),
+            C.c_int(uw), C.c_int64(P), C.c_int(K), C.c_double(dt), C.c_double(xi0), C.c_int(sy),
+            _p(X), _p(Y), _p(cur), _p(fin), _p(stab), _p(bif))
+    if sr:
+        assert w == 8
+        lib().orc_solve_q8s.restype = C.c_int64
+        steps = lib().orc_solve_q8s(*args)
+    else:
+        lib().orc_solve_qn.restype = C.c_int64
+        steps = lib().orc_solve_qn(C.c_int(w), *args)
     return dict(steps=steps, cur=cur, fin=fin, stab=stab, bif=bif.astype(np.int32))
 
 

@@ -267,10 +267,51 @@ void orc_q12_round_state(int64_t n, float *X, float *Y, int sy) {
   orc_qn_round_state(12, n, X, Y, sy);
 }
 
+/* Stochastic Y rounding of the 8-bit form (SIS_FORM_Q8S; no reference
+ * counterpart).  Element (i, a) after step t takes dither byte
+ * j = (a mod 256) / 64 of
+ *   h = mix(key + l 0x85EBCA77),  key = (i Q + q) 0x9E3779B1 xor salt(t),
+ *   salt(t) = t 0x27D4EB2F + 0x165667B1,
+ *   mix: h ^= h >> 15; h *= 0x2C1B3C6D; h ^= h >> 12,
+ * q = a / 256, l = a mod 64, Q = ceil(A / 256), all in wrapping uint32
+ * arithmetic (i is the node, so the storage order of the rows does not
+ * matter).  d = fmaf(byte, 1/256, -127.5/256) lies in (-1/2, 1/2), and the Y
+ * code is the nearest-even integer to fmaf(y, 2^sy, d): floor(y 2^sy + u)
+ * with u = d + 1/2 uniform on 256 levels, so the rounding is unbiased.  No
+ * clamp: a live element has |y| 2^sy <= YM + 1e-5 under the Y bound rule, and
+ * |d| < 1/2 keeps the code within +-YM.  X rounds to nearest even, a wall
+ * element takes the wall code and Y = 0, as in the 8-bit form. */
+static inline uint32_t orc_q8s_hash(int64_t t, int64_t i, int64_t a, int64_t A) {
+  const uint32_t Q = (uint32_t)((A + 255) / 256), q = (uint32_t)(a / 256), l = (uint32_t)(a % 64);
+  const uint32_t salt = (uint32_t)t * 0x27D4EB2Fu + 0x165667B1u;
+  const uint32_t key = ((uint32_t)i * Q + q) * 0x9E3779B1u ^ salt;
+  uint32_t h = key + l * 0x85EBCA77u;
+  h ^= h >> 15;
+  h *= 0x2C1B3C6Du;
+  h ^= h >> 12;
+  return h;
+}
+
+static inline void orc_q8s_round(float *x, float *y, int sy, int64_t t, int64_t i, int64_t a,
+                                 int64_t A) {
+  if (fabsf(*x) > 1.0f) {
+    orc_qn_round(8, x, y, sy);  // the wall code, Y = 0
+    return;
+  }
+  const uint32_t h = orc_q8s_hash(t, i, a, A);
+  const int j = (int)((a % 256) / 64);
+  const float d = fmaf((float)((h >> (8 * j)) & 0xffu), 1.0f / 256.0f, -127.5f / 256.0f);
+  // integer codes, so that a code of 0 is +0 (sign +1) as in the 8-bit form
+  const int yq = (int)rintf(fmaf(*y, ldexpf(1.0f, sy), d));
+  const int xq = (int)rintf(ldexpf(*x, 6));
+  *x = ldexpf((float)xq, -6);
+  *y = ldexpf((float)yq, -sy);
+}
+
 /* precision codes of the FP32 core below: 2 = 12-bit state, 3 = 8-bit state,
- * 4 = 10-bit state */
+ * 4 = 10-bit state, 5 = 8-bit state with stochastic Y rounding */
 static inline int orc_qn_width(int precision) {
-  return precision == 2 ? 12 : precision == 3 ? 8 : 10;
+  return precision == 2 ? 12 : precision == 4 ? 10 : 8;
 }
 static inline int orc_qn_precision(int w) { return w == 12 ? 2 : w == 8 ? 3 : 4; }
 
@@ -343,7 +384,7 @@ void orc_step_f64(int problem, int gather, const int64_t *rp, const int32_t *col
 static void orc_step_f32_core(int problem, int gather, int precision, int sy, const int64_t *rp,
                               const int32_t *col, const float *val, const float *h, int64_t N,
                               int64_t A, const float *Xs, float *Xn, float *Y, const double *cd,
-                              double coupling_d) {
+                              double coupling_d, int64_t t) {
   if (problem == ORC_UNFUSED) {
     orc_unfused_f32(rp, col, val, N, A, Xs, Xn, Y, cd);
     return;
@@ -397,7 +438,8 @@ static void orc_step_f32_core(int problem, int gather, int precision, int sy, co
             y = fmaf(c1, x, y);
           }
         }
-        if (precision >= 2) orc_qn_round(orc_qn_width(precision), &x, &y, sy);
+        if (precision == 5) orc_q8s_round(&x, &y, sy, t, i, a, A);
+        else if (precision >= 2) orc_qn_round(orc_qn_width(precision), &x, &y, sy);
         Xn[e] = x;
         Y[e] = y;
       }
@@ -410,14 +452,20 @@ void orc_step_f32(int problem, int gather, int bf16_gather, const int64_t *rp, c
                   const float *val, const float *h, int64_t N, int64_t A, const float *Xs, float *Xn,
                   float *Y, const double *cd, double coupling_d) {
   orc_step_f32_core(problem, gather, bf16_gather ? 1 : 0, 0, rp, col, val, h, N, A, Xs, Xn, Y, cd,
-                    coupling_d);
+                    coupling_d, 0);
 }
 
 /* Narrow-state dSB step (code width w): Xs / Y hold decoded codes. */
 void orc_step_qn(int w, const int64_t *rp, const int32_t *col, const float *val, int64_t N,
                  int64_t A, const float *Xs, float *Xn, float *Y, const double *cd, int sy) {
   orc_step_f32_core(ORC_MAXCUT, ORC_DSB, orc_qn_precision(w), sy, rp, col, val, NULL, N, A, Xs,
-                    Xn, Y, cd, 1.0);
+                    Xn, Y, cd, 1.0, 0);
+}
+
+/* One step of the 8-bit form with stochastic Y rounding at step index t. */
+void orc_step_q8s(const int64_t *rp, const int32_t *col, const float *val, int64_t N, int64_t A,
+                  const float *Xs, float *Xn, float *Y, const double *cd, int sy, int64_t t) {
+  orc_step_f32_core(ORC_MAXCUT, ORC_DSB, 5, sy, rp, col, val, NULL, N, A, Xs, Xn, Y, cd, 1.0, t);
 }
 
 void orc_step_q12(const int64_t *rp, const int32_t *col, const float *val, int64_t N, int64_t A,
@@ -511,7 +559,7 @@ static int64_t orc_solve_f32_core(int problem, int gather, int precision, int sy
     for (int64_t s = 0; s < P; s++) {
       orc_sched(problem, steps + s, T, dt, xi0, c);
       orc_step_f32_core(problem, gather, precision, sy, rp, col, val, h, N, A, X, Xb, Y, c,
-                        coupling);
+                        coupling, steps + s);
       memcpy(X, Xb, sizeof(float) * (size_t)(N * A));
     }
     steps += P;
@@ -553,6 +601,17 @@ int64_t orc_solve_qn(int w, const int64_t *rp, const int32_t *col, const float *
   orc_qn_round_state(w, N * A, X, Y, sy);
   return orc_solve_f32_core(ORC_MAXCUT, ORC_DSB, orc_qn_precision(w), sy, rp, col, val, NULL, N,
                             A, T, uw, P, K, 0, dt, xi0, 1.0, X, Y, cur, fin, stab, bif, NULL);
+}
+
+/* The same for the 8-bit form with stochastic Y rounding: the initial state
+ * and the pre-step round to nearest even, every step after rounds Y with the
+ * dither of its step index. */
+int64_t orc_solve_q8s(const int64_t *rp, const int32_t *col, const float *val, int64_t N,
+                      int64_t A, int64_t T, int uw, int64_t P, int K, double dt, double xi0, int sy,
+                      float *X, float *Y, int32_t *cur, int32_t *fin, int32_t *stab, uint8_t *bif) {
+  orc_qn_round_state(8, N * A, X, Y, sy);
+  return orc_solve_f32_core(ORC_MAXCUT, ORC_DSB, 5, sy, rp, col, val, NULL, N, A, T, uw, P, K, 0,
+                            dt, xi0, 1.0, X, Y, cur, fin, stab, bif, NULL);
 }
 
 int64_t orc_solve_q12(const int64_t *rp, const int32_t *col, const float *val, int64_t N,
@@ -613,6 +672,19 @@ void orc_advance_qn(int w, const int64_t *rp, const int32_t *col, const float *v
   for (int64_t s = 0; s < nsteps; s++) {
     orc_sched(ORC_MAXCUT, t0 + s, T, dt, xi0, c);
     orc_step_qn(w, rp, col, val, N, A, X, Xb, Y, c, sy);
+    memcpy(X, Xb, sizeof(float) * (size_t)(N * A));
+  }
+  free(Xb);
+}
+
+void orc_advance_q8s(const int64_t *rp, const int32_t *col, const float *val, int64_t N,
+                     int64_t A, int64_t t0, int64_t nsteps, int64_t T, double dt, double xi0, int sy,
+                     float *X, float *Y) {
+  float *Xb = (float *)malloc(sizeof(float) * (size_t)(N * A));
+  double c[4];
+  for (int64_t s = 0; s < nsteps; s++) {
+    orc_sched(ORC_MAXCUT, t0 + s, T, dt, xi0, c);
+    orc_step_q8s(rp, col, val, N, A, X, Xb, Y, c, sy, t0 + s);
     memcpy(X, Xb, sizeof(float) * (size_t)(N * A));
   }
   free(Xb);

@@ -19,10 +19,12 @@
 //     would not reproduce the value exactly, then with 17;
 //   - the seed comes from SIS_SEED (default: the clock, as srand(time(0))).
 // Extra knobs (environment, so the argv layout is unchanged):
-//   SIS_FORM = f32 | f64 | packed | bf16 | q12 | q10 | q8
+//   SIS_FORM = f32 | f64 | packed | bf16 | q12 | q10 | q8 | q8s
 //                                          precision / state form (f32); q12 /
 //                                          q10 / q8 = 12-, 10- or 8-bit X/Y
-//                                          codes, MaxCut dSB only
+//                                          codes, q8s = 8-bit codes with Y
+//                                          rounded stochastically; MaxCut dSB
+//                                          only
 //   SIS_DYNAMICS = dsb | bsb               dynamics (MaxCut dsb, TSP bsb)
 //   SIS_EUC_ROUND = 0 | 1                  EUC_2D truncation (0, reference) or nint
 //   SIS_GRAPH = 0 | 1 | 2                  hipGraph per window period / whole run
@@ -101,6 +103,7 @@ void apply_env(sis_params &p) {
     else if (s == "q12") p.form = SIS_FORM_Q12;
     else if (s == "q10") p.form = SIS_FORM_Q10;
     else if (s == "q8") p.form = SIS_FORM_Q8;
+    else if (s == "q8s") p.form = SIS_FORM_Q8S;
     else p.form = SIS_FORM_F32;
   }
   if (const char *d = env("SIS_DYNAMICS")) p.dynamics = std::string(d) == "bsb" ? SIS_BSB : SIS_DSB;

@@ -377,6 +377,40 @@ __device__ __forceinline__ uint32_t qn_encode(float x, float y, const Q12K &k, b
   return __builtin_fabsf(x) > 1.0f ? (neg ? k.wneg : k.wpos) : in;
 }
 
+// The same with stochastic Y rounding (SIS_FORM_Q8S): the Y code is the
+// nearest-even integer to fma(y, 2^sy, d) for the element's dither d in
+// (-1/2, 1/2), i.e. floor(y 2^sy + u) with u = d + 1/2.  The sum with the
+// rounding addend 2^23 + B is exact before its one rounding.  No clamp: a
+// live element has |y| 2^sy <= YM + 1e-5 under the Y bound rule, so |d| < 1/2
+// keeps the code within +-YM.
+template <int QW>
+__device__ __forceinline__ uint32_t qn_encode_sr(float x, float y, const Q12K &k, bool &neg,
+                                                 float d) {
+  const uint32_t xb = __float_as_uint(__builtin_fmaf(x, (float)(1 << QFmt<QW>::XS), k.mb));
+  const uint32_t yb = __float_as_uint(__builtin_fmaf(y, k.yscale, d) + k.mb);
+  uint32_t in = (yb << QW) | xb;
+  neg = (int32_t)xb < (int32_t)(0x4b000000u | (1u << (QW - 1)));
+  asm volatile("" : "+v"(in));
+  return __builtin_fabsf(x) > 1.0f ? (neg ? k.wneg : k.wpos) : in;
+}
+
+// Dither hash of the stochastic Y rounding: row key (node i, quad q of Q,
+// step salt) and lane l; element j of the lane takes byte j.
+__device__ __forceinline__ uint32_t q8s_salt(int64_t t) {
+  return (uint32_t)t * 0x27D4EB2Fu + 0x165667B1u;
+}
+__device__ __forceinline__ uint32_t q8s_row_key(int64_t node, int64_t Q, int64_t q,
+                                                uint32_t salt) {
+  return ((uint32_t)node * (uint32_t)Q + (uint32_t)q) * 0x9E3779B1u ^ salt;
+}
+__device__ __forceinline__ uint32_t q8s_lane_hash(uint32_t key, int lane) {
+  uint32_t h = key + (uint32_t)lane * 0x85EBCA77u;
+  h ^= h >> 15;
+  h *= 0x2C1B3C6Du;
+  h ^= h >> 12;
+  return h;
+}
+
 template <int QW>
 __device__ __forceinline__ uint32_t qn_encode(float x, float y, const Q12K &k) {
   bool neg;

@@ -362,21 +362,30 @@ __device__ __forceinline__ void qn_st_row(uint32_t *rb, int64_t ro, int64_t ps, 
 // New codes and sign words of one row: decode, fused dSB update, encode,
 // one ballot per sign word.  rb: the row's plane-0 dwords of this quad
 // (wave-uniform), ro: their offset from the planes' base, ps: plane stride,
-// sw: the row's four sign words.
-template <int QW, int NT>
+// sw: the row's four sign words; SR: Y rounded stochastically with the
+// dither of the row key (wave-uniform) and the lane.
+template <int QW, int NT, int SR>
 __device__ __forceinline__ void qn_finish_row(const uint32_t d[QFmt<QW>::NP], const float F[4],
                                               const Q12K &k, uint32_t *rb, int64_t ro, int64_t ps,
-                                              uint64_t *sw, int lane) {
+                                              uint64_t *sw, int lane, uint32_t key) {
   constexpr int NP = QFmt<QW>::NP;
   uint32_t un[4];
   uint64_t b[4];
+  uint32_t h = 0;
+  if constexpr (SR) h = q8s_lane_hash(key, lane);
 #pragma unroll
   for (int j = 0; j < 4; j++) {
     float x, y;
     qn_decode_at<QW>(d, j, k, x, y);
     sb_update_maxcut_sel(x, y, F[j], k.c0, k.c1, k.c2);
     bool neg;
-    un[j] = qn_encode<QW, false>(x, y, k, neg);  // no Y clamp after a fused step
+    if constexpr (SR) {
+      const float dj = __builtin_fmaf((float)((h >> (8 * j)) & 0xffu), 1.0f / 256.0f,
+                                      -127.5f / 256.0f);
+      un[j] = qn_encode_sr<QW>(x, y, k, neg, dj);
+    } else {
+      un[j] = qn_encode<QW, false>(x, y, k, neg);  // no Y clamp after a fused step
+    }
     b[j] = __ballot(neg);
   }
   uint32_t e[NP];
@@ -393,22 +402,31 @@ __device__ __forceinline__ void qn_finish_row(const uint32_t d[QFmt<QW>::NP], co
 
 // Rows whose wavefront holds no entries at all: F = 0, the sum of no terms
 // (the bit-counting path gives fma(0, 2, -0) = +0 too).
-template <int QW, int NT, int R>
+// Stochastic-rounding key of a stored row (0 unless SR).
+template <int SR>
+__device__ __forceinline__ uint32_t qn_row_key(const StepArgs &a, int64_t row, int64_t Q,
+                                               int64_t q, uint32_t salt) {
+  if constexpr (SR) return q8s_row_key(a.rowid ? (int64_t)a.rowid[row] : row, Q, q, salt);
+  return 0u;
+}
+
+template <int QW, int NT, int SR, int R>
 __device__ __forceinline__ void qn_finish_empty(const uint32_t (&d)[R][QFmt<QW>::NP],
-                                                const Q12K &k, uint32_t *P, int64_t Q, int64_t q,
-                                                int64_t ps, uint64_t *Sn, int64_t W, int64_t w0,
-                                                int64_t row0, int rl, int lane) {
+                                                const Q12K &k, const StepArgs &a, uint32_t *P,
+                                                int64_t Q, int64_t q, int64_t ps, uint64_t *Sn,
+                                                int64_t W, int64_t w0, int64_t row0, int rl,
+                                                int lane, uint32_t salt) {
   const float F[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
   for (int r = 0; r < R; r++) {
     if (r >= rl) continue;
     const int64_t row = row0 + r;
-    qn_finish_row<QW, NT>(d[r], F, k, P + (row * Q + q) * 64, (row * Q + q) * 64, ps,
-                          Sn + row * W + w0, lane);
+    qn_finish_row<QW, NT, SR>(d[r], F, k, P + (row * Q + q) * 64, (row * Q + q) * 64, ps,
+                              Sn + row * W + w0, lane, qn_row_key<SR>(a, row, Q, q, salt));
   }
 }
 
-template <int QW, int R, int NT, int VG>
+template <int QW, int R, int NT, int VG, int SR>
 __device__ __forceinline__ void qn_step_body(const StepArgs &a) {
   constexpr int NP = QFmt<QW>::NP;
   const Ctrl *ctrl = a.ctrl;
@@ -423,6 +441,7 @@ __device__ __forceinline__ void qn_step_body(const StepArgs &a) {
   const int64_t t = ctrl->base + a.offset;
   const float *sc = (const float *)a.sched + 4 * t;
   const Q12K kc = qn_consts<QW>(sc[0], sc[1], sc[2], a.qshift);
+  const uint32_t salt = SR ? q8s_salt(t) : 0u;
   const uint64_t *__restrict__ S = (const uint64_t *)a.gsrc;
   const float *__restrict__ val = (const float *)a.val;
   const int32_t *__restrict__ col = a.col;
@@ -465,7 +484,7 @@ __device__ __forceinline__ void qn_step_body(const StepArgs &a) {
       // are never read back); a wavefront without entries loads nothing.
       constexpr int kSlots = kQ12Stage / 64;
       if (ne == 0) {
-        qn_finish_empty<QW, NT>(d, kc, P, Q, q, ps, Sn, W, w0, row0, rl, lane);
+        qn_finish_empty<QW, NT, SR>(d, kc, a, P, Q, q, ps, Sn, W, w0, row0, rl, lane, salt);
         return;
       }
       int64_t ce[kSlots];
@@ -552,8 +571,9 @@ __device__ __forceinline__ void qn_step_body(const StepArgs &a) {
 #pragma unroll
           for (int j = 0; j < 4; j++) F[j] = __builtin_fmaf((float)((acc >> (8 * j)) & 0xffu), 2.0f, md);
           const int64_t row = row0 + r;
-          qn_finish_row<QW, NT>(d[r], F, kc, P + (row * Q + q) * 64, (row * Q + q) * 64, ps,
-                                Sn + row * W + w0, lane);
+          qn_finish_row<QW, NT, SR>(d[r], F, kc, P + (row * Q + q) * 64, (row * Q + q) * 64, ps,
+                                Sn + row * W + w0, lane,
+                                qn_row_key<SR>(a, row, Q, q, salt));
         }
         return;
       }
@@ -580,8 +600,9 @@ __device__ __forceinline__ void qn_step_body(const StepArgs &a) {
         }
         if (k < k1) add(F, lsig[wv][k][half], lval[wv][k]);
         const int64_t row = row0 + r;
-        qn_finish_row<QW, NT>(d[r], F, kc, P + (row * Q + q) * 64, (row * Q + q) * 64, ps,
-                              Sn + row * W + w0, lane);
+        qn_finish_row<QW, NT, SR>(d[r], F, kc, P + (row * Q + q) * 64, (row * Q + q) * 64, ps,
+                              Sn + row * W + w0, lane,
+                              qn_row_key<SR>(a, row, Q, q, salt));
       }
       return;
     }
@@ -621,14 +642,15 @@ __device__ __forceinline__ void qn_step_body(const StepArgs &a) {
         F[u] = ((sw >> lane) & 1) ? F[u] + v : F[u] - v;
       }
     }
-    qn_finish_row<QW, NT>(d[r], F, kc, P + (row * Q + q) * 64, (row * Q + q) * 64, ps,
-                          Sn + row * W + w0, lane);
+    qn_finish_row<QW, NT, SR>(d[r], F, kc, P + (row * Q + q) * 64, (row * Q + q) * 64, ps,
+                          Sn + row * W + w0, lane,
+                          qn_row_key<SR>(a, row, Q, q, salt));
   }
 }
 
-template <int QW, int R, int NT, int VG>
+template <int QW, int R, int NT, int VG, int SR>
 __global__ __launch_bounds__(kBlock) void k_step_qn(StepArgs a) {
-  qn_step_body<QW, R, NT, VG>(a);
+  qn_step_body<QW, R, NT, VG, SR>(a);
 }
 
 // Initial state of the narrow forms: the counter-based stream of
@@ -2294,7 +2316,7 @@ void launch_xy_update(hipStream_t st, bool f64, void *X, void *Y, int64_t n, con
 }
 
 bool step_supported(int form, int prob, int gather, int vec) {
-  if (form == 4 || form == 5 || form == 6) {
+  if (form == 4 || form == 5 || form == 6 || form == 7) {
     // 4 | rows << 8 | non-temporal << 16 | staged gathers << 17
     const int rows = (vec >> 8) & 0xff, vg = (vec >> 17) & 1;
     if (prob != PROB_MAXCUT || gather != G_DSB || (vec & 0xff) != 4 || (vec >> 18) != 0)
@@ -2378,18 +2400,19 @@ void launch_step(hipStream_t st, int form, int prob, int gather, int vec, const 
 #undef SIS_PACKED_V
 #undef SIS_PACKED_R
 #undef SIS_PACKED
-  } else if (form == 4 || form == 5 || form == 6) {
+  } else if (form == 4 || form == 5 || form == 6 || form == 7) {
     const int rows = (vec >> 8) & 0xff, vg = (vec >> 17) & 1;
     const int64_t waves = ((a.N + rows - 1) / rows) * (a.W / 4);
-#define SIS_QN(QW, R, NT, VG)                                   \
-  if (qw == QW && rows == R && nt == NT && vg == VG) {          \
-    launch_waves(st, k_step_qn<QW, R, NT, VG>, waves, a);       \
-    return;                                                     \
+#define SIS_QN(QW, R, NT, VG, SR)                                   \
+  if (qw == QW && sr == SR && rows == R && nt == NT && vg == VG) {  \
+    launch_waves(st, k_step_qn<QW, R, NT, VG, SR>, waves, a);       \
+    return;                                                         \
   }
 #define SIS_QN_NT(R, VG)                                                                    \
-  SIS_QN(12, R, 0, VG) SIS_QN(12, R, 1, VG) SIS_QN(10, R, 0, VG) SIS_QN(10, R, 1, VG) \
-  SIS_QN(8, R, 0, VG) SIS_QN(8, R, 1, VG)
-    const int qw = form == 4 ? 12 : form == 6 ? 10 : 8;
+  SIS_QN(12, R, 0, VG, 0) SIS_QN(12, R, 1, VG, 0) SIS_QN(10, R, 0, VG, 0)                   \
+  SIS_QN(10, R, 1, VG, 0) SIS_QN(8, R, 0, VG, 0) SIS_QN(8, R, 1, VG, 0)                     \
+  SIS_QN(8, R, 0, VG, 1) SIS_QN(8, R, 1, VG, 1)
+    const int qw = form == 4 ? 12 : form == 6 ? 10 : 8, sr = form == 7;
     SIS_QN_NT(1, 0)
     SIS_QN_NT(2, 0)
     SIS_QN_NT(4, 0)

@@ -68,6 +68,7 @@ struct StepArgs {
   double coupling;
   int64_t split;      // rows with more stored entries are left to k_step_split (0: none)
   int32_t qshift;     // narrow-state form: Y codes are round(y 2^qshift)
+  const int32_t *rowid;  // node of each stored row (stochastic rounding), or nullptr
 };
 
 // Row split: a row with more than `chunk` entries is cut into groups of
@@ -171,7 +172,8 @@ void launch_pack_signs_qn(hipStream_t st, int qw, const uint32_t *P, uint64_t *S
                           int64_t lda);
 
 // form: 0 f32, 1 f64, 2 packed, 3 bf16, 4 narrow (12-bit codes), 5 narrow
-// (8-bit codes), 6 narrow (10-bit codes).  vec = agents per lane (f32/f64/bf16); for the packed form
+// (8-bit codes), 6 narrow (10-bit codes), 7 narrow (8-bit codes, Y rounded
+// stochastically).  vec = agents per lane (f32/f64/bf16); for the packed form
 // vec = words per wavefront | rows per wavefront << 8 | non-temporal state
 // access << 16; for the narrow forms 4 | rows per wavefront << 8 |
 // non-temporal << 16 | staged gathers << 17.

@@ -19,7 +19,10 @@ using namespace sis;
 
 // Code width of a narrow-state form (12, 10 or 8 bits), 0 for the others.
 static int qn_width_of(int form) {
-  return form == SIS_FORM_Q12 ? 12 : form == SIS_FORM_Q10 ? 10 : form == SIS_FORM_Q8 ? 8 : 0;
+  return form == SIS_FORM_Q12 ? 12
+         : form == SIS_FORM_Q10 ? 10
+         : (form == SIS_FORM_Q8 || form == SIS_FORM_Q8S) ? 8
+                                                        : 0;
 }
 
 // Dwords of the narrow state per 256-agent quad and lane: 3 (12-bit), 2.5
@@ -637,7 +640,7 @@ int sis_solver_create(const sis_csr *J, const double *field, const sis_params *p
     return fail(SIS_ERR_ARG, "sis_solver_create: the packed-sign form is dSB only");
   if (p.form == SIS_FORM_BF16 && p.dynamics != SIS_BSB)
     return fail(SIS_ERR_ARG, "sis_solver_create: the BF16 form is bSB only");
-  if (p.form < SIS_FORM_F32 || p.form > SIS_FORM_Q10)
+  if (p.form < SIS_FORM_F32 || p.form > SIS_FORM_Q8S)
     return fail(SIS_ERR_ARG, "sis_solver_create: unknown form");
   if (qn_width_of(p.form)) {
     if (p.problem != SIS_MAXCUT || p.dynamics != SIS_DSB || !p.fused)
@@ -1036,11 +1039,13 @@ static void enqueue_step(sis_solver *s, int32_t off) {
     case SIS_FORM_Q12:
     case SIS_FORM_Q10:
     case SIS_FORM_Q8:
+    case SIS_FORM_Q8S:
       a.gsrc = s->sg[cur].p;
       a.gdst = s->sg[nxt].p;
       a.X = s->Xq.p;
       a.Y = nullptr;
       a.qshift = s->qshift;
+      a.rowid = s->node_of_d.p;  // the dither is keyed by node (nullptr: node order)
       break;
   }
   if (s->prob == PROB_UNFUSED)
@@ -1136,7 +1141,8 @@ static void enqueue_window(sis_solver *s, int64_t P, int uw) {
     case SIS_FORM_PACKED:
     case SIS_FORM_Q12:
     case SIS_FORM_Q10:
-    case SIS_FORM_Q8: src = SRC_BITS; x = s->sg[s->par].p; break;
+    case SIS_FORM_Q8:
+    case SIS_FORM_Q8S: src = SRC_BITS; x = s->sg[s->par].p; break;
     case SIS_FORM_F64: src = SRC_F64; x = s->Xcur(); break;
     default: src = SRC_F32; x = s->Xcur(); break;
   }

@@ -14,12 +14,12 @@ LIB_PATH = os.environ.get("SIS_LIB", os.path.join(_HERE, "..", "build", "libsis.
 
 MAXCUT, TSP = 0, 1
 DSB, BSB = 0, 1
-FORM_F32, FORM_F64, FORM_PACKED, FORM_BF16, FORM_Q12, FORM_Q8, FORM_Q10 = 0, 1, 2, 3, 4, 5, 6
+FORM_F32, FORM_F64, FORM_PACKED, FORM_BF16, FORM_Q12, FORM_Q8, FORM_Q10, FORM_Q8S = range(8)
 W_ONE, W_PM1, W_INT3 = 0, 1, 2
 EUC_2D, GEO = 0, 1
 SPINS_CURRENT, SPINS_FINAL = 0, 1
 FORMS = {"f32": FORM_F32, "f64": FORM_F64, "packed": FORM_PACKED, "bf16": FORM_BF16,
-         "q12": FORM_Q12, "q10": FORM_Q10, "q8": FORM_Q8}
+         "q12": FORM_Q12, "q10": FORM_Q10, "q8": FORM_Q8, "q8s": FORM_Q8S}
 
 
 class SisError(RuntimeError):

@@ -73,9 +73,10 @@ def test_maxcut_line_matches_oracle(tmp_path, fused, knobs):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("form,w", [("q12", 12), ("q10", 10), ("q8", 8)])
+@pytest.mark.parametrize("form,w", [("q12", 12), ("q10", 10), ("q8", 8), ("q8s", 8)])
 def test_maxcut_line_narrow_form_matches_oracle(tmp_path, form, w):
-    """SIS_FORM=q12 / q10 / q8 (12-, 10- or 8-bit X/Y codes): the line's cut statistics
+    """SIS_FORM=q12 / q10 / q8 / q8s (12-, 10- or 8-bit X/Y codes; q8s rounds
+    Y stochastically): the line's cut statistics
     equal the oracle's narrow-state solve from the same seed."""
     J = sis.Csr.gen_er(300, 0.05, sis.W_PM1, 3)
     mtx = tmp_path / "g.mtx"
@@ -89,7 +90,7 @@ def test_maxcut_line_narrow_form_matches_oracle(tmp_path, form, w):
     rp, col, val = J.arrays()
     pb = orc.Problem(rp, col, val, S=J.sum)
     X, Y = orc.init_state(seed, orc.MAXCUT, J.n, A)
-    res = orc.solve_qn(pb, X, Y, T, 1, P=100, K=25, dt=dt, xi0=J.xi0(), w=w)
+    res = orc.solve_qn(pb, X, Y, T, 1, P=100, K=25, dt=dt, xi0=J.xi0(), w=w, sr=form == "q8s")
     assert [float(v) for v in f[1:6]] == list(orc.cut_stats(orc.cut(pb, res["cur"])))
 
 

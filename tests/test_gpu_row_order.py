@@ -1,11 +1,12 @@
 """GPU parity of the row storage order (sis_params.row_order).
 
 The locality order stores the rows of the sign-gather forms (packed, 12-, 10-
-and 8-bit codes) grouped by graph neighbourhood; states, spins and results
+and 8-bit codes, the last also with stochastic Y rounding, whose dither is
+keyed by node) grouped by graph neighbourhood; states, spins and results
 still come and go in node order, each row keeps its entries in CSR order and
-the seeded initial state is keyed by node.  Bar: bit-exact -- a solver in the locality
-order equals one in node order and the oracle in every state, spin, lock, step
-count and cut.
+the seeded initial state is keyed by node.  Bar: bit-exact -- a solver in
+the locality order equals one in node order and the oracle in every state,
+spin, lock, step count and cut.
 """
 import numpy as np
 import pytest
@@ -15,12 +16,13 @@ import sis
 
 pytestmark = pytest.mark.gpu
 
-WIDTH = {"q12": 12, "q10": 10, "q8": 8}
+WIDTH = {"q12": 12, "q10": 10, "q8": 8, "q8s": 8}
 
 
 def oracle_solve(pb, form, X, Y, T, uw, dt, xi0):
     if form in WIDTH:
-        return orc.solve_qn(pb, X, Y, T, uw, P=100, K=25, dt=dt, xi0=xi0, w=WIDTH[form])
+        return orc.solve_qn(pb, X, Y, T, uw, P=100, K=25, dt=dt, xi0=xi0, w=WIDTH[form],
+                            sr=form == "q8s")
     return orc.solve(pb, orc.MAXCUT, orc.DSB, X, Y, T, uw, P=100, K=25, dt=dt, xi0=xi0)
 
 
@@ -45,7 +47,7 @@ def same(s1, s0):
     assert np.array_equal(c1, c0) and b1 == b0
 
 
-@pytest.mark.parametrize("form", ["packed", "q12", "q10", "q8"])
+@pytest.mark.parametrize("form", ["packed", "q12", "q10", "q8", "q8s"])
 @pytest.mark.parametrize("graph", [0, 2])
 def test_locality_order_full_run_equals_node_order_and_oracle(form, graph):
     # integer weights in [-3, 3] on a 4-regular graph, 300 agents (ragged),
@@ -69,7 +71,7 @@ def test_locality_order_full_run_equals_node_order_and_oracle(form, graph):
     assert np.array_equal(s1.cuts()[0], orc.cut(pb, r["cur"]))
 
 
-@pytest.mark.parametrize("form", ["packed", "q10", "q8"])
+@pytest.mark.parametrize("form", ["packed", "q10", "q8", "q8s"])
 def test_locality_order_set_state_and_advance(form):
     # explicit states in node order go in and come out in node order; an
     # isolated node and an ER graph with uneven degrees
@@ -105,7 +107,7 @@ def test_locality_order_self_loops_duplicates_and_row_split():
     r_of = np.repeat(np.arange(n), np.diff(rp))
     assert (r_of == col).any()  # self loops present
     pb = orc.Problem(rp, col, val, S=J.sum)
-    for form in ("packed", "q12"):
+    for form in ("packed", "q12", "q8s"):
         s1, s0 = solvers(J, form, xi0=0.2)
         for s in (s1, s0):
             s.init_random(3)

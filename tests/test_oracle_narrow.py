@@ -197,3 +197,94 @@ def test_narrow_dsb_finds_bipartite_max_cut(w):
     r = orc.solve_qn(pb, X, Y, 2000, 0, P=100, dt=0.5, xi0=xi0, w=w)
     cuts = orc.cut(pb, r["cur"])
     assert cuts.max() == n
+
+
+# ---------------------------------------------------------------- 8-bit, stochastic Y rounding
+
+def test_q8s_step_rounds_y_up_with_the_probability_of_its_fraction():
+    # One step from the same on-grid state as the FP32 step: X rounds to
+    # nearest even exactly as in the 8-bit form; each Y code is the floor or
+    # the ceiling of the FP32 step's y 2^sy, and the share rounded up tracks
+    # the fraction (a dither on [0, 1) instead of [-1/2, 1/2) would round up
+    # about half a code too often, nearest-even rounding never for f < 1/2).
+    pb, M, X, Y, sy = _trajectory(8, n=300, A=256, seed=11)
+    xi0 = orc.xi0_maxcut(pb.n, pb.val64)
+    t = 57
+    c = orc.sched(orc.MAXCUT, t, 300, 0.5, xi0)
+    Y32 = Y.copy()
+    X32 = orc.step(pb, orc.MAXCUT, orc.DSB, X, Y32, c)
+    Yq = Y.copy()
+    Xq = orc.step_qn(pb, X, Yq, c, sy, 8)
+    Ys = Y.copy()
+    Xs = orc.step_qn(pb, X, Ys, c, sy, 8, sr=True, t=t)
+    assert np.array_equal(Xs, Xq)
+    live = np.abs(X32) <= 1
+    v = Y32[live].astype(np.float64) * 2.0 ** sy
+    code = Ys[live].astype(np.float64) * 2.0 ** sy
+    assert np.array_equal(code, np.round(code))
+    fl = np.floor(v)
+    assert ((code == fl) | (code == fl + 1)).all()
+    f = v - fl
+    band = (f > 0.2) & (f < 0.3)
+    assert band.sum() > 2000
+    up = (code[band] == fl[band] + 1).mean()
+    assert 0.2 < up < 0.3, up
+    # unbiased on average, where nearest-even rounding is too (the dither
+    # only changes which way each element goes)
+    assert abs((code - v).mean()) < 0.01
+    assert (Ys[~live] == 0).all()
+
+
+def test_q8s_dither_is_a_function_of_step_node_and_agent():
+    pb, M, X, Y, sy = _trajectory(8, n=120, A=300, seed=12)
+    c = orc.sched(orc.MAXCUT, 10, 300, 0.5, 0.3)
+    outs = []
+    for t in (10, 10, 11):
+        Yt = Y.copy()
+        orc.step_qn(pb, X, Yt, c, sy, 8, sr=True, t=t)
+        outs.append(Yt)
+    assert np.array_equal(outs[0], outs[1])  # deterministic
+    assert not np.array_equal(outs[0], outs[2])  # another step index, another dither
+
+
+def test_q8s_kick_is_exact_and_invariants_hold():
+    # c = (0, 0, 1, 0), y = 0: y 2^sy is an integer, and a dither below half a
+    # code rounds it back to itself, so Y = F exactly
+    pb, M = random_problem(41, 0.15, 3, symmetric=False, weights="int")
+    rng = np.random.default_rng(5)
+    X = (rng.integers(-64, 65, (41, 6)) / 64).astype(np.float32)
+    Y = np.zeros_like(X)
+    orc.step_qn(pb, X, Y, np.array([0.0, 0.0, 1.0, 0.0]), 2, 8, sr=True, t=3)
+    F = -(sp.csr_matrix(M) @ np.where(np.signbit(X), -1.0, 1.0))
+    np.testing.assert_array_equal(Y, F)
+    # codes on the grid, walls with Y = 0, live |y| within the bound, over a run
+    pb, M, X, Y, sy = _trajectory(8, seed=13)
+    xi0 = orc.xi0_maxcut(pb.n, pb.val64)
+    ymax, walls = 0.0, 0
+    for t0 in range(0, 400, 50):
+        orc.advance_qn(pb, X, Y, t0, 50, 400, 0.5, xi0, sy, 8, sr=True)
+        xq = X.astype(np.float64) * 64
+        yq = Y.astype(np.float64) * 2.0 ** sy
+        assert np.array_equal(xq, np.round(xq)) and np.abs(xq).max() <= 65
+        assert np.array_equal(yq, np.round(yq)) and np.abs(yq).max() <= 127
+        wall = np.abs(xq) == 65
+        assert (Y[wall] == 0).all()
+        # a zero code is +0: its sign gathers as +1 (a -0 would flip it)
+        assert not np.signbit(X[xq == 0]).any() and not np.signbit(Y[yq == 0]).any()
+        walls += int(wall.sum())
+        ymax = max(ymax, float(np.abs(Y[np.abs(X) <= 1]).max()))
+    assert walls > 0 and ymax <= 2 / 0.5 + 0.5 + 2.0 ** -sy
+
+
+def test_q8s_finds_bipartite_max_cut():
+    n = 40
+    rows = np.arange(n)
+    M = np.zeros((n, n))
+    M[rows, (rows + 1) % n] = 1
+    M = M + M.T
+    csr = sp.csr_matrix(M)
+    pb = orc.Problem(csr.indptr, csr.indices, csr.data, S=n)
+    X, Y = orc.init_state(3, orc.MAXCUT, n, 64)
+    r = orc.solve_qn(pb, X, Y, 2000, 0, P=100, dt=0.5, xi0=orc.xi0_maxcut(n, pb.val64), w=8,
+                     sr=True)
+    assert orc.cut(pb, r["cur"]).max() == n
