@@ -402,27 +402,28 @@ __device__ __forceinline__ void qn_finish_row(const uint32_t d[QFmt<QW>::NP], co
 
 // Rows whose wavefront holds no entries at all: F = 0, the sum of no terms
 // (the bit-counting path gives fma(0, 2, -0) = +0 too).
-// Stochastic-rounding key of a stored row (0 unless SR).
+// Stochastic-rounding key of row r of the wavefront (0 unless SR): its node
+// comes from lane r of nodel, loaded once for the wavefront's rows.
 template <int SR>
-__device__ __forceinline__ uint32_t qn_row_key(const StepArgs &a, int64_t row, int64_t Q,
-                                               int64_t q, uint32_t salt) {
-  if constexpr (SR) return q8s_row_key(a.rowid ? (int64_t)a.rowid[row] : row, Q, q, salt);
+__device__ __forceinline__ uint32_t qn_row_key(int nodel, int r, int64_t Q, int64_t q,
+                                               uint32_t salt) {
+  if constexpr (SR) return q8s_row_key(__builtin_amdgcn_readlane(nodel, r), Q, q, salt);
   return 0u;
 }
 
 template <int QW, int NT, int SR, int R>
 __device__ __forceinline__ void qn_finish_empty(const uint32_t (&d)[R][QFmt<QW>::NP],
-                                                const Q12K &k, const StepArgs &a, uint32_t *P,
-                                                int64_t Q, int64_t q, int64_t ps, uint64_t *Sn,
-                                                int64_t W, int64_t w0, int64_t row0, int rl,
-                                                int lane, uint32_t salt) {
+                                                const Q12K &k, uint32_t *P, int64_t Q, int64_t q,
+                                                int64_t ps, uint64_t *Sn, int64_t W, int64_t w0,
+                                                int64_t row0, int rl, int lane, int nodel,
+                                                uint32_t salt) {
   const float F[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
   for (int r = 0; r < R; r++) {
     if (r >= rl) continue;
     const int64_t row = row0 + r;
     qn_finish_row<QW, NT, SR>(d[r], F, k, P + (row * Q + q) * 64, (row * Q + q) * 64, ps,
-                              Sn + row * W + w0, lane, qn_row_key<SR>(a, row, Q, q, salt));
+                              Sn + row * W + w0, lane, qn_row_key<SR>(nodel, r, Q, q, salt));
   }
 }
 
@@ -461,6 +462,13 @@ __device__ __forceinline__ void qn_step_body(const StepArgs &a) {
   const int rl = (int)(a.N - row0 < R ? a.N - row0 : R);
   int64_t rpl = 0;
   if constexpr (VG) rpl = rp[row0 + (lane < rl ? lane : rl)];
+  // SR: the node of row r in lane r (the dither is keyed by node), loaded
+  // with the row pointers rather than once per row
+  int nodel = 0;
+  if constexpr (SR) {
+    const int64_t rr = row0 + (lane < rl ? lane : rl - 1);
+    nodel = a.rowid ? a.rowid[rr] : (int)rr;
+  }
   uint32_t d[R][NP];
 #pragma unroll
   for (int r = 0; r < R; r++) {
@@ -484,7 +492,7 @@ __device__ __forceinline__ void qn_step_body(const StepArgs &a) {
       // are never read back); a wavefront without entries loads nothing.
       constexpr int kSlots = kQ12Stage / 64;
       if (ne == 0) {
-        qn_finish_empty<QW, NT, SR>(d, kc, a, P, Q, q, ps, Sn, W, w0, row0, rl, lane, salt);
+        qn_finish_empty<QW, NT, SR>(d, kc, P, Q, q, ps, Sn, W, w0, row0, rl, lane, nodel, salt);
         return;
       }
       int64_t ce[kSlots];
@@ -573,7 +581,7 @@ __device__ __forceinline__ void qn_step_body(const StepArgs &a) {
           const int64_t row = row0 + r;
           qn_finish_row<QW, NT, SR>(d[r], F, kc, P + (row * Q + q) * 64, (row * Q + q) * 64, ps,
                                 Sn + row * W + w0, lane,
-                                qn_row_key<SR>(a, row, Q, q, salt));
+                                qn_row_key<SR>(nodel, r, Q, q, salt));
         }
         return;
       }
@@ -602,7 +610,7 @@ __device__ __forceinline__ void qn_step_body(const StepArgs &a) {
         const int64_t row = row0 + r;
         qn_finish_row<QW, NT, SR>(d[r], F, kc, P + (row * Q + q) * 64, (row * Q + q) * 64, ps,
                               Sn + row * W + w0, lane,
-                              qn_row_key<SR>(a, row, Q, q, salt));
+                              qn_row_key<SR>(nodel, r, Q, q, salt));
       }
       return;
     }
@@ -644,7 +652,7 @@ __device__ __forceinline__ void qn_step_body(const StepArgs &a) {
     }
     qn_finish_row<QW, NT, SR>(d[r], F, kc, P + (row * Q + q) * 64, (row * Q + q) * 64, ps,
                           Sn + row * W + w0, lane,
-                          qn_row_key<SR>(a, row, Q, q, salt));
+                          qn_row_key<SR>(nodel, r, Q, q, salt));
   }
 }
 

@@ -25,8 +25,8 @@ def test_more_than_one_gpu_exits_with_message():
 @pytest.mark.gpu
 def test_headline_line_has_the_contract_fields():
     # a small instance of the headline workload: random 4-regular graph,
-    # 10-bit (headline), 12-bit, 8-bit and packed dSB against plain FP32 dSB,
-    # CPU baseline on a 1 s sample
+    # 8-bit with stochastic Y rounding (headline), 10-, 12- and 8-bit and
+    # packed dSB against plain FP32 dSB, CPU baseline on a 1 s sample
     # --steps 150: a run executes whole window periods, so 200 steps run and
     # the line reports 200 (the timed region's real size)
     r = run_bench("--nodes-log2", "14", "--agents", "128", "--steps", "150", "--warmup", "10",
@@ -50,11 +50,12 @@ def test_headline_line_has_the_contract_fields():
     cb = d["cpu_baseline"]
     assert cb["value"] > 0 and cb["cores"] >= 1 and cb["kind"] in ("port", "reference")
     assert cb["sample"]
-    # headline: the 10-bit form; packed and plain dSB follow the same
-    # trajectory (identical best cut), the narrow forms their own
-    assert d["config"]["form"] == "q10" and d["packed"]["best_cut_rel_diff_vs_plain"] == 0.0
+    # headline: the 8-bit form with stochastic Y rounding; packed and plain
+    # dSB follow the same trajectory (identical best cut), the narrow forms
+    # their own
+    assert d["config"]["form"] == "q8s" and d["packed"]["best_cut_rel_diff_vs_plain"] == 0.0
     assert abs(d["best_cut_rel_diff_vs_plain"]) < 0.01
-    for f in ("q12", "q8"):
+    for f in ("q10", "q12", "q8"):
         assert d[f]["form"] == f and abs(d[f]["best_cut_rel_diff_vs_plain"]) < 0.01
     assert d["energy_check"]["max_abs_err_cut"] == 0.0
     bf = d["bf16_bsb"]
