@@ -93,15 +93,16 @@ enum sis_form {
                           exact. */
   SIS_FORM_Q8S = 7     /* as SIS_FORM_Q8 (8-bit codes, 2 bytes per node-agent)
                           with Y rounded stochastically after every step (no
-                          reference counterpart): the Y code is the nearest-
-                          even integer to fma(y, 2^s, d), d = byte/256 -
-                          127.5/256 from a hash of (step index, node, agent),
-                          i.e. floor(y 2^s + u) with u uniform on 256 levels,
-                          so the rounding is unbiased.  X, the initial state
+                          reference counterpart): the Y code is
+                          floor((w + b) / 256), w the nearest-even integer to
+                          y 2^(s+8) and b a byte from a hash of (step index,
+                          node, agent): y 2^s rounded down or up with the
+                          probability of its fraction, so the rounding is
+                          unbiased.  X, the initial state
                           and the pre-step round to nearest even as in
                           SIS_FORM_Q8.  Deterministic: the same seed and
                           steps give the same trajectory in any row order or
-                          tile.  It runs within about 7% of SIS_FORM_Q8 and
+                          tile.  It runs within about 10% of SIS_FORM_Q8 and
                           holds plain FP32's cut quality on the graph
                           families measured, where nearest-even 8-bit
                           rounding falls short on some. */

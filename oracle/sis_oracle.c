@@ -275,12 +275,13 @@ void orc_q12_round_state(int64_t n, float *X, float *Y, int sy) {
  *   mix: h ^= h >> 15; h *= 0x2C1B3C6D; h ^= h >> 12,
  * q = a / 256, l = a mod 64, Q = ceil(A / 256), all in wrapping uint32
  * arithmetic (i is the node, so the storage order of the rows does not
- * matter).  d = fmaf(byte, 1/256, -127.5/256) lies in (-1/2, 1/2), and the Y
- * code is the nearest-even integer to fmaf(y, 2^sy, d): floor(y 2^sy + u)
- * with u = d + 1/2 uniform on 256 levels, so the rounding is unbiased.  No
- * clamp: a live element has |y| 2^sy <= YM + 1e-5 under the Y bound rule, and
- * |d| < 1/2 keeps the code within +-YM.  X rounds to nearest even, a wall
- * element takes the wall code and Y = 0, as in the 8-bit form. */
+ * matter).  With w the nearest-even integer to y 2^(sy+8) and b that byte, the
+ * Y code is floor((w + b) / 256): y 2^sy to 1/256, rounded down or up with
+ * the probability of its fraction (b uniform on 0..255), so the rounding is
+ * unbiased, and a y 2^sy on the code grid stays put.  No clamp: a live
+ * element has |y| 2^sy <= YM + 1e-5 under the Y bound rule, so |w| <= 256 YM
+ * and the code stays within +-YM.  X rounds to nearest even, a wall element
+ * takes the wall code and Y = 0, as in the 8-bit form. */
 static inline uint32_t orc_q8s_hash(int64_t t, int64_t i, int64_t a, int64_t A) {
   const uint32_t Q = (uint32_t)((A + 255) / 256), q = (uint32_t)(a / 256), l = (uint32_t)(a % 64);
   const uint32_t salt = (uint32_t)t * 0x27D4EB2Fu + 0x165667B1u;
@@ -300,9 +301,11 @@ static inline void orc_q8s_round(float *x, float *y, int sy, int64_t t, int64_t 
   }
   const uint32_t h = orc_q8s_hash(t, i, a, A);
   const int j = (int)((a % 256) / 64);
-  const float d = fmaf((float)((h >> (8 * j)) & 0xffu), 1.0f / 256.0f, -127.5f / 256.0f);
-  // integer codes, so that a code of 0 is +0 (sign +1) as in the 8-bit form
-  const int yq = (int)rintf(fmaf(*y, ldexpf(1.0f, sy), d));
+  const int b = (int)((h >> (8 * j)) & 0xffu);
+  const int w = (int)rintf(ldexpf(*y, sy + 8));  /* exact product, |w| < 2^16 */
+  /* integer codes, so that a code of 0 is +0 (sign +1) as in the 8-bit
+   * form; the floor of a quotient of a possibly negative numerator */
+  const int yq = (w + b + 65536) / 256 - 256;
   const int xq = (int)rintf(ldexpf(*x, 6));
   *x = ldexpf((float)xq, -6);
   *y = ldexpf((float)yq, -sy);

@@ -261,6 +261,7 @@ __device__ __forceinline__ void qn_plane_st(uint32_t *P, int64_t ps, int p, int6
 // powers of two, so every product below is exact.
 struct Q12K {
   float c0, c1, c2, yscale, ylim, yinv, xoff, yoff, xbo, ybo, mb;
+  float ys8, mb8;  // 2^(sy+8) and 2^23 + 256 B: the stochastic Y encoder's
   uint32_t mask, wpos, wneg;
 };
 
@@ -280,8 +281,11 @@ __device__ __forceinline__ Q12K qn_consts(float c0, float c1, float c2, int qshi
   k.ybo = -B * k.yinv;                                                   // -B 2^-sy
   uint32_t mask = QFmt<QW>::MASK, wpos = QFmt<QW>::WALL_POS, wneg = QFmt<QW>::WALL_NEG;
   float mb = 8388608.0f + B;  // 2^23 + B: the rounding addend of the encoder
+  float mb8 = 8388608.0f + 256.0f * B;
   asm volatile("" : "+v"(mask), "+v"(wpos), "+v"(wneg), "+v"(xoff), "+v"(xbo), "+v"(mb));
   k.mb = mb;
+  k.ys8 = k.yscale * 256.0f;
+  k.mb8 = mb8;
   k.xoff = xoff;
   k.xbo = xbo;
   k.mask = mask;
@@ -377,18 +381,22 @@ __device__ __forceinline__ uint32_t qn_encode(float x, float y, const Q12K &k, b
   return __builtin_fabsf(x) > 1.0f ? (neg ? k.wneg : k.wpos) : in;
 }
 
-// The same with stochastic Y rounding (SIS_FORM_Q8S): the Y code is the
-// nearest-even integer to fma(y, 2^sy, d) for the element's dither d in
-// (-1/2, 1/2), i.e. floor(y 2^sy + u) with u = d + 1/2.  The sum with the
-// rounding addend 2^23 + B is exact before its one rounding.  No clamp: a
-// live element has |y| 2^sy <= YM + 1e-5 under the Y bound rule, so |d| < 1/2
-// keeps the code within +-YM.
+// The same with stochastic Y rounding (SIS_FORM_Q8S): the Y code is
+// floor((w + b) / 256) for w the nearest-even integer to y 2^(sy+8) and b the
+// element's dither byte, i.e. y 2^sy rounded down or up with the probability
+// of its fraction (to 1/256).  fma(y, 2^(sy+8), 2^23 + 256 B) is exact before
+// its one rounding, so its bits are those of 2^23 plus 256 B + w; b added as
+// an integer carries into bits 8..15, which then hold the biased Y code (the
+// unit's bits 8..15), and the fraction bits 0..7 are masked off.  No clamp: a
+// live element has |y| 2^sy <= YM + 1e-5 under the Y bound rule, so
+// 256 YM + 255 bounds w + b - 256 B from above and -256 YM from below.
 template <int QW>
 __device__ __forceinline__ uint32_t qn_encode_sr(float x, float y, const Q12K &k, bool &neg,
-                                                 float d) {
+                                                 uint32_t b) {
+  static_assert(QW == 8, "stochastic Y rounding is defined for the 8-bit codes");
   const uint32_t xb = __float_as_uint(__builtin_fmaf(x, (float)(1 << QFmt<QW>::XS), k.mb));
-  const uint32_t yb = __float_as_uint(__builtin_fmaf(y, k.yscale, d) + k.mb);
-  uint32_t in = (yb << QW) | xb;
+  const uint32_t ys = __float_as_uint(__builtin_fmaf(y, k.ys8, k.mb8)) + b;
+  uint32_t in = (ys & 0xff00u) | xb;
   neg = (int32_t)xb < (int32_t)(0x4b000000u | (1u << (QW - 1)));
   asm volatile("" : "+v"(in));
   return __builtin_fabsf(x) > 1.0f ? (neg ? k.wneg : k.wpos) : in;

@@ -380,9 +380,7 @@ __device__ __forceinline__ void qn_finish_row(const uint32_t d[QFmt<QW>::NP], co
     sb_update_maxcut_sel(x, y, F[j], k.c0, k.c1, k.c2);
     bool neg;
     if constexpr (SR) {
-      const float dj = __builtin_fmaf((float)((h >> (8 * j)) & 0xffu), 1.0f / 256.0f,
-                                      -127.5f / 256.0f);
-      un[j] = qn_encode_sr<QW>(x, y, k, neg, dj);
+      un[j] = qn_encode_sr<QW>(x, y, k, neg, (h >> (8 * j)) & 0xffu);
     } else {
       un[j] = qn_encode<QW, false>(x, y, k, neg);  // no Y clamp after a fused step
     }

@@ -205,8 +205,9 @@ def test_q8s_step_rounds_y_up_with_the_probability_of_its_fraction():
     # One step from the same on-grid state as the FP32 step: X rounds to
     # nearest even exactly as in the 8-bit form; each Y code is the floor or
     # the ceiling of the FP32 step's y 2^sy, and the share rounded up tracks
-    # the fraction (a dither on [0, 1) instead of [-1/2, 1/2) would round up
-    # about half a code too often, nearest-even rounding never for f < 1/2).
+    # the fraction (rounding the dithered value to nearest instead of down
+    # would round up about half a code too often, nearest-even rounding of y
+    # itself never for f < 1/2).
     pb, M, X, Y, sy = _trajectory(8, n=300, A=256, seed=11)
     xi0 = orc.xi0_maxcut(pb.n, pb.val64)
     t = 57
